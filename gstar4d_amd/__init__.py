@@ -1,0 +1,293 @@
+"""gStar4D for MI355X — host-side Python interface over the C ABI (include/gstar4d.h).
+
+The reference's triangulation interface (GDelaunay/GDelaunay/GDelaunay.h:39-42) is mirrored
+by gdel_init / gdel_compute / get_tetra / gdel_deinit; Triangulator is the object form.
+All computation runs in libgstar4d.so (hand-written gfx950 HIP kernels); when the library
+is missing this module raises instead of falling back to anything else.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+__all__ = ["Triangulator", "Options", "Stats", "G4DError", "load_library", "gdel_init",
+           "gdel_compute", "get_tetra", "gdel_deinit", "canonical", "LIB_PATH"]
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libgstar4d.so")
+
+STATUS = {0: "OK", 1: "INVALID_INPUT", 2: "DUPLICATE_POINTS", 3: "GRID_LIMIT", 4: "OOM",
+          5: "DEVICE", 6: "NOT_CONVERGED", 7: "CAPACITY", 8: "PREDICATE_RANGE", 9: "INTERNAL"}
+
+
+class G4DError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (STATUS.get(code, "?"), code, msg))
+        self.code = code
+        self.status = STATUS.get(code, "?")
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("grid_size", ctypes.c_int32), ("min_workset", ctypes.c_int32),
+                ("max_rounds", ctypes.c_int32), ("index_space", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("compute_adjacency", ctypes.c_int32)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("num_tets", ctypes.c_int64), ("tets", ctypes.POINTER(ctypes.c_int32)),
+                ("opp", ctypes.POINTER(ctypes.c_int32)), ("perm", ctypes.POINTER(ctypes.c_int32)),
+                ("num_points", ctypes.c_int32)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("t_morton_ms", ctypes.c_double), ("t_pba_ms", ctypes.c_double),
+                ("t_pairs_ms", ctypes.c_double), ("t_star_ms", ctypes.c_double),
+                ("t_splay_ms", ctypes.c_double), ("t_out_ms", ctypes.c_double),
+                ("t_adj_ms", ctypes.c_double), ("t_total_ms", ctypes.c_double),
+                ("num_points", ctypes.c_int32), ("grid_size", ctypes.c_int32),
+                ("splay_rounds", ctypes.c_int32), ("max_workset", ctypes.c_int32),
+                ("big_stars", ctypes.c_int32), ("pad0", ctypes.c_int32),
+                ("num_missing", ctypes.c_int64), ("num_grid_pairs", ctypes.c_int64),
+                ("num_workset", ctypes.c_int64), ("num_pumped", ctypes.c_int64),
+                ("splay_insertions", ctypes.c_int64), ("num_tets", ctypes.c_int64),
+                ("exact_calls", ctypes.c_int64), ("sos_calls", ctypes.c_int64),
+                ("pba_bytes", ctypes.c_int64), ("t_pba_pass_ms", ctypes.c_double * 3)]
+
+    def as_dict(self):
+        d = {}
+        for f, _ in self._fields_:
+            v = getattr(self, f)
+            d[f] = list(v) if f == "t_pba_pass_ms" else v
+        d.pop("pad0", None)
+        return d
+
+
+_LIB = None
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int32)
+_up = ctypes.POINTER(ctypes.c_uint64)
+_lp = ctypes.POINTER(ctypes.c_int64)
+
+EXPORTS = ["g4d_default_options", "g4d_create", "g4d_destroy", "g4d_last_error", "g4d_device_count",
+           "g4d_triangulate", "g4d_triangulate_device", "g4d_copy_result", "g4d_upload_points",
+           "g4d_get_stats", "g4d_set_options", "g4d_diag_orient4", "g4d_diag_orient3d",
+           "g4d_diag_morton", "g4d_diag_pba", "g4d_diag_grid_pairs"]
+
+
+def load_library(path=None):
+    """Loads libgstar4d.so (building nothing); raises if it is absent."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    path = path or LIB_PATH
+    if not os.path.exists(path):
+        raise RuntimeError("libgstar4d.so not built (run __graft_entry__.build()); no fallback exists")
+    L = ctypes.CDLL(path)
+    L.g4d_default_options.argtypes = [ctypes.POINTER(Options)]
+    L.g4d_create.argtypes = [ctypes.POINTER(_vp), ctypes.POINTER(Options)]
+    L.g4d_destroy.argtypes = [_vp]
+    L.g4d_destroy.restype = None
+    L.g4d_last_error.argtypes = [_vp]
+    L.g4d_last_error.restype = ctypes.c_char_p
+    L.g4d_triangulate.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.POINTER(Result)]
+    L.g4d_triangulate_device.argtypes = [_vp, _vp, ctypes.c_int32, _lp]
+    L.g4d_copy_result.argtypes = [_vp, _ip, _ip, _ip]
+    L.g4d_upload_points.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.POINTER(_vp)]
+    L.g4d_get_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
+    L.g4d_set_options.argtypes = [_vp, ctypes.POINTER(Options)]
+    L.g4d_diag_orient4.argtypes = [_vp, _dp, _ip, ctypes.c_int64, ctypes.c_int32, _ip, _ip]
+    L.g4d_diag_orient3d.argtypes = [_vp, _dp, ctypes.c_int64, _ip]
+    L.g4d_diag_morton.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.c_int32, _ip, _ip]
+    L.g4d_diag_pba.argtypes = [_vp, _ip, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _ip]
+    L.g4d_diag_grid_pairs.argtypes = [_vp, _ip, ctypes.c_int32, _up, ctypes.c_int64, _lp]
+    _LIB = L
+    return L
+
+
+def _ptr(a, t):
+    return a.ctypes.data_as(t)
+
+
+def default_options(**kw):
+    L = load_library()
+    o = Options()
+    L.g4d_default_options(ctypes.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, int(v))
+    return o
+
+
+class Triangulator:
+    """One device context.  triangulate(xyz) returns (tets, opp, perm): tets (T,4) int32 with
+    orient3d(v0,v1,v2,v3) < 0, opp (T,4) neighbour across the face opposite each vertex or -1,
+    perm (N,) Morton rank -> input index."""
+
+    def __init__(self, grid_size=256, min_workset=8, max_rounds=1000, index_space="input",
+                 device=0, adjacency=True):
+        self._L = load_library()
+        self.opt = default_options(grid_size=grid_size, min_workset=min_workset,
+                                   max_rounds=max_rounds,
+                                   index_space=0 if index_space == "input" else 1,
+                                   device=device, compute_adjacency=1 if adjacency else 0)
+        self._ctx = _vp()
+        rc = self._L.g4d_create(ctypes.byref(self._ctx), ctypes.byref(self.opt))
+        if rc != 0:
+            raise G4DError(rc, "g4d_create failed (no HIP device?)")
+
+    def close(self):
+        if self._ctx:
+            self._L.g4d_destroy(self._ctx)
+            self._ctx = _vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            raise G4DError(rc, self._L.g4d_last_error(self._ctx).decode())
+
+    def set_grid(self, grid_size):
+        self.opt.grid_size = int(grid_size)
+        self._check(self._L.g4d_set_options(self._ctx, ctypes.byref(self.opt)))
+
+    def triangulate(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        if xyz.ndim != 2 or xyz.shape[1] != 3:
+            raise ValueError("xyz must be (N, 3)")
+        res = Result()
+        self._check(self._L.g4d_triangulate(self._ctx, _ptr(xyz, _dp), xyz.shape[0], ctypes.byref(res)))
+        nt = res.num_tets
+        tets = np.ctypeslib.as_array(res.tets, shape=(max(nt, 1), 4))[:nt].copy()
+        opp = (np.ctypeslib.as_array(res.opp, shape=(max(nt, 1), 4))[:nt].copy()
+               if res.opp else None)
+        perm = np.ctypeslib.as_array(res.perm, shape=(xyz.shape[0],)).copy()
+        return tets, opp, perm
+
+    def upload(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        d = _vp()
+        self._check(self._L.g4d_upload_points(self._ctx, _ptr(xyz, _dp), xyz.shape[0], ctypes.byref(d)))
+        return d.value, xyz.shape[0]
+
+    def triangulate_device(self, d_xyz, n):
+        nt = ctypes.c_int64(0)
+        self._check(self._L.g4d_triangulate_device(self._ctx, _vp(d_xyz), int(n), ctypes.byref(nt)))
+        return nt.value
+
+    def copy_result(self, nt, n, want_opp=True):
+        tets = np.empty((max(nt, 1), 4), np.int32)
+        opp = np.empty((max(nt, 1), 4), np.int32) if want_opp else None
+        perm = np.empty(n, np.int32)
+        self._check(self._L.g4d_copy_result(self._ctx, _ptr(tets, _ip),
+                                            _ptr(opp, _ip) if want_opp else None, _ptr(perm, _ip)))
+        return tets[:nt], (opp[:nt] if want_opp else None), perm
+
+    def stats(self):
+        st = Stats()
+        self._check(self._L.g4d_get_stats(self._ctx, ctypes.byref(st)))
+        return st.as_dict()
+
+    # ---- diagnostics ----
+    def diag_orient4(self, pts5, idx5, mode=0):
+        pts5 = np.ascontiguousarray(pts5, dtype=np.float64).reshape(-1, 15)
+        idx5 = np.ascontiguousarray(idx5, dtype=np.int32).reshape(-1, 5)
+        T = pts5.shape[0]
+        sign = np.zeros(T, np.int32)
+        depth = np.zeros(T, np.int32)
+        self._check(self._L.g4d_diag_orient4(self._ctx, _ptr(pts5, _dp), _ptr(idx5, _ip), T, mode,
+                                             _ptr(sign, _ip), _ptr(depth, _ip)))
+        return sign, depth
+
+    def diag_orient3d(self, pts4):
+        pts4 = np.ascontiguousarray(pts4, dtype=np.float64).reshape(-1, 12)
+        sign = np.zeros(pts4.shape[0], np.int32)
+        self._check(self._L.g4d_diag_orient3d(self._ctx, _ptr(pts4, _dp), pts4.shape[0], _ptr(sign, _ip)))
+        return sign
+
+    def diag_morton(self, xyz, grid):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        n = xyz.shape[0]
+        perm = np.empty(n, np.int32)
+        vox = np.empty((n, 3), np.int32)
+        self._check(self._L.g4d_diag_morton(self._ctx, _ptr(xyz, _dp), n, grid, _ptr(perm, _ip), _ptr(vox, _ip)))
+        return perm, vox
+
+    def diag_pba(self, vox_sorted, grid, codes=False):
+        v = np.ascontiguousarray(vox_sorted, dtype=np.int32)
+        out = np.empty(grid ** 3, np.int32)
+        self._check(self._L.g4d_diag_pba(self._ctx, _ptr(v, _ip), v.shape[0], grid, int(codes), _ptr(out, _ip)))
+        return out.reshape(grid, grid, grid)
+
+    def diag_grid_pairs(self, id_grid):
+        g = id_grid.shape[0]
+        grid = np.ascontiguousarray(id_grid, dtype=np.int32)
+        cnt = ctypes.c_int64(0)
+        self._check(self._L.g4d_diag_grid_pairs(self._ctx, _ptr(grid, _ip), g, None, 0, ctypes.byref(cnt)))
+        out = np.empty(max(cnt.value, 1), np.uint64)
+        self._check(self._L.g4d_diag_grid_pairs(self._ctx, _ptr(grid, _ip), g, _ptr(out, _up), out.shape[0],
+                                                ctypes.byref(cnt)))
+        return out[:cnt.value]
+
+
+def canonical(tets, perm=None):
+    """Canonical tetra set: optional index map, each quadruple sorted, rows sorted."""
+    t = np.asarray(tets, dtype=np.int64)
+    if perm is not None:
+        t = np.asarray(perm, dtype=np.int64)[t]
+    t = np.sort(t, axis=1)
+    order = np.lexsort((t[:, 3], t[:, 2], t[:, 1], t[:, 0]))
+    return t[order].astype(np.int32)
+
+
+# ---- the reference's four-call interface (module-level state, like GDelHost.cu's globals) ----
+_STATE = {"tri": None, "points": None, "result": None, "times": None}
+
+
+def gdel_init(config, points, scaled=None):
+    """gdelInit (GD/GDelaunay/GDelaunay.cpp:47-60).  config: dict with grid_size (and optional
+    min_workset / device).  Like the reference, `points` (an (N,3) float64 array) is reordered
+    in place into Morton order when config['sort_in_place'] is true (the default)."""
+    gdel_deinit()
+    tri = Triangulator(grid_size=config.get("grid_size", 256), min_workset=config.get("min_workset", 8),
+                       index_space="morton", device=config.get("device", 0))
+    _STATE.update(tri=tri, points=points, result=None, times=None,
+                  sort=config.get("sort_in_place", True))
+
+
+def gdel_compute():
+    """gdelCompute: returns (t_pba, t_star, t_splay, t_out) in ms."""
+    tri = _STATE["tri"]
+    if tri is None:
+        raise RuntimeError("gdel_init not called")
+    pts = _STATE["points"]
+    tets, opp, perm = tri.triangulate(pts)
+    if _STATE.get("sort", True):
+        pts[:] = pts[perm]  # GD/GDelaunay/GDelData.cu:125-130 rewrites the caller's points
+    _STATE["result"] = (tets, opp, perm)
+    st = tri.stats()
+    _STATE["times"] = (st["t_pba_ms"], st["t_star_ms"], st["t_splay_ms"], st["t_out_ms"])
+    return _STATE["times"]
+
+
+def get_tetra():
+    """getTetraFromGpu: (tets, opp) in Morton index space (the reordered points)."""
+    if _STATE["result"] is None:
+        raise RuntimeError("gdel_compute not called")
+    tets, opp, _ = _STATE["result"]
+    return tets, opp
+
+
+def gdel_deinit():
+    if _STATE.get("tri") is not None:
+        _STATE["tri"].close()
+    _STATE.update(tri=None, points=None, result=None, times=None)

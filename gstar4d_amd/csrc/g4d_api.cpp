@@ -1,0 +1,796 @@
+// gStar4D for MI355X — host orchestration behind the C ABI (include/gstar4d.h).
+//
+// Mirrors the reference's gdelInit / gdelCompute / getTetraFromGpu / gdelDeInit
+// (GD/GDelaunay/GDelaunay.cpp:47-104) with the stages of GD/GDelaunay/GDelHost.cu:
+// Morton sort -> PBA -> pairs and worksets -> star creation -> splaying rounds -> tetra output.
+// All stage data stays in HBM; the host reads back only counters that size the next stage
+// (one small read per stage, where the reference synchronises after every launch).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "gstar4d.h"
+#include "g4d_diag.h"
+#include "g4d_grid.h"
+#include "g4d_prims.h"
+#include "g4d_star.h"
+
+using g4d::DevBuf;
+
+namespace {
+
+struct Pool {
+    DevBuf vnum, voff, vpool, tnum, toff, tpool;
+    void release()
+    {
+        vnum.release(); voff.release(); vpool.release(); tnum.release(); toff.release(); tpool.release();
+    }
+};
+
+struct Hist {
+    DevBuf keys, off;
+    void release()
+    {
+        keys.release();
+        off.release();
+    }
+};
+
+enum { EV_START, EV_MORTON, EV_PBA0, EV_PBA1, EV_PBA2, EV_PBA3, EV_PAIRS, EV_STAR, EV_SPLAY, EV_OUT, EV_ADJ, EV_NUM };
+
+} // namespace
+
+struct g4d_ctx {
+    g4d_options opt;
+    hipStream_t stream = nullptr;
+    std::string err;
+    g4d::Prims P;
+    DevBuf xyz_up, keys, pts, perm, vox, bounds, grid, gtmp, idgrid, pairs, ws_off, cnt;
+    Pool pool[2];
+    Hist hist[2];
+    DevBuf drown, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, big, tetcnt, tetoff, tets, opp, table;
+    std::vector<int32_t> h_tets, h_opp, h_perm;
+    int64_t num_tets = 0;
+    int last_n = 0;
+    bool have_result = false;
+    g4d_stats st;
+    hipEvent_t ev[EV_NUM];
+    bool ev_ok = false;
+};
+
+static int set_err(g4d_ctx *c, int code, const char *fmt, ...)
+{
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->err = buf;
+    return code;
+}
+
+static int hip_fail(g4d_ctx *c, hipError_t e, const char *what, int line)
+{
+    const int code = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? G4D_ERR_OOM : G4D_ERR_DEVICE;
+    return set_err(c, code, "%s failed at g4d_api.cpp:%d: %s", what, line, hipGetErrorString(e));
+}
+
+#define HCK(x)                                                                                                         \
+    do {                                                                                                               \
+        hipError_t e_ = (x);                                                                                           \
+        if (e_ != hipSuccess) return hip_fail(ctx, e_, #x, __LINE__);                                                  \
+    } while (0)
+
+static int bits_for(int64_t n)
+{
+    int b = 1;
+    while (((int64_t)1 << b) < n) b++;
+    return b;
+}
+
+static uint64_t next_pow2(uint64_t v)
+{
+    uint64_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+template <typename T>
+static T *buf(DevBuf &b)
+{
+    return b.as<T>();
+}
+
+// Sort directed keys (a<<32 | b) with a, b < 2^nb.
+static hipError_t sort_pairs(g4d::Prims &P, uint64_t *k, int64_t m, int nb, hipStream_t s)
+{
+    hipError_t e = g4d::radix_sort_u64(P, k, m, 0, nb, s);
+    if (e != hipSuccess) return e;
+    return g4d::radix_sort_u64(P, k, m, 32, 32 + nb, s);
+}
+
+static int read_counters(g4d_ctx *ctx, unsigned long long *h)
+{
+    HCK(hipMemcpyAsync(h, ctx->cnt.ptr, sizeof(unsigned long long) * G4D_CNT_NUM, hipMemcpyDeviceToHost, ctx->stream));
+    HCK(hipStreamSynchronize(ctx->stream));
+    return G4D_OK;
+}
+
+static int check_dev_error(g4d_ctx *ctx, unsigned long long flags, const char *stage)
+{
+    if (!flags) return G4D_OK;
+    if (flags & G4D_DEV_ERR_DUPLICATE) return set_err(ctx, G4D_ERR_DUPLICATE_POINTS, "%s: duplicate input points", stage);
+    if (flags & G4D_DEV_ERR_PREDICATE_RANGE)
+        return set_err(ctx, G4D_ERR_PREDICATE_RANGE, "%s: coordinates outside the exact-arithmetic range", stage);
+    if (flags & G4D_DEV_ERR_STAR_CAPACITY)
+        return set_err(ctx, G4D_ERR_CAPACITY, "%s: a star exceeds %d link vertices", stage, g4d::STAR_VMAX_BIG);
+    return set_err(ctx, G4D_ERR_INTERNAL, "%s: device error flags 0x%llx", stage, flags);
+}
+
+#define CK(x)                                                                                                          \
+    do {                                                                                                               \
+        int r_ = (x);                                                                                                  \
+        if (r_ != G4D_OK) return r_;                                                                                   \
+    } while (0)
+
+static int ensure_pool(g4d_ctx *ctx, Pool &pl, int n, int64_t vtot, int64_t ttot)
+{
+    HCK(pl.vnum.ensure(sizeof(int) * (size_t)n));
+    HCK(pl.tnum.ensure(sizeof(int) * (size_t)n));
+    HCK(pl.voff.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+    HCK(pl.toff.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+    HCK(pl.vpool.ensure(sizeof(int) * (size_t)std::max<int64_t>(vtot, 1)));
+    HCK(pl.tpool.ensure(sizeof(uint64_t) * (size_t)std::max<int64_t>(ttot, 1)));
+    return G4D_OK;
+}
+
+static void ev_record(g4d_ctx *ctx, int e)
+{
+    if (ctx->ev_ok) (void)hipEventRecord(ctx->ev[e], ctx->stream);
+}
+
+static double ev_ms(g4d_ctx *ctx, int a, int b)
+{
+    float ms = 0.f;
+    if (!ctx->ev_ok) return 0.0;
+    if (hipEventElapsedTime(&ms, ctx->ev[a], ctx->ev[b]) != hipSuccess) return 0.0;
+    return ms;
+}
+
+// The whole path on device-resident FP64 points.
+static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
+{
+    hipStream_t s = ctx->stream;
+    g4d::Prims &P = ctx->P;
+    const int G = ctx->opt.grid_size;
+    if (G < 4 || G > 1024) return set_err(ctx, G4D_ERR_GRID_LIMIT, "grid_size %d outside [4, 1024]", G);
+    if (n < 5) return set_err(ctx, G4D_ERR_INVALID_INPUT, "need at least 5 points, got %d", n);
+    if (n > (1 << 29)) return set_err(ctx, G4D_ERR_INVALID_INPUT, "too many points (%d)", n);
+    const int min_ws = ctx->opt.min_workset;
+    if (min_ws < 4 || min_ws > 64) return set_err(ctx, G4D_ERR_INVALID_INPUT, "min_workset %d outside [4, 64]", min_ws);
+    const int nb = bits_for(n);
+    memset(&ctx->st, 0, sizeof(ctx->st));
+    ctx->st.num_points = n;
+    ctx->st.grid_size = G;
+    ctx->have_result = false;
+    unsigned long long hc[G4D_CNT_NUM];
+
+    HCK(ctx->cnt.ensure(sizeof(unsigned long long) * G4D_CNT_NUM));
+    unsigned long long *cnt = buf<unsigned long long>(ctx->cnt);
+    HCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * G4D_CNT_NUM, s));
+    ev_record(ctx, EV_START);
+
+    // ---- B: Morton order (GD/GDelaunay/GDelData.cu:91-131) ----
+    HCK(ctx->bounds.ensure(sizeof(double) * 512));
+    HCK(g4d::launch_bounds(d_xyz, n, buf<double>(ctx->bounds), s));
+    double hb[2];
+    HCK(hipMemcpyAsync(hb, ctx->bounds.ptr, sizeof(hb), hipMemcpyDeviceToHost, s));
+    HCK(hipStreamSynchronize(s));
+    if (!std::isfinite(hb[0]) || !std::isfinite(hb[1]) || !(hb[1] > hb[0]))
+        return set_err(ctx, G4D_ERR_INVALID_INPUT, "coordinates must be finite and not all equal");
+    HCK(ctx->keys.ensure(sizeof(uint64_t) * (size_t)n));
+    HCK(ctx->pts.ensure(sizeof(double) * 3 * (size_t)n));
+    HCK(ctx->perm.ensure(sizeof(int) * (size_t)n));
+    HCK(ctx->vox.ensure(sizeof(int) * (size_t)n));
+    uint64_t *keys = buf<uint64_t>(ctx->keys);
+    HCK(g4d::launch_morton_keys(d_xyz, n, G, buf<double>(ctx->bounds), keys, s));
+    HCK(g4d::radix_sort_u64(P, keys, n, 32, 62, s));
+    double *pts = buf<double>(ctx->pts);
+    int *perm = buf<int>(ctx->perm), *vox = buf<int>(ctx->vox);
+    HCK(g4d::launch_gather_sorted(d_xyz, keys, n, G, buf<double>(ctx->bounds), pts, perm, vox, s));
+    HCK(g4d::launch_dup_check(pts, vox, n, cnt, s));
+    ev_record(ctx, EV_MORTON);
+
+    // ---- C: PBA (GD/PBA/Pba.cu:105-122) ----
+    const int64_t G3 = (int64_t)G * G * G;
+    HCK(ctx->grid.ensure(sizeof(int) * (size_t)G3));
+    HCK(ctx->gtmp.ensure(sizeof(int) * (size_t)G3));
+    HCK(ctx->idgrid.ensure(sizeof(int) * (size_t)G3));
+    int *grid = buf<int>(ctx->grid), *gtmp = buf<int>(ctx->gtmp), *idgrid = buf<int>(ctx->idgrid);
+    HCK(g4d::launch_fill_i32(grid, G3, G4D_MARKER, s));
+    HCK(g4d::launch_scatter_sites(vox, n, G, grid, idgrid, s));
+    ev_record(ctx, EV_PBA0);
+    HCK(g4d::pba_pass(2, G, grid, gtmp, nullptr, s));
+    ev_record(ctx, EV_PBA1);
+    HCK(g4d::pba_pass(1, G, gtmp, grid, nullptr, s));
+    ev_record(ctx, EV_PBA2);
+    HCK(g4d::pba_pass(0, G, grid, gtmp, idgrid, s)); // point ids in gtmp
+    ev_record(ctx, EV_PBA3);
+    const int *idg = gtmp;
+
+    // ---- D: missing points, grid pairs, worksets, pumping (GD/GDelaunay/GDelHost.cu:259-529) ----
+    int64_t cap = std::max<int64_t>(48 * (int64_t)n, 1 << 20);
+    int64_t m = 0, nmiss = 0;
+    for (int attempt = 0; attempt < 3; attempt++) {
+        HCK(ctx->pairs.ensure(sizeof(uint64_t) * (size_t)(cap + 2 * (int64_t)n * min_ws + 64)));
+        uint64_t *pr = buf<uint64_t>(ctx->pairs);
+        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8, s));
+        HCK(g4d::launch_missing(vox, n, G, idg, pr, cnt + G4D_CNT_APPEND0, cap, s));
+        CK(read_counters(ctx, hc));
+        nmiss = (int64_t)hc[G4D_CNT_APPEND0];
+        HCK(g4d::launch_grid_pairs(idg, G, pr, cnt + G4D_CNT_APPEND0, cap, s));
+        CK(read_counters(ctx, hc));
+        m = 2 * (int64_t)hc[G4D_CNT_APPEND0];
+        if (m <= cap) break;
+        cap = m + m / 4;
+        m = -1;
+    }
+    if (m < 0) return set_err(ctx, G4D_ERR_INTERNAL, "pair buffer sizing failed");
+    CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], "morton"));
+    uint64_t *pr = buf<uint64_t>(ctx->pairs);
+    HCK(sort_pairs(P, pr, m, nb, s));
+    HCK(g4d::unique_u64(P, pr, m, &m, s));
+    ctx->st.num_missing = nmiss;
+    ctx->st.num_grid_pairs = (m - 2 * nmiss) / 2;
+    HCK(ctx->ws_off.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+    int64_t *ws_off = buf<int64_t>(ctx->ws_off);
+    HCK(g4d::launch_csr_offsets(pr, m, n, ws_off, s));
+    HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8, s));
+    HCK(g4d::launch_pump(pr, ws_off, n, min_ws, pr + m, cnt + G4D_CNT_APPEND0, 2 * (int64_t)n * min_ws + 64, s));
+    CK(read_counters(ctx, hc));
+    const int64_t npump = (int64_t)hc[G4D_CNT_APPEND0];
+    if (npump > 0) {
+        int64_t m2 = m + 2 * npump;
+        HCK(sort_pairs(P, pr, m2, nb, s));
+        HCK(g4d::unique_u64(P, pr, m2, &m2, s));
+        ctx->st.num_pumped = m2 - m;
+        m = m2;
+        HCK(g4d::launch_csr_offsets(pr, m, n, ws_off, s));
+    }
+    ctx->st.num_workset = m;
+    ev_record(ctx, EV_PAIRS);
+
+    // ---- E: star creation (GD/GDelaunay/GDelHost.cu:543-743) ----
+    HCK(ctx->vcap.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+    HCK(ctx->tcap.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+    HCK(ctx->hcap.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+    HCK(ctx->big.ensure(sizeof(int) * (size_t)n));
+    int64_t *vcap = buf<int64_t>(ctx->vcap), *tcap = buf<int64_t>(ctx->tcap), *hcap = buf<int64_t>(ctx->hcap);
+    const int nblk_star = 4096;
+
+    auto star_pass = [&](int mode, const uint64_t *ins_keys, const int64_t *ins_off, int64_t nins, Pool *in,
+                         Pool &out, const uint64_t *hkeys, const int64_t *hoff, Hist *hout, int64_t *drown_n,
+                         int64_t *tp_n) -> int {
+        HCK(g4d::launch_capacities(n, in ? buf<int>(in->vnum) : nullptr, ins_off, hoff, vcap, tcap, hout ? hcap : nullptr,
+                                   s));
+        HCK(out.voff.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+        HCK(out.toff.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+        int64_t *voff = buf<int64_t>(out.voff), *toff = buf<int64_t>(out.toff);
+        HCK(g4d::scan_excl_i64(P, vcap, voff, n, voff + n, s));
+        HCK(g4d::scan_excl_i64(P, tcap, toff, n, toff + n, s));
+        int64_t tot[2];
+        HCK(hipMemcpyAsync(&tot[0], voff + n, 8, hipMemcpyDeviceToHost, s));
+        HCK(hipMemcpyAsync(&tot[1], toff + n, 8, hipMemcpyDeviceToHost, s));
+        int64_t htot = 0;
+        if (hout) {
+            HCK(hout->off.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+            int64_t *ho = buf<int64_t>(hout->off);
+            HCK(g4d::scan_excl_i64(P, hcap, ho, n, ho + n, s));
+            HCK(hipMemcpyAsync(&htot, ho + n, 8, hipMemcpyDeviceToHost, s));
+        }
+        HCK(hipStreamSynchronize(s));
+        CK(ensure_pool(ctx, out, n, tot[0], tot[1]));
+        if (hout) HCK(hout->keys.ensure(sizeof(uint64_t) * (size_t)std::max<int64_t>(htot, 1)));
+        const int64_t dcap = std::max<int64_t>(nins, 1);
+        const int64_t tpcap = 2 * tot[1] + 4 * dcap + 64;
+        HCK(ctx->drown.ensure(sizeof(uint64_t) * (size_t)dcap));
+        HCK(ctx->cand.ensure(sizeof(uint64_t) * (size_t)tpcap));
+        g4d::StarParams sp;
+        memset(&sp, 0, sizeof(sp));
+        sp.pts = pts;
+        sp.n = n;
+        sp.mode = mode;
+        sp.ins_keys = ins_keys;
+        sp.ins_off = ins_off;
+        if (in) {
+            sp.in_vnum = buf<int>(in->vnum);
+            sp.in_voff = buf<int64_t>(in->voff);
+            sp.in_vpool = buf<int>(in->vpool);
+            sp.in_tnum = buf<int>(in->tnum);
+            sp.in_toff = buf<int64_t>(in->toff);
+            sp.in_tpool = buf<uint64_t>(in->tpool);
+        }
+        sp.out_vnum = buf<int>(out.vnum);
+        sp.out_voff = voff;
+        sp.out_vpool = buf<int>(out.vpool);
+        sp.out_tnum = buf<int>(out.tnum);
+        sp.out_toff = toff;
+        sp.out_tpool = buf<uint64_t>(out.tpool);
+        sp.hist_keys = hkeys;
+        sp.hist_off = hoff;
+        if (hout) {
+            sp.hist_out = buf<uint64_t>(hout->keys);
+            sp.hist_out_off = buf<int64_t>(hout->off);
+        }
+        sp.drown = buf<uint64_t>(ctx->drown);
+        sp.drown_cap = dcap;
+        sp.tpairs = buf<uint64_t>(ctx->cand);
+        sp.tpairs_cap = tpcap;
+        sp.big_list = buf<int>(ctx->big);
+        sp.big_cap = n;
+        sp.cnt = cnt;
+        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8 * 3, s));
+        HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
+        HCK(g4d::launch_star_kernel(sp, false, nblk_star, s));
+        CK(read_counters(ctx, hc));
+        const int64_t nbig = (int64_t)hc[G4D_CNT_APPEND2];
+        if (nbig > 0) {
+            ctx->st.big_stars += (int32_t)nbig;
+            sp.star_list = buf<int>(ctx->big);
+            sp.nlist = (int)nbig;
+            HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
+            HCK(g4d::launch_star_kernel(sp, true, (int)std::min<int64_t>(nbig, 1024), s));
+            CK(read_counters(ctx, hc));
+        }
+        CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], mode ? "splaying" : "star creation"));
+        *drown_n = (int64_t)hc[G4D_CNT_APPEND0];
+        *tp_n = (int64_t)hc[G4D_CNT_APPEND1];
+        return G4D_OK;
+    };
+
+    int cur = 0;
+    int64_t nd = 0, ntp = 0;
+    CK(star_pass(0, pr, ws_off, m, nullptr, ctx->pool[0], nullptr, nullptr, nullptr, &nd, &ntp));
+    const uint64_t *hkeys = pr;
+    const int64_t *hoff = ws_off;
+    ev_record(ctx, EV_STAR);
+
+    // ---- F: splaying rounds (GD/GDelaunay/GDelHost.cu:1389-1470) ----
+    int rounds = 0;
+    bool converged = false;
+    for (int it = 0; it < ctx->opt.max_rounds; it++) {
+        // F1-F3: drowned insertions, mutual removal, containment proofs
+        int64_t ntask = 0;
+        if (nd > 0) {
+            HCK(ctx->dkeys.ensure(sizeof(uint64_t) * (size_t)nd));
+            HCK(ctx->tasks.ensure(sizeof(uint64_t) * (size_t)nd));
+            uint64_t *dk = buf<uint64_t>(ctx->dkeys);
+            HCK(g4d::launch_drown_keys(buf<uint64_t>(ctx->drown), nd, dk, s));
+            HCK(g4d::radix_sort_u64(P, dk, nd, 0, 1 + nb, s));
+            HCK(g4d::radix_sort_u64(P, dk, nd, 33, 33 + nb, s));
+            HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8, s));
+            HCK(g4d::launch_nonmutual(dk, nd, buf<uint64_t>(ctx->tasks), cnt, s));
+            CK(read_counters(ctx, hc));
+            ntask = (int64_t)hc[G4D_CNT_APPEND0];
+            if (ntask > 0) {
+                Pool &pl = ctx->pool[cur];
+                g4d::ProofParams pp;
+                pp.pts = pts;
+                pp.tasks = buf<uint64_t>(ctx->tasks);
+                pp.ntasks = ntask;
+                pp.vnum = buf<int>(pl.vnum);
+                pp.voff = buf<int64_t>(pl.voff);
+                pp.vpool = buf<int>(pl.vpool);
+                pp.tnum = buf<int>(pl.tnum);
+                pp.toff = buf<int64_t>(pl.toff);
+                pp.tpool = buf<uint64_t>(pl.tpool);
+                pp.out = buf<uint64_t>(ctx->cand) + ntp;
+                pp.cnt = cnt;
+                HCK(g4d::launch_proofs(pp, s));
+            }
+        }
+        // F4-F5: candidates = triangle pairs + proofs, history filter, sort-unique
+        const int64_t ncand = ntp + 4 * ntask;
+        int64_t nins = 0;
+        if (ncand > 0) {
+            HCK(ctx->keep.ensure((size_t)ncand));
+            HCK(ctx->ins.ensure(sizeof(uint64_t) * (size_t)(2 * ncand + 2)));
+            uint64_t *ins = buf<uint64_t>(ctx->ins);
+            HCK(g4d::launch_hist_filter(buf<uint64_t>(ctx->cand), ncand, hkeys, hoff, buf<uint8_t>(ctx->keep), s));
+            HCK(g4d::compact_u64(P, buf<uint64_t>(ctx->cand), buf<uint8_t>(ctx->keep), ins, ncand, &nins, s));
+            if (nins > 0) {
+                HCK(sort_pairs(P, ins, nins, nb, s));
+                HCK(g4d::unique_u64(P, ins, nins, &nins, s));
+            }
+        }
+        CK(read_counters(ctx, hc));
+        CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], "proofs"));
+        if (nins == 0) {
+            converged = true;
+            break;
+        }
+        rounds++;
+        // F6-F9: reciprocate, group by star, insert
+        uint64_t *ins = buf<uint64_t>(ctx->ins);
+        HCK(g4d::launch_reciprocate(ins, nins, s));
+        const int64_t nrec = 2 * nins;
+        HCK(sort_pairs(P, ins, nrec, nb, s));
+        ctx->st.splay_insertions += nrec;
+        HCK(ctx->ins_off.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+        int64_t *ioff = buf<int64_t>(ctx->ins_off);
+        HCK(g4d::launch_csr_offsets(ins, nrec, n, ioff, s));
+        const int nxt = cur ^ 1;
+        Hist &hnext = ctx->hist[rounds & 1];
+        CK(star_pass(1, ins, ioff, nrec, &ctx->pool[cur], ctx->pool[nxt], hkeys, hoff, &hnext, &nd, &ntp));
+        hkeys = buf<uint64_t>(hnext.keys);
+        hoff = buf<int64_t>(hnext.off);
+        cur = nxt;
+    }
+    if (!converged) return set_err(ctx, G4D_ERR_NOT_CONVERGED, "splaying did not converge in %d rounds", rounds);
+    ctx->st.splay_rounds = rounds;
+    ev_record(ctx, EV_SPLAY);
+
+    // ---- G: tetrahedra (GD/GDelaunay/GDelHost.cu:1520-1615) ----
+    Pool &pl = ctx->pool[cur];
+    HCK(ctx->tetcnt.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+    HCK(ctx->tetoff.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+    g4d::OutParams op;
+    op.pts = pts;
+    op.n = n;
+    op.vnum = buf<int>(pl.vnum);
+    op.voff = buf<int64_t>(pl.voff);
+    op.vpool = buf<int>(pl.vpool);
+    op.tnum = buf<int>(pl.tnum);
+    op.toff = buf<int64_t>(pl.toff);
+    op.tpool = buf<uint64_t>(pl.tpool);
+    op.star_tet_count = buf<int64_t>(ctx->tetcnt);
+    op.star_tet_off = buf<int64_t>(ctx->tetoff);
+    op.cnt = cnt;
+    op.tets = nullptr;
+    HCK(g4d::launch_out_count(op, s));
+    int64_t *toffs = buf<int64_t>(ctx->tetoff);
+    HCK(g4d::scan_excl_i64(P, buf<int64_t>(ctx->tetcnt), toffs, n, toffs + n, s));
+    int64_t nt = 0;
+    HCK(hipMemcpyAsync(&nt, toffs + n, 8, hipMemcpyDeviceToHost, s));
+    HCK(hipStreamSynchronize(s));
+    HCK(ctx->tets.ensure(sizeof(int) * 4 * (size_t)std::max<int64_t>(nt, 1)));
+    op.tets = buf<int>(ctx->tets);
+    HCK(g4d::launch_out_write(op, s));
+    ev_record(ctx, EV_OUT);
+    if (ctx->opt.compute_adjacency) {
+        const uint64_t tsz = next_pow2((uint64_t)std::max<int64_t>(8 * nt, 1024));
+        HCK(ctx->table.ensure(sizeof(uint32_t) * (size_t)tsz));
+        HCK(ctx->opp.ensure(sizeof(int) * 4 * (size_t)std::max<int64_t>(nt, 1)));
+        HCK(g4d::launch_adjacency(buf<int>(ctx->tets), nt, buf<int>(ctx->opp), buf<uint32_t>(ctx->table), (int64_t)tsz,
+                                  cnt, s));
+    }
+    if (ctx->opt.index_space == G4D_INDEX_INPUT) HCK(g4d::launch_map_perm(buf<int>(ctx->tets), 4 * nt, perm, s));
+    ev_record(ctx, EV_ADJ);
+    CK(read_counters(ctx, hc));
+    CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], "output"));
+    ctx->num_tets = nt;
+    ctx->last_n = n;
+    ctx->have_result = true;
+    ctx->st.num_tets = nt;
+    ctx->st.exact_calls = (int64_t)hc[G4D_CNT_EXACT];
+    ctx->st.sos_calls = (int64_t)hc[G4D_CNT_SOS];
+    ctx->st.pba_bytes = 24 * G3;
+    ctx->st.t_morton_ms = ev_ms(ctx, EV_START, EV_MORTON);
+    ctx->st.t_pba_ms = ev_ms(ctx, EV_MORTON, EV_PBA3);
+    ctx->st.t_pba_pass_ms[0] = ev_ms(ctx, EV_PBA0, EV_PBA1);
+    ctx->st.t_pba_pass_ms[1] = ev_ms(ctx, EV_PBA1, EV_PBA2);
+    ctx->st.t_pba_pass_ms[2] = ev_ms(ctx, EV_PBA2, EV_PBA3);
+    ctx->st.t_pairs_ms = ev_ms(ctx, EV_PBA3, EV_PAIRS);
+    ctx->st.t_star_ms = ev_ms(ctx, EV_PAIRS, EV_STAR);
+    ctx->st.t_splay_ms = ev_ms(ctx, EV_STAR, EV_SPLAY);
+    ctx->st.t_out_ms = ev_ms(ctx, EV_SPLAY, EV_OUT);
+    ctx->st.t_adj_ms = ev_ms(ctx, EV_OUT, EV_ADJ);
+    ctx->st.t_total_ms = ev_ms(ctx, EV_START, EV_ADJ);
+    return G4D_OK;
+}
+
+// ------------------------------------------------------------------------------------
+// C ABI
+// ------------------------------------------------------------------------------------
+
+extern "C" {
+
+int g4d_default_options(g4d_options *opt)
+{
+    if (!opt) return G4D_ERR_INVALID_INPUT;
+    opt->grid_size = 256;
+    opt->min_workset = 8;
+    opt->max_rounds = 1000;
+    opt->index_space = G4D_INDEX_INPUT;
+    opt->device = 0;
+    opt->compute_adjacency = 1;
+    return G4D_OK;
+}
+
+int g4d_device_count(void)
+{
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+    return c;
+}
+
+int g4d_create(g4d_ctx **out, const g4d_options *opt)
+{
+    if (!out) return G4D_ERR_INVALID_INPUT;
+    *out = nullptr;
+    g4d_ctx *ctx = new (std::nothrow) g4d_ctx();
+    if (!ctx) return G4D_ERR_OOM;
+    if (opt) ctx->opt = *opt;
+    else g4d_default_options(&ctx->opt);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        delete ctx;
+        return G4D_ERR_DEVICE;
+    }
+    if (ctx->opt.device < 0 || ctx->opt.device >= ndev) {
+        delete ctx;
+        return G4D_ERR_INVALID_INPUT;
+    }
+    if (hipSetDevice(ctx->opt.device) != hipSuccess || hipStreamCreate(&ctx->stream) != hipSuccess) {
+        delete ctx;
+        return G4D_ERR_DEVICE;
+    }
+    ctx->ev_ok = true;
+    for (int i = 0; i < EV_NUM; i++)
+        if (hipEventCreate(&ctx->ev[i]) != hipSuccess) ctx->ev_ok = false;
+    *out = ctx;
+    return G4D_OK;
+}
+
+void g4d_destroy(g4d_ctx *ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->opt.device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    DevBuf *bufs[] = {&ctx->xyz_up, &ctx->keys, &ctx->pts, &ctx->perm, &ctx->vox, &ctx->bounds, &ctx->grid,
+                      &ctx->gtmp, &ctx->idgrid, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->drown, &ctx->dkeys,
+                      &ctx->tasks, &ctx->cand, &ctx->keep, &ctx->ins, &ctx->ins_off, &ctx->vcap, &ctx->tcap,
+                      &ctx->hcap, &ctx->big, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table};
+    for (DevBuf *b : bufs) b->release();
+    for (auto &p : ctx->pool) p.release();
+    for (auto &h : ctx->hist) h.release();
+    ctx->P.release();
+    if (ctx->ev_ok)
+        for (int i = 0; i < EV_NUM; i++) (void)hipEventDestroy(ctx->ev[i]);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+const char *g4d_last_error(const g4d_ctx *ctx)
+{
+    return ctx ? ctx->err.c_str() : "null context";
+}
+
+int g4d_set_options(g4d_ctx *ctx, const g4d_options *opt)
+{
+    if (!ctx || !opt) return G4D_ERR_INVALID_INPUT;
+    if (opt->device != ctx->opt.device) return set_err(ctx, G4D_ERR_INVALID_INPUT, "device cannot change");
+    ctx->opt = *opt;
+    return G4D_OK;
+}
+
+int g4d_upload_points(g4d_ctx *ctx, const double *xyz, int32_t n, const double **d_xyz)
+{
+    if (!ctx || !xyz || n <= 0 || !d_xyz) return G4D_ERR_INVALID_INPUT;
+    HCK(hipSetDevice(ctx->opt.device));
+    HCK(ctx->xyz_up.ensure(sizeof(double) * 3 * (size_t)n));
+    HCK(hipMemcpyAsync(ctx->xyz_up.ptr, xyz, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, ctx->stream));
+    HCK(hipStreamSynchronize(ctx->stream));
+    *d_xyz = buf<double>(ctx->xyz_up);
+    return G4D_OK;
+}
+
+int g4d_triangulate_device(g4d_ctx *ctx, const double *d_xyz, int32_t n, int64_t *num_tets)
+{
+    if (!ctx || !d_xyz) return G4D_ERR_INVALID_INPUT;
+    HCK(hipSetDevice(ctx->opt.device));
+    const int r = run_device(ctx, d_xyz, n);
+    if (r != G4D_OK) return r;
+    if (num_tets) *num_tets = ctx->num_tets;
+    return G4D_OK;
+}
+
+int g4d_copy_result(g4d_ctx *ctx, int32_t *tets, int32_t *opp, int32_t *perm)
+{
+    if (!ctx || !ctx->have_result) return G4D_ERR_INVALID_INPUT;
+    const int64_t nt = ctx->num_tets;
+    if (tets && nt > 0) HCK(hipMemcpyAsync(tets, ctx->tets.ptr, sizeof(int) * 4 * (size_t)nt, hipMemcpyDeviceToHost, ctx->stream));
+    if (opp && nt > 0) {
+        if (!ctx->opt.compute_adjacency) return set_err(ctx, G4D_ERR_INVALID_INPUT, "adjacency was not computed");
+        HCK(hipMemcpyAsync(opp, ctx->opp.ptr, sizeof(int) * 4 * (size_t)nt, hipMemcpyDeviceToHost, ctx->stream));
+    }
+    if (perm) HCK(hipMemcpyAsync(perm, ctx->perm.ptr, sizeof(int) * (size_t)ctx->last_n, hipMemcpyDeviceToHost, ctx->stream));
+    HCK(hipStreamSynchronize(ctx->stream));
+    return G4D_OK;
+}
+
+int g4d_triangulate(g4d_ctx *ctx, const double *xyz, int32_t n, g4d_result *out)
+{
+    if (!ctx || !xyz || !out) return G4D_ERR_INVALID_INPUT;
+    const double *d_xyz = nullptr;
+    int r = g4d_upload_points(ctx, xyz, n, &d_xyz);
+    if (r != G4D_OK) return r;
+    r = run_device(ctx, d_xyz, n);
+    if (r != G4D_OK) return r;
+    const int64_t nt = ctx->num_tets;
+    ctx->h_tets.resize((size_t)std::max<int64_t>(4 * nt, 1));
+    ctx->h_opp.resize(ctx->opt.compute_adjacency ? (size_t)std::max<int64_t>(4 * nt, 1) : 0);
+    ctx->h_perm.resize((size_t)n);
+    r = g4d_copy_result(ctx, ctx->h_tets.data(), ctx->opt.compute_adjacency ? ctx->h_opp.data() : nullptr,
+                        ctx->h_perm.data());
+    if (r != G4D_OK) return r;
+    out->num_tets = nt;
+    out->tets = ctx->h_tets.data();
+    out->opp = ctx->opt.compute_adjacency ? ctx->h_opp.data() : nullptr;
+    out->perm = ctx->h_perm.data();
+    out->num_points = n;
+    return G4D_OK;
+}
+
+int g4d_get_stats(const g4d_ctx *ctx, g4d_stats *st)
+{
+    if (!ctx || !st) return G4D_ERR_INVALID_INPUT;
+    *st = ctx->st;
+    return G4D_OK;
+}
+
+// ---- diagnostics ----
+
+int g4d_diag_orient4(g4d_ctx *ctx, const double *pts5, const int32_t *idx5, int64_t T, int32_t mode, int32_t *sign_out,
+                     int32_t *depth_out)
+{
+    if (!ctx || !pts5 || !idx5 || !sign_out || T < 0) return G4D_ERR_INVALID_INPUT;
+    if (T == 0) return G4D_OK;
+    HCK(hipSetDevice(ctx->opt.device));
+    DevBuf dp, di, ds, dd;
+    HCK(dp.ensure(sizeof(double) * 15 * (size_t)T));
+    HCK(di.ensure(sizeof(int) * 5 * (size_t)T));
+    HCK(ds.ensure(sizeof(int) * (size_t)T));
+    HCK(dd.ensure(sizeof(int) * (size_t)T));
+    HCK(ctx->cnt.ensure(sizeof(unsigned long long) * G4D_CNT_NUM));
+    HCK(hipMemsetAsync(ctx->cnt.ptr, 0, sizeof(unsigned long long) * G4D_CNT_NUM, ctx->stream));
+    HCK(hipMemcpyAsync(dp.ptr, pts5, sizeof(double) * 15 * (size_t)T, hipMemcpyHostToDevice, ctx->stream));
+    HCK(hipMemcpyAsync(di.ptr, idx5, sizeof(int) * 5 * (size_t)T, hipMemcpyHostToDevice, ctx->stream));
+    HCK(g4d::launch_diag_orient4(buf<double>(dp), buf<int>(di), T, mode, buf<int>(ds), buf<int>(dd),
+                                 buf<unsigned long long>(ctx->cnt), ctx->stream));
+    HCK(hipMemcpyAsync(sign_out, ds.ptr, sizeof(int) * (size_t)T, hipMemcpyDeviceToHost, ctx->stream));
+    if (depth_out) HCK(hipMemcpyAsync(depth_out, dd.ptr, sizeof(int) * (size_t)T, hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long hc[G4D_CNT_NUM];
+    CK(read_counters(ctx, hc));
+    dp.release(); di.release(); ds.release(); dd.release();
+    return check_dev_error(ctx, hc[G4D_CNT_ERROR], "diag_orient4");
+}
+
+int g4d_diag_orient3d(g4d_ctx *ctx, const double *pts4, int64_t T, int32_t *sign_out)
+{
+    if (!ctx || !pts4 || !sign_out || T < 0) return G4D_ERR_INVALID_INPUT;
+    if (T == 0) return G4D_OK;
+    HCK(hipSetDevice(ctx->opt.device));
+    DevBuf dp, ds;
+    HCK(dp.ensure(sizeof(double) * 12 * (size_t)T));
+    HCK(ds.ensure(sizeof(int) * (size_t)T));
+    HCK(ctx->cnt.ensure(sizeof(unsigned long long) * G4D_CNT_NUM));
+    HCK(hipMemsetAsync(ctx->cnt.ptr, 0, sizeof(unsigned long long) * G4D_CNT_NUM, ctx->stream));
+    HCK(hipMemcpyAsync(dp.ptr, pts4, sizeof(double) * 12 * (size_t)T, hipMemcpyHostToDevice, ctx->stream));
+    HCK(g4d::launch_diag_orient3d(buf<double>(dp), T, buf<int>(ds), buf<unsigned long long>(ctx->cnt), ctx->stream));
+    HCK(hipMemcpyAsync(sign_out, ds.ptr, sizeof(int) * (size_t)T, hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long hc[G4D_CNT_NUM];
+    CK(read_counters(ctx, hc));
+    dp.release(); ds.release();
+    return check_dev_error(ctx, hc[G4D_CNT_ERROR], "diag_orient3d");
+}
+
+int g4d_diag_morton(g4d_ctx *ctx, const double *xyz, int32_t n, int32_t grid, int32_t *perm_out, int32_t *vox_out)
+{
+    if (!ctx || !xyz || n <= 0 || !perm_out || !vox_out || grid < 4 || grid > 1024) return G4D_ERR_INVALID_INPUT;
+    HCK(hipSetDevice(ctx->opt.device));
+    hipStream_t s = ctx->stream;
+    const double *d_xyz = nullptr;
+    CK(g4d_upload_points(ctx, xyz, n, &d_xyz));
+    HCK(ctx->bounds.ensure(sizeof(double) * 512));
+    HCK(ctx->keys.ensure(sizeof(uint64_t) * (size_t)n));
+    HCK(ctx->pts.ensure(sizeof(double) * 3 * (size_t)n));
+    HCK(ctx->perm.ensure(sizeof(int) * (size_t)n));
+    HCK(ctx->vox.ensure(sizeof(int) * (size_t)n));
+    HCK(g4d::launch_bounds(d_xyz, n, buf<double>(ctx->bounds), s));
+    HCK(g4d::launch_morton_keys(d_xyz, n, grid, buf<double>(ctx->bounds), buf<uint64_t>(ctx->keys), s));
+    HCK(g4d::radix_sort_u64(ctx->P, buf<uint64_t>(ctx->keys), n, 32, 62, s));
+    HCK(g4d::launch_gather_sorted(d_xyz, buf<uint64_t>(ctx->keys), n, grid, buf<double>(ctx->bounds),
+                                  buf<double>(ctx->pts), buf<int>(ctx->perm), buf<int>(ctx->vox), s));
+    std::vector<int> vs((size_t)n);
+    HCK(hipMemcpyAsync(perm_out, ctx->perm.ptr, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
+    HCK(hipMemcpyAsync(vs.data(), ctx->vox.ptr, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
+    HCK(hipStreamSynchronize(s));
+    for (int k = 0; k < n; k++) {
+        const int i = perm_out[k], c = vs[(size_t)k];
+        vox_out[3 * (int64_t)i] = G4D_DEC_X(c);
+        vox_out[3 * (int64_t)i + 1] = G4D_DEC_Y(c);
+        vox_out[3 * (int64_t)i + 2] = G4D_DEC_Z(c);
+    }
+    return G4D_OK;
+}
+
+int g4d_diag_pba(g4d_ctx *ctx, const int32_t *vox_sorted, int32_t n, int32_t grid, int32_t codes, int32_t *out)
+{
+    if (!ctx || !vox_sorted || n <= 0 || !out || grid < 4 || grid > 1024) return G4D_ERR_INVALID_INPUT;
+    HCK(hipSetDevice(ctx->opt.device));
+    hipStream_t s = ctx->stream;
+    std::vector<int> enc((size_t)n);
+    for (int k = 0; k < n; k++) {
+        const int x = vox_sorted[3 * (int64_t)k], y = vox_sorted[3 * (int64_t)k + 1], z = vox_sorted[3 * (int64_t)k + 2];
+        if (x < 0 || y < 0 || z < 0 || x >= grid || y >= grid || z >= grid)
+            return set_err(ctx, G4D_ERR_INVALID_INPUT, "voxel outside the grid");
+        enc[(size_t)k] = G4D_ENC(x, y, z);
+    }
+    const int64_t G3 = (int64_t)grid * grid * grid;
+    HCK(ctx->vox.ensure(sizeof(int) * (size_t)n));
+    HCK(ctx->grid.ensure(sizeof(int) * (size_t)G3));
+    HCK(ctx->gtmp.ensure(sizeof(int) * (size_t)G3));
+    HCK(ctx->idgrid.ensure(sizeof(int) * (size_t)G3));
+    HCK(hipMemcpyAsync(ctx->vox.ptr, enc.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s));
+    HCK(g4d::launch_fill_i32(buf<int>(ctx->grid), G3, G4D_MARKER, s));
+    HCK(g4d::launch_scatter_sites(buf<int>(ctx->vox), n, grid, buf<int>(ctx->grid), buf<int>(ctx->idgrid), s));
+    HCK(g4d::pba_run(grid, buf<int>(ctx->grid), buf<int>(ctx->gtmp), codes ? nullptr : buf<int>(ctx->idgrid), s));
+    HCK(hipMemcpyAsync(out, ctx->gtmp.ptr, sizeof(int) * (size_t)G3, hipMemcpyDeviceToHost, s));
+    HCK(hipStreamSynchronize(s));
+    return G4D_OK;
+}
+
+int g4d_diag_grid_pairs(g4d_ctx *ctx, const int32_t *id_grid, int32_t grid, uint64_t *out, int64_t cap, int64_t *count)
+{
+    if (!ctx || !id_grid || !count || grid < 4 || grid > 1024) return G4D_ERR_INVALID_INPUT;
+    HCK(hipSetDevice(ctx->opt.device));
+    hipStream_t s = ctx->stream;
+    const int64_t G3 = (int64_t)grid * grid * grid;
+    HCK(ctx->gtmp.ensure(sizeof(int) * (size_t)G3));
+    HCK(hipMemcpyAsync(ctx->gtmp.ptr, id_grid, sizeof(int) * (size_t)G3, hipMemcpyHostToDevice, s));
+    HCK(ctx->cnt.ensure(sizeof(unsigned long long) * G4D_CNT_NUM));
+    unsigned long long *cnt = buf<unsigned long long>(ctx->cnt);
+    int64_t pcap = std::max<int64_t>(64 * G3 / 8, 1 << 16);
+    int64_t m = 0;
+    unsigned long long hc[G4D_CNT_NUM];
+    for (int attempt = 0; attempt < 3; attempt++) {
+        HCK(ctx->pairs.ensure(sizeof(uint64_t) * (size_t)pcap));
+        HCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * G4D_CNT_NUM, s));
+        HCK(g4d::launch_grid_pairs(buf<int>(ctx->gtmp), grid, buf<uint64_t>(ctx->pairs), cnt + G4D_CNT_APPEND0, pcap, s));
+        CK(read_counters(ctx, hc));
+        m = 2 * (int64_t)hc[G4D_CNT_APPEND0];
+        if (m <= pcap) break;
+        pcap = m + 1024;
+        m = -1;
+    }
+    if (m < 0) return set_err(ctx, G4D_ERR_INTERNAL, "pair buffer sizing failed");
+    int maxid = 0;
+    for (int64_t i = 0; i < G3; i++) maxid = std::max(maxid, id_grid[i]);
+    const int nb = bits_for((int64_t)maxid + 1);
+    uint64_t *pr = buf<uint64_t>(ctx->pairs);
+    HCK(sort_pairs(ctx->P, pr, m, nb, s));
+    HCK(g4d::unique_u64(ctx->P, pr, m, &m, s));
+    std::vector<uint64_t> h((size_t)std::max<int64_t>(m, 1));
+    HCK(hipMemcpyAsync(h.data(), pr, sizeof(uint64_t) * (size_t)m, hipMemcpyDeviceToHost, s));
+    HCK(hipStreamSynchronize(s));
+    int64_t k = 0;
+    for (int64_t i = 0; i < m; i++) {
+        const uint32_t a = (uint32_t)(h[(size_t)i] >> 32), b = (uint32_t)h[(size_t)i];
+        if (a < b) {
+            if (out && k < cap) out[k] = h[(size_t)i];
+            k++;
+        }
+    }
+    *count = k;
+    return k > cap && out ? set_err(ctx, G4D_ERR_CAPACITY, "output capacity %lld < %lld", (long long)cap, (long long)k)
+                          : G4D_OK;
+}
+
+} // extern "C"

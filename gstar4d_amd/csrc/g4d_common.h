@@ -1,0 +1,122 @@
+// gStar4D for MI355X — shared device/host definitions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define G4D_WAVE 64
+
+#define G4D_MARKER (-1)
+// Voxel site code: x<<20 | y<<10 | z, 10 bits per axis (grid width <= 1024), as the
+// reference's PBA encoding (GD/PBA/pba3D.h:61-72).
+#define G4D_ENC(x, y, z) (((x) << 20) | ((y) << 10) | (z))
+#define G4D_DEC_X(v) ((v) >> 20)
+#define G4D_DEC_Y(v) (((v) >> 10) & 0x3ff)
+#define G4D_DEC_Z(v) ((v)&0x3ff)
+
+// Device-side error flags (bitwise OR into a counter block word).
+enum g4d_dev_error : uint32_t {
+    G4D_DEV_ERR_NONE = 0,
+    G4D_DEV_ERR_STAR_CAPACITY = 1u << 0,   // star exceeds the LDS star capacity
+    G4D_DEV_ERR_PREDICATE_RANGE = 1u << 1, // coordinates outside the exact-arithmetic range
+    G4D_DEV_ERR_HOLE = 1u << 2,            // hole boundary not a single cycle
+    G4D_DEV_ERR_PROOF = 1u << 3,           // no containment proof found
+    G4D_DEV_ERR_BUFFER = 1u << 4,          // an output buffer overflowed
+    G4D_DEV_ERR_DUPLICATE = 1u << 5,       // duplicate input points
+    G4D_DEV_ERR_CONE = 1u << 6,            // workset smaller than 4
+    G4D_DEV_ERR_FACE = 1u << 7,            // a face shared by more than two tetrahedra
+};
+
+// Counter block layout (int64 words), device resident, read back once per stage.
+enum g4d_counter : int {
+    G4D_CNT_ERROR = 0,
+    G4D_CNT_APPEND0 = 1,  // generic append counters
+    G4D_CNT_APPEND1 = 2,
+    G4D_CNT_APPEND2 = 3,
+    G4D_CNT_ORIENT4 = 4,  // filtered orient4 evaluations
+    G4D_CNT_EXACT = 5,    // exact insphere evaluations
+    G4D_CNT_SOS = 6,      // SoS resolutions
+    G4D_CNT_ORIENT3 = 7,
+    G4D_CNT_MISC = 8,
+    G4D_CNT_WORK = 9,     // work-queue head
+    G4D_CNT_NUM = 16
+};
+
+#define HIP_CHECK_RET(expr)                                                                    \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess) return _e;                                                       \
+    } while (0)
+
+// Wave-level helpers ------------------------------------------------------------------
+
+__device__ __forceinline__ int g4d_lane() { return __lane_id(); }
+
+// Exclusive prefix count of `pred` over lanes below this one; total in *tot.
+__device__ __forceinline__ int g4d_wave_excl_count(bool pred, int *tot)
+{
+    const uint64_t m = __ballot(pred);
+    *tot = __popcll(m);
+    return __popcll(m & ((1ull << __lane_id()) - 1ull));
+}
+
+// Wave-inclusive scan of small non-negative ints via DPP-free shuffles.
+__device__ __forceinline__ int g4d_wave_incl_sum(int v)
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const int o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int g4d_wave_sum(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ int g4d_wave_min(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ uint64_t g4d_wave_min_u64(uint64_t v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const uint64_t o = __shfl_xor(v, off, 64);
+        v = o < v ? o : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ int g4d_wave_max(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+// Wave-aggregated append: one atomic per wave; returns this lane's slot (or -1 if !pred).
+__device__ __forceinline__ long long g4d_wave_append(bool pred, unsigned long long *counter)
+{
+    int tot;
+    const int rank = g4d_wave_excl_count(pred, &tot);
+    unsigned long long base = 0;
+    const int leader = __ffsll((long long)__ballot(1)) - 1;
+    if (__lane_id() == leader && tot > 0) base = atomicAdd(counter, (unsigned long long)tot);
+    base = __shfl(base, leader, 64);
+    return pred ? (long long)(base + rank) : -1;
+}

@@ -1,0 +1,83 @@
+// gStar4D for MI355X — device primitives: scans, stable LSD radix sort, compaction.
+#pragma once
+
+#include <vector>
+
+#include "g4d_common.h"
+
+namespace g4d {
+
+// Growable device buffer owned by a context; never shrinks, reused across calls.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need)
+    {
+        if (need <= bytes) return hipSuccess;
+        if (ptr) {
+            hipError_t e = hipFree(ptr);
+            if (e != hipSuccess) return e;
+            ptr = nullptr;
+            bytes = 0;
+        }
+        size_t cap = need + need / 8 + 256;
+        hipError_t e = hipMalloc(&ptr, cap);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        bytes = cap;
+        return hipSuccess;
+    }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    template <typename T>
+    T *as() const
+    {
+        return reinterpret_cast<T *>(ptr);
+    }
+};
+
+struct Prims {
+    DevBuf scan_tmp[4];
+    DevBuf sort_tmp_keys, sort_hist, flags, pos;
+    void release()
+    {
+        for (auto &b : scan_tmp) b.release();
+        sort_tmp_keys.release();
+        sort_hist.release();
+        flags.release();
+        pos.release();
+    }
+};
+
+// Exclusive prefix sum of n int64 values; out may alias in.  If total != nullptr the device
+// total is written there.
+hipError_t scan_excl_i64(Prims &P, const int64_t *in, int64_t *out, int64_t n, int64_t *total,
+                         hipStream_t s);
+hipError_t scan_excl_u32(Prims &P, const uint32_t *in, uint32_t *out, int64_t n, hipStream_t s);
+
+// Stable LSD radix sort of uint64 keys on bits [lo_bit, hi_bit).  Result in keys.
+hipError_t radix_sort_u64(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int hi_bit, hipStream_t s);
+
+// Order-preserving removal of adjacent duplicates; returns the new count via host sync.
+hipError_t unique_u64(Prims &P, uint64_t *keys, int64_t n, int64_t *n_out, hipStream_t s);
+
+// Order-preserving compaction keep[i] != 0.  Count returned via host sync.
+hipError_t compact_u64(Prims &P, const uint64_t *in, const uint8_t *keep, uint64_t *out, int64_t n,
+                       int64_t *n_out, hipStream_t s);
+
+// Kernel launch helpers
+inline int g4d_blocks(int64_t n, int per_block, int cap = 1 << 20)
+{
+    int64_t b = (n + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (int)b;
+}
+
+} // namespace g4d

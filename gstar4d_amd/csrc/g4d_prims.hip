@@ -1,0 +1,310 @@
+// gStar4D for MI355X — device primitives.
+//
+// The reference leans on thrust for sort_by_key / exclusive_scan / remove_if
+// (GD/GDelaunay/GDelCommon.cu:85-277).  Here: a three-phase tiled scan, a stable LSD radix
+// sort whose per-wave ranking uses ballot-matched digit peers (8 ballots per 8-bit digit),
+// and scan-based order-preserving compaction.
+#include "g4d_prims.h"
+
+namespace g4d {
+
+static constexpr int SCAN_THREADS = 256;
+static constexpr int SCAN_ITEMS = 8;
+static constexpr int SCAN_TILE = SCAN_THREADS * SCAN_ITEMS;
+
+template <typename T>
+__device__ __forceinline__ T wave_incl_scan_t(T v)
+{
+    const int lane = __lane_id();
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const T o = __shfl_up(v, off, 64);
+        if (lane >= off) v += o;
+    }
+    return v;
+}
+
+// Block-wide exclusive scan over 256 threads; returns exclusive prefix, total in *tot.
+template <typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T *lds4, T *tot)
+{
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    const T inc = wave_incl_scan_t<T>(v);
+    if (lane == 63) lds4[wid] = inc;
+    __syncthreads();
+    T wbase = 0, all = 0;
+#pragma unroll
+    for (int w = 0; w < SCAN_THREADS / 64; w++) {
+        const T x = lds4[w];
+        if (w < wid) wbase += x;
+        all += x;
+    }
+    __syncthreads();
+    *tot = all;
+    return wbase + inc - v;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_reduce(const T *in, int64_t n, T *bsum)
+{
+    __shared__ T lds[4];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const int64_t k = base + (int64_t)i * SCAN_THREADS + threadIdx.x;
+        if (k < n) s += in[k];
+    }
+    T tot;
+    (void)block_excl_scan<T>(s, lds, &tot);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(SCAN_THREADS) k_scan_apply(const T *in, T *out, int64_t n,
+                                                             const T *boff, T *total)
+{
+    __shared__ T lds[4];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_ITEMS;
+    T v[SCAN_ITEMS];
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const int64_t k = base + i;
+        v[i] = (k < n) ? in[k] : (T)0;
+        s += v[i];
+    }
+    T tot;
+    T run = block_excl_scan<T>(s, lds, &tot) + (boff ? boff[blockIdx.x] : (T)0);
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; i++) {
+        const int64_t k = base + i;
+        const T x = v[i];
+        if (k < n) out[k] = run;
+        run += x;
+    }
+    if (total && blockIdx.x == gridDim.x - 1 && threadIdx.x == SCAN_THREADS - 1) *total = run;
+}
+
+template <typename T>
+static hipError_t scan_excl_impl(Prims &P, const T *in, T *out, int64_t n, T *total, hipStream_t s, int depth)
+{
+    if (n <= 0) {
+        if (total) return hipMemsetAsync(total, 0, sizeof(T), s);
+        return hipSuccess;
+    }
+    const int64_t nb = (n + SCAN_TILE - 1) / SCAN_TILE;
+    if (nb == 1) {
+        hipLaunchKernelGGL(k_scan_apply<T>, dim3(1), dim3(SCAN_THREADS), 0, s, in, out, n, (const T *)nullptr, total);
+        return hipGetLastError();
+    }
+    if (depth >= 4) return hipErrorInvalidValue;
+    DevBuf &tmp = P.scan_tmp[depth];
+    HIP_CHECK_RET(tmp.ensure(sizeof(T) * (size_t)nb));
+    T *bs = tmp.as<T>();
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, in, n, bs);
+    HIP_CHECK_RET(hipGetLastError());
+    HIP_CHECK_RET(scan_excl_impl<T>(P, bs, bs, nb, (T *)nullptr, s, depth + 1));
+    hipLaunchKernelGGL(k_scan_apply<T>, dim3((unsigned)nb), dim3(SCAN_THREADS), 0, s, in, out, n, (const T *)bs, total);
+    return hipGetLastError();
+}
+
+hipError_t scan_excl_i64(Prims &P, const int64_t *in, int64_t *out, int64_t n, int64_t *total, hipStream_t s)
+{
+    return scan_excl_impl<int64_t>(P, in, out, n, total, s, 0);
+}
+
+hipError_t scan_excl_u32(Prims &P, const uint32_t *in, uint32_t *out, int64_t n, hipStream_t s)
+{
+    return scan_excl_impl<uint32_t>(P, in, out, n, (uint32_t *)nullptr, s, 0);
+}
+
+// ------------------------------------------------------------------------------------
+// Radix sort
+// ------------------------------------------------------------------------------------
+
+static constexpr int RS_THREADS = 256;
+static constexpr int RS_ITEMS = 8;
+static constexpr int RS_TILE = RS_THREADS * RS_ITEMS; // 2048 keys per block
+static constexpr int RS_WAVES = RS_THREADS / 64;
+
+__global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t *keys, int64_t n, int shift, uint32_t *hist,
+                                                        int nblocks)
+{
+    __shared__ uint32_t cnt[256];
+    cnt[threadIdx.x] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+#pragma unroll
+    for (int i = 0; i < RS_ITEMS; i++) {
+        const int64_t k = base + (int64_t)i * RS_THREADS + threadIdx.x;
+        if (k < n) atomicAdd(&cnt[(keys[k] >> shift) & 255], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)threadIdx.x * nblocks + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Each wave owns RS_ITEMS consecutive rounds of 64 keys (wave-contiguous sub-tile), so
+// (wave, round, lane) order equals input order: ranking in that order is stable.
+__global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t *keys, uint64_t *out, int64_t n, int shift,
+                                                           const uint32_t *hist_scan, int nblocks)
+{
+    __shared__ uint32_t cnt[RS_WAVES][256];
+    __shared__ uint32_t goff[256];
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_THREADS) (&cnt[0][0])[i] = 0;
+    goff[threadIdx.x] = hist_scan[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+    __syncthreads();
+    const int64_t wbase = (int64_t)blockIdx.x * RS_TILE + (int64_t)wid * (RS_ITEMS * 64);
+    uint64_t kv[RS_ITEMS];
+    uint32_t rank[RS_ITEMS];
+    int dg[RS_ITEMS];
+    const uint64_t lt = (1ull << lane) - 1ull;
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const int64_t k = wbase + (int64_t)r * 64 + lane;
+        const bool valid = k < n;
+        kv[r] = valid ? keys[k] : 0ull;
+        const int d = valid ? (int)((kv[r] >> shift) & 255) : 256;
+        dg[r] = d;
+        uint64_t peers = __ballot(1);
+#pragma unroll
+        for (int b = 0; b < 9; b++) {
+            const bool bit = (d >> b) & 1;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        uint32_t before = 0;
+        if (valid) before = cnt[wid][d];
+        const uint32_t inwave = (uint32_t)__popcll(peers & lt);
+        rank[r] = before + inwave;
+        // lowest lane of each peer group bumps the counter after everyone read it
+        __builtin_amdgcn_wave_barrier();
+        if (valid && (peers & lt) == 0) cnt[wid][d] = before + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // exclusive prefix over waves, per digit
+    {
+        const int d = threadIdx.x;
+        uint32_t run = 0;
+#pragma unroll
+        for (int w = 0; w < RS_WAVES; w++) {
+            const uint32_t t = cnt[w][d];
+            cnt[w][d] = run;
+            run += t;
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < RS_ITEMS; r++) {
+        const int d = dg[r];
+        if (d < 256) out[goff[d] + cnt[wid][d] + rank[r]] = kv[r];
+    }
+}
+
+hipError_t radix_sort_u64(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int hi_bit, hipStream_t s)
+{
+    if (n <= 1 || hi_bit <= lo_bit) return hipSuccess;
+    const int64_t nb64 = (n + RS_TILE - 1) / RS_TILE;
+    if (nb64 > (1 << 30)) return hipErrorInvalidValue;
+    const int nb = (int)nb64;
+    HIP_CHECK_RET(P.sort_tmp_keys.ensure(sizeof(uint64_t) * (size_t)n));
+    HIP_CHECK_RET(P.sort_hist.ensure(sizeof(uint32_t) * (size_t)256 * nb));
+    uint64_t *src = keys, *dst = P.sort_tmp_keys.as<uint64_t>();
+    uint32_t *hist = P.sort_hist.as<uint32_t>();
+    int passes = 0;
+    for (int shift = lo_bit; shift < hi_bit; shift += 8) {
+        hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, s, src, n, shift, hist, nb);
+        HIP_CHECK_RET(hipGetLastError());
+        HIP_CHECK_RET(scan_excl_u32(P, hist, hist, (int64_t)256 * nb, s));
+        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_THREADS), 0, s, src, dst, n, shift, hist, nb);
+        HIP_CHECK_RET(hipGetLastError());
+        uint64_t *t = src;
+        src = dst;
+        dst = t;
+        passes++;
+    }
+    if (src != keys) HIP_CHECK_RET(hipMemcpyAsync(keys, src, sizeof(uint64_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
+    return hipSuccess;
+}
+
+// ------------------------------------------------------------------------------------
+// Compaction
+// ------------------------------------------------------------------------------------
+
+__global__ void k_unique_flags(const uint64_t *keys, int64_t n, uint32_t *flags)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+__global__ void k_keep_flags(const uint8_t *keep, int64_t n, uint32_t *flags)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        flags[i] = keep[i] ? 1u : 0u;
+}
+
+__global__ void k_scatter_flagged(const uint64_t *in, const uint32_t *flags, const uint32_t *pos, uint64_t *out,
+                                  int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (flags[i]) out[pos[i]] = in[i];
+}
+
+static hipError_t compact_with_flags(Prims &P, const uint64_t *in, uint64_t *out, int64_t n, int64_t *n_out,
+                                     hipStream_t s)
+{
+    uint32_t *flags = P.flags.as<uint32_t>();
+    uint32_t *pos = P.pos.as<uint32_t>();
+    HIP_CHECK_RET(scan_excl_u32(P, flags, pos, n, s));
+    uint32_t last[2];
+    HIP_CHECK_RET(hipMemcpyAsync(&last[0], pos + n - 1, 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK_RET(hipMemcpyAsync(&last[1], flags + n - 1, 4, hipMemcpyDeviceToHost, s));
+    if (in == out) {
+        // scatter through the sort scratch then copy back
+        HIP_CHECK_RET(P.sort_tmp_keys.ensure(sizeof(uint64_t) * (size_t)n));
+        uint64_t *tmp = P.sort_tmp_keys.as<uint64_t>();
+        hipLaunchKernelGGL(k_scatter_flagged, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, in, flags, pos, tmp, n);
+        HIP_CHECK_RET(hipGetLastError());
+        HIP_CHECK_RET(hipStreamSynchronize(s));
+        const int64_t m = (int64_t)last[0] + last[1];
+        if (m > 0) HIP_CHECK_RET(hipMemcpyAsync(out, tmp, sizeof(uint64_t) * (size_t)m, hipMemcpyDeviceToDevice, s));
+        *n_out = m;
+        return hipSuccess;
+    }
+    hipLaunchKernelGGL(k_scatter_flagged, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, in, flags, pos, out, n);
+    HIP_CHECK_RET(hipGetLastError());
+    HIP_CHECK_RET(hipStreamSynchronize(s));
+    *n_out = (int64_t)last[0] + last[1];
+    return hipSuccess;
+}
+
+hipError_t unique_u64(Prims &P, uint64_t *keys, int64_t n, int64_t *n_out, hipStream_t s)
+{
+    if (n <= 1) {
+        *n_out = n;
+        return hipSuccess;
+    }
+    HIP_CHECK_RET(P.flags.ensure(sizeof(uint32_t) * (size_t)n));
+    HIP_CHECK_RET(P.pos.ensure(sizeof(uint32_t) * (size_t)n));
+    hipLaunchKernelGGL(k_unique_flags, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, keys, n, P.flags.as<uint32_t>());
+    HIP_CHECK_RET(hipGetLastError());
+    return compact_with_flags(P, keys, keys, n, n_out, s);
+}
+
+hipError_t compact_u64(Prims &P, const uint64_t *in, const uint8_t *keep, uint64_t *out, int64_t n, int64_t *n_out,
+                       hipStream_t s)
+{
+    if (n <= 0) {
+        *n_out = 0;
+        return hipSuccess;
+    }
+    HIP_CHECK_RET(P.flags.ensure(sizeof(uint32_t) * (size_t)n));
+    HIP_CHECK_RET(P.pos.ensure(sizeof(uint32_t) * (size_t)n));
+    hipLaunchKernelGGL(k_keep_flags, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, keep, n, P.flags.as<uint32_t>());
+    HIP_CHECK_RET(hipGetLastError());
+    return compact_with_flags(P, in, out, n, n_out, s);
+}
+
+} // namespace g4d

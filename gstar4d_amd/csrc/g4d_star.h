@@ -1,0 +1,104 @@
+// gStar4D for MI355X — star stage launchers.
+#pragma once
+
+#include "g4d_prims.h"
+
+namespace g4d {
+
+// Packed star triangle in HBM: local vertex indices (9 bits each), adjacency slots (10 bits
+// each), checked flag.
+static constexpr int STAR_VMAX_SMALL = 128;
+static constexpr int STAR_VMAX_BIG = 512;
+
+struct StarParams {
+    const double *pts;
+    int n;
+    int mode; // 0: create from worksets, 1: splaying insertions
+    // insertion lists: directed keys (star<<32 | vertex), sorted, CSR offsets (n+1)
+    const uint64_t *ins_keys;
+    const int64_t *ins_off;
+    // input stars (mode 1)
+    const int *in_vnum;
+    const int64_t *in_voff;
+    const int *in_vpool;
+    const int *in_tnum;
+    const int64_t *in_toff;
+    const uint64_t *in_tpool;
+    // output stars (capacities come from the offsets)
+    int *out_vnum;
+    const int64_t *out_voff;
+    int *out_vpool;
+    int *out_tnum;
+    const int64_t *out_toff;
+    uint64_t *out_tpool;
+    // history (mode 1): old CSR and the merged output CSR
+    const uint64_t *hist_keys;
+    const int64_t *hist_off;
+    uint64_t *hist_out;
+    const int64_t *hist_out_off;
+    // appends
+    uint64_t *drown;      // (star<<32 | vertex)       counter G4D_CNT_APPEND0
+    int64_t drown_cap;
+    uint64_t *tpairs;     // (a<<32 | b), a < b         counter G4D_CNT_APPEND1
+    int64_t tpairs_cap;
+    int *big_list;        // stars deferred to the big-star kernel, counter G4D_CNT_APPEND2
+    int64_t big_cap;
+    // work list (nullptr = all stars)
+    const int *star_list;
+    int nlist;
+    unsigned long long *cnt;
+};
+
+hipError_t launch_star_kernel(const StarParams &p, bool big, int nblocks, hipStream_t s);
+
+struct ProofParams {
+    const double *pts;
+    const uint64_t *tasks; // (star<<32 | vertex) drowned, non-mutual
+    int64_t ntasks;
+    const int *vnum;
+    const int64_t *voff;
+    const int *vpool;
+    const int *tnum;
+    const int64_t *toff;
+    const uint64_t *tpool;
+    uint64_t *out;        // 4 pairs per task (a<<32|b, a<b)
+    unsigned long long *cnt;
+};
+hipError_t launch_proofs(const ProofParams &p, hipStream_t s);
+
+// Drowned pairs -> keys (min<<33 | max<<1 | dir) for the mutual test.
+hipError_t launch_drown_keys(const uint64_t *drown, int64_t m, uint64_t *keys, hipStream_t s);
+// Non-mutual drowned keys -> proof tasks (star<<32 | vertex); counter G4D_CNT_APPEND0.
+hipError_t launch_nonmutual(const uint64_t *keys, int64_t m, uint64_t *tasks, unsigned long long *cnt, hipStream_t s);
+// keep[i] = pair (a<<32|b) not in history of a.
+hipError_t launch_hist_filter(const uint64_t *pairs, int64_t m, const uint64_t *hist_keys, const int64_t *hist_off,
+                              uint8_t *keep, hipStream_t s);
+// Reciprocate m ordered pairs in place into 2m directed keys.
+hipError_t launch_reciprocate(uint64_t *keys, int64_t m, hipStream_t s);
+// Per-star counts from CSR offsets.
+hipError_t launch_capacities(int n, const int *vnum, const int64_t *ins_off, const int64_t *hist_off,
+                             int64_t *vcap, int64_t *tcap, int64_t *hcap, hipStream_t s);
+
+// Output: per-triangle keep flag for owner triangles on the lower hull (exact orient3d < 0)
+struct OutParams {
+    const double *pts;
+    int n;
+    const int *vnum;
+    const int64_t *voff;
+    const int *vpool;
+    const int *tnum;
+    const int64_t *toff;
+    const uint64_t *tpool;
+    int64_t *star_tet_count; // per star
+    const int64_t *star_tet_off;
+    int *tets;               // 4 per tetra
+    unsigned long long *cnt;
+};
+hipError_t launch_out_count(const OutParams &p, hipStream_t s);
+hipError_t launch_out_write(const OutParams &p, hipStream_t s);
+// Face adjacency via an open-addressing hash of face triples.
+hipError_t launch_adjacency(const int *tets, int64_t nt, int *opp, uint32_t *table, int64_t table_size,
+                            unsigned long long *cnt, hipStream_t s);
+hipError_t launch_map_perm(int *tets, int64_t m4, const int *perm, hipStream_t s);
+
+} // namespace g4d

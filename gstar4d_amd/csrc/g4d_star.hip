@@ -1,0 +1,761 @@
+// gStar4D for MI355X — 4D stars on the lifted points and parallel star splaying.
+//
+// Reference behaviour restated (GD/ = GDelaunay/):
+//   initial cones            GD/GDelaunay/GDelPredKernels.cu:45-231
+//   beneath marking          GD/GDelaunay/GDelPredKernels.cu:233-420
+//   hole stitching           GD/GDelaunay/GDelKernels.cu:518-783
+//   drowned insertions       GD/GDelaunay/GDelKernels.cu:1459-1527, GD/GDelaunay/GDelHost.cu:817-875
+//   containment proofs       GD/GDelaunay/GDelPredKernels.cu:427-626
+//   triangle insertions      GD/GDelaunay/GDelKernels.cu:818-939
+//   history                  GD/GDelaunay/GDelKernels.cu:984-1068,1625-1677
+//   tetra output             GD/GDelaunay/GDelKernels.cu:1097-1191, GD/GDelaunay/GDelPredKernels.cu:628-692
+//
+// MI355X design.  One wave64 owns one star at a time (dynamic work queue).  The star's link
+// (a triangulated 2-sphere) lives in LDS: local vertex table with coordinates relative to the
+// star point, triangles with 9-bit local vertices and 16-bit adjacency.  Each lane owns the
+// triangle slots lane, lane+64, ...; for each slot it keeps in registers the four cofactors
+// of the lifted 4x4 determinant, so the beneath test of a new vertex v against triangle
+// (a,b,c) is one 4-term dot product:
+//     orient4(a,b,c,s,v) = v'x*Mx - v'y*My + v'z*Mz - v'l*Ml      (primes: relative to s)
+// with a forward-error bound from the cofactor permanents; uncertain lanes fall back to the
+// exact integer predicate with SoS.  The hole boundary after freeing beneath triangles is
+// found in parallel (edges of freed triangles whose neighbour survives), slots are assigned
+// with a wave prefix sum over a free-slot stack, and the fan's adjacency is closed through
+// two per-vertex maps (boundary edge starting / ending at each vertex).  The whole insertion
+// list of a star runs inside one kernel, so there is no host loop per insertion index.
+#include "g4d_predicates.h"
+#include "g4d_star.h"
+
+namespace g4d {
+
+static constexpr uint32_t TF_ALIVE = 1u << 27;
+static constexpr uint32_t TF_CHECKED = 1u << 28;
+static constexpr uint32_t TF_VIS = 1u << 29;
+static constexpr uint32_t TF_FRESH = 1u << 30;
+
+__device__ __forceinline__ int tvert(uint32_t t, int i) { return (int)((t >> (9 * i)) & 511u); }
+__device__ __forceinline__ uint32_t tpack(int a, int b, int c, uint32_t fl)
+{
+    return (uint32_t)a | ((uint32_t)b << 9) | ((uint32_t)c << 18) | fl;
+}
+
+// HBM triangle: bits 0..26 vertices, 27..56 adjacency (10 bits each), 57 checked.
+__device__ __forceinline__ uint64_t hpack(int a, int b, int c, int o0, int o1, int o2, bool checked)
+{
+    return (uint64_t)a | ((uint64_t)b << 9) | ((uint64_t)c << 18) | ((uint64_t)o0 << 27) | ((uint64_t)o1 << 37) |
+           ((uint64_t)o2 << 47) | ((uint64_t)(checked ? 1 : 0) << 57);
+}
+__device__ __forceinline__ int hvert(uint64_t t, int i) { return (int)((t >> (9 * i)) & 511u); }
+__device__ __forceinline__ int hopp(uint64_t t, int i) { return (int)((t >> (27 + 10 * i)) & 1023u); }
+__device__ __forceinline__ bool hchecked(uint64_t t) { return (t >> 57) & 1u; }
+
+template <int VMAX>
+struct StarLds {
+    static constexpr int TMAX = 2 * VMAX;
+    int vid[VMAX];
+    double vc[VMAX][4];
+    uint32_t tv[TMAX];
+    uint16_t to[TMAX][3];
+    int16_t vstart[VMAX];
+    int16_t vend[VMAX];
+    uint16_t freel[TMAX];
+    uint16_t be_a[TMAX], be_b[TMAX], be_u[TMAX], be_slot[TMAX];
+    uint8_t be_j[TMAX];
+    int ins[VMAX];
+    int16_t ins_local[VMAX];
+    uint8_t present[VMAX];
+    uint16_t remap[TMAX];
+    uint16_t vremap[VMAX];
+    int sc[4];
+};
+
+// Exclusive scan of small per-lane counts across the wave; returns prefix, total in *tot.
+__device__ __forceinline__ int wave_excl(int v, int *tot)
+{
+    const int inc = g4d_wave_incl_sum(v);
+    *tot = __shfl(inc, 63, 64);
+    return inc - v;
+}
+
+template <int VMAX>
+__global__ void __launch_bounds__(64) k_star(StarParams p, int nitems, int big)
+{
+    constexpr int TMAX = StarLds<VMAX>::TMAX;
+    constexpr int KPL = TMAX / 64;
+    __shared__ StarLds<VMAX> L;
+    const int lane = threadIdx.x;
+    unsigned long long *cnt = p.cnt;
+    double C[KPL][4], P[KPL][4];
+
+    for (;;) {
+        if (lane == 0) L.sc[0] = (int)atomicAdd(&cnt[G4D_CNT_WORK], 1ull);
+        __syncthreads();
+        const int item = L.sc[0];
+        __syncthreads();
+        if (item >= nitems) break;
+        const int s = p.star_list ? p.star_list[item] : item;
+        const int64_t ib = p.ins_off[s];
+        const int m = (int)(p.ins_off[s + 1] - ib);
+        const int vn = p.mode ? p.in_vnum[s] : 0;
+        const int need = vn + m;
+        if (need > VMAX) {
+            if (lane == 0) {
+                if (big) {
+                    atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_STAR_CAPACITY);
+                    p.out_vnum[s] = 0;
+                    p.out_tnum[s] = 0;
+                } else {
+                    const unsigned long long k = atomicAdd(&cnt[G4D_CNT_APPEND2], 1ull);
+                    if ((int64_t)k < p.big_cap) p.big_list[k] = s;
+                    else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
+                }
+            }
+            continue;
+        }
+        if (p.mode == 0 && m < 4) {
+            if (lane == 0) {
+                atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_CONE);
+                p.out_vnum[s] = 0;
+                p.out_tnum[s] = 0;
+            }
+            continue;
+        }
+        if (p.mode == 1 && m == 0) {
+            // untouched star: copy verbatim
+            const int tn = p.in_tnum[s];
+            const int64_t vi = p.in_voff[s], vo = p.out_voff[s], ti = p.in_toff[s], to = p.out_toff[s];
+            for (int i = lane; i < vn; i += 64) p.out_vpool[vo + i] = p.in_vpool[vi + i];
+            for (int i = lane; i < tn; i += 64) p.out_tpool[to + i] = p.in_tpool[ti + i];
+            const int64_t hb = p.hist_off[s], he = p.hist_off[s + 1], ho = p.hist_out_off[s];
+            for (int64_t i = lane; i < he - hb; i += 64) p.hist_out[ho + i] = p.hist_keys[hb + i];
+            if (lane == 0) {
+                p.out_vnum[s] = vn;
+                p.out_tnum[s] = tn;
+            }
+            continue;
+        }
+        const double sx = p.pts[3 * (int64_t)s], sy = p.pts[3 * (int64_t)s + 1], sz = p.pts[3 * (int64_t)s + 2];
+        for (int q = lane; q < m; q += 64) {
+            L.ins[q] = (int)(uint32_t)p.ins_keys[ib + q];
+            L.ins_local[q] = -1;
+        }
+        int T, V, nfree = 0, q0;
+        if (p.mode == 1) {
+            const int tn = p.in_tnum[s];
+            const int64_t vi = p.in_voff[s], ti = p.in_toff[s];
+            for (int v = lane; v < vn; v += 64) {
+                const int g = p.in_vpool[vi + v];
+                L.vid[v] = g;
+                rel_coords(p.pts, g, sx, sy, sz, L.vc[v]);
+            }
+            for (int t = lane; t < tn; t += 64) {
+                const uint64_t h = p.in_tpool[ti + t];
+                L.tv[t] = tpack(hvert(h, 0), hvert(h, 1), hvert(h, 2), TF_ALIVE | (hchecked(h) ? TF_CHECKED : 0u));
+                L.to[t][0] = (uint16_t)hopp(h, 0);
+                L.to[t][1] = (uint16_t)hopp(h, 1);
+                L.to[t][2] = (uint16_t)hopp(h, 2);
+            }
+            T = tn;
+            V = vn;
+            q0 = 0;
+        } else {
+            // initial 4-simplex (GD/GDelaunay/GDelPredKernels.cu:45-158)
+            int l[4];
+            for (int i = 0; i < 4; i++) l[i] = (int)(uint32_t)p.ins_keys[ib + i];
+            const double *pp[5] = {p.pts + 3 * (int64_t)l[0], p.pts + 3 * (int64_t)l[1], p.pts + 3 * (int64_t)l[2],
+                                   p.pts + 3 * (int64_t)l[3], p.pts + 3 * (int64_t)s};
+            const int idx[5] = {l[0], l[1], l[2], l[3], s};
+            const int o = orient4_full(pp, idx, cnt);
+            const bool sw = o < 0;
+            if (sw) {
+                const int t = l[0];
+                l[0] = l[1];
+                l[1] = t;
+            }
+            if (lane < 4) {
+                L.vid[lane] = l[lane];
+                rel_coords(p.pts, l[lane], sx, sy, sz, L.vc[lane]);
+                // cone triangles (link triangles seen from the star point) and adjacency
+                const int lt[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+                L.tv[lane] = tpack(lt[lane][0], lt[lane][1], lt[lane][2], TF_ALIVE);
+                L.to[lane][0] = (uint16_t)lt[lane][0];
+                L.to[lane][1] = (uint16_t)lt[lane][1];
+                L.to[lane][2] = (uint16_t)lt[lane][2];
+                const int local = (lane < 2 && sw) ? 1 - lane : lane;
+                L.ins_local[lane] = (int16_t)local;
+            }
+            T = 4;
+            V = 4;
+            q0 = 4;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KPL; k++) {
+            const int slot = lane + 64 * k;
+            if (slot < T) {
+                const uint32_t t = L.tv[slot];
+                tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], C[k], P[k]);
+            }
+        }
+        bool failed = false;
+        for (int q = q0; q < m; q++) {
+            const int gv = L.ins[q];
+            double r[4];
+            rel_coords(p.pts, gv, sx, sy, sz, r);
+            int nv = 0;
+#pragma unroll
+            for (int k = 0; k < KPL; k++) {
+                const int slot = lane + 64 * k;
+                if (slot < T) {
+                    const uint32_t t = L.tv[slot];
+                    if (t & TF_ALIVE) {
+                        int sg = cofactor_filter_sign(r, C[k], P[k]);
+                        if (sg == 0) {
+                            const int a = L.vid[tvert(t, 0)], b = L.vid[tvert(t, 1)], c = L.vid[tvert(t, 2)];
+                            const double *pp[5] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b,
+                                                   p.pts + 3 * (int64_t)c, p.pts + 3 * (int64_t)s,
+                                                   p.pts + 3 * (int64_t)gv};
+                            const int idx[5] = {a, b, c, s, gv};
+                            sg = orient4_full(pp, idx, cnt);
+                        }
+                        if (sg < 0) {
+                            L.tv[slot] = t | TF_VIS;
+                            nv++;
+                        }
+                    }
+                }
+            }
+            int nvis;
+            const int vbase = wave_excl(nv, &nvis);
+            if (nvis == 0) continue; // drowned at insertion
+            const int Vn = V;
+            if (lane == 0) {
+                L.vid[Vn] = gv;
+                L.vc[Vn][0] = r[0];
+                L.vc[Vn][1] = r[1];
+                L.vc[Vn][2] = r[2];
+                L.vc[Vn][3] = r[3];
+                L.ins_local[q] = (int16_t)Vn;
+            }
+            __syncthreads();
+            // hole boundary: edges of freed triangles whose neighbour survives
+            int nb = 0;
+#pragma unroll
+            for (int k = 0; k < KPL; k++) {
+                const int slot = lane + 64 * k;
+                if (slot < T && (L.tv[slot] & TF_VIS))
+                    for (int i = 0; i < 3; i++) nb += (L.tv[L.to[slot][i]] & TF_VIS) ? 0 : 1;
+            }
+            int nbtot;
+            int e = wave_excl(nb, &nbtot);
+            if (nbtot > TMAX || nbtot < 3) {
+                failed = true;
+                break;
+            }
+            int fpos = nfree + vbase;
+#pragma unroll
+            for (int k = 0; k < KPL; k++) {
+                const int slot = lane + 64 * k;
+                if (slot < T && (L.tv[slot] & TF_VIS)) {
+                    const uint32_t t = L.tv[slot];
+                    for (int i = 0; i < 3; i++) {
+                        const int u = L.to[slot][i];
+                        if (L.tv[u] & TF_VIS) continue;
+                        int j = 0;
+                        while (j < 2 && L.to[u][j] != slot) j++;
+                        L.be_a[e] = (uint16_t)tvert(t, (i + 1) % 3);
+                        L.be_b[e] = (uint16_t)tvert(t, (i + 2) % 3);
+                        L.be_u[e] = (uint16_t)u;
+                        L.be_j[e] = (uint8_t)j;
+                        e++;
+                    }
+                    L.freel[fpos++] = (uint16_t)slot;
+                }
+            }
+            __syncthreads();
+            const int nfree_tot = nfree + nvis;
+#pragma unroll
+            for (int k = 0; k < KPL; k++) {
+                const int slot = lane + 64 * k;
+                if (slot < T && (L.tv[slot] & TF_VIS)) L.tv[slot] = 0u;
+            }
+            for (int k = lane; k < nbtot; k += 64)
+                L.be_slot[k] = (uint16_t)(k < nfree_tot ? L.freel[nfree_tot - 1 - k] : T + (k - nfree_tot));
+            const int newT = T + (nbtot > nfree_tot ? nbtot - nfree_tot : 0);
+            if (newT > TMAX || Vn + 1 > VMAX) {
+                failed = true;
+                break;
+            }
+            __syncthreads();
+            for (int k = lane; k < nbtot; k += 64) {
+                const int slot = L.be_slot[k], a = L.be_a[k], b = L.be_b[k], u = L.be_u[k];
+                L.tv[slot] = tpack(a, b, Vn, TF_ALIVE | TF_FRESH);
+                L.to[slot][2] = (uint16_t)u;
+                L.to[u][L.be_j[k]] = (uint16_t)slot;
+                L.vstart[a] = (int16_t)slot;
+                L.vend[b] = (int16_t)slot;
+            }
+            __syncthreads();
+            for (int k = lane; k < nbtot; k += 64) {
+                const int slot = L.be_slot[k];
+                L.to[slot][0] = (uint16_t)L.vstart[L.be_b[k]];
+                L.to[slot][1] = (uint16_t)L.vend[L.be_a[k]];
+            }
+            nfree = nfree_tot - (nbtot < nfree_tot ? nbtot : nfree_tot);
+            T = newT;
+            V = Vn + 1;
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < KPL; k++) {
+                const int slot = lane + 64 * k;
+                if (slot < T) {
+                    const uint32_t t = L.tv[slot];
+                    if (t & TF_FRESH) {
+                        tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], C[k], P[k]);
+                        L.tv[slot] = t & ~TF_FRESH;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+        if (failed) {
+            if (lane == 0) {
+                atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
+                p.out_vnum[s] = 0;
+                p.out_tnum[s] = 0;
+            }
+            __syncthreads();
+            continue;
+        }
+        // ---- finalize: link set, drowned insertions, compaction, pairs, history ----
+        for (int v = lane; v < V; v += 64) L.present[v] = 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KPL; k++) {
+            const int slot = lane + 64 * k;
+            if (slot < T) {
+                const uint32_t t = L.tv[slot];
+                if (t & TF_ALIVE) {
+                    L.present[tvert(t, 0)] = 1;
+                    L.present[tvert(t, 1)] = 1;
+                    L.present[tvert(t, 2)] = 1;
+                }
+            }
+        }
+        __syncthreads();
+        for (int base = 0; base < m; base += 64) {
+            const int q = base + lane;
+            bool dr = false;
+            if (q < m) {
+                const int lv = L.ins_local[q];
+                dr = lv < 0 || !L.present[lv];
+            }
+            const long long slot = g4d_wave_append(dr, &cnt[G4D_CNT_APPEND0]);
+            if (dr) {
+                if (slot < p.drown_cap) p.drown[slot] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)L.ins[q];
+                else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
+            }
+        }
+        // vertex compaction
+        const int64_t vo = p.out_voff[s], to = p.out_toff[s];
+        int vcount = 0;
+        for (int base = 0; base < V; base += 64) {
+            const int v = base + lane;
+            const bool pr = v < V && L.present[v];
+            int tot;
+            const int rk = g4d_wave_excl_count(pr, &tot);
+            if (pr) {
+                L.vremap[v] = (uint16_t)(vcount + rk);
+                p.out_vpool[vo + vcount + rk] = L.vid[v];
+            }
+            vcount += tot;
+        }
+        int tcount = 0;
+        for (int base = 0; base < T; base += 64) {
+            const int t = base + lane;
+            const bool al = t < T && (L.tv[t] & TF_ALIVE);
+            int tot;
+            const int rk = g4d_wave_excl_count(al, &tot);
+            if (al) L.remap[t] = (uint16_t)(tcount + rk);
+            tcount += tot;
+        }
+        if (tcount > (int)(p.out_toff[s + 1] - to) || tcount != 2 * vcount - 4) {
+            // not a triangulated 2-sphere: report instead of writing past the star's slots
+            if (lane == 0) {
+                atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
+                p.out_vnum[s] = 0;
+                p.out_tnum[s] = 0;
+            }
+            __syncthreads();
+            continue;
+        }
+        __syncthreads();
+        for (int base = 0; base < T; base += 64) {
+            const int t = base + lane;
+            uint32_t tt = 0;
+            if (t < T) tt = L.tv[t];
+            const bool al = t < T && (tt & TF_ALIVE);
+            if (al) {
+                const int a = tvert(tt, 0), b = tvert(tt, 1), c = tvert(tt, 2);
+                p.out_tpool[to + L.remap[t]] = hpack(L.vremap[a], L.vremap[b], L.vremap[c], L.remap[L.to[t][0]],
+                                                     L.remap[L.to[t][1]], L.remap[L.to[t][2]], true);
+            }
+            const bool unchecked = al && !(tt & TF_CHECKED);
+            // pairs from unchecked triangles: each edge (v_i, v_i+1) with v_i < v_i+1
+#pragma unroll
+            for (int i = 0; i < 3; i++) {
+                int ga = 0, gb = 0;
+                if (unchecked) {
+                    ga = L.vid[tvert(tt, i)];
+                    gb = L.vid[tvert(tt, (i + 1) % 3)];
+                }
+                const bool em = unchecked && ga < gb;
+                const long long slot = g4d_wave_append(em, &cnt[G4D_CNT_APPEND1]);
+                if (em) {
+                    if (slot < p.tpairs_cap) p.tpairs[slot] = ((uint64_t)(uint32_t)ga << 32) | (uint32_t)gb;
+                    else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
+                }
+            }
+        }
+        if (lane == 0) {
+            p.out_vnum[s] = vcount;
+            p.out_tnum[s] = tcount;
+        }
+        if (p.mode == 1) {
+            // merged history: old list and this pass's insertions are disjoint sorted lists
+            const int64_t hb = p.hist_off[s], he = p.hist_off[s + 1], ho = p.hist_out_off[s];
+            const int hn = (int)(he - hb);
+            for (int i = lane; i < hn; i += 64) {
+                const int x = (int)(uint32_t)p.hist_keys[hb + i];
+                int lo = 0, hi = m;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (L.ins[mid] < x) lo = mid + 1;
+                    else hi = mid;
+                }
+                p.hist_out[ho + i + lo] = p.hist_keys[hb + i];
+            }
+            for (int i = lane; i < m; i += 64) {
+                const int x = L.ins[i];
+                int64_t lo = hb, hi = he;
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if ((int)(uint32_t)p.hist_keys[mid] < x) lo = mid + 1;
+                    else hi = mid;
+                }
+                p.hist_out[ho + i + (lo - hb)] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)x;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_star_kernel(const StarParams &p, bool big, int nblocks, hipStream_t s)
+{
+    const int nitems = p.star_list ? p.nlist : p.n;
+    if (nitems <= 0) return hipSuccess;
+    if (big)
+        hipLaunchKernelGGL(k_star<STAR_VMAX_BIG>, dim3(nblocks), dim3(64), 0, s, p, nitems, 1);
+    else
+        hipLaunchKernelGGL(k_star<STAR_VMAX_SMALL>, dim3(nblocks), dim3(64), 0, s, p, nitems, 0);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
+// Containment proofs for drowned insertions (GD/GDelaunay/GDelPredKernels.cu:427-520).
+// exVert is the lowest link vertex; the proof triangle is the unique live triangle
+// without exVert whose three edges give one orient4(s, v, exVert, edge) sign.
+// ------------------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(64) k_proofs(ProofParams p)
+{
+    const int lane = threadIdx.x;
+    for (int64_t task = blockIdx.x; task < p.ntasks; task += gridDim.x) {
+        const uint64_t tk = p.tasks[task];
+        const int s = (int)(tk >> 32), v = (int)(uint32_t)tk;
+        const int vn = p.vnum[s], tn = p.tnum[s];
+        const int64_t vo = p.voff[s], to = p.toff[s];
+        int mn = 0x7fffffff;
+        for (int i = lane; i < vn; i += 64) mn = min(mn, p.vpool[vo + i]);
+        const int ex = g4d_wave_min(mn);
+        int k0 = 0x7fffffff, k1 = 0x7fffffff, k2 = 0x7fffffff;
+        for (int t = lane; t < tn; t += 64) {
+            const uint64_t h = p.tpool[to + t];
+            const int a = p.vpool[vo + hvert(h, 0)], b = p.vpool[vo + hvert(h, 1)], c = p.vpool[vo + hvert(h, 2)];
+            if (a == ex || b == ex || c == ex) continue;
+            const int tri[3] = {a, b, c};
+            int o[3];
+            bool ok = true;
+            for (int i = 0; i < 3 && ok; i++) {
+                const int pv = tri[i], tv2 = tri[(i + 1) % 3];
+                const double *pp[5] = {p.pts + 3 * (int64_t)s, p.pts + 3 * (int64_t)v, p.pts + 3 * (int64_t)ex,
+                                       p.pts + 3 * (int64_t)pv, p.pts + 3 * (int64_t)tv2};
+                const int idx[5] = {s, v, ex, pv, tv2};
+                o[i] = orient4_full(pp, idx, p.cnt);
+                if (i > 0 && o[i] != o[0]) ok = false;
+            }
+            if (!ok) continue;
+            int x0 = a, x1 = b, x2 = c, tmp;
+            if (x0 > x1) { tmp = x0; x0 = x1; x1 = tmp; }
+            if (x1 > x2) { tmp = x1; x1 = x2; x2 = tmp; }
+            if (x0 > x1) { tmp = x0; x0 = x1; x1 = tmp; }
+            if (x0 < k0 || (x0 == k0 && (x1 < k1 || (x1 == k1 && x2 < k2)))) {
+                k0 = x0;
+                k1 = x1;
+                k2 = x2;
+            }
+        }
+        const int b0 = g4d_wave_min(k0);
+        const int b1 = g4d_wave_min(k0 == b0 ? k1 : 0x7fffffff);
+        const int b2 = g4d_wave_min((k0 == b0 && k1 == b1) ? k2 : 0x7fffffff);
+        if (lane == 0) {
+            uint64_t *o = p.out + 4 * task;
+            if (b0 == 0x7fffffff) {
+                atomicOr(&p.cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_PROOF);
+                o[0] = o[1] = o[2] = o[3] = ~0ull;
+            } else {
+                const int xs[4] = {ex, b0, b1, b2};
+                for (int j = 0; j < 4; j++) {
+                    const int a = v < xs[j] ? v : xs[j], b = v < xs[j] ? xs[j] : v;
+                    o[j] = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
+                }
+            }
+        }
+    }
+}
+
+hipError_t launch_proofs(const ProofParams &p, hipStream_t s)
+{
+    if (p.ntasks <= 0) return hipSuccess;
+    const int nb = (int)(p.ntasks < 65536 ? p.ntasks : 65536);
+    hipLaunchKernelGGL(k_proofs, dim3(nb), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
+// Drowned pairs, mutual removal (GD/GDelaunay/GDelHost.cu:832-875), history filter,
+// reciprocation, capacities
+// ------------------------------------------------------------------------------------
+
+__global__ void k_drown_keys(const uint64_t *drown, int64_t m, uint64_t *keys)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t s = (uint32_t)(drown[i] >> 32), v = (uint32_t)drown[i];
+        const uint32_t mn = s < v ? s : v, mx = s < v ? v : s;
+        keys[i] = ((uint64_t)mn << 33) | ((uint64_t)mx << 1) | (uint64_t)(s == mn ? 0 : 1);
+    }
+}
+
+hipError_t launch_drown_keys(const uint64_t *drown, int64_t m, uint64_t *keys, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_drown_keys, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, drown, m, keys);
+    return hipGetLastError();
+}
+
+__global__ void k_nonmutual(const uint64_t *keys, int64_t m, uint64_t *tasks, unsigned long long *cnt)
+{
+    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < m; base += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = base + threadIdx.x;
+        bool keep = false;
+        uint64_t task = 0;
+        if (i < m) {
+            const uint64_t k = keys[i] >> 1;
+            const bool mutual = (i > 0 && (keys[i - 1] >> 1) == k) || (i + 1 < m && (keys[i + 1] >> 1) == k);
+            keep = !mutual;
+            const uint32_t mn = (uint32_t)(keys[i] >> 33), mx = (uint32_t)((keys[i] >> 1) & 0xffffffffu);
+            const bool dir = keys[i] & 1;
+            const uint32_t s = dir ? mx : mn, v = dir ? mn : mx;
+            task = ((uint64_t)s << 32) | v;
+        }
+        const long long slot = g4d_wave_append(keep, &cnt[G4D_CNT_APPEND0]);
+        if (keep) tasks[slot] = task;
+    }
+}
+
+hipError_t launch_nonmutual(const uint64_t *keys, int64_t m, uint64_t *tasks, unsigned long long *cnt, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_nonmutual, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, keys, m, tasks, cnt);
+    return hipGetLastError();
+}
+
+__global__ void k_hist_filter(const uint64_t *pairs, int64_t m, const uint64_t *hist, const int64_t *hoff, uint8_t *keep)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = pairs[i];
+        const uint32_t a = (uint32_t)(k >> 32), b = (uint32_t)k;
+        if (!(a < b)) {
+            keep[i] = 0;
+            continue;
+        }
+        int64_t lo = hoff[a], hi = hoff[a + 1];
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (hist[mid] < k) lo = mid + 1;
+            else hi = mid;
+        }
+        keep[i] = (lo < hoff[a + 1] && hist[lo] == k) ? 0 : 1;
+    }
+}
+
+hipError_t launch_hist_filter(const uint64_t *pairs, int64_t m, const uint64_t *hist_keys, const int64_t *hist_off,
+                              uint8_t *keep, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hist_filter, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, pairs, m, hist_keys, hist_off,
+                       keep);
+    return hipGetLastError();
+}
+
+__global__ void k_reciprocate(uint64_t *keys, int64_t m)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i];
+        keys[m + i] = (k << 32) | (k >> 32);
+    }
+}
+
+hipError_t launch_reciprocate(uint64_t *keys, int64_t m, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_reciprocate, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, keys, m);
+    return hipGetLastError();
+}
+
+__global__ void k_capacities(int n, const int *vnum, const int64_t *ins_off, const int64_t *hist_off, int64_t *vcap,
+                             int64_t *tcap, int64_t *hcap)
+{
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const int64_t m = ins_off[s + 1] - ins_off[s];
+        const int64_t v = (vnum ? vnum[s] : 0) + m;
+        vcap[s] = v;
+        tcap[s] = v >= 4 ? 2 * v - 4 : 4;
+        if (hcap) hcap[s] = (hist_off[s + 1] - hist_off[s]) + m;
+    }
+}
+
+hipError_t launch_capacities(int n, const int *vnum, const int64_t *ins_off, const int64_t *hist_off, int64_t *vcap,
+                             int64_t *tcap, int64_t *hcap, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_capacities, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, n, vnum, ins_off, hist_off, vcap,
+                       tcap, hcap);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
+// Output tetrahedra: owner triangles (star id below its three link vertices) whose exact
+// orient3d(v0, v1, v2, star) is negative; upper-hull and flat ones are dropped.
+// ------------------------------------------------------------------------------------
+
+template <bool WRITE>
+__global__ void k_out(OutParams p)
+{
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < p.n; s += gridDim.x * blockDim.x) {
+        const int tn = p.tnum[s];
+        const int64_t vo = p.voff[s], to = p.toff[s];
+        int64_t k = WRITE ? p.star_tet_off[s] : 0;
+        int64_t c = 0;
+        for (int t = 0; t < tn; t++) {
+            const uint64_t h = p.tpool[to + t];
+            const int a = p.vpool[vo + hvert(h, 0)], b = p.vpool[vo + hvert(h, 1)], cc = p.vpool[vo + hvert(h, 2)];
+            if (!(s < a && s < b && s < cc)) continue;
+            const double *pp[4] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b, p.pts + 3 * (int64_t)cc,
+                                   p.pts + 3 * (int64_t)s};
+            if (orient3d_full(pp, p.cnt) >= 0) continue;
+            if (WRITE) {
+                int *o = p.tets + 4 * (k + c);
+                o[0] = a;
+                o[1] = b;
+                o[2] = cc;
+                o[3] = s;
+            }
+            c++;
+        }
+        if (!WRITE) p.star_tet_count[s] = c;
+    }
+}
+
+hipError_t launch_out_count(const OutParams &p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_out<false>, dim3(g4d_blocks(p.n, 64, 65536)), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_out_write(const OutParams &p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_out<true>, dim3(g4d_blocks(p.n, 64, 65536)), dim3(64), 0, s, p);
+    return hipGetLastError();
+}
+
+__device__ __forceinline__ void face_of(const int *tt, int i, int *f)
+{
+    int q = 0;
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        if (j != i) f[q++] = tt[j];
+    int tmp;
+    if (f[0] > f[1]) { tmp = f[0]; f[0] = f[1]; f[1] = tmp; }
+    if (f[1] > f[2]) { tmp = f[1]; f[1] = f[2]; f[2] = tmp; }
+    if (f[0] > f[1]) { tmp = f[0]; f[0] = f[1]; f[1] = tmp; }
+}
+
+__global__ void k_adjacency(const int *tets, int64_t nt, int *opp, uint32_t *table, uint64_t mask,
+                            unsigned long long *cnt)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (int64_t)gridDim.x * blockDim.x) {
+        const int *tt = tets + 4 * t;
+        for (int i = 0; i < 4; i++) {
+            int f[3];
+            face_of(tt, i, f);
+            uint64_t h = (uint64_t)(uint32_t)f[0] * 0x9E3779B97F4A7C15ull ^ (uint64_t)(uint32_t)f[1] * 0xC2B2AE3D27D4EB4Full ^
+                         (uint64_t)(uint32_t)f[2] * 0x165667B19E3779F9ull;
+            h ^= h >> 29;
+            h *= 0xBF58476D1CE4E5B9ull;
+            h ^= h >> 32;
+            uint64_t slot = h & mask;
+            const uint32_t mine = (uint32_t)((t << 2) | i) + 1u;
+            for (uint64_t probe = 0; probe <= mask; probe++) {
+                uint32_t cur = atomicCAS(&table[slot], 0u, mine);
+                if (cur == 0u) break; // first occurrence stored
+                const int64_t t2 = (int64_t)((cur - 1u) >> 2);
+                const int i2 = (int)((cur - 1u) & 3u);
+                int g[3];
+                face_of(tets + 4 * t2, i2, g);
+                if (g[0] == f[0] && g[1] == f[1] && g[2] == f[2]) {
+                    if (atomicCAS(&opp[4 * t2 + i2], -1, (int)t) != -1)
+                        atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_FACE);
+                    opp[4 * t + i] = (int)t2;
+                    break;
+                }
+                slot = (slot + 1) & mask;
+            }
+        }
+    }
+}
+
+hipError_t launch_adjacency(const int *tets, int64_t nt, int *opp, uint32_t *table, int64_t table_size,
+                            unsigned long long *cnt, hipStream_t s)
+{
+    if (nt <= 0) return hipSuccess;
+    HIP_CHECK_RET(hipMemsetAsync(table, 0, sizeof(uint32_t) * (size_t)table_size, s));
+    HIP_CHECK_RET(hipMemsetAsync(opp, 0xff, sizeof(int) * 4 * (size_t)nt, s));
+    hipLaunchKernelGGL(k_adjacency, dim3(g4d_blocks(nt, 256, 65536)), dim3(256), 0, s, tets, nt, opp, table,
+                       (uint64_t)(table_size - 1), cnt);
+    return hipGetLastError();
+}
+
+__global__ void k_map_perm(int *a, int64_t m, const int *perm)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        a[i] = perm[a[i]];
+}
+
+hipError_t launch_map_perm(int *tets, int64_t m4, const int *perm, hipStream_t s)
+{
+    if (m4 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_map_perm, dim3(g4d_blocks(m4, 256, 65536)), dim3(256), 0, s, tets, m4, perm);
+    return hipGetLastError();
+}
+
+} // namespace g4d

@@ -1,0 +1,124 @@
+/*
+ * gStar4D for MI355X — C ABI.
+ *
+ * Drop-in boundary for the reference's triangulation call (GDelaunay/GDelaunay/GDelaunay.h:39-42):
+ *   gdelInit(const Config&, Point3HVec& points, Point3HVec& scaled)   -> g4d_create + input
+ *   gdelCompute(double& tPba, double& tStar, double& tSplay, double& tOut) -> g4d_triangulate
+ *   getTetraFromGpu(TetraMesh&)                                        -> g4d_result / g4d_copy_result
+ *   gdelDeInit()                                                       -> g4d_destroy
+ * The C++ shim with those four functions is include/gdelaunay_shim.hpp.
+ *
+ * Points are FP64 xyz (N x 3, array of structs).  Output tetrahedra are int32 quadruples with
+ * orient3d(v0, v1, v2, v3) < 0 (the reference's convention, GD/GDelaunay/GDelPredKernels.cu:648-661;
+ * GD/Common/Geometry.cu:319-344) and opp[i] = tetrahedron across the face opposite v[i] or -1
+ * (GD/Common/Geometry.h:147-151).  The tetrahedron set equals the reference's: the exact
+ * Delaunay tetrahedralisation of the points lifted to the paraboloid, degeneracies resolved by
+ * Simulation of Simplicity over Morton-order point ranks, flat tetrahedra dropped.
+ *
+ * Errors: every entry point returns a g4d_status; g4d_last_error() gives a message.  Nothing
+ * calls exit().  One context per host thread; calls on one context are serialised.
+ */
+#ifndef GSTAR4D_H
+#define GSTAR4D_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct g4d_ctx g4d_ctx;
+
+typedef enum {
+    G4D_OK = 0,
+    G4D_ERR_INVALID_INPUT = 1,
+    G4D_ERR_DUPLICATE_POINTS = 2,
+    G4D_ERR_GRID_LIMIT = 3,
+    G4D_ERR_OOM = 4,
+    G4D_ERR_DEVICE = 5,
+    G4D_ERR_NOT_CONVERGED = 6,
+    G4D_ERR_CAPACITY = 7,
+    G4D_ERR_PREDICATE_RANGE = 8,
+    G4D_ERR_INTERNAL = 9
+} g4d_status;
+
+typedef enum {
+    G4D_INDEX_INPUT = 0,  /* vertex ids refer to the caller's input order */
+    G4D_INDEX_MORTON = 1  /* vertex ids refer to Morton order, as the reference's gdelInit
+                             leaves them (GD/GDelaunay/GDelData.cu:125-130) */
+} g4d_index_space;
+
+typedef struct {
+    int32_t grid_size;          /* PBA grid width G, 4..1024 (Config::_gridSize) */
+    int32_t min_workset;        /* MinWorksetSize (GD/GDelaunay/GDelCommon.h:49), 4..64 */
+    int32_t max_rounds;         /* splaying round guard */
+    int32_t index_space;        /* g4d_index_space */
+    int32_t device;             /* HIP device ordinal */
+    int32_t compute_adjacency;  /* 1: fill opp */
+} g4d_options;
+
+typedef struct {
+    int64_t num_tets;
+    const int32_t *tets; /* 4*num_tets, owned by the context until the next call */
+    const int32_t *opp;  /* 4*num_tets or NULL */
+    const int32_t *perm; /* num_points: perm[k] = input index of the k-th point in Morton order */
+    int32_t num_points;
+} g4d_result;
+
+typedef struct {
+    double t_morton_ms, t_pba_ms, t_pairs_ms, t_star_ms, t_splay_ms, t_out_ms, t_adj_ms, t_total_ms;
+    int32_t num_points;
+    int32_t grid_size;
+    int32_t splay_rounds;      /* LogLoopNum in the reference */
+    int32_t max_workset;
+    int32_t big_stars;         /* stars routed to the large-star kernel */
+    int32_t pad0;
+    int64_t num_missing;
+    int64_t num_grid_pairs;    /* sort-unique Voronoi-neighbour pairs (one direction) */
+    int64_t num_workset;       /* workset entries, both directions, after pumping */
+    int64_t num_pumped;
+    int64_t splay_insertions;  /* reciprocated insertions over all rounds */
+    int64_t num_tets;
+    int64_t exact_calls;       /* orient4 filter failures resolved exactly */
+    int64_t sos_calls;         /* exact zeros resolved by SoS */
+    int64_t pba_bytes;         /* algorithmic PBA bytes: 24 G^3 */
+    double t_pba_pass_ms[3];   /* per axis pass (z, y, x) */
+} g4d_stats;
+
+int g4d_default_options(g4d_options *opt);
+int g4d_create(g4d_ctx **ctx, const g4d_options *opt);
+void g4d_destroy(g4d_ctx *ctx);
+const char *g4d_last_error(const g4d_ctx *ctx);
+int g4d_device_count(void);
+
+/* Host points in, host tetrahedra out (valid until the next call on ctx). */
+int g4d_triangulate(g4d_ctx *ctx, const double *xyz, int32_t n, g4d_result *out);
+/* Device-resident points in, device-resident result kept in ctx (for timing on HBM-resident
+   input); g4d_copy_result copies the last result to host buffers (any may be NULL). */
+int g4d_triangulate_device(g4d_ctx *ctx, const double *d_xyz, int32_t n, int64_t *num_tets);
+int g4d_copy_result(g4d_ctx *ctx, int32_t *tets, int32_t *opp, int32_t *perm);
+/* Upload host points into a device buffer owned by ctx and return its device address. */
+int g4d_upload_points(g4d_ctx *ctx, const double *xyz, int32_t n, const double **d_xyz);
+int g4d_get_stats(const g4d_ctx *ctx, g4d_stats *st);
+int g4d_set_options(g4d_ctx *ctx, const g4d_options *opt);
+
+/* ---- Kernel-level diagnostics (used by parity tests; no reference counterpart) ---- */
+/* orient4 of T tuples (15 doubles, 5 indices each).  mode 0: filter + exact + SoS;
+   1: exact + SoS only; 2: the star kernel's cached-cofactor filter with exact fallback,
+   tuples ordered (a, b, c, s, v).  depth_out (may be NULL): 0 filter/exact, else SoS depth. */
+int g4d_diag_orient4(g4d_ctx *ctx, const double *pts5, const int32_t *idx5, int64_t T, int32_t mode,
+                     int32_t *sign_out, int32_t *depth_out);
+int g4d_diag_orient3d(g4d_ctx *ctx, const double *pts4, int64_t T, int32_t *sign_out);
+/* Morton order and voxel coordinates (3 per input point). */
+int g4d_diag_morton(g4d_ctx *ctx, const double *xyz, int32_t n, int32_t grid, int32_t *perm_out, int32_t *vox_out);
+/* PBA of sites given as Morton-sorted voxel coordinates (3n); out: G^3 nearest-site point ids,
+   index (z*G + y)*G + x.  codes != 0 returns site codes instead of ids. */
+int g4d_diag_pba(g4d_ctx *ctx, const int32_t *vox_sorted, int32_t n, int32_t grid, int32_t codes, int32_t *out);
+/* Sorted unique Voronoi-neighbour pairs (min<<32|max) from an id grid. */
+int g4d_diag_grid_pairs(g4d_ctx *ctx, const int32_t *id_grid, int32_t grid, uint64_t *out, int64_t cap,
+                        int64_t *count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
