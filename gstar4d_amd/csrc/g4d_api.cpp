@@ -187,7 +187,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
     ev_record(ctx, EV_START);
 
     // ---- B: Morton order (GD/GDelaunay/GDelData.cu:91-131) ----
-    HCK(ctx->bounds.ensure(sizeof(double) * 512));
+    HCK(ctx->bounds.ensure(sizeof(double) * 768));
     HCK(g4d::launch_bounds(d_xyz, n, buf<double>(ctx->bounds), s));
     double hb[2];
     HCK(hipMemcpyAsync(hb, ctx->bounds.ptr, sizeof(hb), hipMemcpyDeviceToHost, s));
@@ -698,7 +698,7 @@ int g4d_diag_morton(g4d_ctx *ctx, const double *xyz, int32_t n, int32_t grid, in
     hipStream_t s = ctx->stream;
     const double *d_xyz = nullptr;
     CK(g4d_upload_points(ctx, xyz, n, &d_xyz));
-    HCK(ctx->bounds.ensure(sizeof(double) * 512));
+    HCK(ctx->bounds.ensure(sizeof(double) * 768));
     HCK(ctx->keys.ensure(sizeof(uint64_t) * (size_t)n));
     HCK(ctx->pts.ensure(sizeof(double) * 3 * (size_t)n));
     HCK(ctx->perm.ensure(sizeof(int) * (size_t)n));

@@ -5,7 +5,7 @@
 
 namespace g4d {
 
-hipError_t launch_bounds(const double *xyz, int n, double *part /* >= 512 doubles */, hipStream_t s);
+hipError_t launch_bounds(const double *xyz, int n, double *part /* >= 768 doubles */, hipStream_t s);
 hipError_t launch_morton_keys(const double *xyz, int n, int g, const double *bounds, uint64_t *keys, hipStream_t s);
 hipError_t launch_gather_sorted(const double *xyz, const uint64_t *keys, int n, int g, const double *bounds,
                                 double *pts, int *perm, int *vox, hipStream_t s);

@@ -24,12 +24,18 @@ namespace g4d {
 __global__ void __launch_bounds__(256) k_bounds_partial(const double *xyz, int64_t m, double *part)
 {
     __shared__ double smin[256], smax[256];
+    __shared__ int sbad;
+    if (threadIdx.x == 0) sbad = 0;
+    __syncthreads();
     double lo = 1e308, hi = -1e308;
+    bool bad = false;
     for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < m; i += (int64_t)gridDim.x * 256) {
         const double v = xyz[i];
+        bad |= !isfinite(v);
         lo = fmin(lo, v);
         hi = fmax(hi, v);
     }
+    if (bad) atomicOr(&sbad, 1);
     smin[threadIdx.x] = lo;
     smax[threadIdx.x] = hi;
     __syncthreads();
@@ -41,20 +47,25 @@ __global__ void __launch_bounds__(256) k_bounds_partial(const double *xyz, int64
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        part[2 * blockIdx.x] = smin[0];
-        part[2 * blockIdx.x + 1] = smax[0];
+        part[3 * blockIdx.x] = smin[0];
+        part[3 * blockIdx.x + 1] = smax[0];
+        part[3 * blockIdx.x + 2] = sbad ? 1.0 : 0.0;
     }
 }
 
+// Reduces the per-block partials into part[0] = min, part[1] = max; a non-finite input
+// coordinate makes part[0] NaN so the host rejects the input.
 __global__ void k_bounds_final(double *part, int nb)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     double lo = part[0], hi = part[1];
+    bool bad = part[2] != 0.0;
     for (int b = 1; b < nb; b++) {
-        lo = fmin(lo, part[2 * b]);
-        hi = fmax(hi, part[2 * b + 1]);
+        lo = fmin(lo, part[3 * b]);
+        hi = fmax(hi, part[3 * b + 1]);
+        bad |= part[3 * b + 2] != 0.0;
     }
-    part[0] = lo;
+    part[0] = bad ? __longlong_as_double(0x7ff8000000000000ll) : lo;
     part[1] = hi;
 }
 

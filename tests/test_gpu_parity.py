@@ -1,0 +1,215 @@
+"""GPU parity: every stage of the HIP library against the CPU oracle, through the C ABI."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import gstar4d_amd as g
+from gstar4d_amd import datasets
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLD_PRED = os.path.join(HERE, "golden", "predicates_golden.npz")
+GOLD_PIPE = os.path.join(HERE, "golden", "pipeline_golden.json")
+
+
+def _h(tets):
+    return hashlib.sha256(np.ascontiguousarray(tets, "<i4").tobytes()).hexdigest()
+
+
+# ---------------------------------------------------------------- predicates
+
+def test_orient3d_golden_vectors(tri):
+    d = np.load(GOLD_PRED, allow_pickle=False)
+    got = tri.diag_orient3d(d["o3_pts"].reshape(-1, 12))
+    assert (got == d["o3_sign"]).all()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_insphere_golden_vectors(tri, mode):
+    d = np.load(GOLD_PRED, allow_pickle=False)
+    pts = d["is_pts"]
+    idx = np.tile(np.arange(5, dtype=np.int32), (pts.shape[0], 1))
+    sign, depth = tri.diag_orient4(pts.reshape(-1, 15), idx, mode=mode)
+    exact = np.where(depth > 0, 0, sign)  # depth > 0: the exact determinant was zero
+    assert (exact == d["is_sign"]).all()
+
+
+def _tuples(rng, n):
+    out = [rng.random((n, 5, 3))]
+    out.append(rng.integers(0, 3, size=(n, 5, 3)).astype(np.float64) / 4.0)
+    out.append(rng.integers(0, 80, size=(n, 5, 3)).astype(np.float64) / 80.0)
+    w = rng.random((n, 5, 3))
+    w[:, 0] *= 2.0 ** -40  # forces the 4-limb exact path
+    out.append(w)
+    t = np.concatenate(out)
+    keep = np.array([len({tuple(r) for r in q}) == 5 for q in t])
+    return t[keep]
+
+
+def test_orient4_sos_matches_oracle(tri, orc):
+    rng = np.random.default_rng(11)
+    t = _tuples(rng, 1500)
+    idx = np.array([rng.permutation(10 ** 6)[:5] for _ in range(t.shape[0])], np.int32)
+    sign, depth = tri.diag_orient4(t.reshape(-1, 15), idx, mode=0)
+    ref = [orc.orient4_sos(t[j], idx[j]) for j in range(t.shape[0])]
+    assert (sign == np.array([r[0] for r in ref])).all()
+    assert (depth == np.array([r[1] for r in ref])).all()
+    assert (depth > 0).sum() > 100
+
+
+def test_cofactor_filter_never_wrong(tri, orc):
+    rng = np.random.default_rng(12)
+    t = _tuples(rng, 3000)
+    idx = np.tile(np.arange(5, dtype=np.int32), (t.shape[0], 1))
+    filt, _ = tri.diag_orient4(t.reshape(-1, 15), idx, mode=2)
+    full, _ = tri.diag_orient4(t.reshape(-1, 15), idx, mode=0)
+    nz = filt != 0
+    assert (filt[nz] == full[nz]).all()
+    assert nz.sum() > t.shape[0] // 2
+    # near-cospherical points stress the bound
+    v = rng.normal(size=(4000, 5, 3))
+    v = 0.5 + 0.3 * v / np.linalg.norm(v, axis=2, keepdims=True)
+    idx = np.tile(np.arange(5, dtype=np.int32), (4000, 1))
+    filt, _ = tri.diag_orient4(v.reshape(-1, 15), idx, mode=2)
+    full, _ = tri.diag_orient4(v.reshape(-1, 15), idx, mode=0)
+    nz = filt != 0
+    assert (filt[nz] == full[nz]).all()
+
+
+# ---------------------------------------------------------------- grid stages
+
+@pytest.mark.parametrize("g_,n", [(64, 5000), (128, 20000), (1024, 3000)])
+def test_morton_matches_oracle(tri, orc, g_, n):
+    x = datasets.hard_mix(n, seed=2)
+    pg, vg = tri.diag_morton(x, g_)
+    po, vo = orc.morton_order(x, g_)
+    assert (pg == po).all() and (vg == vo).all()
+
+
+@pytest.mark.parametrize("g_,n", [(16, 40), (64, 3000), (96, 20000), (128, 8000)])
+def test_pba_matches_oracle(tri, orc, g_, n):
+    x = datasets.hard_mix(n, seed=3)
+    perm, vox = orc.morton_order(x, g_)
+    vs = vox[perm]
+    assert (tri.diag_pba(vs, g_, codes=True) == orc.pba(vs, g_, codes=True)).all()
+    ids = tri.diag_pba(vs, g_)
+    assert (ids == orc.pba(vs, g_)).all()
+    pairs = tri.diag_grid_pairs(ids)
+    assert (pairs == orc.grid_pairs(ids)).all()
+
+
+# ---------------------------------------------------------------- full pipeline
+
+def _parity(orc, x, grid, **kw):
+    with g.Triangulator(grid_size=grid, **kw) as t:
+        tets, opp, perm = t.triangulate(x)
+        st = t.stats()
+    ref, rperm, rst = orc.triangulate(x, grid, min_workset=kw.get("min_workset", 8))
+    assert (perm == rperm).all()
+    got = g.canonical(tets)
+    assert got.shape == ref.shape and (got == ref).all()
+    return tets, opp, perm, st, rst
+
+
+@pytest.mark.parametrize("n,grid", [(5, 8), (6, 8), (30, 16), (500, 16), (2000, 32), (20000, 64)])
+def test_uniform_small(orc, n, grid):
+    x = datasets.uniform_cube(n, seed=n)
+    tets, opp, perm, st, rst = _parity(orc, x, grid)
+    assert st["splay_rounds"] == rst["splay_rounds"]
+
+
+def test_orientation_and_adjacency(orc):
+    x = datasets.uniform_cube(3000, seed=9)
+    tets, opp, perm, st, _ = _parity(orc, x, 32)
+    for j in range(0, tets.shape[0], 13):
+        assert orc.orient3d(*x[tets[j]]) < 0
+    ref_opp, bad = orc.adjacency(tets, x.shape[0])
+    assert bad == 0 and (opp == ref_opp).all()
+
+
+def test_c1_full_size_against_golden(orc):
+    gold = json.load(open(GOLD_PIPE))["c1_uniform_100k_g128"]
+    x = datasets.make("c1_uniform_100k_g128")
+    with g.Triangulator(grid_size=128) as t:
+        tets, opp, perm = t.triangulate(x)
+        st = t.stats()
+    got = g.canonical(tets)
+    assert got.shape[0] == gold["num_tets"] and _h(got) == gold["sha256"]
+    assert st["splay_rounds"] == gold["splay_rounds"]
+
+
+@pytest.mark.parametrize("n,grid", [(20000, 128), (100000, 256)])
+def test_hard_mix_parity(orc, n, grid):
+    x = datasets.hard_mix(n, seed=1)
+    _, _, _, st, rst = _parity(orc, x, grid)
+    assert st["sos_calls"] > 0
+
+
+def test_clustered_parity(orc):
+    x = datasets.clustered_mix(60000, seed=2)
+    _parity(orc, x, 128)
+
+
+def test_lattice_degenerate_parity(orc):
+    x = np.stack(np.meshgrid(*[np.arange(9)] * 3, indexing="ij"), -1).reshape(-1, 3) / 9.0
+    _parity(orc, x, 32)
+
+
+def test_min_workset_extremes_same_output(orc):
+    x = datasets.uniform_cube(4000, seed=4)
+    a = _parity(orc, x, 32, min_workset=4)[0]
+    b = _parity(orc, x, 32, min_workset=40)[0]
+    assert (g.canonical(a) == g.canonical(b)).all()
+
+
+def test_morton_index_space_and_reuse(orc):
+    x = datasets.uniform_cube(5000, seed=5)
+    with g.Triangulator(grid_size=32, index_space="morton") as t:
+        tm, _, perm = t.triangulate(x)
+        tm2, _, _ = t.triangulate(x)  # context reuse
+    assert (tm == tm2).all()
+    ref, _, _ = orc.triangulate(x, 32)
+    assert (g.canonical(tm, perm) == ref).all()
+
+
+def test_errors_are_reported(orc):
+    with g.Triangulator(grid_size=32) as t:
+        x = datasets.uniform_cube(100, seed=1)
+        x[50] = x[10]
+        with pytest.raises(g.G4DError) as e:
+            t.triangulate(x)
+        assert e.value.status == "DUPLICATE_POINTS"
+        with pytest.raises(g.G4DError):
+            t.triangulate(x[:4])
+        y = datasets.uniform_cube(100, seed=2)
+        y[3, 1] = np.nan
+        with pytest.raises(g.G4DError):
+            t.triangulate(y)
+        t.set_grid(2000)
+        with pytest.raises(g.G4DError) as e:
+            t.triangulate(datasets.uniform_cube(100, seed=3))
+        assert e.value.status == "GRID_LIMIT"
+        t.set_grid(32)
+        tets, _, _ = t.triangulate(datasets.uniform_cube(100, seed=3))  # still usable
+        assert tets.shape[0] > 0
+
+
+def test_full_size_goldens(orc):
+    """C2 (4M uniform, G=512) and C3 (2M hard mix, G=512) against oracle hashes."""
+    gold = json.load(open(GOLD_PIPE))
+    ran = 0
+    for name in ("c2_uniform_4m_g512", "c3_hard_2m_g512"):
+        if name not in gold:
+            continue
+        x = datasets.make(name)
+        with g.Triangulator(grid_size=gold[name]["grid"], adjacency=False) as t:
+            tets, _, _ = t.triangulate(x)
+        got = g.canonical(tets)
+        assert got.shape[0] == gold[name]["num_tets"], name
+        assert _h(got) == gold[name]["sha256"], name
+        ran += 1
+    assert ran > 0
