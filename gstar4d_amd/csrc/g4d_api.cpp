@@ -11,6 +11,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -63,6 +64,25 @@ struct g4d_ctx {
     hipEvent_t ev[EV_NUM];
     bool ev_ok = false;
 };
+
+// Stage trace on stderr when the environment sets G4D_TRACE (flushed per line).
+static bool trace_on()
+{
+    static const bool on = std::getenv("G4D_TRACE") != nullptr;
+    return on;
+}
+
+static void trace(const char *fmt, ...)
+{
+    if (!trace_on()) return;
+    va_list ap;
+    va_start(ap, fmt);
+    fprintf(stderr, "[g4d] ");
+    vfprintf(stderr, fmt, ap);
+    fprintf(stderr, "\n");
+    fflush(stderr);
+    va_end(ap);
+}
 
 static int set_err(g4d_ctx *c, int code, const char *fmt, ...)
 {
@@ -264,6 +284,8 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
         HCK(g4d::launch_csr_offsets(pr, m, n, ws_off, s));
     }
     ctx->st.num_workset = m;
+    trace("n %d grid %d: missing %lld, grid pairs %lld, worksets %lld keys (%lld pumped)", n, G, (long long)nmiss,
+          (long long)ctx->st.num_grid_pairs, (long long)m, (long long)ctx->st.num_pumped);
     ev_record(ctx, EV_PAIRS);
 
     // ---- E: star creation (GD/GDelaunay/GDelHost.cu:543-743) ----
@@ -348,6 +370,8 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
             HCK(g4d::launch_star_kernel(sp, true, (int)std::min<int64_t>(nbig, 1024), s));
             CK(read_counters(ctx, hc));
         }
+        trace("star pass mode %d: %lld insertions, large stars %llu, drowned %llu, pairs %llu, err 0x%llx", mode,
+              (long long)nins, hc[G4D_CNT_APPEND2], hc[G4D_CNT_APPEND0], hc[G4D_CNT_APPEND1], hc[G4D_CNT_ERROR]);
         CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], mode ? "splaying" : "star creation"));
         *drown_n = (int64_t)hc[G4D_CNT_APPEND0];
         *tp_n = (int64_t)hc[G4D_CNT_APPEND1];
@@ -410,6 +434,8 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
             }
         }
         CK(read_counters(ctx, hc));
+        trace("round %d: %lld drowned, %lld proofs, %lld candidates, %lld new pairs", rounds + 1, (long long)nd,
+              (long long)ntask, (long long)ncand, (long long)nins);
         CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], "proofs"));
         if (nins == 0) {
             converged = true;
