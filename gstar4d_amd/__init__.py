@@ -30,7 +30,8 @@ class G4DError(RuntimeError):
 class Options(ctypes.Structure):
     _fields_ = [("grid_size", ctypes.c_int32), ("min_workset", ctypes.c_int32),
                 ("max_rounds", ctypes.c_int32), ("index_space", ctypes.c_int32),
-                ("device", ctypes.c_int32), ("compute_adjacency", ctypes.c_int32)]
+                ("device", ctypes.c_int32), ("compute_adjacency", ctypes.c_int32),
+                ("lds_star_limit", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
@@ -46,7 +47,7 @@ class Stats(ctypes.Structure):
                 ("t_adj_ms", ctypes.c_double), ("t_total_ms", ctypes.c_double),
                 ("num_points", ctypes.c_int32), ("grid_size", ctypes.c_int32),
                 ("splay_rounds", ctypes.c_int32), ("max_workset", ctypes.c_int32),
-                ("big_stars", ctypes.c_int32), ("pad0", ctypes.c_int32),
+                ("big_stars", ctypes.c_int32), ("huge_stars", ctypes.c_int32),
                 ("num_missing", ctypes.c_int64), ("num_grid_pairs", ctypes.c_int64),
                 ("num_workset", ctypes.c_int64), ("num_pumped", ctypes.c_int64),
                 ("splay_insertions", ctypes.c_int64), ("num_tets", ctypes.c_int64),
@@ -58,7 +59,6 @@ class Stats(ctypes.Structure):
         for f, _ in self._fields_:
             v = getattr(self, f)
             d[f] = list(v) if f == "t_pba_pass_ms" else v
-        d.pop("pad0", None)
         return d
 
 
@@ -124,12 +124,13 @@ class Triangulator:
     perm (N,) Morton rank -> input index."""
 
     def __init__(self, grid_size=256, min_workset=8, max_rounds=1000, index_space="input",
-                 device=0, adjacency=True):
+                 device=0, adjacency=True, lds_star_limit=0):
         self._L = load_library()
         self.opt = default_options(grid_size=grid_size, min_workset=min_workset,
                                    max_rounds=max_rounds,
                                    index_space=0 if index_space == "input" else 1,
-                                   device=device, compute_adjacency=1 if adjacency else 0)
+                                   device=device, compute_adjacency=1 if adjacency else 0,
+                                   lds_star_limit=lds_star_limit)
         self._ctx = _vp()
         rc = self._L.g4d_create(ctypes.byref(self._ctx), ctypes.byref(self.opt))
         if rc != 0:

@@ -55,7 +55,8 @@ struct g4d_ctx {
     DevBuf xyz_up, keys, pts, perm, vox, bounds, grid, gtmp, idgrid, pairs, ws_off, cnt;
     Pool pool[2];
     Hist hist[2];
-    DevBuf drown, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, big, tetcnt, tetoff, tets, opp, table;
+    DevBuf drown, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, big, huge, gwork, tetcnt, tetoff, tets, opp,
+        table;
     std::vector<int32_t> h_tets, h_opp, h_perm;
     int64_t num_tets = 0;
     int last_n = 0;
@@ -149,7 +150,7 @@ static int check_dev_error(g4d_ctx *ctx, unsigned long long flags, const char *s
     if (flags & G4D_DEV_ERR_PREDICATE_RANGE)
         return set_err(ctx, G4D_ERR_PREDICATE_RANGE, "%s: coordinates outside the exact-arithmetic range", stage);
     if (flags & G4D_DEV_ERR_STAR_CAPACITY)
-        return set_err(ctx, G4D_ERR_CAPACITY, "%s: a star exceeds %d link vertices", stage, g4d::STAR_VMAX_BIG);
+        return set_err(ctx, G4D_ERR_CAPACITY, "%s: a star exceeds its workspace", stage);
     return set_err(ctx, G4D_ERR_INTERNAL, "%s: device error flags 0x%llx", stage, flags);
 }
 
@@ -192,6 +193,8 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
     if (G < 4 || G > 1024) return set_err(ctx, G4D_ERR_GRID_LIMIT, "grid_size %d outside [4, 1024]", G);
     if (n < 5) return set_err(ctx, G4D_ERR_INVALID_INPUT, "need at least 5 points, got %d", n);
     if (n > (1 << 29)) return set_err(ctx, G4D_ERR_INVALID_INPUT, "too many points (%d)", n);
+    if (ctx->opt.lds_star_limit != 0 && ctx->opt.lds_star_limit < 4)
+        return set_err(ctx, G4D_ERR_INVALID_INPUT, "lds_star_limit %d below 4", ctx->opt.lds_star_limit);
     const int min_ws = ctx->opt.min_workset;
     if (min_ws < 4 || min_ws > 64) return set_err(ctx, G4D_ERR_INVALID_INPUT, "min_workset %d outside [4, 64]", min_ws);
     const int nb = bits_for(n);
@@ -293,8 +296,11 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
     HCK(ctx->tcap.ensure(sizeof(int64_t) * (size_t)(n + 1)));
     HCK(ctx->hcap.ensure(sizeof(int64_t) * (size_t)(n + 1)));
     HCK(ctx->big.ensure(sizeof(int) * (size_t)n));
+    HCK(ctx->huge.ensure(sizeof(int) * (size_t)n));
     int64_t *vcap = buf<int64_t>(ctx->vcap), *tcap = buf<int64_t>(ctx->tcap), *hcap = buf<int64_t>(ctx->hcap);
     const int nblk_star = 4096;
+    const int lds_limit = std::min(g4d::STAR_LDS_LIMIT, ctx->opt.lds_star_limit > 0 ? ctx->opt.lds_star_limit
+                                                                                  : g4d::STAR_LDS_LIMIT);
 
     auto star_pass = [&](int mode, const uint64_t *ins_keys, const int64_t *ins_off, int64_t nins, Pool *in,
                          Pool &out, const uint64_t *hkeys, const int64_t *hoff, Hist *hout, int64_t *drown_n,
@@ -354,24 +360,47 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
         sp.drown_cap = dcap;
         sp.tpairs = buf<uint64_t>(ctx->cand);
         sp.tpairs_cap = tpcap;
-        sp.big_list = buf<int>(ctx->big);
-        sp.big_cap = n;
         sp.cnt = cnt;
+        sp.lds_limit = lds_limit;
+        sp.defer_list = buf<int>(ctx->big);
+        sp.defer_cap = n;
+        sp.defer_counter = G4D_CNT_APPEND2;
         HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8 * 3, s));
+        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND3, 0, 8, s));
         HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
-        HCK(g4d::launch_star_kernel(sp, false, nblk_star, s));
+        HCK(g4d::launch_star_kernel(sp, 0, nblk_star, s));
         CK(read_counters(ctx, hc));
         const int64_t nbig = (int64_t)hc[G4D_CNT_APPEND2];
+        int64_t nhuge = 0;
         if (nbig > 0) {
+            // stars beyond the small kernel's capacity: the big LDS kernel, then the global one
             ctx->st.big_stars += (int32_t)nbig;
             sp.star_list = buf<int>(ctx->big);
             sp.nlist = (int)nbig;
+            sp.defer_list = buf<int>(ctx->huge);
+            sp.defer_counter = G4D_CNT_APPEND3;
             HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
-            HCK(g4d::launch_star_kernel(sp, true, (int)std::min<int64_t>(nbig, 1024), s));
+            HCK(g4d::launch_star_kernel(sp, 1, (int)std::min<int64_t>(nbig, 1024), s));
+            CK(read_counters(ctx, hc));
+            nhuge = (int64_t)hc[G4D_CNT_APPEND3];
+        }
+        if (nhuge > 0) {
+            ctx->st.huge_stars += (int32_t)nhuge;
+            HCK(hipMemsetAsync(cnt + G4D_CNT_MISC, 0, 8, s));
+            HCK(g4d::launch_need_max(sp, buf<int>(ctx->huge), (int)nhuge, cnt + G4D_CNT_MISC, s));
+            CK(read_counters(ctx, hc));
+            const int vcap_g = (int)hc[G4D_CNT_MISC];
+            const int nblk = (int)std::min<int64_t>(nhuge, 64);
+            const int64_t stride = (g4d::star_global_bytes(vcap_g) + 4095) & ~(int64_t)4095;
+            HCK(ctx->gwork.ensure((size_t)(stride * nblk)));
+            HCK(g4d::launch_star_global(sp, buf<int>(ctx->huge), (int)nhuge, buf<char>(ctx->gwork), stride, vcap_g,
+                                        nblk, s));
             CK(read_counters(ctx, hc));
         }
-        trace("star pass mode %d: %lld insertions, large stars %llu, drowned %llu, pairs %llu, err 0x%llx", mode,
-              (long long)nins, hc[G4D_CNT_APPEND2], hc[G4D_CNT_APPEND0], hc[G4D_CNT_APPEND1], hc[G4D_CNT_ERROR]);
+        trace("star pass mode %d: %lld insertions, large stars %lld (global-memory %lld), drowned %llu, pairs %llu, "
+              "err 0x%llx",
+              mode, (long long)nins, (long long)nbig, (long long)nhuge, hc[G4D_CNT_APPEND0], hc[G4D_CNT_APPEND1],
+              hc[G4D_CNT_ERROR]);
         CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], mode ? "splaying" : "star creation"));
         *drown_n = (int64_t)hc[G4D_CNT_APPEND0];
         *tp_n = (int64_t)hc[G4D_CNT_APPEND1];
@@ -536,6 +565,7 @@ int g4d_default_options(g4d_options *opt)
     opt->index_space = G4D_INDEX_INPUT;
     opt->device = 0;
     opt->compute_adjacency = 1;
+    opt->lds_star_limit = 0;
     return G4D_OK;
 }
 
@@ -582,7 +612,7 @@ void g4d_destroy(g4d_ctx *ctx)
     DevBuf *bufs[] = {&ctx->xyz_up, &ctx->keys, &ctx->pts, &ctx->perm, &ctx->vox, &ctx->bounds, &ctx->grid,
                       &ctx->gtmp, &ctx->idgrid, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->drown, &ctx->dkeys,
                       &ctx->tasks, &ctx->cand, &ctx->keep, &ctx->ins, &ctx->ins_off, &ctx->vcap, &ctx->tcap,
-                      &ctx->hcap, &ctx->big, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table};
+                      &ctx->hcap, &ctx->big, &ctx->huge, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table};
     for (DevBuf *b : bufs) b->release();
     for (auto &p : ctx->pool) p.release();
     for (auto &h : ctx->hist) h.release();

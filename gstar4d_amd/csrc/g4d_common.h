@@ -39,6 +39,7 @@ enum g4d_counter : int {
     G4D_CNT_ORIENT3 = 7,
     G4D_CNT_MISC = 8,
     G4D_CNT_WORK = 9,     // work-queue head
+    G4D_CNT_APPEND3 = 10,
     G4D_CNT_NUM = 16
 };
 

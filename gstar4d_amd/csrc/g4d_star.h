@@ -5,10 +5,16 @@
 
 namespace g4d {
 
-// Packed star triangle in HBM: local vertex indices (9 bits each), adjacency slots (10 bits
-// each), checked flag.
+// Star triangles in HBM.  A star with fewer than STAR_WIDE_MIN link vertices stores each
+// triangle in one word: local vertex indices (9 bits each), adjacency slots (10 bits each),
+// checked flag (bit 57).  A larger star stores two words per triangle: local vertices
+// (21 bits each, checked flag in bit 63), then adjacency slots (21 bits each).
 static constexpr int STAR_VMAX_SMALL = 128;
 static constexpr int STAR_VMAX_BIG = 512;
+static constexpr int STAR_WIDE_MIN = 512;
+// Stars whose link (plus this pass's insertions) exceeds the LDS kernels' limit run in the
+// global-memory star kernel, which has no fixed capacity.
+static constexpr int STAR_LDS_LIMIT = STAR_WIDE_MIN - 1;
 
 struct StarParams {
     const double *pts;
@@ -41,15 +47,25 @@ struct StarParams {
     int64_t drown_cap;
     uint64_t *tpairs;     // (a<<32 | b), a < b         counter G4D_CNT_APPEND1
     int64_t tpairs_cap;
-    int *big_list;        // stars deferred to the big-star kernel, counter G4D_CNT_APPEND2
-    int64_t big_cap;
+    int *defer_list;      // stars that exceed this kernel's capacity, counter `defer_counter`
+    int64_t defer_cap;
+    int defer_counter;
+    int lds_limit;        // largest vn + m handled in LDS (<= STAR_LDS_LIMIT)
     // work list (nullptr = all stars)
     const int *star_list;
     int nlist;
     unsigned long long *cnt;
 };
 
-hipError_t launch_star_kernel(const StarParams &p, bool big, int nblocks, hipStream_t s);
+// tier 0: LDS kernel for up to STAR_VMAX_SMALL link vertices; tier 1: up to STAR_LDS_LIMIT.
+hipError_t launch_star_kernel(const StarParams &p, int tier, int nblocks, hipStream_t s);
+// Global-memory star kernel for the stars in `list` (any size up to vcap link vertices);
+// each block owns a workspace of star_global_bytes(vcap) bytes at ws + blockIdx * stride.
+int64_t star_global_bytes(int vcap);
+hipError_t launch_star_global(const StarParams &p, const int *list, int nitems, char *ws, int64_t ws_stride, int vcap,
+                              int nblocks, hipStream_t s);
+// Largest vn + m over the stars in `list`, atomically maxed into *out.
+hipError_t launch_need_max(const StarParams &p, const int *list, int nitems, unsigned long long *out, hipStream_t s);
 
 struct ProofParams {
     const double *pts;
@@ -75,7 +91,8 @@ hipError_t launch_hist_filter(const uint64_t *pairs, int64_t m, const uint64_t *
                               uint8_t *keep, hipStream_t s);
 // Reciprocate m ordered pairs in place into 2m directed keys.
 hipError_t launch_reciprocate(uint64_t *keys, int64_t m, hipStream_t s);
-// Per-star counts from CSR offsets.
+// Per-star capacities from CSR offsets: v = vn + m link vertices, triangle words 2v - 4
+// (twice that for stars that may need the wide format), history hn + m.
 hipError_t launch_capacities(int n, const int *vnum, const int64_t *ins_off, const int64_t *hist_off,
                              int64_t *vcap, int64_t *tcap, int64_t *hcap, hipStream_t s);
 

@@ -49,6 +49,30 @@ __device__ __forceinline__ int hvert(uint64_t t, int i) { return (int)((t >> (9 
 __device__ __forceinline__ int hopp(uint64_t t, int i) { return (int)((t >> (27 + 10 * i)) & 1023u); }
 __device__ __forceinline__ bool hchecked(uint64_t t) { return (t >> 57) & 1u; }
 
+// Wide format (stars with >= STAR_WIDE_MIN link vertices): two words per triangle.
+static constexpr uint64_t W21 = (1ull << 21) - 1;
+__device__ __forceinline__ uint64_t wpack3(int a, int b, int c)
+{
+    return (uint64_t)a | ((uint64_t)b << 21) | ((uint64_t)c << 42);
+}
+__device__ __forceinline__ int wfield(uint64_t w, int i) { return (int)((w >> (21 * i)) & W21); }
+
+// Local vertex indices of triangle t of a star stored at tpool + to.
+__device__ __forceinline__ void star_tri_verts(const uint64_t *tpool, int64_t to, int t, bool wide, int *v)
+{
+    if (wide) {
+        const uint64_t w = tpool[to + 2 * (int64_t)t];
+        v[0] = wfield(w, 0);
+        v[1] = wfield(w, 1);
+        v[2] = wfield(w, 2);
+    } else {
+        const uint64_t h = tpool[to + t];
+        v[0] = hvert(h, 0);
+        v[1] = hvert(h, 1);
+        v[2] = hvert(h, 2);
+    }
+}
+
 template <int VMAX>
 struct StarLds {
     static constexpr int TMAX = 2 * VMAX;
@@ -78,7 +102,7 @@ __device__ __forceinline__ int wave_excl(int v, int *tot)
 }
 
 template <int VMAX>
-__global__ void __launch_bounds__(64) k_star(StarParams p, int nitems, int big)
+__global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
 {
     constexpr int TMAX = StarLds<VMAX>::TMAX;
     constexpr int KPL = TMAX / 64;
@@ -98,17 +122,12 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems, int big)
         const int m = (int)(p.ins_off[s + 1] - ib);
         const int vn = p.mode ? p.in_vnum[s] : 0;
         const int need = vn + m;
-        if (need > VMAX) {
+        const int lim = VMAX < p.lds_limit ? VMAX : p.lds_limit;
+        if (need > lim) {
             if (lane == 0) {
-                if (big) {
-                    atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_STAR_CAPACITY);
-                    p.out_vnum[s] = 0;
-                    p.out_tnum[s] = 0;
-                } else {
-                    const unsigned long long k = atomicAdd(&cnt[G4D_CNT_APPEND2], 1ull);
-                    if ((int64_t)k < p.big_cap) p.big_list[k] = s;
-                    else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
-                }
+                const unsigned long long k = atomicAdd(&cnt[p.defer_counter], 1ull);
+                if ((int64_t)k < p.defer_cap) p.defer_list[k] = s;
+                else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
             }
             continue;
         }
@@ -450,14 +469,437 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems, int big)
     }
 }
 
-hipError_t launch_star_kernel(const StarParams &p, bool big, int nblocks, hipStream_t s)
+hipError_t launch_star_kernel(const StarParams &p, int tier, int nblocks, hipStream_t s)
 {
     const int nitems = p.star_list ? p.nlist : p.n;
     if (nitems <= 0) return hipSuccess;
-    if (big)
-        hipLaunchKernelGGL(k_star<STAR_VMAX_BIG>, dim3(nblocks), dim3(64), 0, s, p, nitems, 1);
+    if (p.lds_limit > STAR_LDS_LIMIT) return hipErrorInvalidValue; // local ids must fit 9 bits
+    if (tier == 1)
+        hipLaunchKernelGGL(k_star<STAR_VMAX_BIG>, dim3(nblocks), dim3(64), 0, s, p, nitems);
     else
-        hipLaunchKernelGGL(k_star<STAR_VMAX_SMALL>, dim3(nblocks), dim3(64), 0, s, p, nitems, 0);
+        hipLaunchKernelGGL(k_star<STAR_VMAX_SMALL>, dim3(nblocks), dim3(64), 0, s, p, nitems);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
+// Global-memory star kernel: the same insertion / finalisation as k_star for stars whose
+// link exceeds the LDS kernels' capacity.  One wave per star; the star's working arrays
+// (32-bit local indices, cached cofactors per triangle slot) live in a per-block HBM
+// workspace sized for the largest such star of the pass.
+// ------------------------------------------------------------------------------------
+
+struct GWork {
+    int *vid, *vstart, *vend, *ins, *ins_local, *vremap;
+    double *vc;   // 4 per vertex
+    int *tva;     // 3 local vertices per triangle slot
+    uint32_t *tfl;
+    int *to;      // 3 adjacency slots per triangle slot
+    double *cof;  // 8 per triangle slot: cofactors, permanents
+    int *freel, *be_a, *be_b, *be_u, *be_slot, *be_j, *remap;
+    uint8_t *present;
+};
+
+__host__ __device__ inline int64_t gwork_layout(char *base, int vcap, GWork *w)
+{
+    const int64_t V = vcap, T = 2 * (int64_t)vcap;
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) -> char * {
+        char *r = base ? base + off : nullptr;
+        off += (bytes + 255) & ~(int64_t)255;
+        return r;
+    };
+    GWork t;
+    t.vc = (double *)take(32 * V);
+    t.cof = (double *)take(64 * T);
+    t.vid = (int *)take(4 * V);
+    t.vstart = (int *)take(4 * V);
+    t.vend = (int *)take(4 * V);
+    t.ins = (int *)take(4 * V);
+    t.ins_local = (int *)take(4 * V);
+    t.vremap = (int *)take(4 * V);
+    t.tva = (int *)take(12 * T);
+    t.tfl = (uint32_t *)take(4 * T);
+    t.to = (int *)take(12 * T);
+    t.freel = (int *)take(4 * T);
+    t.be_a = (int *)take(4 * T);
+    t.be_b = (int *)take(4 * T);
+    t.be_u = (int *)take(4 * T);
+    t.be_slot = (int *)take(4 * T);
+    t.be_j = (int *)take(4 * T);
+    t.remap = (int *)take(4 * T);
+    t.present = (uint8_t *)take(V);
+    if (w) *w = t;
+    return off;
+}
+
+int64_t star_global_bytes(int vcap) { return gwork_layout(nullptr, vcap, nullptr); }
+
+__global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *list, int nitems, char *ws,
+                                                    int64_t ws_stride, int vcap)
+{
+    const int lane = threadIdx.x;
+    unsigned long long *cnt = p.cnt;
+    GWork L;
+    gwork_layout(ws + (int64_t)blockIdx.x * ws_stride, vcap, &L);
+    const int TMAX = 2 * vcap;
+
+    for (int item = blockIdx.x; item < nitems; item += gridDim.x) {
+        const int s = list[item];
+        const int64_t ib = p.ins_off[s];
+        const int m = (int)(p.ins_off[s + 1] - ib);
+        const int vn = p.mode ? p.in_vnum[s] : 0;
+        const bool in_wide = vn >= STAR_WIDE_MIN;
+        if (vn + m > vcap || (p.mode == 0 && m < 4)) {
+            if (lane == 0) {
+                atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)(vn + m > vcap ? G4D_DEV_ERR_STAR_CAPACITY
+                                                                                  : G4D_DEV_ERR_CONE));
+                p.out_vnum[s] = 0;
+                p.out_tnum[s] = 0;
+            }
+            continue;
+        }
+        if (p.mode == 1 && m == 0) {
+            // untouched star: copy verbatim
+            const int tn = p.in_tnum[s];
+            const int64_t words = in_wide ? 2 * (int64_t)tn : tn;
+            const int64_t vi = p.in_voff[s], vo = p.out_voff[s], ti = p.in_toff[s], to = p.out_toff[s];
+            for (int i = lane; i < vn; i += 64) p.out_vpool[vo + i] = p.in_vpool[vi + i];
+            for (int64_t i = lane; i < words; i += 64) p.out_tpool[to + i] = p.in_tpool[ti + i];
+            const int64_t hb = p.hist_off[s], he = p.hist_off[s + 1], ho = p.hist_out_off[s];
+            for (int64_t i = lane; i < he - hb; i += 64) p.hist_out[ho + i] = p.hist_keys[hb + i];
+            if (lane == 0) {
+                p.out_vnum[s] = vn;
+                p.out_tnum[s] = tn;
+            }
+            continue;
+        }
+        const double sx = p.pts[3 * (int64_t)s], sy = p.pts[3 * (int64_t)s + 1], sz = p.pts[3 * (int64_t)s + 2];
+        for (int q = lane; q < m; q += 64) {
+            L.ins[q] = (int)(uint32_t)p.ins_keys[ib + q];
+            L.ins_local[q] = -1;
+        }
+        int T, V, nfree = 0, q0;
+        if (p.mode == 1) {
+            const int tn = p.in_tnum[s];
+            const int64_t vi = p.in_voff[s], ti = p.in_toff[s];
+            for (int v = lane; v < vn; v += 64) {
+                const int g = p.in_vpool[vi + v];
+                L.vid[v] = g;
+                rel_coords(p.pts, g, sx, sy, sz, L.vc + 4 * v);
+            }
+            for (int t = lane; t < tn; t += 64) {
+                int a[3], o[3];
+                bool chk;
+                if (in_wide) {
+                    const uint64_t w0 = p.in_tpool[ti + 2 * (int64_t)t], w1 = p.in_tpool[ti + 2 * (int64_t)t + 1];
+                    for (int i = 0; i < 3; i++) {
+                        a[i] = wfield(w0, i);
+                        o[i] = wfield(w1, i);
+                    }
+                    chk = (w0 >> 63) & 1u;
+                } else {
+                    const uint64_t h = p.in_tpool[ti + t];
+                    for (int i = 0; i < 3; i++) {
+                        a[i] = hvert(h, i);
+                        o[i] = hopp(h, i);
+                    }
+                    chk = hchecked(h);
+                }
+                for (int i = 0; i < 3; i++) {
+                    L.tva[3 * t + i] = a[i];
+                    L.to[3 * t + i] = o[i];
+                }
+                L.tfl[t] = TF_ALIVE | (chk ? TF_CHECKED : 0u);
+            }
+            T = tn;
+            V = vn;
+            q0 = 0;
+        } else {
+            // initial 4-simplex (GD/GDelaunay/GDelPredKernels.cu:45-158)
+            int l[4];
+            for (int i = 0; i < 4; i++) l[i] = (int)(uint32_t)p.ins_keys[ib + i];
+            const double *pp[5] = {p.pts + 3 * (int64_t)l[0], p.pts + 3 * (int64_t)l[1], p.pts + 3 * (int64_t)l[2],
+                                   p.pts + 3 * (int64_t)l[3], p.pts + 3 * (int64_t)s};
+            const int idx[5] = {l[0], l[1], l[2], l[3], s};
+            const bool sw = orient4_full(pp, idx, cnt) < 0;
+            if (sw) {
+                const int t = l[0];
+                l[0] = l[1];
+                l[1] = t;
+            }
+            if (lane < 4) {
+                L.vid[lane] = l[lane];
+                rel_coords(p.pts, l[lane], sx, sy, sz, L.vc + 4 * lane);
+                const int lt[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+                for (int i = 0; i < 3; i++) {
+                    L.tva[3 * lane + i] = lt[lane][i];
+                    L.to[3 * lane + i] = lt[lane][i];
+                }
+                L.tfl[lane] = TF_ALIVE;
+                L.ins_local[lane] = (lane < 2 && sw) ? 1 - lane : lane;
+            }
+            T = 4;
+            V = 4;
+            q0 = 4;
+        }
+        __syncthreads();
+        for (int slot = lane; slot < T; slot += 64)
+            tri_cofactors(L.vc + 4 * L.tva[3 * slot], L.vc + 4 * L.tva[3 * slot + 1], L.vc + 4 * L.tva[3 * slot + 2],
+                          L.cof + 8 * slot, L.cof + 8 * slot + 4);
+        bool failed = false;
+        for (int q = q0; q < m; q++) {
+            const int gv = L.ins[q];
+            double r[4];
+            rel_coords(p.pts, gv, sx, sy, sz, r);
+            int nv = 0;
+            for (int slot = lane; slot < T; slot += 64) {
+                const uint32_t f = L.tfl[slot];
+                if (!(f & TF_ALIVE)) continue;
+                int sg = cofactor_filter_sign(r, L.cof + 8 * slot, L.cof + 8 * slot + 4);
+                if (sg == 0) {
+                    const int a = L.vid[L.tva[3 * slot]], b = L.vid[L.tva[3 * slot + 1]], c = L.vid[L.tva[3 * slot + 2]];
+                    const double *pp[5] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b, p.pts + 3 * (int64_t)c,
+                                           p.pts + 3 * (int64_t)s, p.pts + 3 * (int64_t)gv};
+                    const int idx[5] = {a, b, c, s, gv};
+                    sg = orient4_full(pp, idx, cnt);
+                }
+                if (sg < 0) {
+                    L.tfl[slot] = f | TF_VIS;
+                    nv++;
+                }
+            }
+            int nvis;
+            const int vbase = wave_excl(nv, &nvis);
+            if (nvis == 0) continue; // drowned at insertion
+            const int Vn = V;
+            if (lane == 0) {
+                L.vid[Vn] = gv;
+                for (int i = 0; i < 4; i++) L.vc[4 * Vn + i] = r[i];
+                L.ins_local[q] = Vn;
+            }
+            __syncthreads();
+            // hole boundary: edges of freed triangles whose neighbour survives
+            int nb = 0;
+            for (int slot = lane; slot < T; slot += 64)
+                if (L.tfl[slot] & TF_VIS)
+                    for (int i = 0; i < 3; i++) nb += (L.tfl[L.to[3 * slot + i]] & TF_VIS) ? 0 : 1;
+            int nbtot;
+            int e = wave_excl(nb, &nbtot);
+            if (nbtot > TMAX || nbtot < 3) {
+                failed = true;
+                break;
+            }
+            int fpos = nfree + vbase;
+            for (int slot = lane; slot < T; slot += 64) {
+                if (!(L.tfl[slot] & TF_VIS)) continue;
+                for (int i = 0; i < 3; i++) {
+                    const int u = L.to[3 * slot + i];
+                    if (L.tfl[u] & TF_VIS) continue;
+                    int j = 0;
+                    while (j < 2 && L.to[3 * u + j] != slot) j++;
+                    L.be_a[e] = L.tva[3 * slot + (i + 1) % 3];
+                    L.be_b[e] = L.tva[3 * slot + (i + 2) % 3];
+                    L.be_u[e] = u;
+                    L.be_j[e] = j;
+                    e++;
+                }
+                L.freel[fpos++] = slot;
+            }
+            __syncthreads();
+            const int nfree_tot = nfree + nvis;
+            for (int slot = lane; slot < T; slot += 64)
+                if (L.tfl[slot] & TF_VIS) L.tfl[slot] = 0u;
+            for (int k = lane; k < nbtot; k += 64)
+                L.be_slot[k] = k < nfree_tot ? L.freel[nfree_tot - 1 - k] : T + (k - nfree_tot);
+            const int newT = T + (nbtot > nfree_tot ? nbtot - nfree_tot : 0);
+            if (newT > TMAX || Vn + 1 > vcap) {
+                failed = true;
+                break;
+            }
+            __syncthreads();
+            for (int k = lane; k < nbtot; k += 64) {
+                const int slot = L.be_slot[k], a = L.be_a[k], b = L.be_b[k], u = L.be_u[k];
+                L.tva[3 * slot] = a;
+                L.tva[3 * slot + 1] = b;
+                L.tva[3 * slot + 2] = Vn;
+                L.tfl[slot] = TF_ALIVE | TF_FRESH;
+                L.to[3 * slot + 2] = u;
+                L.to[3 * u + L.be_j[k]] = slot;
+                L.vstart[a] = slot;
+                L.vend[b] = slot;
+            }
+            __syncthreads();
+            for (int k = lane; k < nbtot; k += 64) {
+                const int slot = L.be_slot[k];
+                L.to[3 * slot] = L.vstart[L.be_b[k]];
+                L.to[3 * slot + 1] = L.vend[L.be_a[k]];
+            }
+            nfree = nfree_tot - (nbtot < nfree_tot ? nbtot : nfree_tot);
+            T = newT;
+            V = Vn + 1;
+            __syncthreads();
+            for (int slot = lane; slot < T; slot += 64) {
+                const uint32_t f = L.tfl[slot];
+                if (f & TF_FRESH) {
+                    tri_cofactors(L.vc + 4 * L.tva[3 * slot], L.vc + 4 * L.tva[3 * slot + 1],
+                                  L.vc + 4 * L.tva[3 * slot + 2], L.cof + 8 * slot, L.cof + 8 * slot + 4);
+                    L.tfl[slot] = f & ~TF_FRESH;
+                }
+            }
+            __syncthreads();
+        }
+        if (failed) {
+            if (lane == 0) {
+                atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
+                p.out_vnum[s] = 0;
+                p.out_tnum[s] = 0;
+            }
+            __syncthreads();
+            continue;
+        }
+        // ---- finalize: link set, drowned insertions, compaction, pairs, history ----
+        for (int v = lane; v < V; v += 64) L.present[v] = 0;
+        __syncthreads();
+        for (int slot = lane; slot < T; slot += 64)
+            if (L.tfl[slot] & TF_ALIVE)
+                for (int i = 0; i < 3; i++) L.present[L.tva[3 * slot + i]] = 1;
+        __syncthreads();
+        for (int base = 0; base < m; base += 64) {
+            const int q = base + lane;
+            bool dr = false;
+            if (q < m) {
+                const int lv = L.ins_local[q];
+                dr = lv < 0 || !L.present[lv];
+            }
+            const long long slot = g4d_wave_append(dr, &cnt[G4D_CNT_APPEND0]);
+            if (dr) {
+                if (slot < p.drown_cap) p.drown[slot] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)L.ins[q];
+                else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
+            }
+        }
+        const int64_t vo = p.out_voff[s], to = p.out_toff[s];
+        int vcount = 0;
+        for (int base = 0; base < V; base += 64) {
+            const int v = base + lane;
+            const bool pr = v < V && L.present[v];
+            int tot;
+            const int rk = g4d_wave_excl_count(pr, &tot);
+            if (pr) {
+                L.vremap[v] = vcount + rk;
+                p.out_vpool[vo + vcount + rk] = L.vid[v];
+            }
+            vcount += tot;
+        }
+        int tcount = 0;
+        for (int base = 0; base < T; base += 64) {
+            const int t = base + lane;
+            const bool al = t < T && (L.tfl[t] & TF_ALIVE);
+            int tot;
+            const int rk = g4d_wave_excl_count(al, &tot);
+            if (al) L.remap[t] = tcount + rk;
+            tcount += tot;
+        }
+        const bool wide = vcount >= STAR_WIDE_MIN;
+        const int64_t words = wide ? 2 * (int64_t)tcount : tcount;
+        if (words > p.out_toff[s + 1] - to || tcount != 2 * vcount - 4) {
+            if (lane == 0) {
+                atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
+                p.out_vnum[s] = 0;
+                p.out_tnum[s] = 0;
+            }
+            __syncthreads();
+            continue;
+        }
+        __syncthreads();
+        for (int base = 0; base < T; base += 64) {
+            const int t = base + lane;
+            const bool al = t < T && (L.tfl[t] & TF_ALIVE);
+            int a = 0, b = 0, c = 0;
+            uint32_t f = 0;
+            if (al) {
+                f = L.tfl[t];
+                a = L.tva[3 * t];
+                b = L.tva[3 * t + 1];
+                c = L.tva[3 * t + 2];
+                const int ra = L.vremap[a], rb = L.vremap[b], rc = L.vremap[c];
+                const int o0 = L.remap[L.to[3 * t]], o1 = L.remap[L.to[3 * t + 1]], o2 = L.remap[L.to[3 * t + 2]];
+                const int64_t d = L.remap[t];
+                if (wide) {
+                    p.out_tpool[to + 2 * d] = wpack3(ra, rb, rc) | (1ull << 63);
+                    p.out_tpool[to + 2 * d + 1] = wpack3(o0, o1, o2);
+                } else {
+                    p.out_tpool[to + d] = hpack(ra, rb, rc, o0, o1, o2, true);
+                }
+            }
+            const bool unchecked = al && !(f & TF_CHECKED);
+            const int tv3[3] = {a, b, c};
+            for (int i = 0; i < 3; i++) {
+                int ga = 0, gb = 0;
+                if (unchecked) {
+                    ga = L.vid[tv3[i]];
+                    gb = L.vid[tv3[(i + 1) % 3]];
+                }
+                const bool em = unchecked && ga < gb;
+                const long long slot = g4d_wave_append(em, &cnt[G4D_CNT_APPEND1]);
+                if (em) {
+                    if (slot < p.tpairs_cap) p.tpairs[slot] = ((uint64_t)(uint32_t)ga << 32) | (uint32_t)gb;
+                    else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
+                }
+            }
+        }
+        if (lane == 0) {
+            p.out_vnum[s] = vcount;
+            p.out_tnum[s] = tcount;
+        }
+        if (p.mode == 1) {
+            const int64_t hb = p.hist_off[s], he = p.hist_off[s + 1], ho = p.hist_out_off[s];
+            const int hn = (int)(he - hb);
+            for (int i = lane; i < hn; i += 64) {
+                const int x = (int)(uint32_t)p.hist_keys[hb + i];
+                int lo = 0, hi = m;
+                while (lo < hi) {
+                    const int mid = (lo + hi) >> 1;
+                    if (L.ins[mid] < x) lo = mid + 1;
+                    else hi = mid;
+                }
+                p.hist_out[ho + i + lo] = p.hist_keys[hb + i];
+            }
+            for (int i = lane; i < m; i += 64) {
+                const int x = L.ins[i];
+                int64_t lo = hb, hi = he;
+                while (lo < hi) {
+                    const int64_t mid = (lo + hi) >> 1;
+                    if ((int)(uint32_t)p.hist_keys[mid] < x) lo = mid + 1;
+                    else hi = mid;
+                }
+                p.hist_out[ho + i + (lo - hb)] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)x;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_star_global(const StarParams &p, const int *list, int nitems, char *ws, int64_t ws_stride, int vcap,
+                              int nblocks, hipStream_t s)
+{
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_star_global, dim3(nblocks), dim3(64), 0, s, p, list, nitems, ws, ws_stride, vcap);
+    return hipGetLastError();
+}
+
+__global__ void k_need_max(StarParams p, const int *list, int nitems, unsigned long long *out)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += gridDim.x * blockDim.x) {
+        const int s = list[i];
+        const int64_t need = (p.ins_off[s + 1] - p.ins_off[s]) + (p.mode ? p.in_vnum[s] : 0);
+        atomicMax(out, (unsigned long long)need);
+    }
+}
+
+hipError_t launch_need_max(const StarParams &p, const int *list, int nitems, unsigned long long *out, hipStream_t s)
+{
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_need_max, dim3(g4d_blocks(nitems, 256, 1024)), dim3(256), 0, s, p, list, nitems, out);
     return hipGetLastError();
 }
 
@@ -474,14 +916,16 @@ __global__ void __launch_bounds__(64) k_proofs(ProofParams p)
         const uint64_t tk = p.tasks[task];
         const int s = (int)(tk >> 32), v = (int)(uint32_t)tk;
         const int vn = p.vnum[s], tn = p.tnum[s];
+        const bool wide = vn >= STAR_WIDE_MIN;
         const int64_t vo = p.voff[s], to = p.toff[s];
         int mn = 0x7fffffff;
         for (int i = lane; i < vn; i += 64) mn = min(mn, p.vpool[vo + i]);
         const int ex = g4d_wave_min(mn);
         int k0 = 0x7fffffff, k1 = 0x7fffffff, k2 = 0x7fffffff;
         for (int t = lane; t < tn; t += 64) {
-            const uint64_t h = p.tpool[to + t];
-            const int a = p.vpool[vo + hvert(h, 0)], b = p.vpool[vo + hvert(h, 1)], c = p.vpool[vo + hvert(h, 2)];
+            int lv[3];
+            star_tri_verts(p.tpool, to, t, wide, lv);
+            const int a = p.vpool[vo + lv[0]], b = p.vpool[vo + lv[1]], c = p.vpool[vo + lv[2]];
             if (a == ex || b == ex || c == ex) continue;
             const int tri[3] = {a, b, c};
             int o[3];
@@ -630,7 +1074,8 @@ __global__ void k_capacities(int n, const int *vnum, const int64_t *ins_off, con
         const int64_t m = ins_off[s + 1] - ins_off[s];
         const int64_t v = (vnum ? vnum[s] : 0) + m;
         vcap[s] = v;
-        tcap[s] = v >= 4 ? 2 * v - 4 : 4;
+        const int64_t t = v >= 4 ? 2 * v - 4 : 4;
+        tcap[s] = v >= STAR_WIDE_MIN ? 2 * t : t;
         if (hcap) hcap[s] = (hist_off[s + 1] - hist_off[s]) + m;
     }
 }
@@ -653,12 +1098,14 @@ __global__ void k_out(OutParams p)
 {
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < p.n; s += gridDim.x * blockDim.x) {
         const int tn = p.tnum[s];
+        const bool wide = p.vnum[s] >= STAR_WIDE_MIN;
         const int64_t vo = p.voff[s], to = p.toff[s];
         int64_t k = WRITE ? p.star_tet_off[s] : 0;
         int64_t c = 0;
         for (int t = 0; t < tn; t++) {
-            const uint64_t h = p.tpool[to + t];
-            const int a = p.vpool[vo + hvert(h, 0)], b = p.vpool[vo + hvert(h, 1)], cc = p.vpool[vo + hvert(h, 2)];
+            int lv[3];
+            star_tri_verts(p.tpool, to, t, wide, lv);
+            const int a = p.vpool[vo + lv[0]], b = p.vpool[vo + lv[1]], cc = p.vpool[vo + lv[2]];
             if (!(s < a && s < b && s < cc)) continue;
             const double *pp[4] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b, p.pts + 3 * (int64_t)cc,
                                    p.pts + 3 * (int64_t)s};

@@ -55,6 +55,9 @@ typedef struct {
     int32_t index_space;        /* g4d_index_space */
     int32_t device;             /* HIP device ordinal */
     int32_t compute_adjacency;  /* 1: fill opp */
+    int32_t lds_star_limit;     /* 0: default.  Stars whose link plus insertions exceed this
+                                   (at most 511) run in the global-memory star kernel; lower
+                                   values route more stars there (testing). */
 } g4d_options;
 
 typedef struct {
@@ -71,8 +74,8 @@ typedef struct {
     int32_t grid_size;
     int32_t splay_rounds;      /* LogLoopNum in the reference */
     int32_t max_workset;
-    int32_t big_stars;         /* stars routed to the large-star kernel */
-    int32_t pad0;
+    int32_t big_stars;         /* star passes routed to the large-star LDS kernel */
+    int32_t huge_stars;        /* star passes routed to the global-memory star kernel */
     int64_t num_missing;
     int64_t num_grid_pairs;    /* sort-unique Voronoi-neighbour pairs (one direction) */
     int64_t num_workset;       /* workset entries, both directions, after pumping */

@@ -166,6 +166,27 @@ def test_min_workset_extremes_same_output(orc):
     assert (g.canonical(a) == g.canonical(b)).all()
 
 
+@pytest.mark.parametrize("limit", [12, 4])
+def test_global_memory_star_path_forced(orc, limit):
+    """Stars above lds_star_limit run in the global-memory star kernel (packed output)."""
+    x = datasets.uniform_cube(3000, seed=21)
+    _, _, _, st, rst = _parity(orc, x, 32, lds_star_limit=limit)
+    assert st["huge_stars"] > 0
+    assert st["splay_rounds"] == rst["splay_rounds"]
+
+
+def test_star_beyond_lds_capacity(orc):
+    """A point at the centre of 1500 points on a sphere: its link holds every sphere point,
+    so its star exceeds the LDS kernels and uses the wide triangle format."""
+    rng = np.random.default_rng(5)
+    v = rng.normal(size=(1500, 3))
+    v /= np.linalg.norm(v, axis=1)[:, None]
+    x = np.concatenate([0.5 + 0.4 * v, [[0.5, 0.5, 0.5]]])
+    tets, _, _, st, _ = _parity(orc, x, 32)
+    assert st["huge_stars"] > 0
+    assert np.bincount(tets.ravel()).max() == tets.shape[0]  # every tetrahedron uses the centre
+
+
 def test_morton_index_space_and_reuse(orc):
     x = datasets.uniform_cube(5000, seed=5)
     with g.Triangulator(grid_size=32, index_space="morton") as t:
