@@ -52,6 +52,7 @@ class Stats(ctypes.Structure):
                 ("num_workset", ctypes.c_int64), ("num_pumped", ctypes.c_int64),
                 ("splay_insertions", ctypes.c_int64), ("num_tets", ctypes.c_int64),
                 ("exact_calls", ctypes.c_int64), ("sos_calls", ctypes.c_int64),
+                ("exact_stars", ctypes.c_int64),
                 ("pba_bytes", ctypes.c_int64), ("t_pba_pass_ms", ctypes.c_double * 3)]
 
     def as_dict(self):
@@ -72,7 +73,7 @@ _lp = ctypes.POINTER(ctypes.c_int64)
 EXPORTS = ["g4d_default_options", "g4d_create", "g4d_destroy", "g4d_last_error", "g4d_device_count",
            "g4d_triangulate", "g4d_triangulate_device", "g4d_copy_result", "g4d_upload_points",
            "g4d_get_stats", "g4d_set_options", "g4d_diag_orient4", "g4d_diag_orient3d",
-           "g4d_diag_morton", "g4d_diag_pba", "g4d_diag_grid_pairs"]
+           "g4d_diag_morton", "g4d_diag_pba", "g4d_diag_grid_pairs", "g4d_diag_copy"]
 
 
 def load_library(path=None):
@@ -101,6 +102,7 @@ def load_library(path=None):
     L.g4d_diag_morton.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.c_int32, _ip, _ip]
     L.g4d_diag_pba.argtypes = [_vp, _ip, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _ip]
     L.g4d_diag_grid_pairs.argtypes = [_vp, _ip, ctypes.c_int32, _up, ctypes.c_int64, _lp]
+    L.g4d_diag_copy.argtypes = [_vp, ctypes.c_int64, _dp]
     _LIB = L
     return L
 
@@ -238,6 +240,12 @@ class Triangulator:
         self._check(self._L.g4d_diag_grid_pairs(self._ctx, _ptr(grid, _ip), g, _ptr(out, _up), out.shape[0],
                                                 ctypes.byref(cnt)))
         return out[:cnt.value]
+
+    def diag_copy(self, nbytes):
+        """Plain device copy of nbytes; returns the kernel time in ms."""
+        ms = ctypes.c_double(0.0)
+        self._check(self._L.g4d_diag_copy(self._ctx, int(nbytes), ctypes.byref(ms)))
+        return ms.value
 
 
 def canonical(tets, perm=None):

@@ -52,11 +52,11 @@ struct g4d_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     g4d::Prims P;
-    DevBuf xyz_up, keys, pts, perm, vox, bounds, grid, gtmp, idgrid, pairs, ws_off, cnt;
+    DevBuf xyz_up, keys, pts, perm, vox, bounds, grid, gtmp, idgrid, pbaspill, pcount, pairs, ws_off, cnt;
     Pool pool[2];
     Hist hist[2];
-    DevBuf drown, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, big, huge, gwork, tetcnt, tetoff, tets, opp,
-        table;
+    DevBuf drown, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, big, huge, xstars, xtasks, xtris, gwork,
+        tetcnt, tetoff, tets, opp, table;
     std::vector<int32_t> h_tets, h_opp, h_perm;
     int64_t num_tets = 0;
     int last_n = 0;
@@ -239,38 +239,46 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
     HCK(g4d::launch_fill_i32(grid, G3, G4D_MARKER, s));
     HCK(g4d::launch_scatter_sites(vox, n, G, grid, idgrid, s));
     ev_record(ctx, EV_PBA0);
-    HCK(g4d::pba_pass(2, G, grid, gtmp, nullptr, s));
+    HCK(ctx->pbaspill.ensure(sizeof(int) * (size_t)g4d::pba_spill_ints(G)));
+    int *spill = buf<int>(ctx->pbaspill);
+    HCK(g4d::pba_pass(2, G, grid, gtmp, nullptr, spill, s));
     ev_record(ctx, EV_PBA1);
-    HCK(g4d::pba_pass(1, G, gtmp, grid, nullptr, s));
+    HCK(g4d::pba_pass(1, G, gtmp, grid, nullptr, spill, s));
     ev_record(ctx, EV_PBA2);
-    HCK(g4d::pba_pass(0, G, grid, gtmp, idgrid, s)); // point ids in gtmp
+    HCK(g4d::pba_pass(0, G, grid, gtmp, idgrid, spill, s)); // point ids in gtmp
     ev_record(ctx, EV_PBA3);
     const int *idg = gtmp;
 
     // ---- D: missing points, grid pairs, worksets, pumping (GD/GDelaunay/GDelHost.cu:259-529) ----
-    int64_t cap = std::max<int64_t>(48 * (int64_t)n, 1 << 20);
-    int64_t m = 0, nmiss = 0;
-    for (int attempt = 0; attempt < 3; attempt++) {
-        HCK(ctx->pairs.ensure(sizeof(uint64_t) * (size_t)(cap + 2 * (int64_t)n * min_ws + 64)));
-        uint64_t *pr = buf<uint64_t>(ctx->pairs);
-        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8, s));
-        HCK(g4d::launch_missing(vox, n, G, idg, pr, cnt + G4D_CNT_APPEND0, cap, s));
-        CK(read_counters(ctx, hc));
-        nmiss = (int64_t)hc[G4D_CNT_APPEND0];
-        HCK(g4d::launch_grid_pairs(idg, G, pr, cnt + G4D_CNT_APPEND0, cap, s));
-        CK(read_counters(ctx, hc));
-        m = 2 * (int64_t)hc[G4D_CNT_APPEND0];
-        if (m <= cap) break;
-        cap = m + m / 4;
-        m = -1;
-    }
-    if (m < 0) return set_err(ctx, G4D_ERR_INTERNAL, "pair buffer sizing failed");
-    CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], "morton"));
+    // Pairs are extracted once per unordered pair (min<<32 | max), deduplicated, then
+    // reciprocated into directed workset keys (star<<32 | vertex) and grouped by star.
+    const int zc = G > 512 ? 32 : 16;
+    const int64_t nthr = g4d::grid_pair_threads(G, zc);
+    HCK(ctx->pcount.ensure(sizeof(int64_t) * (size_t)(nthr + 1)));
+    int64_t *pcnt = buf<int64_t>(ctx->pcount);
+    HCK(g4d::launch_grid_pairs(idg, G, zc, pcnt, nullptr, false, s));
+    HCK(g4d::scan_excl_i64(P, pcnt, pcnt, nthr, pcnt + nthr, s));
+    int64_t ngp = 0;
+    HCK(hipMemcpyAsync(&ngp, pcnt + nthr, 8, hipMemcpyDeviceToHost, s));
+    HCK(hipStreamSynchronize(s));
+    const int64_t ucap = ngp + n;
+    HCK(ctx->pairs.ensure(sizeof(uint64_t) * (size_t)(2 * ucap + 2 * (int64_t)n * min_ws + 64)));
     uint64_t *pr = buf<uint64_t>(ctx->pairs);
-    HCK(sort_pairs(P, pr, m, nb, s));
-    HCK(g4d::unique_u64(P, pr, m, &m, s));
+    HCK(g4d::launch_grid_pairs(idg, G, zc, pcnt, pr, true, s));
+    HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8, s));
+    HCK(g4d::launch_missing(vox, n, G, idg, pr + ngp, cnt + G4D_CNT_APPEND0, n, s));
+    CK(read_counters(ctx, hc));
+    CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], "morton"));
+    const int64_t nmiss = (int64_t)hc[G4D_CNT_APPEND0];
+    trace("n %d grid %d: grid pairs %lld, missing %lld", n, G, (long long)ngp, (long long)nmiss);
+    int64_t u = ngp + nmiss;
+    HCK(sort_pairs(P, pr, u, nb, s));
+    HCK(g4d::unique_u64(P, pr, u, &u, s));
     ctx->st.num_missing = nmiss;
-    ctx->st.num_grid_pairs = (m - 2 * nmiss) / 2;
+    ctx->st.num_grid_pairs = u - nmiss;
+    HCK(g4d::launch_reciprocate(pr, u, s));
+    int64_t m = 2 * u;
+    HCK(sort_pairs(P, pr, m, nb, s));
     HCK(ctx->ws_off.ensure(sizeof(int64_t) * (size_t)(n + 1)));
     int64_t *ws_off = buf<int64_t>(ctx->ws_off);
     HCK(g4d::launch_csr_offsets(pr, m, n, ws_off, s));
@@ -287,8 +295,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
         HCK(g4d::launch_csr_offsets(pr, m, n, ws_off, s));
     }
     ctx->st.num_workset = m;
-    trace("n %d grid %d: missing %lld, grid pairs %lld, worksets %lld keys (%lld pumped)", n, G, (long long)nmiss,
-          (long long)ctx->st.num_grid_pairs, (long long)m, (long long)ctx->st.num_pumped);
+    trace("worksets %lld keys (%lld pumped)", (long long)m, (long long)ctx->st.num_pumped);
     ev_record(ctx, EV_PAIRS);
 
     // ---- E: star creation (GD/GDelaunay/GDelHost.cu:543-743) ----
@@ -297,6 +304,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
     HCK(ctx->hcap.ensure(sizeof(int64_t) * (size_t)(n + 1)));
     HCK(ctx->big.ensure(sizeof(int) * (size_t)n));
     HCK(ctx->huge.ensure(sizeof(int) * (size_t)n));
+    HCK(ctx->xstars.ensure(sizeof(int) * (size_t)n));
     int64_t *vcap = buf<int64_t>(ctx->vcap), *tcap = buf<int64_t>(ctx->tcap), *hcap = buf<int64_t>(ctx->hcap);
     const int nblk_star = 4096;
     const int lds_limit = std::min(g4d::STAR_LDS_LIMIT, ctx->opt.lds_star_limit > 0 ? ctx->opt.lds_star_limit
@@ -365,42 +373,56 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
         sp.defer_list = buf<int>(ctx->big);
         sp.defer_cap = n;
         sp.defer_counter = G4D_CNT_APPEND2;
+        sp.exact_list = buf<int>(ctx->xstars);
+        sp.exact_cap = n;
+        sp.exact_counter = G4D_CNT_APPEND4;
         HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8 * 3, s));
-        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND3, 0, 8, s));
+        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND3, 0, 8 * 2, s));
         HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
-        HCK(g4d::launch_star_kernel(sp, 0, nblk_star, s));
+        // filter-only LDS kernels; stars they cannot decide go to the exact-capable kernels
+        HCK(g4d::launch_star_kernel(sp, 0, false, nblk_star, s));
         CK(read_counters(ctx, hc));
         const int64_t nbig = (int64_t)hc[G4D_CNT_APPEND2];
-        int64_t nhuge = 0;
         if (nbig > 0) {
-            // stars beyond the small kernel's capacity: the big LDS kernel, then the global one
             ctx->st.big_stars += (int32_t)nbig;
             sp.star_list = buf<int>(ctx->big);
             sp.nlist = (int)nbig;
             sp.defer_list = buf<int>(ctx->huge);
             sp.defer_counter = G4D_CNT_APPEND3;
             HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
-            HCK(g4d::launch_star_kernel(sp, 1, (int)std::min<int64_t>(nbig, 1024), s));
+            HCK(g4d::launch_star_kernel(sp, 1, false, 1024, s));
             CK(read_counters(ctx, hc));
-            nhuge = (int64_t)hc[G4D_CNT_APPEND3];
         }
+        const int64_t nexact = (int64_t)hc[G4D_CNT_APPEND4];
+        if (nexact > 0) {
+            ctx->st.exact_stars += nexact;
+            sp.star_list = buf<int>(ctx->xstars);
+            sp.nlist = (int)nexact;
+            sp.defer_list = buf<int>(ctx->huge);
+            sp.defer_counter = G4D_CNT_APPEND3;
+            HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
+            HCK(g4d::launch_star_kernel(sp, 1, true, 1024, s));
+            CK(read_counters(ctx, hc));
+        }
+        const int64_t nhuge = (int64_t)hc[G4D_CNT_APPEND3];
         if (nhuge > 0) {
             ctx->st.huge_stars += (int32_t)nhuge;
             HCK(hipMemsetAsync(cnt + G4D_CNT_MISC, 0, 8, s));
             HCK(g4d::launch_need_max(sp, buf<int>(ctx->huge), (int)nhuge, cnt + G4D_CNT_MISC, s));
             CK(read_counters(ctx, hc));
             const int vcap_g = (int)hc[G4D_CNT_MISC];
-            const int nblk = (int)std::min<int64_t>(nhuge, 64);
             const int64_t stride = (g4d::star_global_bytes(vcap_g) + 4095) & ~(int64_t)4095;
+            // bounded workspace: at most 256 blocks and 1 GiB
+            const int nblk = (int)std::max<int64_t>(1, std::min<int64_t>({nhuge, 256, ((int64_t)1 << 30) / stride}));
             HCK(ctx->gwork.ensure((size_t)(stride * nblk)));
             HCK(g4d::launch_star_global(sp, buf<int>(ctx->huge), (int)nhuge, buf<char>(ctx->gwork), stride, vcap_g,
                                         nblk, s));
             CK(read_counters(ctx, hc));
         }
-        trace("star pass mode %d: %lld insertions, large stars %lld (global-memory %lld), drowned %llu, pairs %llu, "
-              "err 0x%llx",
-              mode, (long long)nins, (long long)nbig, (long long)nhuge, hc[G4D_CNT_APPEND0], hc[G4D_CNT_APPEND1],
-              hc[G4D_CNT_ERROR]);
+        trace("star pass mode %d: %lld insertions, large stars %lld, exact %lld, global-memory %lld, drowned %llu, "
+              "pairs %llu, err 0x%llx",
+              mode, (long long)nins, (long long)nbig, (long long)nexact, (long long)nhuge, hc[G4D_CNT_APPEND0],
+              hc[G4D_CNT_APPEND1], hc[G4D_CNT_ERROR]);
         CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], mode ? "splaying" : "star creation"));
         *drown_n = (int64_t)hc[G4D_CNT_APPEND0];
         *tp_n = (int64_t)hc[G4D_CNT_APPEND1];
@@ -445,7 +467,15 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
                 pp.tpool = buf<uint64_t>(pl.tpool);
                 pp.out = buf<uint64_t>(ctx->cand) + ntp;
                 pp.cnt = cnt;
-                HCK(g4d::launch_proofs(pp, s));
+                HCK(ctx->xtasks.ensure(sizeof(int64_t) * (size_t)ntask));
+                pp.xlist = buf<int64_t>(ctx->xtasks);
+                pp.xcap = ntask;
+                pp.nx = 0;
+                HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND5, 0, 8, s));
+                HCK(g4d::launch_proofs(pp, false, s));
+                CK(read_counters(ctx, hc));
+                pp.nx = (int64_t)hc[G4D_CNT_APPEND5];
+                HCK(g4d::launch_proofs(pp, true, s));
             }
         }
         // F4-F5: candidates = triangle pairs + proofs, history filter, sort-unique
@@ -508,6 +538,19 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
     op.star_tet_off = buf<int64_t>(ctx->tetoff);
     op.cnt = cnt;
     op.tets = nullptr;
+    // owner triangles the orient3d filter cannot decide: exact pass marks the ones to keep
+    int64_t xcap = std::max<int64_t>(n / 16, 4096);
+    for (int attempt = 0; attempt < 2; attempt++) {
+        HCK(ctx->xtris.ensure(sizeof(uint64_t) * (size_t)xcap));
+        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND5, 0, 8, s));
+        HCK(g4d::launch_out_flag(op, buf<uint64_t>(ctx->xtris), xcap, s));
+        CK(read_counters(ctx, hc));
+        if ((int64_t)hc[G4D_CNT_APPEND5] <= xcap) break;
+        xcap = (int64_t)hc[G4D_CNT_APPEND5];
+    }
+    const int64_t nxt = (int64_t)hc[G4D_CNT_APPEND5];
+    if (nxt > xcap) return set_err(ctx, G4D_ERR_INTERNAL, "output exact list sizing failed");
+    HCK(g4d::launch_out_exact(op, buf<uint64_t>(ctx->xtris), nxt, s));
     HCK(g4d::launch_out_count(op, s));
     int64_t *toffs = buf<int64_t>(ctx->tetoff);
     HCK(g4d::scan_excl_i64(P, buf<int64_t>(ctx->tetcnt), toffs, n, toffs + n, s));
@@ -610,9 +653,9 @@ void g4d_destroy(g4d_ctx *ctx)
     (void)hipSetDevice(ctx->opt.device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->xyz_up, &ctx->keys, &ctx->pts, &ctx->perm, &ctx->vox, &ctx->bounds, &ctx->grid,
-                      &ctx->gtmp, &ctx->idgrid, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->drown, &ctx->dkeys,
+                      &ctx->gtmp, &ctx->idgrid, &ctx->pbaspill, &ctx->pcount, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->drown, &ctx->dkeys,
                       &ctx->tasks, &ctx->cand, &ctx->keep, &ctx->ins, &ctx->ins_off, &ctx->vcap, &ctx->tcap,
-                      &ctx->hcap, &ctx->big, &ctx->huge, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table};
+                      &ctx->hcap, &ctx->big, &ctx->huge, &ctx->xstars, &ctx->xtasks, &ctx->xtris, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table};
     for (DevBuf *b : bufs) b->release();
     for (auto &p : ctx->pool) p.release();
     for (auto &h : ctx->hist) h.release();
@@ -797,9 +840,30 @@ int g4d_diag_pba(g4d_ctx *ctx, const int32_t *vox_sorted, int32_t n, int32_t gri
     HCK(hipMemcpyAsync(ctx->vox.ptr, enc.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s));
     HCK(g4d::launch_fill_i32(buf<int>(ctx->grid), G3, G4D_MARKER, s));
     HCK(g4d::launch_scatter_sites(buf<int>(ctx->vox), n, grid, buf<int>(ctx->grid), buf<int>(ctx->idgrid), s));
-    HCK(g4d::pba_run(grid, buf<int>(ctx->grid), buf<int>(ctx->gtmp), codes ? nullptr : buf<int>(ctx->idgrid), s));
+    HCK(ctx->pbaspill.ensure(sizeof(int) * (size_t)g4d::pba_spill_ints(grid)));
+    HCK(g4d::pba_run(grid, buf<int>(ctx->grid), buf<int>(ctx->gtmp), codes ? nullptr : buf<int>(ctx->idgrid),
+                     buf<int>(ctx->pbaspill), s));
     HCK(hipMemcpyAsync(out, ctx->gtmp.ptr, sizeof(int) * (size_t)G3, hipMemcpyDeviceToHost, s));
     HCK(hipStreamSynchronize(s));
+    return G4D_OK;
+}
+
+int g4d_diag_copy(g4d_ctx *ctx, int64_t bytes, double *ms)
+{
+    if (!ctx || bytes < 16) return G4D_ERR_INVALID_INPUT;
+    HCK(hipSetDevice(ctx->opt.device));
+    const int64_t n16 = bytes / 16;
+    DevBuf a, b;
+    HCK(a.ensure((size_t)n16 * 16));
+    HCK(b.ensure((size_t)n16 * 16));
+    HCK(hipMemsetAsync(a.ptr, 1, (size_t)n16 * 16, ctx->stream));
+    if (ctx->ev_ok) HCK(hipEventRecord(ctx->ev[EV_START], ctx->stream));
+    HCK(g4d::launch_diag_copy(buf<int4>(a), buf<int4>(b), n16, ctx->stream));
+    if (ctx->ev_ok) HCK(hipEventRecord(ctx->ev[EV_MORTON], ctx->stream));
+    HCK(hipStreamSynchronize(ctx->stream));
+    if (ms) *ms = ev_ms(ctx, EV_START, EV_MORTON);
+    a.release();
+    b.release();
     return G4D_OK;
 }
 
@@ -813,24 +877,21 @@ int g4d_diag_grid_pairs(g4d_ctx *ctx, const int32_t *id_grid, int32_t grid, uint
     HCK(hipMemcpyAsync(ctx->gtmp.ptr, id_grid, sizeof(int) * (size_t)G3, hipMemcpyHostToDevice, s));
     HCK(ctx->cnt.ensure(sizeof(unsigned long long) * G4D_CNT_NUM));
     unsigned long long *cnt = buf<unsigned long long>(ctx->cnt);
-    int64_t pcap = std::max<int64_t>(64 * G3 / 8, 1 << 16);
+    const int zc = 16;
+    const int64_t nthr = g4d::grid_pair_threads(grid, zc);
+    HCK(ctx->pcount.ensure(sizeof(int64_t) * (size_t)(nthr + 1)));
+    int64_t *pcnt = buf<int64_t>(ctx->pcount);
+    HCK(g4d::launch_grid_pairs(buf<int>(ctx->gtmp), grid, zc, pcnt, nullptr, false, s));
+    HCK(g4d::scan_excl_i64(ctx->P, pcnt, pcnt, nthr, pcnt + nthr, s));
     int64_t m = 0;
-    unsigned long long hc[G4D_CNT_NUM];
-    for (int attempt = 0; attempt < 3; attempt++) {
-        HCK(ctx->pairs.ensure(sizeof(uint64_t) * (size_t)pcap));
-        HCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * G4D_CNT_NUM, s));
-        HCK(g4d::launch_grid_pairs(buf<int>(ctx->gtmp), grid, buf<uint64_t>(ctx->pairs), cnt + G4D_CNT_APPEND0, pcap, s));
-        CK(read_counters(ctx, hc));
-        m = 2 * (int64_t)hc[G4D_CNT_APPEND0];
-        if (m <= pcap) break;
-        pcap = m + 1024;
-        m = -1;
-    }
-    if (m < 0) return set_err(ctx, G4D_ERR_INTERNAL, "pair buffer sizing failed");
+    HCK(hipMemcpyAsync(&m, pcnt + nthr, 8, hipMemcpyDeviceToHost, s));
+    HCK(hipStreamSynchronize(s));
+    HCK(ctx->pairs.ensure(sizeof(uint64_t) * (size_t)std::max<int64_t>(m, 1)));
+    uint64_t *pr = buf<uint64_t>(ctx->pairs);
+    HCK(g4d::launch_grid_pairs(buf<int>(ctx->gtmp), grid, zc, pcnt, pr, true, s));
     int maxid = 0;
     for (int64_t i = 0; i < G3; i++) maxid = std::max(maxid, id_grid[i]);
     const int nb = bits_for((int64_t)maxid + 1);
-    uint64_t *pr = buf<uint64_t>(ctx->pairs);
     HCK(sort_pairs(ctx->P, pr, m, nb, s));
     HCK(g4d::unique_u64(ctx->P, pr, m, &m, s));
     std::vector<uint64_t> h((size_t)std::max<int64_t>(m, 1));
@@ -838,11 +899,8 @@ int g4d_diag_grid_pairs(g4d_ctx *ctx, const int32_t *id_grid, int32_t grid, uint
     HCK(hipStreamSynchronize(s));
     int64_t k = 0;
     for (int64_t i = 0; i < m; i++) {
-        const uint32_t a = (uint32_t)(h[(size_t)i] >> 32), b = (uint32_t)h[(size_t)i];
-        if (a < b) {
-            if (out && k < cap) out[k] = h[(size_t)i];
-            k++;
-        }
+        if (out && k < cap) out[k] = h[(size_t)i];
+        k++;
     }
     *count = k;
     return k > cap && out ? set_err(ctx, G4D_ERR_CAPACITY, "output capacity %lld < %lld", (long long)cap, (long long)k)

@@ -40,6 +40,8 @@ enum g4d_counter : int {
     G4D_CNT_MISC = 8,
     G4D_CNT_WORK = 9,     // work-queue head
     G4D_CNT_APPEND3 = 10,
+    G4D_CNT_APPEND4 = 11,
+    G4D_CNT_APPEND5 = 12,
     G4D_CNT_NUM = 16
 };
 

@@ -74,4 +74,17 @@ hipError_t launch_diag_orient3d(const double *pts4, int64_t T, int *sign, unsign
     return hipGetLastError();
 }
 
+__global__ void k_diag_copy(const int4 *in, int4 *out, int64_t n16)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n16; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[i];
+}
+
+hipError_t launch_diag_copy(const int4 *in, int4 *out, int64_t n16, hipStream_t s)
+{
+    if (n16 <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_diag_copy, dim3(g4d_blocks(n16, 256, 8192)), dim3(256), 0, s, in, out, n16);
+    return hipGetLastError();
+}
+
 } // namespace g4d

@@ -201,111 +201,156 @@ __device__ __forceinline__ int64_t pba_gidx(int g, int fixed, int c0, int c, int
     return ((int64_t)fixed * G + c0 + c) * G + q;                  // z=fixed, y=c0+c, x=q
 }
 
+// One wave per tile of 64 columns along the sweep axis, one lane per column.  The column is
+// streamed through an LDS tile of 64 positions x 64 columns (coalesced loads and stores for
+// every axis), the lower envelope of the candidate sites (Maurer stack, exact integer
+// dominance test) is kept per lane in LDS -- entries beyond PBA_STACK spill to a per-column
+// region of `spill` -- and the colour walk writes the nearest site of every voxel back through
+// the same tile.  Ties: a later site along the axis wins (colour walk advances on <=).
+static constexpr int PBA_COLS = 64;
+static constexpr int PBA_CHUNK = 64;
+static constexpr int PBA_STACK = 64;
+
+struct PbaLds {
+    int tile[PBA_CHUNK][PBA_COLS + 1];
+    int scode[PBA_STACK][PBA_COLS];
+};
+
 template <int AXIS>
-__global__ void __launch_bounds__(64) k_pba_pass(const int *in, int *out, int g, int tx, const int *idgrid)
+__device__ __forceinline__ int64_t pba_gidx64(int g, int fixed, int c, int q)
 {
-    extern __shared__ int lds[];
-    int *col = lds;            // [g][tx] candidates, then the in-place stack
-    int *res = lds + g * tx;   // [g][tx] coloured output
-    const int ncb = g / tx;
-    const int fixed = blockIdx.x / ncb;
-    const int c0 = (blockIdx.x % ncb) * tx;
-    const int total = g * tx;
-    // coalesced load
-    for (int i = threadIdx.x; i < total; i += 64) {
-        int c, q;
-        if (AXIS == 0) { c = i / g; q = i % g; } else { q = i / tx; c = i % tx; }
-        col[q * tx + c] = in[pba_gidx<AXIS>(g, fixed, c0, c, q)];
-    }
-    __syncthreads();
-    if (threadIdx.x < tx) {
-        const int c = threadIdx.x;
-        const int ca = (AXIS == 2 || AXIS == 1) ? c0 + c : c0 + c; // x for Z/Y passes, y for X pass
-        const int cb = fixed;                                       // y (Z), z (Y), z (X)
-        int m = 0;
-        for (int q = 0; q < g; q++) {
-            const int s = col[q * tx + c];
-            if (s == G4D_MARKER) continue;
-            int qs;
-            int64_t ds;
-            site_pos_off<AXIS>(s, ca, cb, &qs, &ds);
-            while (m >= 2) {
-                int qu, qv;
-                int64_t du, dv;
-                site_pos_off<AXIS>(col[(m - 2) * tx + c], ca, cb, &qu, &du);
-                site_pos_off<AXIS>(col[(m - 1) * tx + c], ca, cb, &qv, &dv);
-                if (env_dominated(qu, du, qv, dv, qs, ds)) m--;
-                else break;
-            }
-            col[m * tx + c] = s;
-            m++;
+    const int64_t G = g;
+    if (AXIS == 2) return ((int64_t)q * G + fixed) * G + c;   // z = q, y = fixed, x = c
+    if (AXIS == 1) return ((int64_t)fixed * G + q) * G + c;   // z = fixed, y = q, x = c
+    return ((int64_t)fixed * G + c) * G + q;                  // z = fixed, y = c, x = q
+}
+
+template <int AXIS>
+__global__ void __launch_bounds__(64) k_pba_pass(const int *in, int *out, int g, const int *idgrid, int *spill)
+{
+    __shared__ PbaLds L;
+    const int lane = threadIdx.x;
+    const int ntiles = (g + PBA_COLS - 1) / PBA_COLS;
+    const int fixed = blockIdx.x / ntiles;
+    const int c0 = (blockIdx.x % ntiles) * PBA_COLS;
+    const int ncols = min(PBA_COLS, g - c0);
+    const bool active = lane < ncols;
+    const int c = c0 + lane;
+    const int ca = c, cb = fixed; // x (Z, Y passes) or y (X pass); y (Z), z (Y), z (X)
+    int *sp = spill + ((int64_t)blockIdx.x * PBA_COLS + lane) * g;
+
+    // ---- lower envelope ----
+    int m = 0;
+    for (int q0 = 0; q0 < g; q0 += PBA_CHUNK) {
+        const int nq = min(PBA_CHUNK, g - q0);
+        for (int i = lane; i < PBA_CHUNK * PBA_COLS; i += 64) {
+            int cc, qq;
+            if (AXIS == 0) { cc = i / PBA_CHUNK; qq = i % PBA_CHUNK; } else { qq = i / PBA_COLS; cc = i % PBA_COLS; }
+            if (qq < nq && cc < ncols) L.tile[qq][cc] = in[pba_gidx64<AXIS>(g, fixed, c0 + cc, q0 + qq)];
         }
-        int idx = 0;
-        int qa = 0, qb = 0;
-        int64_t da = 0, db = 0;
-        if (m > 0) site_pos_off<AXIS>(col[c], ca, cb, &qa, &da);
-        if (m > 1) site_pos_off<AXIS>(col[tx + c], ca, cb, &qb, &db);
-        for (int t = 0; t < g; t++) {
-            if (m == 0) {
-                res[t * tx + c] = G4D_MARKER;
-                continue;
-            }
-            while (idx + 1 < m) {
-                const int64_t fa = (int64_t)(t - qa) * (t - qa) + da;
-                const int64_t fb = (int64_t)(t - qb) * (t - qb) + db;
-                if (fb <= fa) {
-                    idx++;
-                    qa = qb;
-                    da = db;
-                    if (idx + 1 < m) site_pos_off<AXIS>(col[(idx + 1) * tx + c], ca, cb, &qb, &db);
-                } else {
-                    break;
+        __syncthreads();
+        if (active) {
+            for (int qq = 0; qq < nq; qq++) {
+                const int s = L.tile[qq][lane];
+                if (s == G4D_MARKER) continue;
+                int qs;
+                int64_t ds;
+                site_pos_off<AXIS>(s, ca, cb, &qs, &ds);
+                while (m >= 2) {
+                    int qu, qv;
+                    int64_t du, dv;
+                    const int su = m - 2 < PBA_STACK ? L.scode[m - 2][lane] : sp[m - 2];
+                    const int sv = m - 1 < PBA_STACK ? L.scode[m - 1][lane] : sp[m - 1];
+                    site_pos_off<AXIS>(su, ca, cb, &qu, &du);
+                    site_pos_off<AXIS>(sv, ca, cb, &qv, &dv);
+                    if (env_dominated(qu, du, qv, dv, qs, ds)) m--;
+                    else break;
                 }
+                if (m < PBA_STACK) L.scode[m][lane] = s;
+                else sp[m] = s;
+                m++;
             }
-            res[t * tx + c] = col[idx * tx + c];
         }
+        __syncthreads();
     }
-    __syncthreads();
-    for (int i = threadIdx.x; i < total; i += 64) {
-        int c, q;
-        if (AXIS == 0) { c = i / g; q = i % g; } else { q = i / tx; c = i % tx; }
-        int v = res[q * tx + c];
-        if (idgrid) {
-            v = (v == G4D_MARKER) ? -1
-                                  : idgrid[((int64_t)G4D_DEC_Z(v) * g + G4D_DEC_Y(v)) * g + G4D_DEC_X(v)];
+    // ---- colour walk ----
+    int idx = 0, qa = 0, qb = 0, sa = G4D_MARKER;
+    int64_t da = 0, db = 0;
+    if (active && m > 0) {
+        sa = L.scode[0][lane];
+        site_pos_off<AXIS>(sa, ca, cb, &qa, &da);
+    }
+    if (active && m > 1) {
+        const int sb = 1 < PBA_STACK ? L.scode[1][lane] : sp[1];
+        site_pos_off<AXIS>(sb, ca, cb, &qb, &db);
+    }
+    for (int t0 = 0; t0 < g; t0 += PBA_CHUNK) {
+        const int nt = min(PBA_CHUNK, g - t0);
+        if (active) {
+            for (int tt = 0; tt < nt; tt++) {
+                const int t = t0 + tt;
+                if (m == 0) {
+                    L.tile[tt][lane] = G4D_MARKER;
+                    continue;
+                }
+                while (idx + 1 < m) {
+                    const int64_t fa = (int64_t)(t - qa) * (t - qa) + da;
+                    const int64_t fb = (int64_t)(t - qb) * (t - qb) + db;
+                    if (fb <= fa) {
+                        idx++;
+                        qa = qb;
+                        da = db;
+                        sa = idx < PBA_STACK ? L.scode[idx][lane] : sp[idx];
+                        if (idx + 1 < m) {
+                            const int sb = idx + 1 < PBA_STACK ? L.scode[idx + 1][lane] : sp[idx + 1];
+                            site_pos_off<AXIS>(sb, ca, cb, &qb, &db);
+                        }
+                    } else {
+                        break;
+                    }
+                }
+                L.tile[tt][lane] = sa;
+            }
         }
-        out[pba_gidx<AXIS>(g, fixed, c0, c, q)] = v;
+        __syncthreads();
+        for (int i = lane; i < PBA_CHUNK * PBA_COLS; i += 64) {
+            int cc, qq;
+            if (AXIS == 0) { cc = i / PBA_CHUNK; qq = i % PBA_CHUNK; } else { qq = i / PBA_COLS; cc = i % PBA_COLS; }
+            if (qq < nt && cc < ncols) {
+                int v = L.tile[qq][cc];
+                if (idgrid)
+                    v = (v == G4D_MARKER) ? -1
+                                          : idgrid[((int64_t)G4D_DEC_Z(v) * g + G4D_DEC_Y(v)) * g + G4D_DEC_X(v)];
+                out[pba_gidx64<AXIS>(g, fixed, c0 + cc, t0 + qq)] = v;
+            }
+        }
+        __syncthreads();
     }
 }
 
-int pba_tile_columns(int g)
+int64_t pba_spill_ints(int g)
 {
-    int tx = 8192 / g; // two int buffers of g*tx in 64 KiB of LDS
-    if (tx > 64) tx = 64;
-    if (tx < 1) tx = 1;
-    while (g % tx) tx--;
-    return tx;
+    const int64_t ntiles = (g + PBA_COLS - 1) / PBA_COLS;
+    return (int64_t)g * ntiles * PBA_COLS * g;
 }
 
-hipError_t pba_pass(int axis, int g, const int *in, int *out, const int *idgrid, hipStream_t s)
+hipError_t pba_pass(int axis, int g, const int *in, int *out, const int *idgrid, int *spill, hipStream_t s)
 {
-    const int tx = pba_tile_columns(g);
-    const size_t lds = sizeof(int) * 2 * (size_t)g * tx;
-    const int nblk = g * (g / tx);
+    const int nblk = g * ((g + PBA_COLS - 1) / PBA_COLS);
     if (axis == 2)
-        hipLaunchKernelGGL(k_pba_pass<2>, dim3(nblk), dim3(64), lds, s, in, out, g, tx, idgrid);
+        hipLaunchKernelGGL(k_pba_pass<2>, dim3(nblk), dim3(64), 0, s, in, out, g, idgrid, spill);
     else if (axis == 1)
-        hipLaunchKernelGGL(k_pba_pass<1>, dim3(nblk), dim3(64), lds, s, in, out, g, tx, idgrid);
+        hipLaunchKernelGGL(k_pba_pass<1>, dim3(nblk), dim3(64), 0, s, in, out, g, idgrid, spill);
     else
-        hipLaunchKernelGGL(k_pba_pass<0>, dim3(nblk), dim3(64), lds, s, in, out, g, tx, idgrid);
+        hipLaunchKernelGGL(k_pba_pass<0>, dim3(nblk), dim3(64), 0, s, in, out, g, idgrid, spill);
     return hipGetLastError();
 }
 
-hipError_t pba_run(int g, int *grid, int *tmp, const int *idgrid, hipStream_t s)
+hipError_t pba_run(int g, int *grid, int *tmp, const int *idgrid, int *spill, hipStream_t s)
 {
-    HIP_CHECK_RET(pba_pass(2, g, grid, tmp, nullptr, s));
-    HIP_CHECK_RET(pba_pass(1, g, tmp, grid, nullptr, s));
-    return pba_pass(0, g, grid, tmp, idgrid, s);
+    HIP_CHECK_RET(pba_pass(2, g, grid, tmp, nullptr, spill, s));
+    HIP_CHECK_RET(pba_pass(1, g, tmp, grid, nullptr, spill, s));
+    return pba_pass(0, g, grid, tmp, idgrid, spill, s);
 }
 
 // ------------------------------------------------------------------------------------
@@ -313,18 +358,13 @@ hipError_t pba_run(int g, int *grid, int *tmp, const int *idgrid, hipStream_t s)
 // directions are emitted (the reference reciprocates at GD/GDelaunay/GDelHost.cu:380-386).
 // ------------------------------------------------------------------------------------
 
-__device__ __forceinline__ void emit2(uint64_t *out, unsigned long long *counter, int64_t cap, bool pred, int a, int b)
+__device__ __forceinline__ uint64_t upair(int a, int b)
 {
-    const long long slot = g4d_wave_append(pred, counter);
-    if (pred) {
-        const int64_t k = 2 * (int64_t)slot;
-        if (k + 1 < cap) {
-            out[k] = ((uint64_t)(uint32_t)a << 32) | (uint32_t)b;
-            out[k + 1] = ((uint64_t)(uint32_t)b << 32) | (uint32_t)a;
-        }
-    }
+    return a < b ? (((uint64_t)(uint32_t)a << 32) | (uint32_t)b) : (((uint64_t)(uint32_t)b << 32) | (uint32_t)a);
 }
 
+// Missing points (GD/GDelaunay/GDelKernels.cu:109-135): a point whose voxel is led by another
+// point pairs with that leader.  Emits ordered pairs (min<<32 | max).
 __global__ void k_missing(const int *vox, int n, int g, const int *grid, uint64_t *out, unsigned long long *counter,
                           int64_t cap)
 {
@@ -337,7 +377,8 @@ __global__ void k_missing(const int *vox, int n, int g, const int *grid, uint64_
             leader = grid[((int64_t)G4D_DEC_Z(c) * g + G4D_DEC_Y(c)) * g + G4D_DEC_X(c)];
             miss = leader != k;
         }
-        emit2(out, counter, cap, miss, k, leader);
+        const long long slot = g4d_wave_append(miss, counter);
+        if (miss && slot < cap) out[slot] = upair(k, leader);
     }
 }
 
@@ -347,61 +388,92 @@ __device__ __forceinline__ int grid_at(const int *grid, int g, int x, int y, int
     return grid[((int64_t)z * g + y) * g + x];
 }
 
-// One thread per (x, y) column of 2x2x2 cubes, sweeping z (GD/GDelaunay/GDelKernels.cu:181-290).
-// Corner c = ix + 2 iy + 4 iz; root = corner 0, opp = corner 7; link cycle 6,2,3,1,5,4.
-__global__ void __launch_bounds__(256) k_grid_pairs(const int *grid, int g, uint64_t *out, unsigned long long *counter,
-                                                    int64_t cap)
+// Voronoi-neighbour pairs of one 2x2x2 cube (GD/GDelaunay/GDelKernels.cu:181-290): corner
+// c = ix + 2 iy + 4 iz, root = corner 0, opp = corner 7, link cycle 6,2,3,1,5,4.  Slot
+// 5i+{0..4} holds the pairs of link edge i (root-a, opp-a, root-b, opp-b, a-b); slot 30 is
+// root-opp.  Returns the bit mask of emitted slots.
+__device__ __forceinline__ uint32_t cube_pairs(const int *v, uint64_t *pr)
+{
+    const int root = v[0], opp = v[7];
+    if (root == opp) return 0u;
+    const bool has_marker = (root == G4D_MARKER) || (opp == G4D_MARKER);
+    const int link[7] = {v[6], v[2], v[3], v[1], v[5], v[4], v[6]};
+    uint32_t mask = 0;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int a = link[i], b = link[i + 1];
+        if (a != b && a != root && a != opp && b != root && b != opp) {
+            // grabPair skips a pair whose first value is the Marker (:155-176)
+            if (root != G4D_MARKER) { mask |= 1u << (5 * i + 0); pr[5 * i + 0] = upair(root, a); }
+            if (opp != G4D_MARKER) { mask |= 1u << (5 * i + 1); pr[5 * i + 1] = upair(opp, a); }
+            if (root != G4D_MARKER) { mask |= 1u << (5 * i + 2); pr[5 * i + 2] = upair(root, b); }
+            if (opp != G4D_MARKER) { mask |= 1u << (5 * i + 3); pr[5 * i + 3] = upair(opp, b); }
+            mask |= 1u << (5 * i + 4);
+            pr[5 * i + 4] = upair(a, b);
+        }
+    }
+    if (mask && !has_marker) {
+        mask |= 1u << 30;
+        pr[30] = upair(root, opp);
+    }
+    return mask;
+}
+
+// One thread per (x, y) column of cubes and a chunk of zc cubes along z; pass 0 counts the
+// thread's pairs, pass 1 writes them at the scanned offset (deterministic order, no atomics).
+template <bool WRITE>
+__global__ void __launch_bounds__(256) k_grid_pairs(const int *grid, int g, int zc, int64_t *count, uint64_t *out)
 {
     const int w = g + 1;
     const int64_t ncol = (int64_t)w * w;
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < ncol; base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t col = base + threadIdx.x;
-        const bool active = col < ncol;
-        const int x = active ? (int)(col % w) - 1 : -1;
-        const int y = active ? (int)(col / w) - 1 : -1;
+    const int nzc = (w + zc - 1) / zc;
+    const int64_t nthr = ncol * nzc;
+    for (int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; tid < nthr; tid += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t col = tid % ncol;
+        const int zb = (int)(tid / ncol);
+        const int x = (int)(col % w) - 1, y = (int)(col / w) - 1;
+        const int z0 = zb * zc - 1, z1 = min(z0 + zc, g);
         int v[8];
-        v[0] = active ? grid_at(grid, g, x, y, -1) : G4D_MARKER;
-        v[1] = active ? grid_at(grid, g, x + 1, y, -1) : G4D_MARKER;
-        v[2] = active ? grid_at(grid, g, x, y + 1, -1) : G4D_MARKER;
-        v[3] = active ? grid_at(grid, g, x + 1, y + 1, -1) : G4D_MARKER;
-        for (int z = -1; z < g; z++) {
-            v[4] = active ? grid_at(grid, g, x, y, z + 1) : G4D_MARKER;
-            v[5] = active ? grid_at(grid, g, x + 1, y, z + 1) : G4D_MARKER;
-            v[6] = active ? grid_at(grid, g, x, y + 1, z + 1) : G4D_MARKER;
-            v[7] = active ? grid_at(grid, g, x + 1, y + 1, z + 1) : G4D_MARKER;
-            const int root = v[0], opp = v[7];
-            const bool differ = active && root != opp;
-            const bool has_marker = (root == G4D_MARKER) || (opp == G4D_MARKER);
-            bool has_quad = false;
-            const int link[7] = {v[6], v[2], v[3], v[1], v[5], v[4], v[6]};
+        v[0] = grid_at(grid, g, x, y, z0);
+        v[1] = grid_at(grid, g, x + 1, y, z0);
+        v[2] = grid_at(grid, g, x, y + 1, z0);
+        v[3] = grid_at(grid, g, x + 1, y + 1, z0);
+        int64_t k = WRITE ? count[tid] : 0;
+        uint64_t pr[31];
+        for (int z = z0; z < z1; z++) {
+            v[4] = grid_at(grid, g, x, y, z + 1);
+            v[5] = grid_at(grid, g, x + 1, y, z + 1);
+            v[6] = grid_at(grid, g, x, y + 1, z + 1);
+            v[7] = grid_at(grid, g, x + 1, y + 1, z + 1);
+            const uint32_t mask = cube_pairs(v, pr);
+            if (WRITE) {
 #pragma unroll
-            for (int i = 0; i < 6; i++) {
-                const int a = link[i], b = link[i + 1];
-                const bool quad = differ && a != b && a != root && a != opp && b != root && b != opp;
-                has_quad |= quad;
-                // grabPair skips a pair whose first value is the Marker (:155-176)
-                emit2(out, counter, cap, quad && root != G4D_MARKER, root, a);
-                emit2(out, counter, cap, quad && opp != G4D_MARKER, opp, a);
-                emit2(out, counter, cap, quad && root != G4D_MARKER, root, b);
-                emit2(out, counter, cap, quad && opp != G4D_MARKER, opp, b);
-                emit2(out, counter, cap, quad, a, b);
+                for (int j = 0; j < 31; j++)
+                    if (mask & (1u << j)) out[k++] = pr[j];
+            } else {
+                k += __popc(mask);
             }
-            emit2(out, counter, cap, has_quad && !has_marker, root, opp);
             v[0] = v[4];
             v[1] = v[5];
             v[2] = v[6];
             v[3] = v[7];
         }
+        if (!WRITE) count[tid] = k;
     }
 }
 
 // CSR offsets of sorted directed keys: off[s] = first index with star >= s (off has n+1).
 __global__ void k_csr_offsets(const uint64_t *keys, int64_t m, int n, int64_t *off)
 {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i <= m; i += (int64_t)gridDim.x * blockDim.x) {
-        const int prev = (i == 0) ? -1 : (int)(keys[i - 1] >> 32);
-        const int cur = (i == m) ? n : (int)(keys[i] >> 32);
-        for (int s = prev + 1; s <= cur; s++) off[s] = i;
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s <= n; s += gridDim.x * blockDim.x) {
+        const uint64_t want = (uint64_t)(uint32_t)s << 32;
+        int64_t lo = 0, hi = m;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (keys[mid] < want) lo = mid + 1;
+            else hi = mid;
+        }
+        off[s] = lo;
     }
 }
 
@@ -503,17 +575,25 @@ hipError_t launch_missing(const int *vox, int n, int g, const int *grid, uint64_
     return hipGetLastError();
 }
 
-hipError_t launch_grid_pairs(const int *grid, int g, uint64_t *out, unsigned long long *counter, int64_t cap,
-                             hipStream_t s)
+int64_t grid_pair_threads(int g, int zc)
 {
-    const int64_t ncol = (int64_t)(g + 1) * (g + 1);
-    hipLaunchKernelGGL(k_grid_pairs, dim3(g4d_blocks(ncol, 256, 65536)), dim3(256), 0, s, grid, g, out, counter, cap);
+    const int64_t w = g + 1;
+    return w * w * ((w + zc - 1) / zc);
+}
+
+hipError_t launch_grid_pairs(const int *grid, int g, int zc, int64_t *count, uint64_t *out, bool write, hipStream_t s)
+{
+    const int64_t nthr = grid_pair_threads(g, zc);
+    if (write)
+        hipLaunchKernelGGL(k_grid_pairs<true>, dim3(g4d_blocks(nthr, 256, 65536)), dim3(256), 0, s, grid, g, zc, count, out);
+    else
+        hipLaunchKernelGGL(k_grid_pairs<false>, dim3(g4d_blocks(nthr, 256, 65536)), dim3(256), 0, s, grid, g, zc, count, out);
     return hipGetLastError();
 }
 
 hipError_t launch_csr_offsets(const uint64_t *keys, int64_t m, int n, int64_t *off, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_csr_offsets, dim3(g4d_blocks(m + 1, 256, 65536)), dim3(256), 0, s, keys, m, n, off);
+    hipLaunchKernelGGL(k_csr_offsets, dim3(g4d_blocks((int64_t)n + 1, 256, 65536)), dim3(256), 0, s, keys, m, n, off);
     return hipGetLastError();
 }
 

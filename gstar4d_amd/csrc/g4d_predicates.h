@@ -21,124 +21,110 @@
 namespace g4d {
 
 // ------------------------------------------------------------------------------------
-// Fixed-width signed big integers
+// Runtime-width signed big integers (32-bit limbs, two's complement).  The exact path is
+// rare, so it is written for a small register footprint: non-unrolled loops over private
+// arrays, every primitive out of line.  Kernels that can reach it keep their own register
+// budget instead of inheriting a wide unrolled multiprecision body.
 // ------------------------------------------------------------------------------------
 
-template <int N>
-struct BI {
-    uint32_t w[N];
-};
+namespace rt {
 
-template <int N>
-__device__ __forceinline__ BI<N> bi_from_i64(int64_t v)
+constexpr int MAXL = 24; // 768 bits: the widest value (a 3x3 lifted minor at 125-bit inputs)
+
+__device__ __noinline__ void sext(uint32_t *r, int nr, const uint32_t *a, int na)
 {
-    BI<N> r;
-    r.w[0] = (uint32_t)(uint64_t)v;
-    if (N > 1) r.w[1] = (uint32_t)((uint64_t)v >> 32);
-    const uint32_t ext = v < 0 ? 0xffffffffu : 0u;
-#pragma unroll
-    for (int i = 2; i < N; i++) r.w[i] = ext;
-    return r;
+    const uint32_t ext = (a[na - 1] >> 31) ? 0xffffffffu : 0u;
+#pragma unroll 1
+    for (int i = nr - 1; i >= 0; i--) r[i] = i < na ? a[i] : ext;
 }
 
-template <int NR, int N>
-__device__ __forceinline__ BI<NR> bi_ext(const BI<N> &a)
+__device__ __noinline__ void add(uint32_t *r, const uint32_t *a, const uint32_t *b, int n)
 {
-    BI<NR> r;
-    const uint32_t ext = (a.w[N - 1] >> 31) ? 0xffffffffu : 0u;
-#pragma unroll
-    for (int i = 0; i < NR; i++) r.w[i] = i < N ? a.w[i] : ext;
-    return r;
-}
-
-template <int N>
-__device__ __forceinline__ BI<N> bi_add(const BI<N> &a, const BI<N> &b)
-{
-    BI<N> r;
     uint64_t c = 0;
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-        const uint64_t t = (uint64_t)a.w[i] + b.w[i] + c;
-        r.w[i] = (uint32_t)t;
+#pragma unroll 1
+    for (int i = 0; i < n; i++) {
+        const uint64_t t = (uint64_t)a[i] + b[i] + c;
+        r[i] = (uint32_t)t;
         c = t >> 32;
     }
-    return r;
 }
 
-template <int N>
-__device__ __forceinline__ BI<N> bi_sub(const BI<N> &a, const BI<N> &b)
+__device__ __noinline__ void sub(uint32_t *r, const uint32_t *a, const uint32_t *b, int n)
 {
-    BI<N> r;
     uint64_t c = 1;
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-        const uint64_t t = (uint64_t)a.w[i] + (uint32_t)~b.w[i] + c;
-        r.w[i] = (uint32_t)t;
+#pragma unroll 1
+    for (int i = 0; i < n; i++) {
+        const uint64_t t = (uint64_t)a[i] + (uint32_t)~b[i] + c;
+        r[i] = (uint32_t)t;
         c = t >> 32;
     }
-    return r;
 }
 
-template <int N>
-__device__ __forceinline__ BI<N> bi_neg(const BI<N> &a)
+__device__ __noinline__ void neg(uint32_t *r, int n)
 {
-    BI<N> r;
     uint64_t c = 1;
-#pragma unroll
-    for (int i = 0; i < N; i++) {
-        const uint64_t t = (uint64_t)(uint32_t)~a.w[i] + c;
-        r.w[i] = (uint32_t)t;
+#pragma unroll 1
+    for (int i = 0; i < n; i++) {
+        const uint64_t t = (uint64_t)(uint32_t)~r[i] + c;
+        r[i] = (uint32_t)t;
         c = t >> 32;
     }
-    return r;
 }
 
-template <int N>
-__device__ __forceinline__ bool bi_isneg(const BI<N> &a)
+__device__ __noinline__ int sign(const uint32_t *a, int n)
 {
-    return (a.w[N - 1] >> 31) != 0;
-}
-
-template <int N>
-__device__ __forceinline__ int bi_sign(const BI<N> &a)
-{
-    if (bi_isneg(a)) return -1;
+    if (a[n - 1] >> 31) return -1;
     uint32_t o = 0;
-#pragma unroll
-    for (int i = 0; i < N; i++) o |= a.w[i];
+#pragma unroll 1
+    for (int i = 0; i < n; i++) o |= a[i];
     return o ? 1 : 0;
 }
 
-// Exact signed product; |a| < 2^(32NA-1), |b| < 2^(32NB-1) so it fits in NA+NB limbs.
-template <int NA, int NB>
-__device__ __forceinline__ BI<NA + NB> bi_mul(const BI<NA> &a_in, const BI<NB> &b_in)
+// r[na+nb] = a[na] * b[nb], exact and signed (|a| < 2^(32na-1), |b| < 2^(32nb-1)).
+__device__ __noinline__ void mul(uint32_t *r, const uint32_t *a, int na, const uint32_t *b, int nb)
 {
-    const bool na = bi_isneg(a_in), nb = bi_isneg(b_in);
-    const BI<NA> a = na ? bi_neg(a_in) : a_in;
-    const BI<NB> b = nb ? bi_neg(b_in) : b_in;
-    BI<NA + NB> r;
-#pragma unroll
-    for (int i = 0; i < NA + NB; i++) r.w[i] = 0;
-#pragma unroll
-    for (int i = 0; i < NA; i++) {
+    uint32_t ma[MAXL], mb[MAXL];
+    const bool sa = a[na - 1] >> 31, sb = b[nb - 1] >> 31;
+#pragma unroll 1
+    for (int i = 0; i < na; i++) ma[i] = a[i];
+#pragma unroll 1
+    for (int i = 0; i < nb; i++) mb[i] = b[i];
+    if (sa) neg(ma, na);
+    if (sb) neg(mb, nb);
+#pragma unroll 1
+    for (int i = 0; i < na + nb; i++) r[i] = 0;
+#pragma unroll 1
+    for (int i = 0; i < na; i++) {
         uint64_t carry = 0;
-#pragma unroll
-        for (int j = 0; j < NB; j++) {
-            const uint64_t t = (uint64_t)a.w[i] * b.w[j] + r.w[i + j] + carry;
-            r.w[i + j] = (uint32_t)t;
+#pragma unroll 1
+        for (int j = 0; j < nb; j++) {
+            const uint64_t t = (uint64_t)ma[i] * mb[j] + r[i + j] + carry;
+            r[i + j] = (uint32_t)t;
             carry = t >> 32;
         }
-        r.w[i + NB] = (uint32_t)carry;
+        r[i + nb] = (uint32_t)carry;
     }
-    return (na != nb) ? bi_neg(r) : r;
+    if (sa != sb) neg(r, na + nb);
 }
+
+// r[n] = a*b - c*d with a, b, c, d of na limbs each (n = 2na).
+__device__ __noinline__ void det2(uint32_t *r, const uint32_t *a, const uint32_t *b, const uint32_t *c,
+                                  const uint32_t *d, int na)
+{
+    uint32_t t[rt::MAXL];
+    mul(r, a, na, b, na);
+    mul(t, c, na, d, na);
+    sub(r, r, t, 2 * na);
+}
+
+} // namespace rt
 
 // ------------------------------------------------------------------------------------
 // Fixed-point conversion of the FP64 inputs of one predicate
 // ------------------------------------------------------------------------------------
 
-// Decompose finite nonzero x = m * 2^e with m an odd-free-trailing 53-bit integer (trailing
-// zero bits stripped into e).  Returns false for zero.
+// Decompose finite nonzero x = m * 2^e with m an integer of at most 53 bits whose trailing
+// zero bits are moved into e.  Returns false for zero.
 __device__ __forceinline__ bool fp_decompose(double x, int64_t *m, int *e)
 {
     const uint64_t bits = (uint64_t)__double_as_longlong(x);
@@ -160,180 +146,186 @@ __device__ __forceinline__ bool fp_decompose(double x, int64_t *m, int *e)
     return true;
 }
 
-// Integer coordinates of NP points (3 each) on a common scale.  K = limbs per coordinate.
-template <int NP>
+// Integer coordinates of up to 5 points (3 each) on a common scale 2^emin.
 struct FixedPts {
-    int64_t m[NP][3];
-    int e[NP][3];
-    bool nz[NP][3];
+    int64_t m[5][3];
+    int e[5][3];
     int emin;
     int bits; // bits of the largest magnitude after scaling
+    int limbs; // 2 (bits <= 61) or 4 (bits <= 125); 0 when out of range
 };
 
-template <int NP>
-__device__ __forceinline__ void fixed_analyse(const double *const *p, FixedPts<NP> &f)
+__device__ __noinline__ void fixed_analyse(const double *const *p, int np, FixedPts &f)
 {
     f.emin = 1 << 20;
     int top = -(1 << 20);
-#pragma unroll
-    for (int i = 0; i < NP; i++)
-#pragma unroll
+#pragma unroll 1
+    for (int i = 0; i < np; i++)
+#pragma unroll 1
         for (int c = 0; c < 3; c++) {
-            f.nz[i][c] = fp_decompose(p[i][c], &f.m[i][c], &f.e[i][c]);
-            if (f.nz[i][c]) {
-                f.emin = min(f.emin, f.e[i][c]);
-                const uint64_t am = (uint64_t)(f.m[i][c] < 0 ? -f.m[i][c] : f.m[i][c]);
-                top = max(top, f.e[i][c] + 64 - __builtin_clzll(am));
+            int64_t m;
+            int e;
+            if (!fp_decompose(p[i][c], &m, &e)) {
+                f.m[i][c] = 0;
+                f.e[i][c] = 0;
+                continue;
             }
+            f.m[i][c] = m;
+            f.e[i][c] = e;
+            f.emin = min(f.emin, e);
+            const uint64_t am = (uint64_t)(m < 0 ? -m : m);
+            top = max(top, e + 64 - __builtin_clzll(am));
         }
     f.bits = (top == -(1 << 20)) ? 0 : top - f.emin;
+    f.limbs = f.bits <= 61 ? 2 : (f.bits <= 125 ? 4 : 0);
 }
 
-// Coordinate as a K-limb signed integer scaled by 2^-emin.
-template <int K, int NP>
-__device__ __forceinline__ BI<K> fixed_coord(const FixedPts<NP> &f, int i, int c)
+// Coordinate (i, c) as a K-limb signed integer scaled by 2^-emin.
+__device__ __noinline__ void fixed_coord(const FixedPts &f, int i, int c, int K, uint32_t *r)
 {
-    BI<K> r;
-#pragma unroll
-    for (int q = 0; q < K; q++) r.w[q] = 0;
-    if (!f.nz[i][c]) return r;
+#pragma unroll 1
+    for (int q = 0; q < K; q++) r[q] = 0;
     const int64_t mm = f.m[i][c];
+    if (mm == 0) return;
     const uint64_t am = (uint64_t)(mm < 0 ? -mm : mm);
-    const int sh = f.e[i][c] - f.emin; // >= 0, and am << sh < 2^(32K-2) by the caller's check
+    const int sh = f.e[i][c] - f.emin; // >= 0; am << sh < 2^(32K-3) by the range check
     const int wq = sh >> 5, bq = sh & 31;
-#pragma unroll
-    for (int q = 0; q < K; q++) {
-        // limb q receives bits [32q, 32q+32) of am << sh
-        uint32_t v = 0;
-        const int lo = q - wq; // which 32-bit chunk of (am << bq)
-        const uint64_t lo64 = am << bq;
-        const uint64_t hi64 = bq ? (am >> (64 - bq)) : 0; // bits 64.. of am << bq
-        if (lo == 0) v = (uint32_t)lo64;
-        else if (lo == 1) v = (uint32_t)(lo64 >> 32);
-        else if (lo == 2) v = (uint32_t)hi64;
-        r.w[q] = v;
-    }
-    return mm < 0 ? bi_neg(r) : r;
+    const uint64_t lo64 = am << bq;
+    const uint64_t hi64 = bq ? (am >> (64 - bq)) : 0;
+    if (wq < K) r[wq] = (uint32_t)lo64;
+    if (wq + 1 < K) r[wq + 1] = (uint32_t)(lo64 >> 32);
+    if (wq + 2 < K) r[wq + 2] = (uint32_t)hi64;
+    if (mm < 0) rt::neg(r, K);
 }
 
 // ------------------------------------------------------------------------------------
-// Exact determinants over K-limb coordinates
+// Exact determinants on K-limb coordinates (K = 2 or 4).  With |coordinate| < 2^(32K-3):
+// differences fit K limbs, 2x2 minors 2K, 3x3 minors 3K, lifts 2K, the lifted 4x4
+// determinant 5K; for SoS minors every entry (coordinate or lift, translated) fits 2K limbs,
+// so 2x2 minors fit 4K and 3x3 minors 6K <= 24 limbs.
 // ------------------------------------------------------------------------------------
 
-// Lifted determinant det[[p |p|^2 1]] of 5 points (rows), via translation by the last row:
-// det4[[d |d|^2]] with d = p_i - p_4.  Sign equals Shewchuk's insphere(p0,p1,p2,p3,p4).
-template <int K>
-__device__ __noinline__ int exact_insphere_k(const FixedPts<5> &f)
+// det[[p |p|^2 1]] over rows 0..4, translated by row 4: det4[[d |d|^2]], d = p_i - p_4.
+// Its sign equals Shewchuk's insphere(p0, p1, p2, p3, p4).
+__device__ __noinline__ int exact_insphere_rt(const FixedPts &f)
 {
-    // |coordinates| < 2^(32K-3) so differences fit K limbs; tight limb counts follow from
-    // |d| < 2^(32K-2): 2x2 minors < 2^(64K-3), 3x3 minors < 2^(96K-3), lifts < 2^(64K-2),
-    // determinant < 2^(160K-1).
-    constexpr int ND = K;
-    BI<ND> d[4][3];
-#pragma unroll
+    const int K = f.limbs;
+    uint32_t d[4][3][4], e[4];
+#pragma unroll 1
     for (int c = 0; c < 3; c++) {
-        const BI<ND> e = bi_ext<ND>(fixed_coord<K>(f, 4, c));
-#pragma unroll
-        for (int i = 0; i < 4; i++) d[i][c] = bi_sub(bi_ext<ND>(fixed_coord<K>(f, i, c)), e);
+        fixed_coord(f, 4, c, K, e);
+#pragma unroll 1
+        for (int i = 0; i < 4; i++) {
+            fixed_coord(f, i, c, K, d[i][c]);
+            rt::sub(d[i][c], d[i][c], e, K);
+        }
     }
-    // 2x2 xy minors
-    constexpr int N2 = 2 * ND;
-    BI<N2> m2[4][4];
-#pragma unroll
+    // 2x2 xy minors for the pairs (i<j), index i*4+j
+    uint32_t m2[16][8];
+#pragma unroll 1
     for (int i = 0; i < 4; i++)
-#pragma unroll
-        for (int j = i + 1; j < 4; j++)
-            m2[i][j] = bi_sub(bi_ext<N2>(bi_mul(d[i][0], d[j][1])), bi_ext<N2>(bi_mul(d[j][0], d[i][1])));
-    // 3x3 xyz minors, index = row left out; expanded along z
-    constexpr int N3 = ND + N2;
-    BI<N3> m3[4];
-#pragma unroll
+#pragma unroll 1
+        for (int j = i + 1; j < 4; j++) rt::det2(m2[4 * i + j], d[i][0], d[j][1], d[j][0], d[i][1], K);
+    // lifted column expansion: det4 = sum_i (-1)^(i+3) l_i M3_i, M3_i the xyz minor without row i
+    uint32_t acc[20], m3[12], t[rt::MAXL], l[8], sq[8];
+#pragma unroll 1
+    for (int q = 0; q < 5 * K; q++) acc[q] = 0;
+#pragma unroll 1
     for (int out = 0; out < 4; out++) {
-        int r[3], q = 0;
-#pragma unroll
+        int r[3], n = 0;
+#pragma unroll 1
         for (int i = 0; i < 4; i++)
-            if (i != out) r[q++] = i;
-        BI<N3> t0 = bi_ext<N3>(bi_mul(d[r[0]][2], m2[r[1]][r[2]]));
-        BI<N3> t1 = bi_ext<N3>(bi_mul(d[r[1]][2], m2[r[0]][r[2]]));
-        BI<N3> t2 = bi_ext<N3>(bi_mul(d[r[2]][2], m2[r[0]][r[1]]));
-        m3[out] = bi_add(bi_sub(t0, t1), t2);
+            if (i != out) r[n++] = i;
+        rt::mul(m3, d[r[0]][2], K, m2[4 * r[1] + r[2]], 2 * K);
+        rt::mul(t, d[r[1]][2], K, m2[4 * r[0] + r[2]], 2 * K);
+        rt::sub(m3, m3, t, 3 * K);
+        rt::mul(t, d[r[2]][2], K, m2[4 * r[0] + r[1]], 2 * K);
+        rt::add(m3, m3, t, 3 * K);
+        rt::mul(l, d[out][0], K, d[out][0], K);
+        rt::mul(sq, d[out][1], K, d[out][1], K);
+        rt::add(l, l, sq, 2 * K);
+        rt::mul(sq, d[out][2], K, d[out][2], K);
+        rt::add(l, l, sq, 2 * K);
+        rt::mul(t, l, 2 * K, m3, 3 * K);
+        if (out & 1) rt::add(acc, acc, t, 5 * K);
+        else rt::sub(acc, acc, t, 5 * K);
     }
-    // lifts
-    constexpr int NL = 2 * ND;
-    BI<NL> l[4];
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-        l[i] = bi_add(bi_add(bi_ext<NL>(bi_mul(d[i][0], d[i][0])), bi_ext<NL>(bi_mul(d[i][1], d[i][1]))),
-                      bi_ext<NL>(bi_mul(d[i][2], d[i][2])));
-    }
-    constexpr int NF = NL + N3;
-    BI<NF> acc = bi_ext<NF>(bi_mul(l[1], m3[1]));
-    acc = bi_sub(acc, bi_ext<NF>(bi_mul(l[0], m3[0])));
-    acc = bi_sub(acc, bi_ext<NF>(bi_mul(l[2], m3[2])));
-    acc = bi_add(acc, bi_ext<NF>(bi_mul(l[3], m3[3])));
-    return bi_sign(acc);
+    return rt::sign(acc, 5 * K);
 }
 
-// orient3d exact: det[[p 1]] over 4 points, translated by the last.
-template <int K>
-__device__ __noinline__ int exact_orient3d_k(const FixedPts<4> &f)
+// orient3d exact: det[[p 1]] over rows 0..3, translated by row 3.
+__device__ __noinline__ int exact_orient3d_rt(const FixedPts &f)
 {
-    constexpr int ND = K;
-    BI<ND> d[3][3];
-#pragma unroll
+    const int K = f.limbs;
+    uint32_t d[3][3][4], e[4];
+#pragma unroll 1
     for (int c = 0; c < 3; c++) {
-        const BI<ND> e = bi_ext<ND>(fixed_coord<K>(f, 3, c));
-#pragma unroll
-        for (int i = 0; i < 3; i++) d[i][c] = bi_sub(bi_ext<ND>(fixed_coord<K>(f, i, c)), e);
+        fixed_coord(f, 3, c, K, e);
+#pragma unroll 1
+        for (int i = 0; i < 3; i++) {
+            fixed_coord(f, i, c, K, d[i][c]);
+            rt::sub(d[i][c], d[i][c], e, K);
+        }
     }
-    constexpr int N2 = 2 * ND;
-    const BI<N2> m12 = bi_sub(bi_mul(d[1][0], d[2][1]), bi_mul(d[2][0], d[1][1]));
-    const BI<N2> m02 = bi_sub(bi_mul(d[0][0], d[2][1]), bi_mul(d[2][0], d[0][1]));
-    const BI<N2> m01 = bi_sub(bi_mul(d[0][0], d[1][1]), bi_mul(d[1][0], d[0][1]));
-    constexpr int N3 = ND + N2;
-    BI<N3> acc = bi_mul(d[0][2], m12);
-    acc = bi_sub(acc, bi_mul(d[1][2], m02));
-    acc = bi_add(acc, bi_mul(d[2][2], m01));
-    return bi_sign(acc);
+    uint32_t m[8], t[rt::MAXL], acc[12];
+    rt::det2(m, d[1][0], d[2][1], d[2][0], d[1][1], K);
+    rt::mul(acc, d[0][2], K, m, 2 * K);
+    rt::det2(m, d[0][0], d[2][1], d[2][0], d[0][1], K);
+    rt::mul(t, d[1][2], K, m, 2 * K);
+    rt::sub(acc, acc, t, 3 * K);
+    rt::det2(m, d[0][0], d[1][1], d[1][0], d[0][1], K);
+    rt::mul(t, d[2][2], K, m, 2 * K);
+    rt::add(acc, acc, t, 3 * K);
+    return rt::sign(acc, 3 * K);
 }
 
-// SoS minor: sign det[[cols... 1]] over k rows (k = 2..4) of the 5 fixed points;
-// rows/cols are packed nibbles (row i = (rows >> 4i) & 15, col c = (cols >> 4c) & 15,
-// col 3 = lift |p|^2 of the absolute coordinates).
-template <int K>
-__device__ __noinline__ int exact_minor_k(const FixedPts<5> &f, int k, uint32_t rows, uint32_t cols)
+// SoS minor: sign det[[cols... 1]] over k rows (k = 2..4) of the fixed points; rows/cols are
+// packed nibbles (row i = (rows >> 4i) & 15, col c = (cols >> 4c) & 15, col 3 = |p|^2).
+__device__ __noinline__ int exact_minor_rt(const FixedPts &f, int k, uint32_t rows, uint32_t cols)
 {
-    // entries as absolute integers: |coord| < 2^(32K-3), |lift| < 2^(64K-4); all
-    // differences then fit 2K limbs, 2x2 minors 4K limbs and 3x3 minors 6K limbs.
-    constexpr int NE = 2 * K;
-    BI<NE> a[4][3];
+    const int K = f.limbs, NE = 2 * K;
+    uint32_t a[4][3][8], x[4], y[4], z[4], t[rt::MAXL];
+#pragma unroll 1
     for (int i = 0; i < k; i++) {
         const int r = (rows >> (4 * i)) & 15;
+#pragma unroll 1
         for (int j = 0; j < k - 1; j++) {
             const int c = (cols >> (4 * j)) & 15;
             if (c < 3) {
-                a[i][j] = bi_ext<NE>(fixed_coord<K>(f, r, c));
+                fixed_coord(f, r, c, K, x);
+                rt::sext(a[i][j], NE, x, K);
             } else {
-                const BI<K> x = fixed_coord<K>(f, r, 0), y = fixed_coord<K>(f, r, 1), z = fixed_coord<K>(f, r, 2);
-                a[i][j] = bi_add(bi_add(bi_ext<NE>(bi_mul(x, x)), bi_ext<NE>(bi_mul(y, y))), bi_ext<NE>(bi_mul(z, z)));
+                fixed_coord(f, r, 0, K, x);
+                fixed_coord(f, r, 1, K, y);
+                fixed_coord(f, r, 2, K, z);
+                rt::mul(a[i][j], x, K, x, K);
+                rt::mul(t, y, K, y, K);
+                rt::add(a[i][j], a[i][j], t, NE);
+                rt::mul(t, z, K, z, K);
+                rt::add(a[i][j], a[i][j], t, NE);
             }
         }
     }
     // translate by the last row (the ones column drops out)
-    BI<NE> d[3][3];
+#pragma unroll 1
     for (int i = 0; i < k - 1; i++)
-        for (int j = 0; j < k - 1; j++) d[i][j] = bi_sub(a[i][j], a[k - 1][j]);
-    if (k == 2) return bi_sign(d[0][0]);
-    if (k == 3) return bi_sign(bi_sub(bi_mul(d[0][0], d[1][1]), bi_mul(d[1][0], d[0][1])));
-    constexpr int N2 = 2 * NE;
-    const BI<N2> m12 = bi_sub(bi_mul(d[1][0], d[2][1]), bi_mul(d[2][0], d[1][1]));
-    const BI<N2> m02 = bi_sub(bi_mul(d[0][0], d[2][1]), bi_mul(d[2][0], d[0][1]));
-    const BI<N2> m01 = bi_sub(bi_mul(d[0][0], d[1][1]), bi_mul(d[1][0], d[0][1]));
-    constexpr int N3 = NE + N2;
-    BI<N3> acc = bi_mul(d[0][2], m12);
-    acc = bi_sub(acc, bi_mul(d[1][2], m02));
-    acc = bi_add(acc, bi_mul(d[2][2], m01));
-    return bi_sign(acc);
+#pragma unroll 1
+        for (int j = 0; j < k - 1; j++) rt::sub(a[i][j], a[i][j], a[k - 1][j], NE);
+    if (k == 2) return rt::sign(a[0][0], NE);
+    uint32_t m[16], acc[rt::MAXL];
+    if (k == 3) {
+        rt::det2(m, a[0][0], a[1][1], a[1][0], a[0][1], NE);
+        return rt::sign(m, 2 * NE);
+    }
+    rt::det2(m, a[1][0], a[2][1], a[2][0], a[1][1], NE);
+    rt::mul(acc, a[0][2], NE, m, 2 * NE);
+    rt::det2(m, a[0][0], a[2][1], a[2][0], a[0][1], NE);
+    rt::mul(t, a[1][2], NE, m, 2 * NE);
+    rt::sub(acc, acc, t, 3 * NE);
+    rt::det2(m, a[0][0], a[1][1], a[1][0], a[0][1], NE);
+    rt::mul(t, a[2][2], NE, m, 2 * NE);
+    rt::add(acc, acc, t, 3 * NE);
+    return rt::sign(acc, 3 * NE);
 }
 
 // ------------------------------------------------------------------------------------
@@ -380,20 +372,18 @@ __device__ __forceinline__ void pred_flag_error(unsigned long long *cnt, uint32_
 // Exact insphere sign of 5 points; flags range errors.
 __device__ __noinline__ int exact_insphere(const double *const *p, unsigned long long *cnt)
 {
-    FixedPts<5> f;
-    fixed_analyse<5>(p, f);
-    if (f.bits <= 61) return exact_insphere_k<2>(f);
-    if (f.bits <= 125) return exact_insphere_k<4>(f);
+    FixedPts f;
+    fixed_analyse(p, 5, f);
+    if (f.limbs) return exact_insphere_rt(f);
     pred_flag_error(cnt, G4D_DEV_ERR_PREDICATE_RANGE);
     return 0;
 }
 
 __device__ __noinline__ int exact_orient3d(const double *const *p, unsigned long long *cnt)
 {
-    FixedPts<4> f;
-    fixed_analyse<4>(p, f);
-    if (f.bits <= 61) return exact_orient3d_k<2>(f);
-    if (f.bits <= 125) return exact_orient3d_k<4>(f);
+    FixedPts f;
+    fixed_analyse(p, 4, f);
+    if (f.limbs) return exact_orient3d_rt(f);
     pred_flag_error(cnt, G4D_DEV_ERR_PREDICATE_RANGE);
     return 0;
 }
@@ -404,30 +394,29 @@ __device__ __noinline__ int sos_orient4(const double *const *pin, const int *idx
 {
     int idx[5];
     const double *pt[5];
-#pragma unroll
+#pragma unroll 1
     for (int i = 0; i < 5; i++) {
         idx[i] = idx_in[i];
         pt[i] = pin[i];
     }
     int parity = 0;
+#pragma unroll 1
     for (int i = 1; i < 5; i++)
+#pragma unroll 1
         for (int j = i; j > 0 && idx[j] < idx[j - 1]; j--) {
             const int ti = idx[j]; idx[j] = idx[j - 1]; idx[j - 1] = ti;
             const double *tp = pt[j]; pt[j] = pt[j - 1]; pt[j - 1] = tp;
             parity ^= 1;
         }
-    FixedPts<5> f;
-    fixed_analyse<5>(pt, f);
-    const int kk = f.bits <= 61 ? 2 : (f.bits <= 125 ? 4 : 0);
-    if (kk == 0) pred_flag_error(cnt, G4D_DEV_ERR_PREDICATE_RANGE);
+    FixedPts f;
+    fixed_analyse(pt, 5, f);
+    if (!f.limbs) pred_flag_error(cnt, G4D_DEV_ERR_PREDICATE_RANGE);
     int sign = 0, depth = 49;
-    for (int d = 1; d <= 48; d++) {
+#pragma unroll 1
+    for (int d = 1; d <= 48 && f.limbs; d++) {
         const uint32_t e = kSosTable[d];
         const int k = (e >> 28) & 7;
-        const uint32_t rows = e & 0xffff, cols = (e >> 16) & 0xfff;
-        int s = 0;
-        if (kk == 2) s = exact_minor_k<2>(f, k, rows, cols);
-        else if (kk == 4) s = exact_minor_k<4>(f, k, rows, cols);
+        const int s = exact_minor_rt(f, k, e & 0xffff, (e >> 16) & 0xfff);
         if (s != 0) {
             sign = (e >> 31) ? -s : s;
             depth = d;

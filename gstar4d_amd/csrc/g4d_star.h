@@ -9,8 +9,8 @@ namespace g4d {
 // triangle in one word: local vertex indices (9 bits each), adjacency slots (10 bits each),
 // checked flag (bit 57).  A larger star stores two words per triangle: local vertices
 // (21 bits each, checked flag in bit 63), then adjacency slots (21 bits each).
-static constexpr int STAR_VMAX_SMALL = 128;
-static constexpr int STAR_VMAX_BIG = 512;
+static constexpr int STAR_VMAX_SMALL = 64;
+static constexpr int STAR_VMAX_BIG = 256;
 static constexpr int STAR_WIDE_MIN = 512;
 // Stars whose link (plus this pass's insertions) exceeds the LDS kernels' limit run in the
 // global-memory star kernel, which has no fixed capacity.
@@ -50,6 +50,9 @@ struct StarParams {
     int *defer_list;      // stars that exceed this kernel's capacity, counter `defer_counter`
     int64_t defer_cap;
     int defer_counter;
+    int *exact_list;      // stars whose filters were inconclusive (filter-only kernels),
+    int64_t exact_cap;    // counter `exact_counter`
+    int exact_counter;
     int lds_limit;        // largest vn + m handled in LDS (<= STAR_LDS_LIMIT)
     // work list (nullptr = all stars)
     const int *star_list;
@@ -57,8 +60,11 @@ struct StarParams {
     unsigned long long *cnt;
 };
 
-// tier 0: LDS kernel for up to STAR_VMAX_SMALL link vertices; tier 1: up to STAR_LDS_LIMIT.
-hipError_t launch_star_kernel(const StarParams &p, int tier, int nblocks, hipStream_t s);
+// tier 0: LDS kernel for up to STAR_VMAX_SMALL link vertices; tier 1: up to STAR_VMAX_BIG.
+// Both are filter-only (no exact arithmetic, no private scratch): a star with an
+// inconclusive orientation goes to exact_list.  exact = true runs the STAR_VMAX_BIG kernel
+// with the exact predicates and SoS on the stars of star_list.
+hipError_t launch_star_kernel(const StarParams &p, int tier, bool exact, int nblocks, hipStream_t s);
 // Global-memory star kernel for the stars in `list` (any size up to vcap link vertices);
 // each block owns a workspace of star_global_bytes(vcap) bytes at ws + blockIdx * stride.
 int64_t star_global_bytes(int vcap);
@@ -71,6 +77,9 @@ struct ProofParams {
     const double *pts;
     const uint64_t *tasks; // (star<<32 | vertex) drowned, non-mutual
     int64_t ntasks;
+    int64_t *xlist;       // filter-only pass: tasks needing exact predicates, counter G4D_CNT_APPEND5;
+    int64_t nx;           // exact pass: the task indices to redo
+    int64_t xcap;
     const int *vnum;
     const int64_t *voff;
     const int *vpool;
@@ -80,7 +89,8 @@ struct ProofParams {
     uint64_t *out;        // 4 pairs per task (a<<32|b, a<b)
     unsigned long long *cnt;
 };
-hipError_t launch_proofs(const ProofParams &p, hipStream_t s);
+// exact = false: filter-only pass over all tasks; exact = true: the tasks in xlist[0, nx).
+hipError_t launch_proofs(const ProofParams &p, bool exact, hipStream_t s);
 
 // Drowned pairs -> keys (min<<33 | max<<1 | dir) for the mutual test.
 hipError_t launch_drown_keys(const uint64_t *drown, int64_t m, uint64_t *keys, hipStream_t s);
@@ -105,12 +115,16 @@ struct OutParams {
     const int *vpool;
     const int *tnum;
     const int64_t *toff;
-    const uint64_t *tpool;
+    uint64_t *tpool;       // keep bits of inconclusive owner triangles are set in place
     int64_t *star_tet_count; // per star
     const int64_t *star_tet_off;
     int *tets;               // 4 per tetra
     unsigned long long *cnt;
 };
+// Owner triangles whose orient3d filter is inconclusive -> list (star<<32 | triangle),
+// counter G4D_CNT_APPEND5; then the exact pass marks the ones to keep in their triangle word.
+hipError_t launch_out_flag(const OutParams &p, uint64_t *list, int64_t cap, hipStream_t s);
+hipError_t launch_out_exact(const OutParams &p, const uint64_t *list, int64_t m, hipStream_t s);
 hipError_t launch_out_count(const OutParams &p, hipStream_t s);
 hipError_t launch_out_write(const OutParams &p, hipStream_t s);
 // Face adjacency via an open-addressing hash of face triples.

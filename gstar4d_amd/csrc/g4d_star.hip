@@ -13,8 +13,8 @@
 // MI355X design.  One wave64 owns one star at a time (dynamic work queue).  The star's link
 // (a triangulated 2-sphere) lives in LDS: local vertex table with coordinates relative to the
 // star point, triangles with 9-bit local vertices and 16-bit adjacency.  Each lane owns the
-// triangle slots lane, lane+64, ...; for each slot it keeps in registers the four cofactors
-// of the lifted 4x4 determinant, so the beneath test of a new vertex v against triangle
+// triangle slots lane, lane+64, ...; for each slot LDS caches the four cofactors of the
+// lifted 4x4 determinant, so the beneath test of a new vertex v against triangle
 // (a,b,c) is one 4-term dot product:
 //     orient4(a,b,c,s,v) = v'x*Mx - v'y*My + v'z*Mz - v'l*Ml      (primes: relative to s)
 // with a forward-error bound from the cofactor permanents; uncertain lanes fall back to the
@@ -76,6 +76,7 @@ __device__ __forceinline__ void star_tri_verts(const uint64_t *tpool, int64_t to
 template <int VMAX>
 struct StarLds {
     static constexpr int TMAX = 2 * VMAX;
+    double cof[TMAX][8]; // per triangle slot: cofactors C[4], permanents P[4]
     int vid[VMAX];
     double vc[VMAX][4];
     uint32_t tv[TMAX];
@@ -101,7 +102,21 @@ __device__ __forceinline__ int wave_excl(int v, int *tot)
     return inc - v;
 }
 
-template <int VMAX>
+// Appends star s to a list (lane 0), flagging a buffer error when the list is full.
+__device__ __forceinline__ void star_list_push(int *list, int64_t cap, unsigned long long *counter, int s,
+                                               unsigned long long *cnt)
+{
+    if (__lane_id() == 0) {
+        const unsigned long long k = atomicAdd(counter, 1ull);
+        if ((int64_t)k < cap) list[k] = s;
+        else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
+    }
+}
+
+// EXACT = false: filters only.  A star with an inconclusive orientation is handed to
+// p.exact_list untouched (its output is written only at the end), so the kernel carries no
+// multiprecision code and no private scratch.
+template <int VMAX, bool EXACT>
 __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
 {
     constexpr int TMAX = StarLds<VMAX>::TMAX;
@@ -109,7 +124,6 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
     __shared__ StarLds<VMAX> L;
     const int lane = threadIdx.x;
     unsigned long long *cnt = p.cnt;
-    double C[KPL][4], P[KPL][4];
 
     for (;;) {
         if (lane == 0) L.sc[0] = (int)atomicAdd(&cnt[G4D_CNT_WORK], 1ull);
@@ -124,11 +138,7 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
         const int need = vn + m;
         const int lim = VMAX < p.lds_limit ? VMAX : p.lds_limit;
         if (need > lim) {
-            if (lane == 0) {
-                const unsigned long long k = atomicAdd(&cnt[p.defer_counter], 1ull);
-                if ((int64_t)k < p.defer_cap) p.defer_list[k] = s;
-                else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
-            }
+            star_list_push(p.defer_list, p.defer_cap, &cnt[p.defer_counter], s, cnt);
             continue;
         }
         if (p.mode == 0 && m < 4) {
@@ -184,7 +194,11 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
             const double *pp[5] = {p.pts + 3 * (int64_t)l[0], p.pts + 3 * (int64_t)l[1], p.pts + 3 * (int64_t)l[2],
                                    p.pts + 3 * (int64_t)l[3], p.pts + 3 * (int64_t)s};
             const int idx[5] = {l[0], l[1], l[2], l[3], s};
-            const int o = orient4_full(pp, idx, cnt);
+            const int o = EXACT ? orient4_full(pp, idx, cnt) : filter_insphere(pp);
+            if (!EXACT && o == 0) {
+                star_list_push(p.exact_list, p.exact_cap, &cnt[p.exact_counter], s, cnt);
+                continue;
+            }
             const bool sw = o < 0;
             if (sw) {
                 const int t = l[0];
@@ -208,34 +222,39 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
             q0 = 4;
         }
         __syncthreads();
-#pragma unroll
+#pragma unroll 1
         for (int k = 0; k < KPL; k++) {
             const int slot = lane + 64 * k;
             if (slot < T) {
                 const uint32_t t = L.tv[slot];
-                tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], C[k], P[k]);
+                tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], L.cof[slot], L.cof[slot] + 4);
             }
         }
-        bool failed = false;
+        bool failed = false, unsure = false;
         for (int q = q0; q < m; q++) {
             const int gv = L.ins[q];
             double r[4];
             rel_coords(p.pts, gv, sx, sy, sz, r);
             int nv = 0;
-#pragma unroll
+            bool unc = false;
+#pragma unroll 1
             for (int k = 0; k < KPL; k++) {
                 const int slot = lane + 64 * k;
                 if (slot < T) {
                     const uint32_t t = L.tv[slot];
                     if (t & TF_ALIVE) {
-                        int sg = cofactor_filter_sign(r, C[k], P[k]);
+                        int sg = cofactor_filter_sign(r, L.cof[slot], L.cof[slot] + 4);
                         if (sg == 0) {
-                            const int a = L.vid[tvert(t, 0)], b = L.vid[tvert(t, 1)], c = L.vid[tvert(t, 2)];
-                            const double *pp[5] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b,
-                                                   p.pts + 3 * (int64_t)c, p.pts + 3 * (int64_t)s,
-                                                   p.pts + 3 * (int64_t)gv};
-                            const int idx[5] = {a, b, c, s, gv};
-                            sg = orient4_full(pp, idx, cnt);
+                            if (EXACT) {
+                                const int a = L.vid[tvert(t, 0)], b = L.vid[tvert(t, 1)], c = L.vid[tvert(t, 2)];
+                                const double *pp[5] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b,
+                                                       p.pts + 3 * (int64_t)c, p.pts + 3 * (int64_t)s,
+                                                       p.pts + 3 * (int64_t)gv};
+                                const int idx[5] = {a, b, c, s, gv};
+                                sg = orient4_full(pp, idx, cnt);
+                            } else {
+                                unc = true;
+                            }
                         }
                         if (sg < 0) {
                             L.tv[slot] = t | TF_VIS;
@@ -243,6 +262,10 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
                         }
                     }
                 }
+            }
+            if (!EXACT && __ballot(unc)) {
+                unsure = true;
+                break;
             }
             int nvis;
             const int vbase = wave_excl(nv, &nvis);
@@ -259,7 +282,7 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
             __syncthreads();
             // hole boundary: edges of freed triangles whose neighbour survives
             int nb = 0;
-#pragma unroll
+#pragma unroll 1
             for (int k = 0; k < KPL; k++) {
                 const int slot = lane + 64 * k;
                 if (slot < T && (L.tv[slot] & TF_VIS))
@@ -272,7 +295,7 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
                 break;
             }
             int fpos = nfree + vbase;
-#pragma unroll
+#pragma unroll 1
             for (int k = 0; k < KPL; k++) {
                 const int slot = lane + 64 * k;
                 if (slot < T && (L.tv[slot] & TF_VIS)) {
@@ -293,7 +316,7 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
             }
             __syncthreads();
             const int nfree_tot = nfree + nvis;
-#pragma unroll
+#pragma unroll 1
             for (int k = 0; k < KPL; k++) {
                 const int slot = lane + 64 * k;
                 if (slot < T && (L.tv[slot] & TF_VIS)) L.tv[slot] = 0u;
@@ -324,18 +347,24 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
             T = newT;
             V = Vn + 1;
             __syncthreads();
-#pragma unroll
+#pragma unroll 1
             for (int k = 0; k < KPL; k++) {
                 const int slot = lane + 64 * k;
                 if (slot < T) {
                     const uint32_t t = L.tv[slot];
                     if (t & TF_FRESH) {
-                        tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], C[k], P[k]);
+                        tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], L.cof[slot],
+                                      L.cof[slot] + 4);
                         L.tv[slot] = t & ~TF_FRESH;
                     }
                 }
             }
             __syncthreads();
+        }
+        if (unsure) {
+            star_list_push(p.exact_list, p.exact_cap, &cnt[p.exact_counter], s, cnt);
+            __syncthreads();
+            continue;
         }
         if (failed) {
             if (lane == 0) {
@@ -349,7 +378,7 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
         // ---- finalize: link set, drowned insertions, compaction, pairs, history ----
         for (int v = lane; v < V; v += 64) L.present[v] = 0;
         __syncthreads();
-#pragma unroll
+#pragma unroll 1
         for (int k = 0; k < KPL; k++) {
             const int slot = lane + 64 * k;
             if (slot < T) {
@@ -469,15 +498,18 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
     }
 }
 
-hipError_t launch_star_kernel(const StarParams &p, int tier, int nblocks, hipStream_t s)
+hipError_t launch_star_kernel(const StarParams &p, int tier, bool exact, int nblocks, hipStream_t s)
 {
     const int nitems = p.star_list ? p.nlist : p.n;
     if (nitems <= 0) return hipSuccess;
     if (p.lds_limit > STAR_LDS_LIMIT) return hipErrorInvalidValue; // local ids must fit 9 bits
-    if (tier == 1)
-        hipLaunchKernelGGL(k_star<STAR_VMAX_BIG>, dim3(nblocks), dim3(64), 0, s, p, nitems);
+    if (nblocks > nitems) nblocks = nitems;
+    if (exact)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_star<STAR_VMAX_BIG, true>), dim3(nblocks), dim3(64), 0, s, p, nitems);
+    else if (tier == 1)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_star<STAR_VMAX_BIG, false>), dim3(nblocks), dim3(64), 0, s, p, nitems);
     else
-        hipLaunchKernelGGL(k_star<STAR_VMAX_SMALL>, dim3(nblocks), dim3(64), 0, s, p, nitems);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_star<STAR_VMAX_SMALL, false>), dim3(nblocks), dim3(64), 0, s, p, nitems);
     return hipGetLastError();
 }
 
@@ -646,12 +678,13 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
         for (int slot = lane; slot < T; slot += 64)
             tri_cofactors(L.vc + 4 * L.tva[3 * slot], L.vc + 4 * L.tva[3 * slot + 1], L.vc + 4 * L.tva[3 * slot + 2],
                           L.cof + 8 * slot, L.cof + 8 * slot + 4);
-        bool failed = false;
+        bool failed = false, unsure = false;
         for (int q = q0; q < m; q++) {
             const int gv = L.ins[q];
             double r[4];
             rel_coords(p.pts, gv, sx, sy, sz, r);
             int nv = 0;
+            bool unc = false;
             for (int slot = lane; slot < T; slot += 64) {
                 const uint32_t f = L.tfl[slot];
                 if (!(f & TF_ALIVE)) continue;
@@ -747,6 +780,11 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
                 }
             }
             __syncthreads();
+        }
+        if (unsure) {
+            star_list_push(p.exact_list, p.exact_cap, &cnt[p.exact_counter], s, cnt);
+            __syncthreads();
+            continue;
         }
         if (failed) {
             if (lane == 0) {
@@ -909,10 +947,15 @@ hipError_t launch_need_max(const StarParams &p, const int *list, int nitems, uns
 // without exVert whose three edges give one orient4(s, v, exVert, edge) sign.
 // ------------------------------------------------------------------------------------
 
+// EXACT = false: filters only, over all tasks; a task with an inconclusive orientation goes to
+// p.xlist.  EXACT = true: the tasks listed in p.xlist, with exact predicates and SoS.
+template <bool EXACT>
 __global__ void __launch_bounds__(64) k_proofs(ProofParams p)
 {
     const int lane = threadIdx.x;
-    for (int64_t task = blockIdx.x; task < p.ntasks; task += gridDim.x) {
+    const int64_t nt = EXACT ? p.nx : p.ntasks;
+    for (int64_t it = blockIdx.x; it < nt; it += gridDim.x) {
+        const int64_t task = EXACT ? p.xlist[it] : it;
         const uint64_t tk = p.tasks[task];
         const int s = (int)(tk >> 32), v = (int)(uint32_t)tk;
         const int vn = p.vnum[s], tn = p.tnum[s];
@@ -922,6 +965,7 @@ __global__ void __launch_bounds__(64) k_proofs(ProofParams p)
         for (int i = lane; i < vn; i += 64) mn = min(mn, p.vpool[vo + i]);
         const int ex = g4d_wave_min(mn);
         int k0 = 0x7fffffff, k1 = 0x7fffffff, k2 = 0x7fffffff;
+        bool unc = false;
         for (int t = lane; t < tn; t += 64) {
             int lv[3];
             star_tri_verts(p.tpool, to, t, wide, lv);
@@ -934,8 +978,16 @@ __global__ void __launch_bounds__(64) k_proofs(ProofParams p)
                 const int pv = tri[i], tv2 = tri[(i + 1) % 3];
                 const double *pp[5] = {p.pts + 3 * (int64_t)s, p.pts + 3 * (int64_t)v, p.pts + 3 * (int64_t)ex,
                                        p.pts + 3 * (int64_t)pv, p.pts + 3 * (int64_t)tv2};
-                const int idx[5] = {s, v, ex, pv, tv2};
-                o[i] = orient4_full(pp, idx, p.cnt);
+                if (EXACT) {
+                    const int idx[5] = {s, v, ex, pv, tv2};
+                    o[i] = orient4_full(pp, idx, p.cnt);
+                } else {
+                    o[i] = filter_insphere(pp);
+                    if (o[i] == 0) {
+                        unc = true;
+                        ok = false;
+                    }
+                }
                 if (i > 0 && o[i] != o[0]) ok = false;
             }
             if (!ok) continue;
@@ -948,6 +1000,14 @@ __global__ void __launch_bounds__(64) k_proofs(ProofParams p)
                 k1 = x1;
                 k2 = x2;
             }
+        }
+        if (!EXACT && __ballot(unc)) {
+            if (lane == 0) {
+                const unsigned long long k = atomicAdd(&p.cnt[G4D_CNT_APPEND5], 1ull);
+                if ((int64_t)k < p.xcap) p.xlist[k] = task;
+                else atomicOr(&p.cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
+            }
+            continue;
         }
         const int b0 = g4d_wave_min(k0);
         const int b1 = g4d_wave_min(k0 == b0 ? k1 : 0x7fffffff);
@@ -968,11 +1028,17 @@ __global__ void __launch_bounds__(64) k_proofs(ProofParams p)
     }
 }
 
-hipError_t launch_proofs(const ProofParams &p, hipStream_t s)
+hipError_t launch_proofs(const ProofParams &p, bool exact, hipStream_t s)
 {
-    if (p.ntasks <= 0) return hipSuccess;
-    const int nb = (int)(p.ntasks < 65536 ? p.ntasks : 65536);
-    hipLaunchKernelGGL(k_proofs, dim3(nb), dim3(64), 0, s, p);
+    const int64_t nt = exact ? p.nx : p.ntasks;
+    if (nt <= 0) return hipSuccess;
+    if (exact) {
+        const int nb = (int)(nt < 1024 ? nt : 1024);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_proofs<true>), dim3(nb), dim3(64), 0, s, p);
+    } else {
+        const int nb = (int)(nt < 65536 ? nt : 65536);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_proofs<false>), dim3(nb), dim3(64), 0, s, p);
+    }
     return hipGetLastError();
 }
 
@@ -1093,6 +1159,51 @@ hipError_t launch_capacities(int n, const int *vnum, const int64_t *ins_off, con
 // orient3d(v0, v1, v2, star) is negative; upper-hull and flat ones are dropped.
 // ------------------------------------------------------------------------------------
 
+// Keep flag of an owner triangle decided by the exact orient3d (bit 58 of a packed word, bit
+// 63 of the second word of a wide one); clear as written by the star kernels.
+__device__ __forceinline__ uint64_t *out_keep_word(const OutParams &p, int64_t to, int t, bool wide)
+{
+    return p.tpool + to + (wide ? 2 * (int64_t)t + 1 : (int64_t)t);
+}
+__device__ __forceinline__ uint64_t out_keep_bit(bool wide) { return wide ? (1ull << 63) : (1ull << 58); }
+
+// Filter pass: owner triangles whose orient3d filter is inconclusive go to the list.
+__global__ void k_out_flag(OutParams p, uint64_t *list, int64_t cap)
+{
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < p.n; s += gridDim.x * blockDim.x) {
+        const int tn = p.tnum[s];
+        const bool wide = p.vnum[s] >= STAR_WIDE_MIN;
+        const int64_t vo = p.voff[s], to = p.toff[s];
+        for (int t = 0; t < tn; t++) {
+            int lv[3];
+            star_tri_verts(p.tpool, to, t, wide, lv);
+            const int a = p.vpool[vo + lv[0]], b = p.vpool[vo + lv[1]], cc = p.vpool[vo + lv[2]];
+            if (!(s < a && s < b && s < cc)) continue;
+            const double *pp[4] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b, p.pts + 3 * (int64_t)cc,
+                                   p.pts + 3 * (int64_t)s};
+            if (filter_orient3d(pp) != 0) continue;
+            const unsigned long long k = atomicAdd(&p.cnt[G4D_CNT_APPEND5], 1ull);
+            if ((int64_t)k < cap) list[k] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)t;
+        }
+    }
+}
+
+// Exact pass over the inconclusive owner triangles: mark those with orient3d < 0.
+__global__ void k_out_exact(OutParams p, const uint64_t *list, int64_t m)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const int s = (int)(list[i] >> 32), t = (int)(uint32_t)list[i];
+        const bool wide = p.vnum[s] >= STAR_WIDE_MIN;
+        const int64_t vo = p.voff[s], to = p.toff[s];
+        int lv[3];
+        star_tri_verts(p.tpool, to, t, wide, lv);
+        const int a = p.vpool[vo + lv[0]], b = p.vpool[vo + lv[1]], cc = p.vpool[vo + lv[2]];
+        const double *pp[4] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b, p.pts + 3 * (int64_t)cc,
+                               p.pts + 3 * (int64_t)s};
+        if (exact_orient3d(pp, p.cnt) < 0) *out_keep_word(p, to, t, wide) |= out_keep_bit(wide);
+    }
+}
+
 template <bool WRITE>
 __global__ void k_out(OutParams p)
 {
@@ -1109,18 +1220,33 @@ __global__ void k_out(OutParams p)
             if (!(s < a && s < b && s < cc)) continue;
             const double *pp[4] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b, p.pts + 3 * (int64_t)cc,
                                    p.pts + 3 * (int64_t)s};
-            if (orient3d_full(pp, p.cnt) >= 0) continue;
+            const int o = filter_orient3d(pp);
+            const bool keep = o < 0 || (o == 0 && (*out_keep_word(p, to, t, wide) & out_keep_bit(wide)));
+            if (!keep) continue;
             if (WRITE) {
-                int *o = p.tets + 4 * (k + c);
-                o[0] = a;
-                o[1] = b;
-                o[2] = cc;
-                o[3] = s;
+                int *out = p.tets + 4 * (k + c);
+                out[0] = a;
+                out[1] = b;
+                out[2] = cc;
+                out[3] = s;
             }
             c++;
         }
         if (!WRITE) p.star_tet_count[s] = c;
     }
+}
+
+hipError_t launch_out_flag(const OutParams &p, uint64_t *list, int64_t cap, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_out_flag, dim3(g4d_blocks(p.n, 64, 65536)), dim3(64), 0, s, p, list, cap);
+    return hipGetLastError();
+}
+
+hipError_t launch_out_exact(const OutParams &p, const uint64_t *list, int64_t m, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_out_exact, dim3(g4d_blocks(m, 64, 1024)), dim3(64), 0, s, p, list, m);
+    return hipGetLastError();
 }
 
 hipError_t launch_out_count(const OutParams &p, hipStream_t s)

@@ -84,6 +84,7 @@ typedef struct {
     int64_t num_tets;
     int64_t exact_calls;       /* orient4 filter failures resolved exactly */
     int64_t sos_calls;         /* exact zeros resolved by SoS */
+    int64_t exact_stars;       /* star passes handed to the exact-capable star kernel */
     int64_t pba_bytes;         /* algorithmic PBA bytes: 24 G^3 */
     double t_pba_pass_ms[3];   /* per axis pass (z, y, x) */
 } g4d_stats;
@@ -117,6 +118,9 @@ int g4d_diag_morton(g4d_ctx *ctx, const double *xyz, int32_t n, int32_t grid, in
 /* PBA of sites given as Morton-sorted voxel coordinates (3n); out: G^3 nearest-site point ids,
    index (z*G + y)*G + x.  codes != 0 returns site codes instead of ids. */
 int g4d_diag_pba(g4d_ctx *ctx, const int32_t *vox_sorted, int32_t n, int32_t grid, int32_t codes, int32_t *out);
+/* Plain device-to-device copy of `bytes` (rounded down to 16) between two buffers the context
+   allocates; *ms receives the kernel time.  Calibrates HBM traffic counters and bandwidth. */
+int g4d_diag_copy(g4d_ctx *ctx, int64_t bytes, double *ms);
 /* Sorted unique Voronoi-neighbour pairs (min<<32|max) from an id grid. */
 int g4d_diag_grid_pairs(g4d_ctx *ctx, const int32_t *id_grid, int32_t grid, uint64_t *out, int64_t cap,
                         int64_t *count);
