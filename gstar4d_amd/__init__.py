@@ -10,7 +10,7 @@ import os
 
 import numpy as np
 
-__all__ = ["Triangulator", "Options", "Stats", "G4DError", "load_library", "gdel_init",
+__all__ = ["Triangulator", "Options", "Stats", "CheckReport", "G4DError", "load_library", "gdel_init",
            "gdel_compute", "get_tetra", "gdel_deinit", "canonical", "LIB_PATH"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
@@ -63,6 +63,17 @@ class Stats(ctypes.Structure):
         return d
 
 
+class CheckReport(ctypes.Structure):
+    """g4d_check_report: TetraMesh::check (GD/Common/Geometry.cu:200-393) plus adjacency and
+    empty-sphere counts."""
+    _fields_ = [(f, ctypes.c_int64) for f in (
+        "num_tets", "num_vertices", "num_edges", "num_faces", "hull_faces", "euler", "orientation_failures",
+        "flat_tets", "insphere_failures", "adjacency_failures", "empty_sphere_failures", "empty_sphere_tests")]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 _LIB = None
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -73,7 +84,8 @@ _lp = ctypes.POINTER(ctypes.c_int64)
 EXPORTS = ["g4d_default_options", "g4d_create", "g4d_destroy", "g4d_last_error", "g4d_device_count",
            "g4d_triangulate", "g4d_triangulate_device", "g4d_copy_result", "g4d_upload_points",
            "g4d_get_stats", "g4d_set_options", "g4d_diag_orient4", "g4d_diag_orient3d",
-           "g4d_diag_morton", "g4d_diag_pba", "g4d_diag_grid_pairs", "g4d_diag_copy"]
+           "g4d_diag_morton", "g4d_diag_pba", "g4d_diag_grid_pairs", "g4d_diag_copy",
+           "g4d_check_result", "g4d_check_mesh"]
 
 
 def load_library(path=None):
@@ -103,6 +115,9 @@ def load_library(path=None):
     L.g4d_diag_pba.argtypes = [_vp, _ip, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _ip]
     L.g4d_diag_grid_pairs.argtypes = [_vp, _ip, ctypes.c_int32, _up, ctypes.c_int64, _lp]
     L.g4d_diag_copy.argtypes = [_vp, ctypes.c_int64, _dp]
+    L.g4d_check_result.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(CheckReport)]
+    L.g4d_check_mesh.argtypes = [_vp, _dp, ctypes.c_int32, _ip, _ip, ctypes.c_int64, ctypes.c_int32,
+                                 ctypes.POINTER(CheckReport)]
     _LIB = L
     return L
 
@@ -194,6 +209,23 @@ class Triangulator:
         self._check(self._L.g4d_copy_result(self._ctx, _ptr(tets, _ip),
                                             _ptr(opp, _ip) if want_opp else None, _ptr(perm, _ip)))
         return tets[:nt], (opp[:nt] if want_opp else None), perm
+
+    def check(self, full_empty_sphere=False):
+        """Validate the last result on the GPU; returns the report as a dict."""
+        rep = CheckReport()
+        self._check(self._L.g4d_check_result(self._ctx, int(full_empty_sphere), ctypes.byref(rep)))
+        return rep.as_dict()
+
+    def check_mesh(self, xyz, tets, opp=None, full_empty_sphere=True):
+        """Validate any tetrahedral mesh of the points xyz on the GPU."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        t = np.ascontiguousarray(tets, dtype=np.int32)
+        o = None if opp is None else np.ascontiguousarray(opp, dtype=np.int32)
+        rep = CheckReport()
+        self._check(self._L.g4d_check_mesh(self._ctx, _ptr(xyz, _dp), xyz.shape[0], _ptr(t, _ip),
+                                           None if o is None else _ptr(o, _ip), t.shape[0],
+                                           int(full_empty_sphere), ctypes.byref(rep)))
+        return rep.as_dict()
 
     def stats(self):
         st = Stats()

@@ -12,11 +12,13 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
+#include <ctime>
 #include <cstring>
 #include <string>
 #include <vector>
 
 #include "gstar4d.h"
+#include "g4d_check.h"
 #include "g4d_diag.h"
 #include "g4d_grid.h"
 #include "g4d_prims.h"
@@ -58,6 +60,7 @@ struct g4d_ctx {
     DevBuf drown, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, big, huge, xstars, xtasks, xtris, gwork,
         tetcnt, tetoff, tets, opp, table;
     std::vector<int32_t> h_tets, h_opp, h_perm;
+    const double *last_xyz = nullptr; // device points of the last triangulation (input order)
     int64_t num_tets = 0;
     int last_n = 0;
     bool have_result = false;
@@ -76,9 +79,15 @@ static bool trace_on()
 static void trace(const char *fmt, ...)
 {
     if (!trace_on()) return;
+    // wall time since the previous trace line (the traced stages end in a host synchronisation)
+    static double last = 0.0;
+    timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    const double now = ts.tv_sec * 1e3 + ts.tv_nsec * 1e-6;
     va_list ap;
     va_start(ap, fmt);
-    fprintf(stderr, "[g4d] ");
+    fprintf(stderr, "[g4d +%.1f ms] ", last > 0.0 ? now - last : 0.0);
+    last = now;
     vfprintf(stderr, fmt, ap);
     fprintf(stderr, "\n");
     fflush(stderr);
@@ -187,6 +196,7 @@ static double ev_ms(g4d_ctx *ctx, int a, int b)
 // The whole path on device-resident FP64 points.
 static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
 {
+    ctx->last_xyz = d_xyz;
     hipStream_t s = ctx->stream;
     g4d::Prims &P = ctx->P;
     const int G = ctx->opt.grid_size;
@@ -846,6 +856,144 @@ int g4d_diag_pba(g4d_ctx *ctx, const int32_t *vox_sorted, int32_t n, int32_t gri
     HCK(hipMemcpyAsync(out, ctx->gtmp.ptr, sizeof(int) * (size_t)G3, hipMemcpyDeviceToHost, s));
     HCK(hipStreamSynchronize(s));
     return G4D_OK;
+}
+
+} // extern "C"
+
+// Validation of a device-resident mesh; see g4d_check_report.
+static int check_device(g4d_ctx *ctx, const double *pts, int n, const int *tets, const int *opp, int64_t nt,
+                        bool full, g4d_check_report *rep)
+{
+    hipStream_t s = ctx->stream;
+    g4d::Prims &P = ctx->P;
+    memset(rep, 0, sizeof(*rep));
+    rep->num_tets = nt;
+    rep->empty_sphere_failures = -1;
+    if (nt <= 0) return G4D_OK;
+    HCK(ctx->cnt.ensure(sizeof(unsigned long long) * G4D_CNT_NUM));
+    unsigned long long *cnt = buf<unsigned long long>(ctx->cnt);
+    HCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * G4D_CNT_NUM, s));
+    DevBuf acc_b, used_b, ref_b, table_b, keys_b;
+    HCK(acc_b.ensure(sizeof(unsigned long long) * g4d::CHECK_NUM));
+    unsigned long long *acc = buf<unsigned long long>(acc_b);
+    HCK(hipMemsetAsync(acc, 0, sizeof(unsigned long long) * g4d::CHECK_NUM, s));
+    // independent face matching
+    const uint64_t tsz = next_pow2((uint64_t)std::max<int64_t>(8 * nt, 1024));
+    HCK(table_b.ensure(sizeof(uint32_t) * (size_t)tsz));
+    HCK(ref_b.ensure(sizeof(int) * 4 * (size_t)nt));
+    HCK(g4d::launch_adjacency(tets, nt, buf<int>(ref_b), buf<uint32_t>(table_b), (int64_t)tsz, cnt, s));
+    HCK(used_b.ensure((size_t)n));
+    HCK(hipMemsetAsync(used_b.ptr, 0, (size_t)n, s));
+    HCK(g4d::launch_check_tets(pts, tets, opp, buf<int>(ref_b), nt, n, buf<unsigned char>(used_b), acc, cnt, s));
+    HCK(g4d::launch_check_count_used(buf<unsigned char>(used_b), n, acc, s));
+    // distinct edges
+    HCK(keys_b.ensure(sizeof(uint64_t) * 6 * (size_t)nt));
+    uint64_t *keys = buf<uint64_t>(keys_b);
+    HCK(g4d::launch_check_edges(tets, nt, keys, s));
+    const int nb = bits_for(n);
+    int64_t ne = 6 * nt;
+    HCK(sort_pairs(P, keys, ne, nb, s));
+    HCK(g4d::unique_u64(P, keys, ne, &ne, s));
+    unsigned long long h[g4d::CHECK_NUM];
+    if (full) {
+        // cell grid over the bounding box
+        HCK(ctx->bounds.ensure(sizeof(double) * 768));
+        HCK(g4d::launch_bounds(pts, n, buf<double>(ctx->bounds), s));
+        double hb[2];
+        HCK(hipMemcpyAsync(hb, ctx->bounds.ptr, sizeof(hb), hipMemcpyDeviceToHost, s));
+        HCK(hipStreamSynchronize(s));
+        const double box[6] = {hb[0], hb[0], hb[0], hb[1], hb[1], hb[1]};
+        int R = (int)std::cbrt((double)n / 2.0);
+        R = std::max(1, std::min(R, 256));
+        const int64_t ncell = (int64_t)R * R * R;
+        DevBuf box_b, cid_b, ccnt_b, start_b, fill_b, items_b;
+        HCK(box_b.ensure(sizeof(box)));
+        HCK(hipMemcpyAsync(box_b.ptr, box, sizeof(box), hipMemcpyHostToDevice, s));
+        HCK(cid_b.ensure(sizeof(int) * (size_t)n));
+        HCK(ccnt_b.ensure(sizeof(int64_t) * (size_t)ncell));
+        HCK(start_b.ensure(sizeof(int64_t) * (size_t)(ncell + 1)));
+        HCK(fill_b.ensure(sizeof(int) * (size_t)ncell));
+        HCK(items_b.ensure(sizeof(int) * (size_t)n));
+        HCK(hipMemsetAsync(ccnt_b.ptr, 0, sizeof(int64_t) * (size_t)ncell, s));
+        HCK(hipMemsetAsync(fill_b.ptr, 0, sizeof(int) * (size_t)ncell, s));
+        HCK(g4d::launch_check_cells(pts, n, buf<double>(box_b), R, buf<int>(cid_b), buf<unsigned long long>(ccnt_b),
+                                    s));
+        int64_t *start = buf<int64_t>(start_b);
+        HCK(g4d::scan_excl_i64(P, buf<int64_t>(ccnt_b), start, ncell, start + ncell, s));
+        HCK(g4d::launch_check_cell_fill(buf<int>(cid_b), n, start, buf<int>(fill_b), buf<int>(items_b), s));
+        HCK(g4d::launch_check_empty(pts, tets, nt, buf<double>(box_b), R, start, buf<int>(items_b), acc, cnt, s));
+        HCK(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, s));
+        HCK(hipStreamSynchronize(s));
+        box_b.release();
+        cid_b.release();
+        ccnt_b.release();
+        start_b.release();
+        fill_b.release();
+        items_b.release();
+    } else {
+        HCK(hipMemcpyAsync(h, acc, sizeof(h), hipMemcpyDeviceToHost, s));
+        HCK(hipStreamSynchronize(s));
+    }
+    unsigned long long hc[G4D_CNT_NUM];
+    CK(read_counters(ctx, hc));
+    acc_b.release();
+    used_b.release();
+    ref_b.release();
+    table_b.release();
+    keys_b.release();
+    const unsigned long long other = hc[G4D_CNT_ERROR] & ~(unsigned long long)G4D_DEV_ERR_FACE;
+    if (other) CK(check_dev_error(ctx, other, "check"));
+    rep->num_vertices = (int64_t)h[g4d::CHECK_VERTS];
+    rep->num_edges = ne;
+    rep->hull_faces = (int64_t)h[g4d::CHECK_HULL];
+    rep->num_faces = rep->hull_faces + (int64_t)h[g4d::CHECK_SHARED] / 2;
+    rep->euler = rep->num_vertices - rep->num_edges + rep->num_faces - nt;
+    rep->orientation_failures = (int64_t)h[g4d::CHECK_ORIENT];
+    rep->flat_tets = (int64_t)h[g4d::CHECK_FLAT];
+    rep->insphere_failures = (int64_t)h[g4d::CHECK_INSPHERE];
+    rep->adjacency_failures = (int64_t)h[g4d::CHECK_ADJ] + ((hc[G4D_CNT_ERROR] & G4D_DEV_ERR_FACE) ? 1 : 0);
+    if (full) {
+        rep->empty_sphere_failures = (int64_t)h[g4d::CHECK_EMPTY];
+        rep->empty_sphere_tests = (int64_t)h[g4d::CHECK_EMPTY_TESTS];
+    }
+    return G4D_OK;
+}
+
+extern "C" {
+
+int g4d_check_result(g4d_ctx *ctx, int32_t full_empty_sphere, g4d_check_report *rep)
+{
+    if (!ctx || !rep || !ctx->have_result) return G4D_ERR_INVALID_INPUT;
+    HCK(hipSetDevice(ctx->opt.device));
+    const double *pts = ctx->opt.index_space == G4D_INDEX_MORTON ? buf<double>(ctx->pts) : ctx->last_xyz;
+    const int *opp = ctx->opt.compute_adjacency ? buf<int>(ctx->opp) : nullptr;
+    return check_device(ctx, pts, ctx->last_n, buf<int>(ctx->tets), opp, ctx->num_tets, full_empty_sphere != 0, rep);
+}
+
+int g4d_check_mesh(g4d_ctx *ctx, const double *xyz, int32_t n, const int32_t *tets, const int32_t *opp,
+                   int64_t num_tets, int32_t full_empty_sphere, g4d_check_report *rep)
+{
+    if (!ctx || !xyz || n <= 0 || !rep || num_tets < 0 || (num_tets > 0 && !tets)) return G4D_ERR_INVALID_INPUT;
+    HCK(hipSetDevice(ctx->opt.device));
+    hipStream_t s = ctx->stream;
+    DevBuf dp, dt, dop;
+    HCK(dp.ensure(sizeof(double) * 3 * (size_t)n));
+    HCK(hipMemcpyAsync(dp.ptr, xyz, sizeof(double) * 3 * (size_t)n, hipMemcpyHostToDevice, s));
+    if (num_tets > 0) {
+        HCK(dt.ensure(sizeof(int) * 4 * (size_t)num_tets));
+        HCK(hipMemcpyAsync(dt.ptr, tets, sizeof(int) * 4 * (size_t)num_tets, hipMemcpyHostToDevice, s));
+        if (opp) {
+            HCK(dop.ensure(sizeof(int) * 4 * (size_t)num_tets));
+            HCK(hipMemcpyAsync(dop.ptr, opp, sizeof(int) * 4 * (size_t)num_tets, hipMemcpyHostToDevice, s));
+        }
+    }
+    HCK(hipStreamSynchronize(s));
+    const int r = check_device(ctx, buf<double>(dp), n, buf<int>(dt), opp ? buf<int>(dop) : nullptr, num_tets,
+                               full_empty_sphere != 0, rep);
+    dp.release();
+    dt.release();
+    dop.release();
+    return r;
 }
 
 int g4d_diag_copy(g4d_ctx *ctx, int64_t bytes, double *ms)

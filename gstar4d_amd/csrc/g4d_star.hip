@@ -520,6 +520,27 @@ hipError_t launch_star_kernel(const StarParams &p, int tier, bool exact, int nbl
 // workspace sized for the largest such star of the pass.
 // ------------------------------------------------------------------------------------
 
+static constexpr int GSTAR_NT = 512; // threads per star in the global-memory kernel
+
+// Exclusive prefix sum of v over the work-group's threads in thread order; total in *tot.
+// red: GSTAR_NT / 64 ints of LDS.  Every thread of the work-group must call it.
+__device__ __forceinline__ int block_excl(int v, int *tot, int *red)
+{
+    const int lane = __lane_id(), w = threadIdx.x >> 6;
+    const int inc = g4d_wave_incl_sum(v);
+    if (lane == 63) red[w] = inc;
+    __syncthreads();
+    int base = 0, all = 0;
+    for (int i = 0; i < GSTAR_NT / 64; i++) {
+        const int x = red[i];
+        if (i < w) base += x;
+        all += x;
+    }
+    __syncthreads();
+    *tot = all;
+    return base + inc - v;
+}
+
 struct GWork {
     int *vid, *vstart, *vend, *ins, *ins_local, *vremap;
     double *vc;   // 4 per vertex
@@ -566,10 +587,11 @@ __host__ __device__ inline int64_t gwork_layout(char *base, int vcap, GWork *w)
 
 int64_t star_global_bytes(int vcap) { return gwork_layout(nullptr, vcap, nullptr); }
 
-__global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *list, int nitems, char *ws,
-                                                    int64_t ws_stride, int vcap)
+__global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const int *list, int nitems, char *ws,
+                                                          int64_t ws_stride, int vcap)
 {
-    const int lane = threadIdx.x;
+    __shared__ int red[GSTAR_NT / 64];
+    const int tid = threadIdx.x;
     unsigned long long *cnt = p.cnt;
     GWork L;
     gwork_layout(ws + (int64_t)blockIdx.x * ws_stride, vcap, &L);
@@ -582,7 +604,7 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
         const int vn = p.mode ? p.in_vnum[s] : 0;
         const bool in_wide = vn >= STAR_WIDE_MIN;
         if (vn + m > vcap || (p.mode == 0 && m < 4)) {
-            if (lane == 0) {
+            if (tid == 0) {
                 atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)(vn + m > vcap ? G4D_DEV_ERR_STAR_CAPACITY
                                                                                   : G4D_DEV_ERR_CONE));
                 p.out_vnum[s] = 0;
@@ -595,18 +617,18 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
             const int tn = p.in_tnum[s];
             const int64_t words = in_wide ? 2 * (int64_t)tn : tn;
             const int64_t vi = p.in_voff[s], vo = p.out_voff[s], ti = p.in_toff[s], to = p.out_toff[s];
-            for (int i = lane; i < vn; i += 64) p.out_vpool[vo + i] = p.in_vpool[vi + i];
-            for (int64_t i = lane; i < words; i += 64) p.out_tpool[to + i] = p.in_tpool[ti + i];
+            for (int i = tid; i < vn; i += GSTAR_NT) p.out_vpool[vo + i] = p.in_vpool[vi + i];
+            for (int64_t i = tid; i < words; i += GSTAR_NT) p.out_tpool[to + i] = p.in_tpool[ti + i];
             const int64_t hb = p.hist_off[s], he = p.hist_off[s + 1], ho = p.hist_out_off[s];
-            for (int64_t i = lane; i < he - hb; i += 64) p.hist_out[ho + i] = p.hist_keys[hb + i];
-            if (lane == 0) {
+            for (int64_t i = tid; i < he - hb; i += GSTAR_NT) p.hist_out[ho + i] = p.hist_keys[hb + i];
+            if (tid == 0) {
                 p.out_vnum[s] = vn;
                 p.out_tnum[s] = tn;
             }
             continue;
         }
         const double sx = p.pts[3 * (int64_t)s], sy = p.pts[3 * (int64_t)s + 1], sz = p.pts[3 * (int64_t)s + 2];
-        for (int q = lane; q < m; q += 64) {
+        for (int q = tid; q < m; q += GSTAR_NT) {
             L.ins[q] = (int)(uint32_t)p.ins_keys[ib + q];
             L.ins_local[q] = -1;
         }
@@ -614,12 +636,12 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
         if (p.mode == 1) {
             const int tn = p.in_tnum[s];
             const int64_t vi = p.in_voff[s], ti = p.in_toff[s];
-            for (int v = lane; v < vn; v += 64) {
+            for (int v = tid; v < vn; v += GSTAR_NT) {
                 const int g = p.in_vpool[vi + v];
                 L.vid[v] = g;
                 rel_coords(p.pts, g, sx, sy, sz, L.vc + 4 * v);
             }
-            for (int t = lane; t < tn; t += 64) {
+            for (int t = tid; t < tn; t += GSTAR_NT) {
                 int a[3], o[3];
                 bool chk;
                 if (in_wide) {
@@ -659,23 +681,23 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
                 l[0] = l[1];
                 l[1] = t;
             }
-            if (lane < 4) {
-                L.vid[lane] = l[lane];
-                rel_coords(p.pts, l[lane], sx, sy, sz, L.vc + 4 * lane);
+            if (tid < 4) {
+                L.vid[tid] = l[tid];
+                rel_coords(p.pts, l[tid], sx, sy, sz, L.vc + 4 * tid);
                 const int lt[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
                 for (int i = 0; i < 3; i++) {
-                    L.tva[3 * lane + i] = lt[lane][i];
-                    L.to[3 * lane + i] = lt[lane][i];
+                    L.tva[3 * tid + i] = lt[tid][i];
+                    L.to[3 * tid + i] = lt[tid][i];
                 }
-                L.tfl[lane] = TF_ALIVE;
-                L.ins_local[lane] = (lane < 2 && sw) ? 1 - lane : lane;
+                L.tfl[tid] = TF_ALIVE;
+                L.ins_local[tid] = (tid < 2 && sw) ? 1 - tid : tid;
             }
             T = 4;
             V = 4;
             q0 = 4;
         }
         __syncthreads();
-        for (int slot = lane; slot < T; slot += 64)
+        for (int slot = tid; slot < T; slot += GSTAR_NT)
             tri_cofactors(L.vc + 4 * L.tva[3 * slot], L.vc + 4 * L.tva[3 * slot + 1], L.vc + 4 * L.tva[3 * slot + 2],
                           L.cof + 8 * slot, L.cof + 8 * slot + 4);
         bool failed = false, unsure = false;
@@ -685,7 +707,7 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
             rel_coords(p.pts, gv, sx, sy, sz, r);
             int nv = 0;
             bool unc = false;
-            for (int slot = lane; slot < T; slot += 64) {
+            for (int slot = tid; slot < T; slot += GSTAR_NT) {
                 const uint32_t f = L.tfl[slot];
                 if (!(f & TF_ALIVE)) continue;
                 int sg = cofactor_filter_sign(r, L.cof + 8 * slot, L.cof + 8 * slot + 4);
@@ -702,10 +724,10 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
                 }
             }
             int nvis;
-            const int vbase = wave_excl(nv, &nvis);
+            const int vbase = block_excl(nv, &nvis, red);
             if (nvis == 0) continue; // drowned at insertion
             const int Vn = V;
-            if (lane == 0) {
+            if (tid == 0) {
                 L.vid[Vn] = gv;
                 for (int i = 0; i < 4; i++) L.vc[4 * Vn + i] = r[i];
                 L.ins_local[q] = Vn;
@@ -713,17 +735,17 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
             __syncthreads();
             // hole boundary: edges of freed triangles whose neighbour survives
             int nb = 0;
-            for (int slot = lane; slot < T; slot += 64)
+            for (int slot = tid; slot < T; slot += GSTAR_NT)
                 if (L.tfl[slot] & TF_VIS)
                     for (int i = 0; i < 3; i++) nb += (L.tfl[L.to[3 * slot + i]] & TF_VIS) ? 0 : 1;
             int nbtot;
-            int e = wave_excl(nb, &nbtot);
+            int e = block_excl(nb, &nbtot, red);
             if (nbtot > TMAX || nbtot < 3) {
                 failed = true;
                 break;
             }
             int fpos = nfree + vbase;
-            for (int slot = lane; slot < T; slot += 64) {
+            for (int slot = tid; slot < T; slot += GSTAR_NT) {
                 if (!(L.tfl[slot] & TF_VIS)) continue;
                 for (int i = 0; i < 3; i++) {
                     const int u = L.to[3 * slot + i];
@@ -740,9 +762,9 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
             }
             __syncthreads();
             const int nfree_tot = nfree + nvis;
-            for (int slot = lane; slot < T; slot += 64)
+            for (int slot = tid; slot < T; slot += GSTAR_NT)
                 if (L.tfl[slot] & TF_VIS) L.tfl[slot] = 0u;
-            for (int k = lane; k < nbtot; k += 64)
+            for (int k = tid; k < nbtot; k += GSTAR_NT)
                 L.be_slot[k] = k < nfree_tot ? L.freel[nfree_tot - 1 - k] : T + (k - nfree_tot);
             const int newT = T + (nbtot > nfree_tot ? nbtot - nfree_tot : 0);
             if (newT > TMAX || Vn + 1 > vcap) {
@@ -750,7 +772,7 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
                 break;
             }
             __syncthreads();
-            for (int k = lane; k < nbtot; k += 64) {
+            for (int k = tid; k < nbtot; k += GSTAR_NT) {
                 const int slot = L.be_slot[k], a = L.be_a[k], b = L.be_b[k], u = L.be_u[k];
                 L.tva[3 * slot] = a;
                 L.tva[3 * slot + 1] = b;
@@ -762,7 +784,7 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
                 L.vend[b] = slot;
             }
             __syncthreads();
-            for (int k = lane; k < nbtot; k += 64) {
+            for (int k = tid; k < nbtot; k += GSTAR_NT) {
                 const int slot = L.be_slot[k];
                 L.to[3 * slot] = L.vstart[L.be_b[k]];
                 L.to[3 * slot + 1] = L.vend[L.be_a[k]];
@@ -771,7 +793,7 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
             T = newT;
             V = Vn + 1;
             __syncthreads();
-            for (int slot = lane; slot < T; slot += 64) {
+            for (int slot = tid; slot < T; slot += GSTAR_NT) {
                 const uint32_t f = L.tfl[slot];
                 if (f & TF_FRESH) {
                     tri_cofactors(L.vc + 4 * L.tva[3 * slot], L.vc + 4 * L.tva[3 * slot + 1],
@@ -787,7 +809,7 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
             continue;
         }
         if (failed) {
-            if (lane == 0) {
+            if (tid == 0) {
                 atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
                 p.out_vnum[s] = 0;
                 p.out_tnum[s] = 0;
@@ -796,14 +818,14 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
             continue;
         }
         // ---- finalize: link set, drowned insertions, compaction, pairs, history ----
-        for (int v = lane; v < V; v += 64) L.present[v] = 0;
+        for (int v = tid; v < V; v += GSTAR_NT) L.present[v] = 0;
         __syncthreads();
-        for (int slot = lane; slot < T; slot += 64)
+        for (int slot = tid; slot < T; slot += GSTAR_NT)
             if (L.tfl[slot] & TF_ALIVE)
                 for (int i = 0; i < 3; i++) L.present[L.tva[3 * slot + i]] = 1;
         __syncthreads();
-        for (int base = 0; base < m; base += 64) {
-            const int q = base + lane;
+        for (int base = 0; base < m; base += GSTAR_NT) {
+            const int q = base + tid;
             bool dr = false;
             if (q < m) {
                 const int lv = L.ins_local[q];
@@ -817,11 +839,11 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
         }
         const int64_t vo = p.out_voff[s], to = p.out_toff[s];
         int vcount = 0;
-        for (int base = 0; base < V; base += 64) {
-            const int v = base + lane;
+        for (int base = 0; base < V; base += GSTAR_NT) {
+            const int v = base + tid;
             const bool pr = v < V && L.present[v];
             int tot;
-            const int rk = g4d_wave_excl_count(pr, &tot);
+            const int rk = block_excl(pr ? 1 : 0, &tot, red);
             if (pr) {
                 L.vremap[v] = vcount + rk;
                 p.out_vpool[vo + vcount + rk] = L.vid[v];
@@ -829,18 +851,18 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
             vcount += tot;
         }
         int tcount = 0;
-        for (int base = 0; base < T; base += 64) {
-            const int t = base + lane;
+        for (int base = 0; base < T; base += GSTAR_NT) {
+            const int t = base + tid;
             const bool al = t < T && (L.tfl[t] & TF_ALIVE);
             int tot;
-            const int rk = g4d_wave_excl_count(al, &tot);
+            const int rk = block_excl(al ? 1 : 0, &tot, red);
             if (al) L.remap[t] = tcount + rk;
             tcount += tot;
         }
         const bool wide = vcount >= STAR_WIDE_MIN;
         const int64_t words = wide ? 2 * (int64_t)tcount : tcount;
         if (words > p.out_toff[s + 1] - to || tcount != 2 * vcount - 4) {
-            if (lane == 0) {
+            if (tid == 0) {
                 atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
                 p.out_vnum[s] = 0;
                 p.out_tnum[s] = 0;
@@ -849,8 +871,8 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
             continue;
         }
         __syncthreads();
-        for (int base = 0; base < T; base += 64) {
-            const int t = base + lane;
+        for (int base = 0; base < T; base += GSTAR_NT) {
+            const int t = base + tid;
             const bool al = t < T && (L.tfl[t] & TF_ALIVE);
             int a = 0, b = 0, c = 0;
             uint32_t f = 0;
@@ -885,14 +907,14 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
                 }
             }
         }
-        if (lane == 0) {
+        if (tid == 0) {
             p.out_vnum[s] = vcount;
             p.out_tnum[s] = tcount;
         }
         if (p.mode == 1) {
             const int64_t hb = p.hist_off[s], he = p.hist_off[s + 1], ho = p.hist_out_off[s];
             const int hn = (int)(he - hb);
-            for (int i = lane; i < hn; i += 64) {
+            for (int i = tid; i < hn; i += GSTAR_NT) {
                 const int x = (int)(uint32_t)p.hist_keys[hb + i];
                 int lo = 0, hi = m;
                 while (lo < hi) {
@@ -902,7 +924,7 @@ __global__ void __launch_bounds__(64) k_star_global(StarParams p, const int *lis
                 }
                 p.hist_out[ho + i + lo] = p.hist_keys[hb + i];
             }
-            for (int i = lane; i < m; i += 64) {
+            for (int i = tid; i < m; i += GSTAR_NT) {
                 const int x = L.ins[i];
                 int64_t lo = hb, hi = he;
                 while (lo < hi) {
@@ -921,7 +943,7 @@ hipError_t launch_star_global(const StarParams &p, const int *list, int nitems, 
                               int nblocks, hipStream_t s)
 {
     if (nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_star_global, dim3(nblocks), dim3(64), 0, s, p, list, nitems, ws, ws_stride, vcap);
+    hipLaunchKernelGGL(k_star_global, dim3(nblocks), dim3(GSTAR_NT), 0, s, p, list, nitems, ws, ws_stride, vcap);
     return hipGetLastError();
 }
 

@@ -36,5 +36,48 @@ def main(argv=None):
     return 0
 
 
+def _csv_rows(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _find(root, suffix):
+    for dp, _, files in os.walk(root):
+        for fn in sorted(files):
+            if fn.endswith(suffix):
+                return os.path.join(dp, fn)
+    return None
+
+
+def summarize_trace(prof_dir, command, top=25):
+    """Text table of rocprofv3 --kernel-trace --stats output (kernel_stats.csv)."""
+    path = _find(prof_dir, "kernel_stats.csv")
+    rows = _csv_rows(path)
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    out = ["command: " + command, "source: " + os.path.basename(path), "",
+           "%-58s %7s %12s %12s %6s" % ("kernel", "calls", "avg_us", "total_ms", "pct")]
+    for r in rows[:top]:
+        name = r["Name"].split("(")[0][:58]
+        out.append("%-58s %7d %12.2f %12.3f %6.2f" % (name, int(r["Calls"]), float(r["AverageNs"]) / 1e3,
+                                                      float(r["TotalDurationNs"]) / 1e6,
+                                                      100.0 * float(r["TotalDurationNs"]) / total))
+    out.append("%-58s %7s %12s %12.3f" % ("all kernels", "", "", total / 1e6))
+    return "\n".join(out) + "\n"
+
+
+def pmc_per_kernel(prof_dir, counter):
+    """Mean counter value per dispatch, by kernel name, from a --pmc run."""
+    path = _find(prof_dir, "counter_collection.csv")
+    acc = {}
+    for r in _csv_rows(path):
+        if r.get("Counter_Name") != counter:
+            continue
+        name = r["Kernel_Name"].split("(")[0]
+        acc.setdefault(name, []).append(float(r["Counter_Value"]))
+    return {k: sum(v) / len(v) for k, v in acc.items()}
+
+
 if __name__ == "__main__":
     sys.exit(main())

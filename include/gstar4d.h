@@ -89,6 +89,21 @@ typedef struct {
     double t_pba_pass_ms[3];   /* per axis pass (z, y, x) */
 } g4d_stats;
 
+/* Mesh validation report (GDelaunay/Common/Geometry.cu:200-393 plus adjacency and a full
+   empty-sphere test).  A Delaunay tetrahedralisation of points in general position has
+   euler == 1 and every failure count 0. */
+typedef struct {
+    int64_t num_tets, num_vertices, num_edges, num_faces, hull_faces;
+    int64_t euler;                 /* V - E + F - T over distinct vertices, edges, faces */
+    int64_t orientation_failures;  /* exact orient3d(v0, v1, v2, v3) > 0 */
+    int64_t flat_tets;             /* exact orient3d == 0 */
+    int64_t insphere_failures;     /* adjacent pairs: the opposite vertex strictly inside */
+    int64_t adjacency_failures;    /* opp entries that differ from face matching, bad ids,
+                                      faces shared by more than two tetrahedra */
+    int64_t empty_sphere_failures; /* (point, tetrahedron) pairs strictly inside; -1: not run */
+    int64_t empty_sphere_tests;    /* exact in-sphere tests of that check */
+} g4d_check_report;
+
 int g4d_default_options(g4d_options *opt);
 int g4d_create(g4d_ctx **ctx, const g4d_options *opt);
 void g4d_destroy(g4d_ctx *ctx);
@@ -105,6 +120,13 @@ int g4d_copy_result(g4d_ctx *ctx, int32_t *tets, int32_t *opp, int32_t *perm);
 int g4d_upload_points(g4d_ctx *ctx, const double *xyz, int32_t n, const double **d_xyz);
 int g4d_get_stats(const g4d_ctx *ctx, g4d_stats *st);
 int g4d_set_options(g4d_ctx *ctx, const g4d_options *opt);
+
+/* Validate the last result of ctx on the GPU (TetraMesh::check, GD/Common/Geometry.cu:200-393);
+   full_empty_sphere != 0 adds the empty-sphere test of every point near each tetrahedron. */
+int g4d_check_result(g4d_ctx *ctx, int32_t full_empty_sphere, g4d_check_report *rep);
+/* Validate a caller's mesh: points (n x 3), tets and opp (T x 4; opp may be NULL). */
+int g4d_check_mesh(g4d_ctx *ctx, const double *xyz, int32_t n, const int32_t *tets, const int32_t *opp,
+                   int64_t num_tets, int32_t full_empty_sphere, g4d_check_report *rep);
 
 /* ---- Kernel-level diagnostics (used by parity tests; no reference counterpart) ---- */
 /* orient4 of T tuples (15 doubles, 5 indices each).  mode 0: filter + exact + SoS;

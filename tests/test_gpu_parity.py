@@ -215,6 +215,85 @@ def test_star_beyond_lds_capacity(orc):
     assert np.bincount(tets.ravel()).max() == tets.shape[0]  # every tetrahedron uses the centre
 
 
+# ---------------------------------------------------------------- validator
+
+def _orc_counts(orc, x, tets):
+    flags, c = orc.validate(x, tets, full_empty=True)
+    return c
+
+
+@pytest.mark.parametrize("index_space", ["input", "morton"])
+def test_validator_on_results(orc, index_space):
+    x = datasets.hard_mix(6000, seed=7)
+    with g.Triangulator(grid_size=64, index_space=index_space) as t:
+        tets, opp, perm = t.triangulate(x)
+        rep = t.check(full_empty_sphere=True)
+    pts = x if index_space == "input" else x[perm]
+    ref = _orc_counts(orc, pts, tets)
+    assert rep["num_tets"] == tets.shape[0]
+    assert rep["euler"] == ref["euler"] == 1
+    assert rep["orientation_failures"] == 0 and rep["flat_tets"] == ref["flat"] == 0
+    assert rep["insphere_failures"] == ref["adjacent_insphere"] == 0
+    assert rep["adjacency_failures"] == 0
+    assert rep["empty_sphere_failures"] == ref["empty_sphere"] == 0
+    assert rep["empty_sphere_tests"] > 0
+
+
+def test_validator_finds_planted_defects(orc):
+    """Defects planted in a mesh on the host (the GPU code runs unchanged): counts match the
+    oracle's validator and the reference's checks fail."""
+    x = datasets.uniform_cube(3000, seed=17)
+    with g.Triangulator(grid_size=32) as t:
+        tets, opp, _ = t.triangulate(x)
+        good = t.check_mesh(x, tets, opp)
+        assert good["euler"] == 1 and good["empty_sphere_failures"] == 0 and good["adjacency_failures"] == 0
+        bad = tets.copy()
+        bad[5, [0, 1]] = bad[5, [1, 0]]            # flipped orientation
+        rep = t.check_mesh(x, bad, opp)
+        assert rep["orientation_failures"] == 1
+        fewer = np.delete(tets, [11, 12, 13], axis=0)  # holes: Euler and the empty sphere break
+        rep = t.check_mesh(x, fewer, None)
+        ref = _orc_counts(orc, x, fewer)
+        assert rep["euler"] == ref["euler"] != 1
+        wrong = opp.copy()
+        wrong[7, 0] = -1 if wrong[7, 0] >= 0 else 0
+        rep = t.check_mesh(x, tets, wrong)
+        assert rep["adjacency_failures"] >= 1
+        # a valid but non-Delaunay mesh: one 2-3 flip of an interior face
+        flipped = _flip23(orc, x, tets, opp)
+        rep = t.check_mesh(x, flipped, None)
+        ref = _orc_counts(orc, x, flipped)
+        assert rep["euler"] == 1 and rep["orientation_failures"] == 0 and rep["adjacency_failures"] == 0
+        assert rep["insphere_failures"] > 0 and ref["adjacent_insphere"] > 0
+        assert rep["empty_sphere_failures"] > 0 and ref["empty_sphere"] > 0
+
+
+def _flip23(orc, x, tets, opp):
+    """Replace two tetrahedra sharing face abc by (a,b,d,e), (b,c,d,e), (c,a,d,e) where the
+    segment de crosses abc (exact orientations), each oriented with orient3d < 0."""
+    for t1 in range(tets.shape[0]):
+        for i in range(4):
+            t2 = opp[t1, i]
+            if t2 < 0:
+                continue
+            d = tets[t1, i]
+            abc = [v for v in tets[t1] if v != d]
+            e = [v for v in tets[t2] if v not in tets[t1]][0]
+            a, b, c = abc
+            o = [orc.orient3d(*x[[p, q, d, e]]) for p, q in ((a, b), (b, c), (c, a))]
+            if 0 in o or len(set(o)) != 1:
+                continue
+            new = []
+            for p, q in ((a, b), (b, c), (c, a)):
+                tt = [p, q, d, e]
+                if orc.orient3d(*x[tt]) > 0:
+                    tt = [q, p, d, e]
+                new.append(tt)
+            keep = np.delete(tets, [t1, t2], axis=0)
+            return np.concatenate([keep, np.array(new, np.int32)])
+    raise AssertionError("no flippable face")
+
+
 def test_morton_index_space_and_reuse(orc):
     x = datasets.uniform_cube(5000, seed=5)
     with g.Triangulator(grid_size=32, index_space="morton") as t:
@@ -255,8 +334,11 @@ def test_full_size_goldens(orc):
         if name not in gold:
             continue
         x = datasets.make(name)
+        print("%s: %d points generated" % (name, x.shape[0]), flush=True)
         with g.Triangulator(grid_size=gold[name]["grid"], adjacency=False) as t:
             tets, _, _ = t.triangulate(x)
+            print("%s: %d tetrahedra, %.1f ms on the GPU" % (name, tets.shape[0], t.stats()["t_total_ms"]),
+                  flush=True)
         got = g.canonical(tets)
         assert got.shape[0] == gold[name]["num_tets"], name
         assert _h(got) == gold[name]["sha256"], name
