@@ -79,5 +79,24 @@ def pmc_per_kernel(prof_dir, counter):
     return {k: sum(v) / len(v) for k, v in acc.items()}
 
 
+def pba_traffic(fetch_dir, write_dir, copy_bytes, grid):
+    """HBM bytes per triangulation of the three k_pba_pass launches from FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch), each scaled by what the counter reads for the plain copy
+    kernel of copy_bytes (which reads and writes copy_bytes once)."""
+    f = pmc_per_kernel(fetch_dir, "FETCH_SIZE")
+    w = pmc_per_kernel(write_dir, "WRITE_SIZE")
+    ck = [k for k in f if "k_diag_copy" in k][0]
+    sf = copy_bytes / (f[ck] * 1024.0)
+    sw = copy_bytes / (w[ck] * 1024.0)
+    passes = {}
+    for k in f:
+        if "k_pba_pass" in k:
+            passes[k] = {"fetch_bytes": f[k] * 1024.0 * sf, "write_bytes": w.get(k, 0.0) * 1024.0 * sw}
+    total = sum(v["fetch_bytes"] + v["write_bytes"] for v in passes.values())
+    return {"grid": grid, "copy_bytes": copy_bytes, "fetch_scale": sf, "write_scale": sw,
+            "copy_fetch_kib": f[ck], "copy_write_kib": w[ck], "passes": passes,
+            "hbm_bytes_per_triangulation": total, "algorithmic_bytes": 24 * grid ** 3}
+
+
 if __name__ == "__main__":
     sys.exit(main())
