@@ -53,7 +53,9 @@ class Stats(ctypes.Structure):
                 ("splay_insertions", ctypes.c_int64), ("num_tets", ctypes.c_int64),
                 ("exact_calls", ctypes.c_int64), ("sos_calls", ctypes.c_int64),
                 ("exact_stars", ctypes.c_int64),
-                ("pba_bytes", ctypes.c_int64), ("t_pba_pass_ms", ctypes.c_double * 3)]
+                ("pba_bytes", ctypes.c_int64), ("t_pba_pass_ms", ctypes.c_double * 3),
+                ("star_store_bytes", ctypes.c_int64), ("store_repacks", ctypes.c_int32),
+                ("star_passes", ctypes.c_int32)]
 
     def as_dict(self):
         d = {}

@@ -28,22 +28,28 @@ using g4d::DevBuf;
 
 namespace {
 
-struct Pool {
-    DevBuf vnum, voff, vpool, tnum, toff, tpool;
+// Star storage: per-star records over three append-only arenas (link vertices as int,
+// triangle words as uint64, history vertices as int).  top[k] counts the elements in use.
+struct Store {
+    static constexpr size_t esz[3] = {sizeof(int), sizeof(uint64_t), sizeof(int)};
+    DevBuf vnum, voff, tnum, toff, hnum, hoff, nvoff, ntoff, nhoff;
+    DevBuf arena[3], spare[3], live[3];
+    int64_t top[3] = {0, 0, 0};
+    int32_t repacks = 0;
+    int64_t capacity(int k) const { return (int64_t)(arena[k].bytes / esz[k]); }
     void release()
     {
-        vnum.release(); voff.release(); vpool.release(); tnum.release(); toff.release(); tpool.release();
+        DevBuf *b[] = {&vnum, &voff, &tnum, &toff, &hnum, &hoff, &nvoff, &ntoff, &nhoff};
+        for (DevBuf *x : b) x->release();
+        for (int k = 0; k < 3; k++) {
+            arena[k].release();
+            spare[k].release();
+            live[k].release();
+        }
+        top[0] = top[1] = top[2] = 0;
     }
 };
-
-struct Hist {
-    DevBuf keys, off;
-    void release()
-    {
-        keys.release();
-        off.release();
-    }
-};
+constexpr size_t Store::esz[3];
 
 enum { EV_START, EV_MORTON, EV_PBA0, EV_PBA1, EV_PBA2, EV_PBA3, EV_PAIRS, EV_STAR, EV_SPLAY, EV_OUT, EV_ADJ, EV_NUM };
 
@@ -55,10 +61,9 @@ struct g4d_ctx {
     std::string err;
     g4d::Prims P;
     DevBuf xyz_up, keys, pts, perm, vox, bounds, grid, gtmp, idgrid, pbaspill, pcount, pairs, ws_off, cnt;
-    Pool pool[2];
-    Hist hist[2];
-    DevBuf drown, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, big, huge, xstars, xtasks, xtris, gwork,
-        tetcnt, tetoff, tets, opp, table;
+    Store store;
+    DevBuf drown, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, aflags, apos, active, big, huge, xstars,
+        xtasks, xtris, gwork, tetcnt, tetoff, tets, opp, table;
     std::vector<int32_t> h_tets, h_opp, h_perm;
     const double *last_xyz = nullptr; // device points of the last triangulation (input order)
     int64_t num_tets = 0;
@@ -169,14 +174,45 @@ static int check_dev_error(g4d_ctx *ctx, unsigned long long flags, const char *s
         if (r_ != G4D_OK) return r_;                                                                                   \
     } while (0)
 
-static int ensure_pool(g4d_ctx *ctx, Pool &pl, int n, int64_t vtot, int64_t ttot)
+// Moves every star's live regions into fresh arenas with room for `need` more elements each
+// (half as much again as live + need), dropping the regions earlier passes left behind.
+static int repack_store(g4d_ctx *ctx, int n, const int64_t need[3])
 {
-    HCK(pl.vnum.ensure(sizeof(int) * (size_t)n));
-    HCK(pl.tnum.ensure(sizeof(int) * (size_t)n));
-    HCK(pl.voff.ensure(sizeof(int64_t) * (size_t)(n + 1)));
-    HCK(pl.toff.ensure(sizeof(int64_t) * (size_t)(n + 1)));
-    HCK(pl.vpool.ensure(sizeof(int) * (size_t)std::max<int64_t>(vtot, 1)));
-    HCK(pl.tpool.ensure(sizeof(uint64_t) * (size_t)std::max<int64_t>(ttot, 1)));
+    Store &st = ctx->store;
+    hipStream_t s = ctx->stream;
+    int64_t *sz[3];
+    for (int k = 0; k < 3; k++) {
+        HCK(st.live[k].ensure(sizeof(int64_t) * (size_t)(n + 1)));
+        sz[k] = buf<int64_t>(st.live[k]);
+    }
+    HCK(g4d::launch_live_sizes(n, buf<int>(st.vnum), buf<int>(st.tnum), buf<int>(st.hnum), sz[0], sz[1], sz[2], s));
+    int64_t live[3];
+    for (int k = 0; k < 3; k++) {
+        HCK(g4d::scan_excl_i64(ctx->P, sz[k], sz[k], n, sz[k] + n, s));
+        HCK(hipMemcpyAsync(&live[k], sz[k] + n, 8, hipMemcpyDeviceToHost, s));
+    }
+    HCK(hipStreamSynchronize(s));
+    for (int k = 0; k < 3; k++) {
+        const int64_t cap = std::max<int64_t>((live[k] + need[k]) * 3 / 2, 1024);
+        HCK(st.spare[k].ensure(Store::esz[k] * (size_t)cap));
+    }
+    HCK(g4d::launch_repack(n, buf<int>(st.vnum), buf<int64_t>(st.voff), buf<int>(st.tnum), buf<int64_t>(st.toff),
+                           buf<int>(st.hnum), buf<int64_t>(st.hoff), buf<int>(st.arena[0]), buf<uint64_t>(st.arena[1]),
+                           buf<int>(st.arena[2]), sz[0], sz[1], sz[2], buf<int>(st.spare[0]),
+                           buf<uint64_t>(st.spare[1]), buf<int>(st.spare[2]), s));
+    HCK(hipStreamSynchronize(s));
+    std::swap(st.voff, st.live[0]);
+    std::swap(st.toff, st.live[1]);
+    std::swap(st.hoff, st.live[2]);
+    for (int k = 0; k < 3; k++) {
+        std::swap(st.arena[k], st.spare[k]);
+        st.spare[k].release();
+        st.top[k] = live[k];
+    }
+    st.repacks++;
+    trace("star store repacked: live %lld / %lld / %lld, capacity %lld / %lld / %lld", (long long)live[0],
+          (long long)live[1], (long long)live[2], (long long)st.capacity(0), (long long)st.capacity(1),
+          (long long)st.capacity(2));
     return G4D_OK;
 }
 
@@ -320,31 +356,67 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
     const int lds_limit = std::min(g4d::STAR_LDS_LIMIT, ctx->opt.lds_star_limit > 0 ? ctx->opt.lds_star_limit
                                                                                   : g4d::STAR_LDS_LIMIT);
 
-    auto star_pass = [&](int mode, const uint64_t *ins_keys, const int64_t *ins_off, int64_t nins, Pool *in,
-                         Pool &out, const uint64_t *hkeys, const int64_t *hoff, Hist *hout, int64_t *drown_n,
+    Store &store = ctx->store;
+    HCK(store.vnum.ensure(sizeof(int) * (size_t)n));
+    HCK(store.tnum.ensure(sizeof(int) * (size_t)n));
+    HCK(store.hnum.ensure(sizeof(int) * (size_t)n));
+    HCK(store.voff.ensure(sizeof(int64_t) * (size_t)n));
+    HCK(store.toff.ensure(sizeof(int64_t) * (size_t)n));
+    HCK(store.hoff.ensure(sizeof(int64_t) * (size_t)n));
+    HCK(store.nvoff.ensure(sizeof(int64_t) * (size_t)n));
+    HCK(store.ntoff.ensure(sizeof(int64_t) * (size_t)n));
+    HCK(store.nhoff.ensure(sizeof(int64_t) * (size_t)n));
+    // empty records until a star is built (a star that fails is reported, never read)
+    HCK(hipMemsetAsync(store.vnum.ptr, 0, sizeof(int) * (size_t)n, s));
+    HCK(hipMemsetAsync(store.tnum.ptr, 0, sizeof(int) * (size_t)n, s));
+    HCK(hipMemsetAsync(store.hnum.ptr, 0, sizeof(int) * (size_t)n, s));
+    store.top[0] = store.top[1] = store.top[2] = 0;
+    store.repacks = 0;
+
+    // One pass over the stars with insertions (mode 0: every star, from its workset).  Each
+    // such star gets a fresh region at the arenas' tops; the others are not touched.
+    auto star_pass = [&](int mode, const uint64_t *ins_keys, const int64_t *ins_off, int64_t nins, int64_t *drown_n,
                          int64_t *tp_n) -> int {
-        HCK(g4d::launch_capacities(n, in ? buf<int>(in->vnum) : nullptr, ins_off, hoff, vcap, tcap, hout ? hcap : nullptr,
-                                   s));
-        HCK(out.voff.ensure(sizeof(int64_t) * (size_t)(n + 1)));
-        HCK(out.toff.ensure(sizeof(int64_t) * (size_t)(n + 1)));
-        int64_t *voff = buf<int64_t>(out.voff), *toff = buf<int64_t>(out.toff);
-        HCK(g4d::scan_excl_i64(P, vcap, voff, n, voff + n, s));
-        HCK(g4d::scan_excl_i64(P, tcap, toff, n, toff + n, s));
-        int64_t tot[2];
-        HCK(hipMemcpyAsync(&tot[0], voff + n, 8, hipMemcpyDeviceToHost, s));
-        HCK(hipMemcpyAsync(&tot[1], toff + n, 8, hipMemcpyDeviceToHost, s));
-        int64_t htot = 0;
-        if (hout) {
-            HCK(hout->off.ensure(sizeof(int64_t) * (size_t)(n + 1)));
-            int64_t *ho = buf<int64_t>(hout->off);
-            HCK(g4d::scan_excl_i64(P, hcap, ho, n, ho + n, s));
-            HCK(hipMemcpyAsync(&htot, ho + n, 8, hipMemcpyDeviceToHost, s));
+        const int *list = nullptr;
+        int nitems = n;
+        if (mode == 1) {
+            HCK(ctx->aflags.ensure(sizeof(uint32_t) * (size_t)n));
+            HCK(ctx->apos.ensure(sizeof(uint32_t) * (size_t)n));
+            HCK(ctx->active.ensure(sizeof(int) * (size_t)n));
+            uint32_t *fl = buf<uint32_t>(ctx->aflags), *pos = buf<uint32_t>(ctx->apos);
+            HCK(g4d::launch_active_flags(ins_off, n, fl, s));
+            HCK(g4d::scan_excl_u32(P, fl, pos, n, s));
+            HCK(g4d::launch_active_scatter(fl, pos, n, buf<int>(ctx->active), s));
+            uint32_t last[2];
+            HCK(hipMemcpyAsync(&last[0], pos + n - 1, 4, hipMemcpyDeviceToHost, s));
+            HCK(hipMemcpyAsync(&last[1], fl + n - 1, 4, hipMemcpyDeviceToHost, s));
+            HCK(hipStreamSynchronize(s));
+            nitems = (int)(last[0] + last[1]);
+            list = buf<int>(ctx->active);
+        }
+        HCK(g4d::launch_capacities(list, nitems, mode ? buf<int>(store.vnum) : nullptr,
+                                   mode ? buf<int>(store.hnum) : nullptr, ins_off, vcap, tcap, hcap, s));
+        int64_t need[3] = {0, 0, 0};
+        int64_t *caps[3] = {vcap, tcap, hcap};
+        for (int k = 0; k < 3; k++) {
+            HCK(g4d::scan_excl_i64(P, caps[k], caps[k], nitems, caps[k] + nitems, s));
+            HCK(hipMemcpyAsync(&need[k], caps[k] + nitems, 8, hipMemcpyDeviceToHost, s));
         }
         HCK(hipStreamSynchronize(s));
-        CK(ensure_pool(ctx, out, n, tot[0], tot[1]));
-        if (hout) HCK(hout->keys.ensure(sizeof(uint64_t) * (size_t)std::max<int64_t>(htot, 1)));
+        if (mode == 0) {
+            // room for the splaying rounds: their regions come to about twice the creation's
+            for (int k = 0; k < 3; k++)
+                HCK(store.arena[k].ensure(Store::esz[k] * (size_t)std::max<int64_t>(3 * need[k], 1024)));
+        } else if (store.top[0] + need[0] > store.capacity(0) || store.top[1] + need[1] > store.capacity(1) ||
+                   store.top[2] + need[2] > store.capacity(2)) {
+            CK(repack_store(ctx, n, need));
+        }
+        HCK(g4d::launch_assign_regions(list, nitems, vcap, tcap, hcap, store.top[0], store.top[1], store.top[2],
+                                       buf<int64_t>(store.nvoff), buf<int64_t>(store.ntoff), buf<int64_t>(store.nhoff),
+                                       s));
+        for (int k = 0; k < 3; k++) store.top[k] += need[k];
         const int64_t dcap = std::max<int64_t>(nins, 1);
-        const int64_t tpcap = 2 * tot[1] + 4 * dcap + 64;
+        const int64_t tpcap = 2 * need[1] + 4 * dcap + 64;
         HCK(ctx->drown.ensure(sizeof(uint64_t) * (size_t)dcap));
         HCK(ctx->cand.ensure(sizeof(uint64_t) * (size_t)tpcap));
         g4d::StarParams sp;
@@ -354,26 +426,18 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
         sp.mode = mode;
         sp.ins_keys = ins_keys;
         sp.ins_off = ins_off;
-        if (in) {
-            sp.in_vnum = buf<int>(in->vnum);
-            sp.in_voff = buf<int64_t>(in->voff);
-            sp.in_vpool = buf<int>(in->vpool);
-            sp.in_tnum = buf<int>(in->tnum);
-            sp.in_toff = buf<int64_t>(in->toff);
-            sp.in_tpool = buf<uint64_t>(in->tpool);
-        }
-        sp.out_vnum = buf<int>(out.vnum);
-        sp.out_voff = voff;
-        sp.out_vpool = buf<int>(out.vpool);
-        sp.out_tnum = buf<int>(out.tnum);
-        sp.out_toff = toff;
-        sp.out_tpool = buf<uint64_t>(out.tpool);
-        sp.hist_keys = hkeys;
-        sp.hist_off = hoff;
-        if (hout) {
-            sp.hist_out = buf<uint64_t>(hout->keys);
-            sp.hist_out_off = buf<int64_t>(hout->off);
-        }
+        sp.vnum = buf<int>(store.vnum);
+        sp.voff = buf<int64_t>(store.voff);
+        sp.tnum = buf<int>(store.tnum);
+        sp.toff = buf<int64_t>(store.toff);
+        sp.hnum = buf<int>(store.hnum);
+        sp.hoff = buf<int64_t>(store.hoff);
+        sp.vpool = buf<int>(store.arena[0]);
+        sp.tpool = buf<uint64_t>(store.arena[1]);
+        sp.hpool = buf<int>(store.arena[2]);
+        sp.nvoff = buf<int64_t>(store.nvoff);
+        sp.ntoff = buf<int64_t>(store.ntoff);
+        sp.nhoff = buf<int64_t>(store.nhoff);
         sp.drown = buf<uint64_t>(ctx->drown);
         sp.drown_cap = dcap;
         sp.tpairs = buf<uint64_t>(ctx->cand);
@@ -389,6 +453,8 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
         HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8 * 3, s));
         HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND3, 0, 8 * 2, s));
         HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
+        sp.star_list = list;
+        sp.nlist = nitems;
         // filter-only LDS kernels; stars they cannot decide go to the exact-capable kernels
         HCK(g4d::launch_star_kernel(sp, 0, false, nblk_star, s));
         CK(read_counters(ctx, hc));
@@ -429,9 +495,9 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
                                         nblk, s));
             CK(read_counters(ctx, hc));
         }
-        trace("star pass mode %d: %lld insertions, large stars %lld, exact %lld, global-memory %lld, drowned %llu, "
-              "pairs %llu, err 0x%llx",
-              mode, (long long)nins, (long long)nbig, (long long)nexact, (long long)nhuge, hc[G4D_CNT_APPEND0],
+        trace("star pass mode %d: %lld insertions into %d stars, large stars %lld, exact %lld, global-memory %lld, "
+              "drowned %llu, pairs %llu, err 0x%llx",
+              mode, (long long)nins, nitems, (long long)nbig, (long long)nexact, (long long)nhuge, hc[G4D_CNT_APPEND0],
               hc[G4D_CNT_APPEND1], hc[G4D_CNT_ERROR]);
         CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], mode ? "splaying" : "star creation"));
         *drown_n = (int64_t)hc[G4D_CNT_APPEND0];
@@ -439,11 +505,8 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
         return G4D_OK;
     };
 
-    int cur = 0;
     int64_t nd = 0, ntp = 0;
-    CK(star_pass(0, pr, ws_off, m, nullptr, ctx->pool[0], nullptr, nullptr, nullptr, &nd, &ntp));
-    const uint64_t *hkeys = pr;
-    const int64_t *hoff = ws_off;
+    CK(star_pass(0, pr, ws_off, m, &nd, &ntp));
     ev_record(ctx, EV_STAR);
 
     // ---- F: splaying rounds (GD/GDelaunay/GDelHost.cu:1389-1470) ----
@@ -464,17 +527,16 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
             CK(read_counters(ctx, hc));
             ntask = (int64_t)hc[G4D_CNT_APPEND0];
             if (ntask > 0) {
-                Pool &pl = ctx->pool[cur];
                 g4d::ProofParams pp;
                 pp.pts = pts;
                 pp.tasks = buf<uint64_t>(ctx->tasks);
                 pp.ntasks = ntask;
-                pp.vnum = buf<int>(pl.vnum);
-                pp.voff = buf<int64_t>(pl.voff);
-                pp.vpool = buf<int>(pl.vpool);
-                pp.tnum = buf<int>(pl.tnum);
-                pp.toff = buf<int64_t>(pl.toff);
-                pp.tpool = buf<uint64_t>(pl.tpool);
+                pp.vnum = buf<int>(store.vnum);
+                pp.voff = buf<int64_t>(store.voff);
+                pp.vpool = buf<int>(store.arena[0]);
+                pp.tnum = buf<int>(store.tnum);
+                pp.toff = buf<int64_t>(store.toff);
+                pp.tpool = buf<uint64_t>(store.arena[1]);
                 pp.out = buf<uint64_t>(ctx->cand) + ntp;
                 pp.cnt = cnt;
                 HCK(ctx->xtasks.ensure(sizeof(int64_t) * (size_t)ntask));
@@ -495,7 +557,8 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
             HCK(ctx->keep.ensure((size_t)ncand));
             HCK(ctx->ins.ensure(sizeof(uint64_t) * (size_t)(2 * ncand + 2)));
             uint64_t *ins = buf<uint64_t>(ctx->ins);
-            HCK(g4d::launch_hist_filter(buf<uint64_t>(ctx->cand), ncand, hkeys, hoff, buf<uint8_t>(ctx->keep), s));
+            HCK(g4d::launch_hist_filter(buf<uint64_t>(ctx->cand), ncand, buf<int>(store.arena[2]),
+                                        buf<int64_t>(store.hoff), buf<int>(store.hnum), buf<uint8_t>(ctx->keep), s));
             HCK(g4d::compact_u64(P, buf<uint64_t>(ctx->cand), buf<uint8_t>(ctx->keep), ins, ncand, &nins, s));
             if (nins > 0) {
                 HCK(sort_pairs(P, ins, nins, nb, s));
@@ -520,30 +583,24 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
         HCK(ctx->ins_off.ensure(sizeof(int64_t) * (size_t)(n + 1)));
         int64_t *ioff = buf<int64_t>(ctx->ins_off);
         HCK(g4d::launch_csr_offsets(ins, nrec, n, ioff, s));
-        const int nxt = cur ^ 1;
-        Hist &hnext = ctx->hist[rounds & 1];
-        CK(star_pass(1, ins, ioff, nrec, &ctx->pool[cur], ctx->pool[nxt], hkeys, hoff, &hnext, &nd, &ntp));
-        hkeys = buf<uint64_t>(hnext.keys);
-        hoff = buf<int64_t>(hnext.off);
-        cur = nxt;
+        CK(star_pass(1, ins, ioff, nrec, &nd, &ntp));
     }
     if (!converged) return set_err(ctx, G4D_ERR_NOT_CONVERGED, "splaying did not converge in %d rounds", rounds);
     ctx->st.splay_rounds = rounds;
     ev_record(ctx, EV_SPLAY);
 
     // ---- G: tetrahedra (GD/GDelaunay/GDelHost.cu:1520-1615) ----
-    Pool &pl = ctx->pool[cur];
     HCK(ctx->tetcnt.ensure(sizeof(int64_t) * (size_t)(n + 1)));
     HCK(ctx->tetoff.ensure(sizeof(int64_t) * (size_t)(n + 1)));
     g4d::OutParams op;
     op.pts = pts;
     op.n = n;
-    op.vnum = buf<int>(pl.vnum);
-    op.voff = buf<int64_t>(pl.voff);
-    op.vpool = buf<int>(pl.vpool);
-    op.tnum = buf<int>(pl.tnum);
-    op.toff = buf<int64_t>(pl.toff);
-    op.tpool = buf<uint64_t>(pl.tpool);
+    op.vnum = buf<int>(store.vnum);
+    op.voff = buf<int64_t>(store.voff);
+    op.vpool = buf<int>(store.arena[0]);
+    op.tnum = buf<int>(store.tnum);
+    op.toff = buf<int64_t>(store.toff);
+    op.tpool = buf<uint64_t>(store.arena[1]);
     op.star_tet_count = buf<int64_t>(ctx->tetcnt);
     op.star_tet_off = buf<int64_t>(ctx->tetoff);
     op.cnt = cnt;
@@ -589,6 +646,9 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
     ctx->st.exact_calls = (int64_t)hc[G4D_CNT_EXACT];
     ctx->st.sos_calls = (int64_t)hc[G4D_CNT_SOS];
     ctx->st.pba_bytes = 24 * G3;
+    ctx->st.star_store_bytes = store.top[0] * 4 + store.top[1] * 8 + store.top[2] * 4;
+    ctx->st.store_repacks = store.repacks;
+    ctx->st.star_passes = rounds + 1;
     ctx->st.t_morton_ms = ev_ms(ctx, EV_START, EV_MORTON);
     ctx->st.t_pba_ms = ev_ms(ctx, EV_MORTON, EV_PBA3);
     ctx->st.t_pba_pass_ms[0] = ev_ms(ctx, EV_PBA0, EV_PBA1);
@@ -665,10 +725,9 @@ void g4d_destroy(g4d_ctx *ctx)
     DevBuf *bufs[] = {&ctx->xyz_up, &ctx->keys, &ctx->pts, &ctx->perm, &ctx->vox, &ctx->bounds, &ctx->grid,
                       &ctx->gtmp, &ctx->idgrid, &ctx->pbaspill, &ctx->pcount, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->drown, &ctx->dkeys,
                       &ctx->tasks, &ctx->cand, &ctx->keep, &ctx->ins, &ctx->ins_off, &ctx->vcap, &ctx->tcap,
-                      &ctx->hcap, &ctx->big, &ctx->huge, &ctx->xstars, &ctx->xtasks, &ctx->xtris, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table};
+                      &ctx->hcap, &ctx->aflags, &ctx->apos, &ctx->active, &ctx->big, &ctx->huge, &ctx->xstars, &ctx->xtasks, &ctx->xtris, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table};
     for (DevBuf *b : bufs) b->release();
-    for (auto &p : ctx->pool) p.release();
-    for (auto &h : ctx->hist) h.release();
+    ctx->store.release();
     ctx->P.release();
     if (ctx->ev_ok)
         for (int i = 0; i < EV_NUM; i++) (void)hipEventDestroy(ctx->ev[i]);

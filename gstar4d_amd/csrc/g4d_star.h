@@ -16,6 +16,11 @@ static constexpr int STAR_WIDE_MIN = 512;
 // global-memory star kernel, which has no fixed capacity.
 static constexpr int STAR_LDS_LIMIT = STAR_WIDE_MIN - 1;
 
+// Star storage: per-star records (vertex count/offset, triangle count/offset, history
+// count/offset) over three append-only arenas (link vertices, triangle words, history
+// vertices).  A pass rebuilds only the stars that receive insertions: each is written to a
+// fresh region reserved for it (nvoff/ntoff/nhoff) and its record is switched over when it
+// is complete; untouched stars stay where they are.
 struct StarParams {
     const double *pts;
     int n;
@@ -23,25 +28,21 @@ struct StarParams {
     // insertion lists: directed keys (star<<32 | vertex), sorted, CSR offsets (n+1)
     const uint64_t *ins_keys;
     const int64_t *ins_off;
-    // input stars (mode 1)
-    const int *in_vnum;
-    const int64_t *in_voff;
-    const int *in_vpool;
-    const int *in_tnum;
-    const int64_t *in_toff;
-    const uint64_t *in_tpool;
-    // output stars (capacities come from the offsets)
-    int *out_vnum;
-    const int64_t *out_voff;
-    int *out_vpool;
-    int *out_tnum;
-    const int64_t *out_toff;
-    uint64_t *out_tpool;
-    // history (mode 1): old CSR and the merged output CSR
-    const uint64_t *hist_keys;
-    const int64_t *hist_off;
-    uint64_t *hist_out;
-    const int64_t *hist_out_off;
+    // per-star records (read for the current star, rewritten when it is rebuilt)
+    int *vnum;
+    int64_t *voff;
+    int *tnum;
+    int64_t *toff;
+    int *hnum;
+    int64_t *hoff;
+    // arenas
+    int *vpool;
+    uint64_t *tpool;
+    int *hpool; // per star: sorted vertex ids inserted so far (workset and splaying)
+    // regions reserved for this pass's stars, indexed by star id
+    const int64_t *nvoff;
+    const int64_t *ntoff;
+    const int64_t *nhoff;
     // appends
     uint64_t *drown;      // (star<<32 | vertex)       counter G4D_CNT_APPEND0
     int64_t drown_cap;
@@ -96,15 +97,33 @@ hipError_t launch_proofs(const ProofParams &p, bool exact, hipStream_t s);
 hipError_t launch_drown_keys(const uint64_t *drown, int64_t m, uint64_t *keys, hipStream_t s);
 // Non-mutual drowned keys -> proof tasks (star<<32 | vertex); counter G4D_CNT_APPEND0.
 hipError_t launch_nonmutual(const uint64_t *keys, int64_t m, uint64_t *tasks, unsigned long long *cnt, hipStream_t s);
-// keep[i] = pair (a<<32|b) not in history of a.
-hipError_t launch_hist_filter(const uint64_t *pairs, int64_t m, const uint64_t *hist_keys, const int64_t *hist_off,
+// keep[i] = pair (a<<32|b) with a < b and b not in the history of star a.
+hipError_t launch_hist_filter(const uint64_t *pairs, int64_t m, const int *hpool, const int64_t *hoff, const int *hnum,
                               uint8_t *keep, hipStream_t s);
 // Reciprocate m ordered pairs in place into 2m directed keys.
 hipError_t launch_reciprocate(uint64_t *keys, int64_t m, hipStream_t s);
-// Per-star capacities from CSR offsets: v = vn + m link vertices, triangle words 2v - 4
-// (twice that for stars that may need the wide format), history hn + m.
-hipError_t launch_capacities(int n, const int *vnum, const int64_t *ins_off, const int64_t *hist_off,
+// Stars with at least one insertion (ins_off[s+1] > ins_off[s]): flags[s] in {0,1}; after
+// an exclusive scan of flags into pos, list[pos[s]] = s for the flagged stars (ascending).
+hipError_t launch_active_flags(const int64_t *ins_off, int n, uint32_t *flags, hipStream_t s);
+hipError_t launch_active_scatter(const uint32_t *flags, const uint32_t *pos, int n, int *list, hipStream_t s);
+// Region sizes of the stars of a pass (list = nullptr: stars 0..nitems-1), per item: link
+// vertices v = vn + m, triangle words 2v - 4 (twice that for stars that may need the wide
+// format), history hn + m.
+hipError_t launch_capacities(const int *list, int nitems, const int *vnum, const int *hnum, const int64_t *ins_off,
                              int64_t *vcap, int64_t *tcap, int64_t *hcap, hipStream_t s);
+// nvoff[s] = vbase + vscan[i] (same for t, h) for s = list[i].
+hipError_t launch_assign_regions(const int *list, int nitems, const int64_t *vscan, const int64_t *tscan,
+                                 const int64_t *hscan, int64_t vbase, int64_t tbase, int64_t hbase, int64_t *nvoff,
+                                 int64_t *ntoff, int64_t *nhoff, hipStream_t s);
+// Live region sizes of every star (vertices, triangle words, history), for repacking.
+hipError_t launch_live_sizes(int n, const int *vnum, const int *tnum, const int *hnum, int64_t *vsz, int64_t *tsz,
+                             int64_t *hsz, hipStream_t s);
+// Copies every star's live regions to the offsets vdst/tdst/hdst in the new arenas (the
+// caller then adopts those offsets as the records' own).
+hipError_t launch_repack(int n, const int *vnum, const int64_t *voff, const int *tnum, const int64_t *toff,
+                         const int *hnum, const int64_t *hoff, const int *vpool, const uint64_t *tpool,
+                         const int *hpool, const int64_t *vdst, const int64_t *tdst, const int64_t *hdst, int *vpool2,
+                         uint64_t *tpool2, int *hpool2, hipStream_t s);
 
 // Output: per-triangle keep flag for owner triangles on the lower hull (exact orient3d < 0)
 struct OutParams {

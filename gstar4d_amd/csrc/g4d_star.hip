@@ -113,6 +113,54 @@ __device__ __forceinline__ void star_list_push(int *list, int64_t cap, unsigned 
     }
 }
 
+// New history of star s: its old history (mode 1) merged with this pass's insertions ins[0, m)
+// (sorted, disjoint from the old list) into the star's new region.  Returns the old length.
+__device__ __forceinline__ int star_history_merge(const StarParams &p, int s, const int *ins, int m, int tid, int nt)
+{
+    const int64_t ho = p.nhoff[s];
+    if (p.mode == 0) {
+        for (int i = tid; i < m; i += nt) p.hpool[ho + i] = ins[i];
+        return 0;
+    }
+    const int64_t hb = p.hoff[s];
+    const int hn = p.hnum[s];
+    const int *old = p.hpool + hb;
+    for (int i = tid; i < hn; i += nt) {
+        const int x = old[i];
+        int lo = 0, hi = m;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (ins[mid] < x) lo = mid + 1;
+            else hi = mid;
+        }
+        p.hpool[ho + i + lo] = x;
+    }
+    for (int i = tid; i < m; i += nt) {
+        const int x = ins[i];
+        int lo = 0, hi = hn;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (old[mid] < x) lo = mid + 1;
+            else hi = mid;
+        }
+        p.hpool[ho + i + lo] = x;
+    }
+    return hn;
+}
+
+// Points star s's record at its rebuilt regions (one thread, after every read of the old
+// record and regions).
+__device__ __forceinline__ void star_commit(const StarParams &p, int s, int vcount, int64_t vo, int tcount, int64_t to,
+                                            int hcount)
+{
+    p.vnum[s] = vcount;
+    p.voff[s] = vo;
+    p.tnum[s] = tcount;
+    p.toff[s] = to;
+    p.hnum[s] = hcount;
+    p.hoff[s] = p.nhoff[s];
+}
+
 // EXACT = false: filters only.  A star with an inconclusive orientation is handed to
 // p.exact_list untouched (its output is written only at the end), so the kernel carries no
 // multiprecision code and no private scratch.
@@ -134,7 +182,7 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
         const int s = p.star_list ? p.star_list[item] : item;
         const int64_t ib = p.ins_off[s];
         const int m = (int)(p.ins_off[s + 1] - ib);
-        const int vn = p.mode ? p.in_vnum[s] : 0;
+        const int vn = p.mode ? p.vnum[s] : 0;
         const int need = vn + m;
         const int lim = VMAX < p.lds_limit ? VMAX : p.lds_limit;
         if (need > lim) {
@@ -144,49 +192,43 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
         if (p.mode == 0 && m < 4) {
             if (lane == 0) {
                 atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_CONE);
-                p.out_vnum[s] = 0;
-                p.out_tnum[s] = 0;
+                p.vnum[s] = 0;
+                p.tnum[s] = 0;
             }
             continue;
         }
-        if (p.mode == 1 && m == 0) {
-            // untouched star: copy verbatim
-            const int tn = p.in_tnum[s];
-            const int64_t vi = p.in_voff[s], vo = p.out_voff[s], ti = p.in_toff[s], to = p.out_toff[s];
-            for (int i = lane; i < vn; i += 64) p.out_vpool[vo + i] = p.in_vpool[vi + i];
-            for (int i = lane; i < tn; i += 64) p.out_tpool[to + i] = p.in_tpool[ti + i];
-            const int64_t hb = p.hist_off[s], he = p.hist_off[s + 1], ho = p.hist_out_off[s];
-            for (int64_t i = lane; i < he - hb; i += 64) p.hist_out[ho + i] = p.hist_keys[hb + i];
-            if (lane == 0) {
-                p.out_vnum[s] = vn;
-                p.out_tnum[s] = tn;
-            }
-            continue;
-        }
+        if (p.mode == 1 && m == 0) continue; // untouched star: stays where it is
         const double sx = p.pts[3 * (int64_t)s], sy = p.pts[3 * (int64_t)s + 1], sz = p.pts[3 * (int64_t)s + 2];
+        // Insertion q takes link-vertex slot vn + q; every insertion's coordinates are fetched
+        // here at once instead of one dependent load per insertion step.  (In mode 0 the
+        // first four form the initial simplex, set up below.)
+        const int q0 = p.mode ? 0 : 4;
         for (int q = lane; q < m; q += 64) {
-            L.ins[q] = (int)(uint32_t)p.ins_keys[ib + q];
+            const int g = (int)(uint32_t)p.ins_keys[ib + q];
+            L.ins[q] = g;
             L.ins_local[q] = -1;
+            if (q >= q0) {
+                L.vid[vn + q] = g;
+                rel_coords(p.pts, g, sx, sy, sz, L.vc[vn + q]);
+            }
         }
-        int T, V, nfree = 0, q0;
+        int T, nfree = 0;
         if (p.mode == 1) {
-            const int tn = p.in_tnum[s];
-            const int64_t vi = p.in_voff[s], ti = p.in_toff[s];
+            const int tn = p.tnum[s];
+            const int64_t vi = p.voff[s], ti = p.toff[s];
             for (int v = lane; v < vn; v += 64) {
-                const int g = p.in_vpool[vi + v];
+                const int g = p.vpool[vi + v];
                 L.vid[v] = g;
                 rel_coords(p.pts, g, sx, sy, sz, L.vc[v]);
             }
             for (int t = lane; t < tn; t += 64) {
-                const uint64_t h = p.in_tpool[ti + t];
+                const uint64_t h = p.tpool[ti + t];
                 L.tv[t] = tpack(hvert(h, 0), hvert(h, 1), hvert(h, 2), TF_ALIVE | (hchecked(h) ? TF_CHECKED : 0u));
                 L.to[t][0] = (uint16_t)hopp(h, 0);
                 L.to[t][1] = (uint16_t)hopp(h, 1);
                 L.to[t][2] = (uint16_t)hopp(h, 2);
             }
             T = tn;
-            V = vn;
-            q0 = 0;
         } else {
             // initial 4-simplex (GD/GDelaunay/GDelPredKernels.cu:45-158)
             int l[4];
@@ -218,8 +260,6 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
                 L.ins_local[lane] = (int16_t)local;
             }
             T = 4;
-            V = 4;
-            q0 = 4;
         }
         __syncthreads();
 #pragma unroll 1
@@ -233,8 +273,8 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
         bool failed = false, unsure = false;
         for (int q = q0; q < m; q++) {
             const int gv = L.ins[q];
-            double r[4];
-            rel_coords(p.pts, gv, sx, sy, sz, r);
+            const int Vn = vn + q;
+            const double r[4] = {L.vc[Vn][0], L.vc[Vn][1], L.vc[Vn][2], L.vc[Vn][3]};
             int nv = 0;
             bool unc = false;
 #pragma unroll 1
@@ -270,15 +310,7 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
             int nvis;
             const int vbase = wave_excl(nv, &nvis);
             if (nvis == 0) continue; // drowned at insertion
-            const int Vn = V;
-            if (lane == 0) {
-                L.vid[Vn] = gv;
-                L.vc[Vn][0] = r[0];
-                L.vc[Vn][1] = r[1];
-                L.vc[Vn][2] = r[2];
-                L.vc[Vn][3] = r[3];
-                L.ins_local[q] = (int16_t)Vn;
-            }
+            if (lane == 0) L.ins_local[q] = (int16_t)Vn;
             __syncthreads();
             // hole boundary: edges of freed triangles whose neighbour survives
             int nb = 0;
@@ -345,7 +377,6 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
             }
             nfree = nfree_tot - (nbtot < nfree_tot ? nbtot : nfree_tot);
             T = newT;
-            V = Vn + 1;
             __syncthreads();
 #pragma unroll 1
             for (int k = 0; k < KPL; k++) {
@@ -369,13 +400,14 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
         if (failed) {
             if (lane == 0) {
                 atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
-                p.out_vnum[s] = 0;
-                p.out_tnum[s] = 0;
+                p.vnum[s] = 0;
+                p.tnum[s] = 0;
             }
             __syncthreads();
             continue;
         }
         // ---- finalize: link set, drowned insertions, compaction, pairs, history ----
+        const int V = need; // drowned insertions' slots are never referenced and drop out
         for (int v = lane; v < V; v += 64) L.present[v] = 0;
         __syncthreads();
 #pragma unroll 1
@@ -405,7 +437,7 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
             }
         }
         // vertex compaction
-        const int64_t vo = p.out_voff[s], to = p.out_toff[s];
+        const int64_t vo = p.nvoff[s], to = p.ntoff[s];
         int vcount = 0;
         for (int base = 0; base < V; base += 64) {
             const int v = base + lane;
@@ -414,7 +446,7 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
             const int rk = g4d_wave_excl_count(pr, &tot);
             if (pr) {
                 L.vremap[v] = (uint16_t)(vcount + rk);
-                p.out_vpool[vo + vcount + rk] = L.vid[v];
+                p.vpool[vo + vcount + rk] = L.vid[v];
             }
             vcount += tot;
         }
@@ -427,12 +459,12 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
             if (al) L.remap[t] = (uint16_t)(tcount + rk);
             tcount += tot;
         }
-        if (tcount > (int)(p.out_toff[s + 1] - to) || tcount != 2 * vcount - 4) {
-            // not a triangulated 2-sphere: report instead of writing past the star's slots
+        if (tcount > 2 * need - 4 || tcount != 2 * vcount - 4) {
+            // not a triangulated 2-sphere: report instead of writing past the star's region
             if (lane == 0) {
                 atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
-                p.out_vnum[s] = 0;
-                p.out_tnum[s] = 0;
+                p.vnum[s] = 0;
+                p.tnum[s] = 0;
             }
             __syncthreads();
             continue;
@@ -445,7 +477,7 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
             const bool al = t < T && (tt & TF_ALIVE);
             if (al) {
                 const int a = tvert(tt, 0), b = tvert(tt, 1), c = tvert(tt, 2);
-                p.out_tpool[to + L.remap[t]] = hpack(L.vremap[a], L.vremap[b], L.vremap[c], L.remap[L.to[t][0]],
+                p.tpool[to + L.remap[t]] = hpack(L.vremap[a], L.vremap[b], L.vremap[c], L.remap[L.to[t][0]],
                                                      L.remap[L.to[t][1]], L.remap[L.to[t][2]], true);
             }
             const bool unchecked = al && !(tt & TF_CHECKED);
@@ -465,35 +497,9 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
                 }
             }
         }
-        if (lane == 0) {
-            p.out_vnum[s] = vcount;
-            p.out_tnum[s] = tcount;
-        }
-        if (p.mode == 1) {
-            // merged history: old list and this pass's insertions are disjoint sorted lists
-            const int64_t hb = p.hist_off[s], he = p.hist_off[s + 1], ho = p.hist_out_off[s];
-            const int hn = (int)(he - hb);
-            for (int i = lane; i < hn; i += 64) {
-                const int x = (int)(uint32_t)p.hist_keys[hb + i];
-                int lo = 0, hi = m;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (L.ins[mid] < x) lo = mid + 1;
-                    else hi = mid;
-                }
-                p.hist_out[ho + i + lo] = p.hist_keys[hb + i];
-            }
-            for (int i = lane; i < m; i += 64) {
-                const int x = L.ins[i];
-                int64_t lo = hb, hi = he;
-                while (lo < hi) {
-                    const int64_t mid = (lo + hi) >> 1;
-                    if ((int)(uint32_t)p.hist_keys[mid] < x) lo = mid + 1;
-                    else hi = mid;
-                }
-                p.hist_out[ho + i + (lo - hb)] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)x;
-            }
-        }
+        const int hn = star_history_merge(p, s, L.ins, m, lane, 64);
+        __syncthreads();
+        if (lane == 0) star_commit(p, s, vcount, vo, tcount, to, hn + m);
         __syncthreads();
     }
 }
@@ -601,43 +607,36 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
         const int s = list[item];
         const int64_t ib = p.ins_off[s];
         const int m = (int)(p.ins_off[s + 1] - ib);
-        const int vn = p.mode ? p.in_vnum[s] : 0;
+        const int vn = p.mode ? p.vnum[s] : 0;
         const bool in_wide = vn >= STAR_WIDE_MIN;
         if (vn + m > vcap || (p.mode == 0 && m < 4)) {
             if (tid == 0) {
                 atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)(vn + m > vcap ? G4D_DEV_ERR_STAR_CAPACITY
                                                                                   : G4D_DEV_ERR_CONE));
-                p.out_vnum[s] = 0;
-                p.out_tnum[s] = 0;
+                p.vnum[s] = 0;
+                p.tnum[s] = 0;
             }
             continue;
         }
-        if (p.mode == 1 && m == 0) {
-            // untouched star: copy verbatim
-            const int tn = p.in_tnum[s];
-            const int64_t words = in_wide ? 2 * (int64_t)tn : tn;
-            const int64_t vi = p.in_voff[s], vo = p.out_voff[s], ti = p.in_toff[s], to = p.out_toff[s];
-            for (int i = tid; i < vn; i += GSTAR_NT) p.out_vpool[vo + i] = p.in_vpool[vi + i];
-            for (int64_t i = tid; i < words; i += GSTAR_NT) p.out_tpool[to + i] = p.in_tpool[ti + i];
-            const int64_t hb = p.hist_off[s], he = p.hist_off[s + 1], ho = p.hist_out_off[s];
-            for (int64_t i = tid; i < he - hb; i += GSTAR_NT) p.hist_out[ho + i] = p.hist_keys[hb + i];
-            if (tid == 0) {
-                p.out_vnum[s] = vn;
-                p.out_tnum[s] = tn;
-            }
-            continue;
-        }
+        if (p.mode == 1 && m == 0) continue; // untouched star: stays where it is
         const double sx = p.pts[3 * (int64_t)s], sy = p.pts[3 * (int64_t)s + 1], sz = p.pts[3 * (int64_t)s + 2];
+        // insertion q takes link-vertex slot vn + q (mode 0: the first four form the simplex)
+        const int q0 = p.mode ? 0 : 4;
         for (int q = tid; q < m; q += GSTAR_NT) {
-            L.ins[q] = (int)(uint32_t)p.ins_keys[ib + q];
+            const int g = (int)(uint32_t)p.ins_keys[ib + q];
+            L.ins[q] = g;
             L.ins_local[q] = -1;
+            if (q >= q0) {
+                L.vid[vn + q] = g;
+                rel_coords(p.pts, g, sx, sy, sz, L.vc + 4 * (vn + q));
+            }
         }
-        int T, V, nfree = 0, q0;
+        int T, nfree = 0;
         if (p.mode == 1) {
-            const int tn = p.in_tnum[s];
-            const int64_t vi = p.in_voff[s], ti = p.in_toff[s];
+            const int tn = p.tnum[s];
+            const int64_t vi = p.voff[s], ti = p.toff[s];
             for (int v = tid; v < vn; v += GSTAR_NT) {
-                const int g = p.in_vpool[vi + v];
+                const int g = p.vpool[vi + v];
                 L.vid[v] = g;
                 rel_coords(p.pts, g, sx, sy, sz, L.vc + 4 * v);
             }
@@ -645,14 +644,14 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
                 int a[3], o[3];
                 bool chk;
                 if (in_wide) {
-                    const uint64_t w0 = p.in_tpool[ti + 2 * (int64_t)t], w1 = p.in_tpool[ti + 2 * (int64_t)t + 1];
+                    const uint64_t w0 = p.tpool[ti + 2 * (int64_t)t], w1 = p.tpool[ti + 2 * (int64_t)t + 1];
                     for (int i = 0; i < 3; i++) {
                         a[i] = wfield(w0, i);
                         o[i] = wfield(w1, i);
                     }
                     chk = (w0 >> 63) & 1u;
                 } else {
-                    const uint64_t h = p.in_tpool[ti + t];
+                    const uint64_t h = p.tpool[ti + t];
                     for (int i = 0; i < 3; i++) {
                         a[i] = hvert(h, i);
                         o[i] = hopp(h, i);
@@ -666,8 +665,6 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
                 L.tfl[t] = TF_ALIVE | (chk ? TF_CHECKED : 0u);
             }
             T = tn;
-            V = vn;
-            q0 = 0;
         } else {
             // initial 4-simplex (GD/GDelaunay/GDelPredKernels.cu:45-158)
             int l[4];
@@ -693,8 +690,6 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
                 L.ins_local[tid] = (tid < 2 && sw) ? 1 - tid : tid;
             }
             T = 4;
-            V = 4;
-            q0 = 4;
         }
         __syncthreads();
         for (int slot = tid; slot < T; slot += GSTAR_NT)
@@ -703,8 +698,8 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
         bool failed = false, unsure = false;
         for (int q = q0; q < m; q++) {
             const int gv = L.ins[q];
-            double r[4];
-            rel_coords(p.pts, gv, sx, sy, sz, r);
+            const int Vn = vn + q;
+            const double r[4] = {L.vc[4 * Vn], L.vc[4 * Vn + 1], L.vc[4 * Vn + 2], L.vc[4 * Vn + 3]};
             int nv = 0;
             bool unc = false;
             for (int slot = tid; slot < T; slot += GSTAR_NT) {
@@ -726,12 +721,7 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
             int nvis;
             const int vbase = block_excl(nv, &nvis, red);
             if (nvis == 0) continue; // drowned at insertion
-            const int Vn = V;
-            if (tid == 0) {
-                L.vid[Vn] = gv;
-                for (int i = 0; i < 4; i++) L.vc[4 * Vn + i] = r[i];
-                L.ins_local[q] = Vn;
-            }
+            if (tid == 0) L.ins_local[q] = Vn;
             __syncthreads();
             // hole boundary: edges of freed triangles whose neighbour survives
             int nb = 0;
@@ -791,7 +781,6 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
             }
             nfree = nfree_tot - (nbtot < nfree_tot ? nbtot : nfree_tot);
             T = newT;
-            V = Vn + 1;
             __syncthreads();
             for (int slot = tid; slot < T; slot += GSTAR_NT) {
                 const uint32_t f = L.tfl[slot];
@@ -811,13 +800,14 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
         if (failed) {
             if (tid == 0) {
                 atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
-                p.out_vnum[s] = 0;
-                p.out_tnum[s] = 0;
+                p.vnum[s] = 0;
+                p.tnum[s] = 0;
             }
             __syncthreads();
             continue;
         }
         // ---- finalize: link set, drowned insertions, compaction, pairs, history ----
+        const int V = vn + m; // drowned insertions' slots are never referenced and drop out
         for (int v = tid; v < V; v += GSTAR_NT) L.present[v] = 0;
         __syncthreads();
         for (int slot = tid; slot < T; slot += GSTAR_NT)
@@ -837,7 +827,7 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
                 else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
             }
         }
-        const int64_t vo = p.out_voff[s], to = p.out_toff[s];
+        const int64_t vo = p.nvoff[s], to = p.ntoff[s];
         int vcount = 0;
         for (int base = 0; base < V; base += GSTAR_NT) {
             const int v = base + tid;
@@ -846,7 +836,7 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
             const int rk = block_excl(pr ? 1 : 0, &tot, red);
             if (pr) {
                 L.vremap[v] = vcount + rk;
-                p.out_vpool[vo + vcount + rk] = L.vid[v];
+                p.vpool[vo + vcount + rk] = L.vid[v];
             }
             vcount += tot;
         }
@@ -861,11 +851,12 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
         }
         const bool wide = vcount >= STAR_WIDE_MIN;
         const int64_t words = wide ? 2 * (int64_t)tcount : tcount;
-        if (words > p.out_toff[s + 1] - to || tcount != 2 * vcount - 4) {
+        const int64_t tneed = 2 * (int64_t)(vn + m) - 4;
+        if (words > (vn + m >= STAR_WIDE_MIN ? 2 * tneed : tneed) || tcount != 2 * vcount - 4) {
             if (tid == 0) {
                 atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
-                p.out_vnum[s] = 0;
-                p.out_tnum[s] = 0;
+                p.vnum[s] = 0;
+                p.tnum[s] = 0;
             }
             __syncthreads();
             continue;
@@ -885,10 +876,10 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
                 const int o0 = L.remap[L.to[3 * t]], o1 = L.remap[L.to[3 * t + 1]], o2 = L.remap[L.to[3 * t + 2]];
                 const int64_t d = L.remap[t];
                 if (wide) {
-                    p.out_tpool[to + 2 * d] = wpack3(ra, rb, rc) | (1ull << 63);
-                    p.out_tpool[to + 2 * d + 1] = wpack3(o0, o1, o2);
+                    p.tpool[to + 2 * d] = wpack3(ra, rb, rc) | (1ull << 63);
+                    p.tpool[to + 2 * d + 1] = wpack3(o0, o1, o2);
                 } else {
-                    p.out_tpool[to + d] = hpack(ra, rb, rc, o0, o1, o2, true);
+                    p.tpool[to + d] = hpack(ra, rb, rc, o0, o1, o2, true);
                 }
             }
             const bool unchecked = al && !(f & TF_CHECKED);
@@ -907,34 +898,9 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
                 }
             }
         }
-        if (tid == 0) {
-            p.out_vnum[s] = vcount;
-            p.out_tnum[s] = tcount;
-        }
-        if (p.mode == 1) {
-            const int64_t hb = p.hist_off[s], he = p.hist_off[s + 1], ho = p.hist_out_off[s];
-            const int hn = (int)(he - hb);
-            for (int i = tid; i < hn; i += GSTAR_NT) {
-                const int x = (int)(uint32_t)p.hist_keys[hb + i];
-                int lo = 0, hi = m;
-                while (lo < hi) {
-                    const int mid = (lo + hi) >> 1;
-                    if (L.ins[mid] < x) lo = mid + 1;
-                    else hi = mid;
-                }
-                p.hist_out[ho + i + lo] = p.hist_keys[hb + i];
-            }
-            for (int i = tid; i < m; i += GSTAR_NT) {
-                const int x = L.ins[i];
-                int64_t lo = hb, hi = he;
-                while (lo < hi) {
-                    const int64_t mid = (lo + hi) >> 1;
-                    if ((int)(uint32_t)p.hist_keys[mid] < x) lo = mid + 1;
-                    else hi = mid;
-                }
-                p.hist_out[ho + i + (lo - hb)] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)x;
-            }
-        }
+        const int hn = star_history_merge(p, s, L.ins, m, tid, GSTAR_NT);
+        __syncthreads();
+        if (tid == 0) star_commit(p, s, vcount, vo, tcount, to, hn + m);
         __syncthreads();
     }
 }
@@ -951,7 +917,7 @@ __global__ void k_need_max(StarParams p, const int *list, int nitems, unsigned l
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += gridDim.x * blockDim.x) {
         const int s = list[i];
-        const int64_t need = (p.ins_off[s + 1] - p.ins_off[s]) + (p.mode ? p.in_vnum[s] : 0);
+        const int64_t need = (p.ins_off[s + 1] - p.ins_off[s]) + (p.mode ? p.vnum[s] : 0);
         atomicMax(out, (unsigned long long)need);
     }
 }
@@ -1112,7 +1078,8 @@ hipError_t launch_nonmutual(const uint64_t *keys, int64_t m, uint64_t *tasks, un
     return hipGetLastError();
 }
 
-__global__ void k_hist_filter(const uint64_t *pairs, int64_t m, const uint64_t *hist, const int64_t *hoff, uint8_t *keep)
+__global__ void k_hist_filter(const uint64_t *pairs, int64_t m, const int *hpool, const int64_t *hoff, const int *hnum,
+                              uint8_t *keep)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
         const uint64_t k = pairs[i];
@@ -1121,21 +1088,23 @@ __global__ void k_hist_filter(const uint64_t *pairs, int64_t m, const uint64_t *
             keep[i] = 0;
             continue;
         }
-        int64_t lo = hoff[a], hi = hoff[a + 1];
+        const int *h = hpool + hoff[a];
+        const int hn = hnum[a];
+        int lo = 0, hi = hn;
         while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (hist[mid] < k) lo = mid + 1;
+            const int mid = (lo + hi) >> 1;
+            if ((uint32_t)h[mid] < b) lo = mid + 1;
             else hi = mid;
         }
-        keep[i] = (lo < hoff[a + 1] && hist[lo] == k) ? 0 : 1;
+        keep[i] = (lo < hn && (uint32_t)h[lo] == b) ? 0 : 1;
     }
 }
 
-hipError_t launch_hist_filter(const uint64_t *pairs, int64_t m, const uint64_t *hist_keys, const int64_t *hist_off,
+hipError_t launch_hist_filter(const uint64_t *pairs, int64_t m, const int *hpool, const int64_t *hoff, const int *hnum,
                               uint8_t *keep, hipStream_t s)
 {
     if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_hist_filter, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, pairs, m, hist_keys, hist_off,
+    hipLaunchKernelGGL(k_hist_filter, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, pairs, m, hpool, hoff, hnum,
                        keep);
     return hipGetLastError();
 }
@@ -1155,24 +1124,120 @@ hipError_t launch_reciprocate(uint64_t *keys, int64_t m, hipStream_t s)
     return hipGetLastError();
 }
 
-__global__ void k_capacities(int n, const int *vnum, const int64_t *ins_off, const int64_t *hist_off, int64_t *vcap,
-                             int64_t *tcap, int64_t *hcap)
+__global__ void k_active_flags(const int64_t *ins_off, int n, uint32_t *flags)
 {
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x)
+        flags[s] = ins_off[s + 1] > ins_off[s] ? 1u : 0u;
+}
+
+hipError_t launch_active_flags(const int64_t *ins_off, int n, uint32_t *flags, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_active_flags, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, ins_off, n, flags);
+    return hipGetLastError();
+}
+
+__global__ void k_active_scatter(const uint32_t *flags, const uint32_t *pos, int n, int *list)
+{
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x)
+        if (flags[s]) list[pos[s]] = s;
+}
+
+hipError_t launch_active_scatter(const uint32_t *flags, const uint32_t *pos, int n, int *list, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_active_scatter, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, flags, pos, n, list);
+    return hipGetLastError();
+}
+
+__global__ void k_capacities(const int *list, int nitems, const int *vnum, const int *hnum, const int64_t *ins_off,
+                             int64_t *vcap, int64_t *tcap, int64_t *hcap)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += gridDim.x * blockDim.x) {
+        const int s = list ? list[i] : i;
         const int64_t m = ins_off[s + 1] - ins_off[s];
         const int64_t v = (vnum ? vnum[s] : 0) + m;
-        vcap[s] = v;
+        vcap[i] = v;
         const int64_t t = v >= 4 ? 2 * v - 4 : 4;
-        tcap[s] = v >= STAR_WIDE_MIN ? 2 * t : t;
-        if (hcap) hcap[s] = (hist_off[s + 1] - hist_off[s]) + m;
+        tcap[i] = v >= STAR_WIDE_MIN ? 2 * t : t;
+        hcap[i] = (hnum ? hnum[s] : 0) + m;
     }
 }
 
-hipError_t launch_capacities(int n, const int *vnum, const int64_t *ins_off, const int64_t *hist_off, int64_t *vcap,
-                             int64_t *tcap, int64_t *hcap, hipStream_t s)
+hipError_t launch_capacities(const int *list, int nitems, const int *vnum, const int *hnum, const int64_t *ins_off,
+                             int64_t *vcap, int64_t *tcap, int64_t *hcap, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_capacities, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, n, vnum, ins_off, hist_off, vcap,
-                       tcap, hcap);
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_capacities, dim3(g4d_blocks(nitems, 256, 65536)), dim3(256), 0, s, list, nitems, vnum, hnum,
+                       ins_off, vcap, tcap, hcap);
+    return hipGetLastError();
+}
+
+__global__ void k_assign_regions(const int *list, int nitems, const int64_t *vscan, const int64_t *tscan,
+                                 const int64_t *hscan, int64_t vbase, int64_t tbase, int64_t hbase, int64_t *nvoff,
+                                 int64_t *ntoff, int64_t *nhoff)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += gridDim.x * blockDim.x) {
+        const int s = list ? list[i] : i;
+        nvoff[s] = vbase + vscan[i];
+        ntoff[s] = tbase + tscan[i];
+        nhoff[s] = hbase + hscan[i];
+    }
+}
+
+hipError_t launch_assign_regions(const int *list, int nitems, const int64_t *vscan, const int64_t *tscan,
+                                 const int64_t *hscan, int64_t vbase, int64_t tbase, int64_t hbase, int64_t *nvoff,
+                                 int64_t *ntoff, int64_t *nhoff, hipStream_t s)
+{
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_assign_regions, dim3(g4d_blocks(nitems, 256, 65536)), dim3(256), 0, s, list, nitems, vscan,
+                       tscan, hscan, vbase, tbase, hbase, nvoff, ntoff, nhoff);
+    return hipGetLastError();
+}
+
+__global__ void k_live_sizes(int n, const int *vnum, const int *tnum, const int *hnum, int64_t *vsz, int64_t *tsz,
+                             int64_t *hsz)
+{
+    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
+        const int v = vnum[s];
+        vsz[s] = v;
+        tsz[s] = v >= STAR_WIDE_MIN ? 2 * (int64_t)tnum[s] : tnum[s];
+        hsz[s] = hnum[s];
+    }
+}
+
+hipError_t launch_live_sizes(int n, const int *vnum, const int *tnum, const int *hnum, int64_t *vsz, int64_t *tsz,
+                             int64_t *hsz, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_live_sizes, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, n, vnum, tnum, hnum, vsz, tsz,
+                       hsz);
+    return hipGetLastError();
+}
+
+// One wave per star.
+__global__ void __launch_bounds__(256) k_repack(int n, const int *vnum, const int64_t *voff, const int *tnum,
+                                                const int64_t *toff, const int *hnum, const int64_t *hoff,
+                                                const int *vpool, const uint64_t *tpool, const int *hpool,
+                                                const int64_t *vdst, const int64_t *tdst, const int64_t *hdst,
+                                                int *vpool2, uint64_t *tpool2, int *hpool2)
+{
+    const int lane = threadIdx.x & 63;
+    const int w0 = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nw = (gridDim.x * blockDim.x) >> 6;
+    for (int s = w0; s < n; s += nw) {
+        const int v = vnum[s], h = hnum[s];
+        const int64_t tw = v >= STAR_WIDE_MIN ? 2 * (int64_t)tnum[s] : tnum[s];
+        const int64_t vi = voff[s], ti = toff[s], hi = hoff[s], vo = vdst[s], to = tdst[s], ho = hdst[s];
+        for (int i = lane; i < v; i += 64) vpool2[vo + i] = vpool[vi + i];
+        for (int64_t i = lane; i < tw; i += 64) tpool2[to + i] = tpool[ti + i];
+        for (int i = lane; i < h; i += 64) hpool2[ho + i] = hpool[hi + i];
+    }
+}
+
+hipError_t launch_repack(int n, const int *vnum, const int64_t *voff, const int *tnum, const int64_t *toff,
+                         const int *hnum, const int64_t *hoff, const int *vpool, const uint64_t *tpool,
+                         const int *hpool, const int64_t *vdst, const int64_t *tdst, const int64_t *hdst, int *vpool2,
+                         uint64_t *tpool2, int *hpool2, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_repack, dim3(g4d_blocks(n, 4, 65536)), dim3(256), 0, s, n, vnum, voff, tnum, toff, hnum, hoff,
+                       vpool, tpool, hpool, vdst, tdst, hdst, vpool2, tpool2, hpool2);
     return hipGetLastError();
 }
 

@@ -87,6 +87,9 @@ typedef struct {
     int64_t exact_stars;       /* star passes handed to the exact-capable star kernel */
     int64_t pba_bytes;         /* algorithmic PBA bytes: 24 G^3 */
     double t_pba_pass_ms[3];   /* per axis pass (z, y, x) */
+    int64_t star_store_bytes;  /* star arenas in use at the end (vertices, triangles, history) */
+    int32_t store_repacks;     /* times the star arenas were compacted into larger ones */
+    int32_t star_passes;       /* star kernel passes: creation plus one per splaying round */
 } g4d_stats;
 
 /* Mesh validation report (GDelaunay/Common/Geometry.cu:200-393 plus adjacency and a full
