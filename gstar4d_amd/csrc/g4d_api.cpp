@@ -62,7 +62,7 @@ struct g4d_ctx {
     g4d::Prims P;
     DevBuf xyz_up, keys, pts, perm, vox, bounds, grid, gtmp, idgrid, pbaspill, pcount, pairs, ws_off, cnt;
     Store store;
-    DevBuf drown, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, aflags, apos, active, big, huge, xstars,
+    DevBuf drown, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, aflags, apos, active, mid, big, huge, xstars,
         xtasks, xtris, gwork, tetcnt, tetoff, tets, opp, table;
     std::vector<int32_t> h_tets, h_opp, h_perm;
     const double *last_xyz = nullptr; // device points of the last triangulation (input order)
@@ -348,11 +348,12 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
     HCK(ctx->vcap.ensure(sizeof(int64_t) * (size_t)(n + 1)));
     HCK(ctx->tcap.ensure(sizeof(int64_t) * (size_t)(n + 1)));
     HCK(ctx->hcap.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+    HCK(ctx->mid.ensure(sizeof(int) * (size_t)n));
     HCK(ctx->big.ensure(sizeof(int) * (size_t)n));
     HCK(ctx->huge.ensure(sizeof(int) * (size_t)n));
     HCK(ctx->xstars.ensure(sizeof(int) * (size_t)n));
     int64_t *vcap = buf<int64_t>(ctx->vcap), *tcap = buf<int64_t>(ctx->tcap), *hcap = buf<int64_t>(ctx->hcap);
-    const int nblk_star = 4096;
+    const int nblk_star = 8192; // 256 CUs x 16 resident waves of the 32-vertex tier, twice over
     const int lds_limit = std::min(g4d::STAR_LDS_LIMIT, ctx->opt.lds_star_limit > 0 ? ctx->opt.lds_star_limit
                                                                                   : g4d::STAR_LDS_LIMIT);
 
@@ -444,20 +445,33 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
         sp.tpairs_cap = tpcap;
         sp.cnt = cnt;
         sp.lds_limit = lds_limit;
-        sp.defer_list = buf<int>(ctx->big);
+        sp.defer_list = buf<int>(ctx->mid);
         sp.defer_cap = n;
-        sp.defer_counter = G4D_CNT_APPEND2;
+        sp.defer_counter = G4D_CNT_APPEND6;
         sp.exact_list = buf<int>(ctx->xstars);
         sp.exact_cap = n;
         sp.exact_counter = G4D_CNT_APPEND4;
         HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8 * 3, s));
         HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND3, 0, 8 * 2, s));
+        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND6, 0, 8, s));
         HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
         sp.star_list = list;
         sp.nlist = nitems;
-        // filter-only LDS kernels; stars they cannot decide go to the exact-capable kernels
-        HCK(g4d::launch_star_kernel(sp, 0, false, nblk_star, s));
+        // Filter-only LDS kernels by size tier (32 -> 64 -> 256 link vertices); each passes the
+        // stars it cannot hold to the next tier's list and the ones its filters cannot decide
+        // to the exact list.
+        HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T0, false, nblk_star, s));
         CK(read_counters(ctx, hc));
+        const int64_t nmid = (int64_t)hc[G4D_CNT_APPEND6];
+        if (nmid > 0) {
+            sp.star_list = buf<int>(ctx->mid);
+            sp.nlist = (int)nmid;
+            sp.defer_list = buf<int>(ctx->big);
+            sp.defer_counter = G4D_CNT_APPEND2;
+            HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
+            HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T1, false, 4096, s));
+            CK(read_counters(ctx, hc));
+        }
         const int64_t nbig = (int64_t)hc[G4D_CNT_APPEND2];
         if (nbig > 0) {
             ctx->st.big_stars += (int32_t)nbig;
@@ -466,19 +480,43 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
             sp.defer_list = buf<int>(ctx->huge);
             sp.defer_counter = G4D_CNT_APPEND3;
             HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
-            HCK(g4d::launch_star_kernel(sp, 1, false, 1024, s));
+            HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T2, false, 1024, s));
             CK(read_counters(ctx, hc));
         }
+        // Exact-capable kernels over the stars the filters could not decide, by the same size
+        // tiers (the filter tiers' lists are reused), at most 1,024 work-groups each.
         const int64_t nexact = (int64_t)hc[G4D_CNT_APPEND4];
         if (nexact > 0) {
             ctx->st.exact_stars += nexact;
             sp.star_list = buf<int>(ctx->xstars);
             sp.nlist = (int)nexact;
-            sp.defer_list = buf<int>(ctx->huge);
-            sp.defer_counter = G4D_CNT_APPEND3;
+            sp.defer_list = buf<int>(ctx->mid);
+            sp.defer_counter = G4D_CNT_APPEND6;
+            HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND6, 0, 8, s));
             HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
-            HCK(g4d::launch_star_kernel(sp, 1, true, 1024, s));
+            HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T0, true, 1024, s));
             CK(read_counters(ctx, hc));
+            const int64_t xmid = (int64_t)hc[G4D_CNT_APPEND6];
+            if (xmid > 0) {
+                sp.star_list = buf<int>(ctx->mid);
+                sp.nlist = (int)xmid;
+                sp.defer_list = buf<int>(ctx->big);
+                sp.defer_counter = G4D_CNT_APPEND2;
+                HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND2, 0, 8, s));
+                HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
+                HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T1, true, 1024, s));
+                CK(read_counters(ctx, hc));
+                const int64_t xbig = (int64_t)hc[G4D_CNT_APPEND2];
+                if (xbig > 0) {
+                    sp.star_list = buf<int>(ctx->big);
+                    sp.nlist = (int)xbig;
+                    sp.defer_list = buf<int>(ctx->huge);
+                    sp.defer_counter = G4D_CNT_APPEND3;
+                    HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
+                    HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T2, true, 1024, s));
+                    CK(read_counters(ctx, hc));
+                }
+            }
         }
         const int64_t nhuge = (int64_t)hc[G4D_CNT_APPEND3];
         if (nhuge > 0) {
@@ -495,10 +533,10 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
                                         nblk, s));
             CK(read_counters(ctx, hc));
         }
-        trace("star pass mode %d: %lld insertions into %d stars, large stars %lld, exact %lld, global-memory %lld, "
-              "drowned %llu, pairs %llu, err 0x%llx",
-              mode, (long long)nins, nitems, (long long)nbig, (long long)nexact, (long long)nhuge, hc[G4D_CNT_APPEND0],
-              hc[G4D_CNT_APPEND1], hc[G4D_CNT_ERROR]);
+        trace("star pass mode %d: %lld insertions into %d stars, over 32: %lld, over 64: %lld, exact %lld, "
+              "global-memory %lld, drowned %llu, pairs %llu, err 0x%llx",
+              mode, (long long)nins, nitems, (long long)nmid, (long long)nbig, (long long)nexact, (long long)nhuge,
+              hc[G4D_CNT_APPEND0], hc[G4D_CNT_APPEND1], hc[G4D_CNT_ERROR]);
         CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], mode ? "splaying" : "star creation"));
         *drown_n = (int64_t)hc[G4D_CNT_APPEND0];
         *tp_n = (int64_t)hc[G4D_CNT_APPEND1];
@@ -725,7 +763,7 @@ void g4d_destroy(g4d_ctx *ctx)
     DevBuf *bufs[] = {&ctx->xyz_up, &ctx->keys, &ctx->pts, &ctx->perm, &ctx->vox, &ctx->bounds, &ctx->grid,
                       &ctx->gtmp, &ctx->idgrid, &ctx->pbaspill, &ctx->pcount, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->drown, &ctx->dkeys,
                       &ctx->tasks, &ctx->cand, &ctx->keep, &ctx->ins, &ctx->ins_off, &ctx->vcap, &ctx->tcap,
-                      &ctx->hcap, &ctx->aflags, &ctx->apos, &ctx->active, &ctx->big, &ctx->huge, &ctx->xstars, &ctx->xtasks, &ctx->xtris, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table};
+                      &ctx->hcap, &ctx->aflags, &ctx->apos, &ctx->active, &ctx->mid, &ctx->big, &ctx->huge, &ctx->xstars, &ctx->xtasks, &ctx->xtris, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table};
     for (DevBuf *b : bufs) b->release();
     ctx->store.release();
     ctx->P.release();

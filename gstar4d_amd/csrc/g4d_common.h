@@ -42,6 +42,7 @@ enum g4d_counter : int {
     G4D_CNT_APPEND3 = 10,
     G4D_CNT_APPEND4 = 11,
     G4D_CNT_APPEND5 = 12,
+    G4D_CNT_APPEND6 = 13,
     G4D_CNT_NUM = 16
 };
 

@@ -120,6 +120,126 @@ __device__ __noinline__ void det2(uint32_t *r, const uint32_t *a, const uint32_t
 } // namespace rt
 
 // ------------------------------------------------------------------------------------
+// Fixed-width signed integers of N 64-bit limbs (two's complement), fully unrolled in
+// registers.  Used when every coordinate of a predicate fits 61 bits on the common scale
+// (the usual case: FP64 inputs spanning fewer than 8 binades), where the widths below are
+// known at compile time; wider inputs take the runtime-width path above.
+// ------------------------------------------------------------------------------------
+
+namespace fx {
+
+typedef unsigned __int128 u128;
+
+template <int N>
+struct I {
+    uint64_t w[N];
+};
+
+template <int N>
+__device__ __forceinline__ I<N> from64(int64_t v)
+{
+    I<N> r;
+    r.w[0] = (uint64_t)v;
+#pragma unroll
+    for (int i = 1; i < N; i++) r.w[i] = v < 0 ? ~0ull : 0ull;
+    return r;
+}
+
+template <int N, int M>
+__device__ __forceinline__ I<N> ext(const I<M> &a)
+{
+    I<N> r;
+    const uint64_t e = (a.w[M - 1] >> 63) ? ~0ull : 0ull;
+#pragma unroll
+    for (int i = 0; i < N; i++) r.w[i] = i < M ? a.w[i] : e;
+    return r;
+}
+
+template <int N>
+__device__ __forceinline__ I<N> add(const I<N> &a, const I<N> &b)
+{
+    I<N> r;
+    uint64_t c = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const u128 t = (u128)a.w[i] + b.w[i] + c;
+        r.w[i] = (uint64_t)t;
+        c = (uint64_t)(t >> 64);
+    }
+    return r;
+}
+
+template <int N>
+__device__ __forceinline__ I<N> sub(const I<N> &a, const I<N> &b)
+{
+    I<N> r;
+    uint64_t c = 1;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const u128 t = (u128)a.w[i] + (~b.w[i]) + c;
+        r.w[i] = (uint64_t)t;
+        c = (uint64_t)(t >> 64);
+    }
+    return r;
+}
+
+template <int N>
+__device__ __forceinline__ I<N> neg(const I<N> &a)
+{
+    I<N> r;
+    uint64_t c = 1;
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const u128 t = (u128)(~a.w[i]) + c;
+        r.w[i] = (uint64_t)t;
+        c = (uint64_t)(t >> 64);
+    }
+    return r;
+}
+
+template <int N>
+__device__ __forceinline__ int sign(const I<N> &a)
+{
+    if (a.w[N - 1] >> 63) return -1;
+    uint64_t o = 0;
+#pragma unroll
+    for (int i = 0; i < N; i++) o |= a.w[i];
+    return o ? 1 : 0;
+}
+
+// Exact signed product (|a| < 2^(64NA-1), |b| < 2^(64NB-1)): magnitudes, schoolbook, sign.
+template <int NA, int NB>
+__device__ __forceinline__ I<NA + NB> mul(const I<NA> &a, const I<NB> &b)
+{
+    const bool sa = a.w[NA - 1] >> 63, sb = b.w[NB - 1] >> 63;
+    const I<NA> ma = sa ? neg(a) : a;
+    const I<NB> mb = sb ? neg(b) : b;
+    I<NA + NB> r;
+#pragma unroll
+    for (int i = 0; i < NA + NB; i++) r.w[i] = 0;
+#pragma unroll
+    for (int i = 0; i < NA; i++) {
+        uint64_t carry = 0;
+#pragma unroll
+        for (int j = 0; j < NB; j++) {
+            const u128 t = (u128)ma.w[i] * mb.w[j] + r.w[i + j] + carry;
+            r.w[i + j] = (uint64_t)t;
+            carry = (uint64_t)(t >> 64);
+        }
+        r.w[i + NB] = carry;
+    }
+    return sa != sb ? neg(r) : r;
+}
+
+template <int N>
+__device__ __forceinline__ I<2 * N> det2(const I<N> &a, const I<N> &b, const I<N> &c, const I<N> &d)
+{
+    return sub(mul(a, b), mul(c, d));
+}
+
+} // namespace fx
+
+// ------------------------------------------------------------------------------------
 // Fixed-point conversion of the FP64 inputs of one predicate
 // ------------------------------------------------------------------------------------
 
@@ -329,6 +449,90 @@ __device__ __noinline__ int exact_minor_rt(const FixedPts &f, int k, uint32_t ro
 }
 
 // ------------------------------------------------------------------------------------
+// The same determinants at compile-time widths for coordinates of at most 61 bits on the
+// common scale (FixedPts::limbs == 2): a coordinate is one 64-bit limb, a translated
+// coordinate fits 62 bits, a 2x2 minor 125, a 3x3 minor 189, a lift 126 and the lifted 4x4
+// determinant 317 bits.
+// ------------------------------------------------------------------------------------
+
+__device__ __forceinline__ int64_t fixed_coord64(const FixedPts &f, int i, int c)
+{
+    const int64_t mm = f.m[i][c];
+    return mm == 0 ? 0 : mm * ((int64_t)1 << (f.e[i][c] - f.emin));
+}
+
+__device__ __noinline__ int exact_insphere_fx(const FixedPts &f)
+{
+    fx::I<1> d[4][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int64_t e = fixed_coord64(f, 4, c);
+#pragma unroll
+        for (int i = 0; i < 4; i++) d[i][c] = fx::from64<1>(fixed_coord64(f, i, c) - e);
+    }
+    fx::I<5> acc = fx::from64<5>(0);
+#pragma unroll
+    for (int out = 0; out < 4; out++) {
+        const int r0 = out == 0 ? 1 : 0, r1 = out <= 1 ? 2 : 1, r2 = out <= 2 ? 3 : 2;
+        const fx::I<3> m3 = fx::add(fx::sub(fx::mul(d[r0][2], fx::det2(d[r1][0], d[r2][1], d[r2][0], d[r1][1])),
+                                            fx::mul(d[r1][2], fx::det2(d[r0][0], d[r2][1], d[r2][0], d[r0][1]))),
+                                    fx::mul(d[r2][2], fx::det2(d[r0][0], d[r1][1], d[r1][0], d[r0][1])));
+        const fx::I<2> l = fx::add(fx::add(fx::mul(d[out][0], d[out][0]), fx::mul(d[out][1], d[out][1])),
+                                   fx::mul(d[out][2], d[out][2]));
+        const fx::I<5> t = fx::mul(l, m3);
+        acc = (out & 1) ? fx::add(acc, t) : fx::sub(acc, t);
+    }
+    return fx::sign(acc);
+}
+
+__device__ __noinline__ int exact_orient3d_fx(const FixedPts &f)
+{
+    fx::I<1> d[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+        const int64_t e = fixed_coord64(f, 3, c);
+#pragma unroll
+        for (int i = 0; i < 3; i++) d[i][c] = fx::from64<1>(fixed_coord64(f, i, c) - e);
+    }
+    const fx::I<3> acc = fx::add(fx::sub(fx::mul(d[0][2], fx::det2(d[1][0], d[2][1], d[2][0], d[1][1])),
+                                         fx::mul(d[1][2], fx::det2(d[0][0], d[2][1], d[2][0], d[0][1]))),
+                                 fx::mul(d[2][2], fx::det2(d[0][0], d[1][1], d[1][0], d[0][1])));
+    return fx::sign(acc);
+}
+
+// SoS minor (as exact_minor_rt) with every entry, coordinate or lift, in two limbs.
+__device__ __noinline__ int exact_minor_fx(const FixedPts &f, int k, uint32_t rows, uint32_t cols)
+{
+    fx::I<2> a[4][3];
+#pragma unroll 1
+    for (int i = 0; i < k; i++) {
+        const int r = (rows >> (4 * i)) & 15;
+#pragma unroll 1
+        for (int j = 0; j < k - 1; j++) {
+            const int c = (cols >> (4 * j)) & 15;
+            if (c < 3) {
+                a[i][j] = fx::from64<2>(fixed_coord64(f, r, c));
+            } else {
+                const fx::I<1> x = fx::from64<1>(fixed_coord64(f, r, 0)), y = fx::from64<1>(fixed_coord64(f, r, 1)),
+                               z = fx::from64<1>(fixed_coord64(f, r, 2));
+                a[i][j] = fx::add(fx::add(fx::mul(x, x), fx::mul(y, y)), fx::mul(z, z));
+            }
+        }
+    }
+#pragma unroll 1
+    for (int i = 0; i < k - 1; i++)
+#pragma unroll 1
+        for (int j = 0; j < k - 1; j++) a[i][j] = fx::sub(a[i][j], a[k - 1][j]);
+    if (k == 2) return fx::sign(a[0][0]);
+    if (k == 3) return fx::sign(fx::det2(a[0][0], a[1][1], a[1][0], a[0][1]));
+    const fx::I<6> acc =
+        fx::add(fx::sub(fx::mul(a[0][2], fx::det2(a[1][0], a[2][1], a[2][0], a[1][1])),
+                        fx::mul(a[1][2], fx::det2(a[0][0], a[2][1], a[2][0], a[0][1]))),
+                fx::mul(a[2][2], fx::det2(a[0][0], a[1][1], a[1][0], a[0][1])));
+    return fx::sign(acc);
+}
+
+// ------------------------------------------------------------------------------------
 // SoS depth table (GD/GDelaunay/GDelPredDevice.h:194-660): for depth 1..48, the rows of the
 // index-sorted points, the columns (0=x 1=y 2=z 3=lift) and whether the sign flips.
 // Packed: bits 0..15 rows (nibbles), 16..27 cols (nibbles), 28..30 k (rows), 31 flip.
@@ -374,6 +578,7 @@ __device__ __noinline__ int exact_insphere(const double *const *p, unsigned long
 {
     FixedPts f;
     fixed_analyse(p, 5, f);
+    if (f.limbs == 2) return exact_insphere_fx(f);
     if (f.limbs) return exact_insphere_rt(f);
     pred_flag_error(cnt, G4D_DEV_ERR_PREDICATE_RANGE);
     return 0;
@@ -383,6 +588,7 @@ __device__ __noinline__ int exact_orient3d(const double *const *p, unsigned long
 {
     FixedPts f;
     fixed_analyse(p, 4, f);
+    if (f.limbs == 2) return exact_orient3d_fx(f);
     if (f.limbs) return exact_orient3d_rt(f);
     pred_flag_error(cnt, G4D_DEV_ERR_PREDICATE_RANGE);
     return 0;
@@ -416,7 +622,8 @@ __device__ __noinline__ int sos_orient4(const double *const *pin, const int *idx
     for (int d = 1; d <= 48 && f.limbs; d++) {
         const uint32_t e = kSosTable[d];
         const int k = (e >> 28) & 7;
-        const int s = exact_minor_rt(f, k, e & 0xffff, (e >> 16) & 0xfff);
+        const int s = f.limbs == 2 ? exact_minor_fx(f, k, e & 0xffff, (e >> 16) & 0xfff)
+                                   : exact_minor_rt(f, k, e & 0xffff, (e >> 16) & 0xfff);
         if (s != 0) {
             sign = (e >> 31) ? -s : s;
             depth = d;

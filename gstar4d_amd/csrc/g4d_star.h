@@ -9,8 +9,10 @@ namespace g4d {
 // triangle in one word: local vertex indices (9 bits each), adjacency slots (10 bits each),
 // checked flag (bit 57).  A larger star stores two words per triangle: local vertices
 // (21 bits each, checked flag in bit 63), then adjacency slots (21 bits each).
-static constexpr int STAR_VMAX_SMALL = 64;
-static constexpr int STAR_VMAX_BIG = 256;
+// LDS star kernel tiers by link vertices (plus this pass's insertions).
+static constexpr int STAR_VMAX_T0 = 32;
+static constexpr int STAR_VMAX_T1 = 64;
+static constexpr int STAR_VMAX_T2 = 256;
 static constexpr int STAR_WIDE_MIN = 512;
 // Stars whose link (plus this pass's insertions) exceeds the LDS kernels' limit run in the
 // global-memory star kernel, which has no fixed capacity.
@@ -61,11 +63,11 @@ struct StarParams {
     unsigned long long *cnt;
 };
 
-// tier 0: LDS kernel for up to STAR_VMAX_SMALL link vertices; tier 1: up to STAR_VMAX_BIG.
-// Both are filter-only (no exact arithmetic, no private scratch): a star with an
-// inconclusive orientation goes to exact_list.  exact = true runs the STAR_VMAX_BIG kernel
-// with the exact predicates and SoS on the stars of star_list.
-hipError_t launch_star_kernel(const StarParams &p, int tier, bool exact, int nblocks, hipStream_t s);
+// LDS star kernel for stars of up to vmax (STAR_VMAX_T0, T1 or T2) link vertices plus
+// insertions; larger ones go to p.defer_list.  exact = false: filters only (no multiprecision
+// code, no private scratch), a star with an inconclusive orientation goes to p.exact_list
+// untouched.  exact = true: exact predicates and SoS, for the stars of star_list.
+hipError_t launch_star_kernel(const StarParams &p, int vmax, bool exact, int nblocks, hipStream_t s);
 // Global-memory star kernel for the stars in `list` (any size up to vcap link vertices);
 // each block owns a workspace of star_global_bytes(vcap) bytes at ws + blockIdx * stride.
 int64_t star_global_bytes(int vcap);

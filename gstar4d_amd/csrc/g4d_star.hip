@@ -165,7 +165,7 @@ __device__ __forceinline__ void star_commit(const StarParams &p, int s, int vcou
 // p.exact_list untouched (its output is written only at the end), so the kernel carries no
 // multiprecision code and no private scratch.
 template <int VMAX, bool EXACT>
-__global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
+__device__ __forceinline__ void star_body(const StarParams &p, int nitems)
 {
     constexpr int TMAX = StarLds<VMAX>::TMAX;
     constexpr int KPL = TMAX / 64;
@@ -504,18 +504,43 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
     }
 }
 
-hipError_t launch_star_kernel(const StarParams &p, int tier, bool exact, int nblocks, hipStream_t s)
+// The 32-vertex tier (7 KB of LDS) is compiled for 4 waves per SIMD; the larger tiers are
+// limited by their LDS footprint instead.
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) k_star32(StarParams p, int nitems)
+{
+    star_body<STAR_VMAX_T0, false>(p, nitems);
+}
+
+template <int VMAX, bool EXACT>
+__global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
+{
+    star_body<VMAX, EXACT>(p, nitems);
+}
+
+hipError_t launch_star_kernel(const StarParams &p, int vmax, bool exact, int nblocks, hipStream_t s)
 {
     const int nitems = p.star_list ? p.nlist : p.n;
     if (nitems <= 0) return hipSuccess;
     if (p.lds_limit > STAR_LDS_LIMIT) return hipErrorInvalidValue; // local ids must fit 9 bits
     if (nblocks > nitems) nblocks = nitems;
-    if (exact)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_star<STAR_VMAX_BIG, true>), dim3(nblocks), dim3(64), 0, s, p, nitems);
-    else if (tier == 1)
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_star<STAR_VMAX_BIG, false>), dim3(nblocks), dim3(64), 0, s, p, nitems);
-    else
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_star<STAR_VMAX_SMALL, false>), dim3(nblocks), dim3(64), 0, s, p, nitems);
+    if (exact) {
+        if (vmax == STAR_VMAX_T0)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_star<STAR_VMAX_T0, true>), dim3(nblocks), dim3(64), 0, s, p, nitems);
+        else if (vmax == STAR_VMAX_T1)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_star<STAR_VMAX_T1, true>), dim3(nblocks), dim3(64), 0, s, p, nitems);
+        else if (vmax == STAR_VMAX_T2)
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_star<STAR_VMAX_T2, true>), dim3(nblocks), dim3(64), 0, s, p, nitems);
+        else
+            return hipErrorInvalidValue;
+    } else if (vmax == STAR_VMAX_T0) {
+        hipLaunchKernelGGL(k_star32, dim3(nblocks), dim3(64), 0, s, p, nitems);
+    } else if (vmax == STAR_VMAX_T1) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_star<STAR_VMAX_T1, false>), dim3(nblocks), dim3(64), 0, s, p, nitems);
+    } else if (vmax == STAR_VMAX_T2) {
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_star<STAR_VMAX_T2, false>), dim3(nblocks), dim3(64), 0, s, p, nitems);
+    } else {
+        return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
