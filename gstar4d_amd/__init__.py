@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 __all__ = ["Triangulator", "Options", "Stats", "CheckReport", "G4DError", "load_library", "gdel_init",
-           "gdel_compute", "get_tetra", "gdel_deinit", "canonical", "LIB_PATH"]
+           "gdel_compute", "get_tetra", "gdel_deinit", "canonical", "LIB_PATH", "PointFile", "read_asc",
+           "write_asc", "write_ply"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgstar4d.so")
@@ -65,6 +66,13 @@ class Stats(ctypes.Structure):
         return d
 
 
+class PointFile(ctypes.Structure):
+    """g4d_point_file: points of an .asc file (GD/Main/Application.cpp:475-585)."""
+    _fields_ = [("num_points", ctypes.c_int32), ("pad", ctypes.c_int32), ("num_read", ctypes.c_int64),
+                ("num_duplicates", ctypes.c_int64), ("min_val", ctypes.c_float), ("max_val", ctypes.c_float),
+                ("points", ctypes.POINTER(ctypes.c_double)), ("scaled", ctypes.POINTER(ctypes.c_double))]
+
+
 class CheckReport(ctypes.Structure):
     """g4d_check_report: TetraMesh::check (GD/Common/Geometry.cu:200-393) plus adjacency and
     empty-sphere counts."""
@@ -87,7 +95,8 @@ EXPORTS = ["g4d_default_options", "g4d_create", "g4d_destroy", "g4d_last_error",
            "g4d_triangulate", "g4d_triangulate_device", "g4d_copy_result", "g4d_upload_points",
            "g4d_get_stats", "g4d_set_options", "g4d_diag_orient4", "g4d_diag_orient3d",
            "g4d_diag_morton", "g4d_diag_pba", "g4d_diag_grid_pairs", "g4d_diag_copy",
-           "g4d_check_result", "g4d_check_mesh"]
+           "g4d_check_result", "g4d_check_mesh", "g4d_read_asc", "g4d_free_point_file", "g4d_write_asc",
+           "g4d_write_ply"]
 
 
 def load_library(path=None):
@@ -120,8 +129,50 @@ def load_library(path=None):
     L.g4d_check_result.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(CheckReport)]
     L.g4d_check_mesh.argtypes = [_vp, _dp, ctypes.c_int32, _ip, _ip, ctypes.c_int64, ctypes.c_int32,
                                  ctypes.POINTER(CheckReport)]
+    L.g4d_read_asc.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(PointFile)]
+    L.g4d_free_point_file.argtypes = [ctypes.POINTER(PointFile)]
+    L.g4d_free_point_file.restype = None
+    L.g4d_write_asc.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int32]
+    L.g4d_write_ply.argtypes = [ctypes.c_char_p, _dp, ctypes.c_int32, _ip, ctypes.c_int64]
     _LIB = L
     return L
+
+
+def read_asc(path, grid_size):
+    """Reads an .asc point file as the reference's Application::readPoints does.  Returns
+    (points, scaled, info): (N,3) float64 arrays and a dict with num_read, num_duplicates,
+    min_val, max_val."""
+    L = load_library()
+    f = PointFile()
+    rc = L.g4d_read_asc(os.fsencode(path), int(grid_size), ctypes.byref(f))
+    try:
+        if rc != 0:
+            raise G4DError(rc, "g4d_read_asc(%s) failed" % path)
+        n = f.num_points
+        pts = np.ctypeslib.as_array(f.points, shape=(max(n, 1) * 3,))[:3 * n].reshape(n, 3).copy()
+        sc = np.ctypeslib.as_array(f.scaled, shape=(max(n, 1) * 3,))[:3 * n].reshape(n, 3).copy()
+        info = {"num_read": f.num_read, "num_duplicates": f.num_duplicates, "min_val": f.min_val,
+                "max_val": f.max_val}
+    finally:
+        L.g4d_free_point_file(ctypes.byref(f))
+    return pts, sc, info
+
+
+def write_asc(path, xyz):
+    L = load_library()
+    x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    rc = L.g4d_write_asc(os.fsencode(path), _ptr(x, _dp), x.shape[0])
+    if rc != 0:
+        raise G4DError(rc, "g4d_write_asc(%s) failed" % path)
+
+
+def write_ply(path, xyz, tets):
+    L = load_library()
+    x = np.ascontiguousarray(xyz, np.float64).reshape(-1, 3)
+    t = np.ascontiguousarray(tets, np.int32).reshape(-1, 4)
+    rc = L.g4d_write_ply(os.fsencode(path), _ptr(x, _dp), x.shape[0], _ptr(t, _ip), t.shape[0])
+    if rc != 0:
+        raise G4DError(rc, "g4d_write_ply(%s) failed" % path)
 
 
 def _ptr(a, t):

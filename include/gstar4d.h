@@ -39,7 +39,8 @@ typedef enum {
     G4D_ERR_NOT_CONVERGED = 6,
     G4D_ERR_CAPACITY = 7,
     G4D_ERR_PREDICATE_RANGE = 8,
-    G4D_ERR_INTERNAL = 9
+    G4D_ERR_INTERNAL = 9,
+    G4D_ERR_IO = 10
 } g4d_status;
 
 typedef enum {
@@ -130,6 +131,31 @@ int g4d_check_result(g4d_ctx *ctx, int32_t full_empty_sphere, g4d_check_report *
 /* Validate a caller's mesh: points (n x 3), tets and opp (T x 4; opp may be NULL). */
 int g4d_check_mesh(g4d_ctx *ctx, const double *xyz, int32_t n, const int32_t *tets, const int32_t *opp,
                    int64_t num_tets, int32_t full_empty_sphere, g4d_check_report *rep);
+
+/* ---- Point and mesh files (host only; no context, no device work) ---- */
+
+/* Points of an .asc file (GD/Main/Application.cpp:475-566): whitespace-separated tokens read as
+   floats, three per point; exact duplicates dropped (first occurrence kept); `scaled` is
+   Application::_scalePoint (GD/Main/Application.cpp:568-585) into [1, G - 2] with the float
+   min/max over all coordinates.  Arrays are malloc'ed; release with g4d_free_point_file.
+   Returns G4D_ERR_IO when the file cannot be opened, G4D_ERR_DUPLICATE_POINTS when scaling
+   merges points (the reference asserts it does not). */
+typedef struct {
+    int32_t num_points;     /* unique points */
+    int32_t pad;
+    int64_t num_read;       /* triples in the file */
+    int64_t num_duplicates; /* num_read - num_points */
+    float min_val, max_val; /* over every coordinate; 999 / -999 for an empty file */
+    double *points;         /* num_points x 3, as read */
+    double *scaled;         /* num_points x 3 */
+} g4d_point_file;
+int g4d_read_asc(const char *path, int32_t grid_size, g4d_point_file *out);
+void g4d_free_point_file(g4d_point_file *f);
+/* "x y z" per point, as the reference's Input-Points.asc dump (GD/Main/Application.cpp:231-243). */
+int g4d_write_asc(const char *path, const double *xyz, int32_t n);
+/* ASCII PLY of the points and three faces per tetrahedron (TetraMesh::writeToFile,
+   GD/Common/Geometry.cu:395-466). */
+int g4d_write_ply(const char *path, const double *xyz, int32_t n, const int32_t *tets, int64_t num_tets);
 
 /* ---- Kernel-level diagnostics (used by parity tests; no reference counterpart) ---- */
 /* orient4 of T tuples (15 doubles, 5 indices each).  mode 0: filter + exact + SoS;
