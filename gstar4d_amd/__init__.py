@@ -12,7 +12,7 @@ import numpy as np
 
 __all__ = ["Triangulator", "Options", "Stats", "CheckReport", "G4DError", "load_library", "gdel_init",
            "gdel_compute", "get_tetra", "gdel_deinit", "canonical", "LIB_PATH", "PointFile", "read_asc",
-           "write_asc", "write_ply"]
+           "write_asc", "write_ply", "BatchResult", "batch_grid_size"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgstar4d.so")
@@ -73,6 +73,14 @@ class PointFile(ctypes.Structure):
                 ("points", ctypes.POINTER(ctypes.c_double)), ("scaled", ctypes.POINTER(ctypes.c_double))]
 
 
+class BatchResult(ctypes.Structure):
+    """g4d_batch_result."""
+    _fields_ = [("num_sets", ctypes.c_int32), ("pad", ctypes.c_int32), ("num_points", ctypes.c_int64),
+                ("num_tets", ctypes.c_int64), ("tet_offsets", ctypes.POINTER(ctypes.c_int64)),
+                ("tets", ctypes.POINTER(ctypes.c_int32)), ("opp", ctypes.POINTER(ctypes.c_int32)),
+                ("grid_sizes", ctypes.POINTER(ctypes.c_int32))]
+
+
 class CheckReport(ctypes.Structure):
     """g4d_check_report: TetraMesh::check (GD/Common/Geometry.cu:200-393) plus adjacency and
     empty-sphere counts."""
@@ -96,7 +104,7 @@ EXPORTS = ["g4d_default_options", "g4d_create", "g4d_destroy", "g4d_last_error",
            "g4d_get_stats", "g4d_set_options", "g4d_diag_orient4", "g4d_diag_orient3d",
            "g4d_diag_morton", "g4d_diag_pba", "g4d_diag_grid_pairs", "g4d_diag_copy",
            "g4d_check_result", "g4d_check_mesh", "g4d_read_asc", "g4d_free_point_file", "g4d_write_asc",
-           "g4d_write_ply"]
+           "g4d_write_ply", "g4d_triangulate_batch", "g4d_triangulate_batch_device", "g4d_batch_grid_size"]
 
 
 def load_library(path=None):
@@ -129,6 +137,9 @@ def load_library(path=None):
     L.g4d_check_result.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(CheckReport)]
     L.g4d_check_mesh.argtypes = [_vp, _dp, ctypes.c_int32, _ip, _ip, ctypes.c_int64, ctypes.c_int32,
                                  ctypes.POINTER(CheckReport)]
+    L.g4d_triangulate_batch.argtypes = [_vp, _dp, _lp, ctypes.c_int32, _ip, ctypes.POINTER(BatchResult)]
+    L.g4d_triangulate_batch_device.argtypes = [_vp, _vp, _lp, ctypes.c_int32, _ip, _lp]
+    L.g4d_batch_grid_size.argtypes = [ctypes.c_int64]
     L.g4d_read_asc.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(PointFile)]
     L.g4d_free_point_file.argtypes = [ctypes.POINTER(PointFile)]
     L.g4d_free_point_file.restype = None
@@ -156,6 +167,11 @@ def read_asc(path, grid_size):
     finally:
         L.g4d_free_point_file(ctypes.byref(f))
     return pts, sc, info
+
+
+def batch_grid_size(n):
+    """The PBA grid width the batched mode picks for a set of n points."""
+    return int(load_library().g4d_batch_grid_size(int(n)))
 
 
 def write_asc(path, xyz):
@@ -243,6 +259,33 @@ class Triangulator:
                if res.opp else None)
         perm = np.ctypeslib.as_array(res.perm, shape=(xyz.shape[0],)).copy()
         return tets, opp, perm
+
+    def triangulate_batch(self, xyz, set_offsets, grids=None):
+        """Independent point sets in one pass of each stage.  set_offsets (S+1,) int64 with
+        set_offsets[0] = 0.  Returns (tets, opp, tet_offsets, grid_sizes): set k's tetrahedra are
+        tets[tet_offsets[k]:tet_offsets[k+1]] with vertex ids into that set's points and opp ids
+        within that range (-1 on the hull)."""
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
+        off = np.ascontiguousarray(set_offsets, dtype=np.int64)
+        S = off.shape[0] - 1
+        gr = None if grids is None else np.ascontiguousarray(grids, dtype=np.int32)
+        res = BatchResult()
+        self._check(self._L.g4d_triangulate_batch(self._ctx, _ptr(xyz, _dp), _ptr(off, _lp), S,
+                                                  None if gr is None else _ptr(gr, _ip), ctypes.byref(res)))
+        nt = res.num_tets
+        tets = np.ctypeslib.as_array(res.tets, shape=(max(nt, 1), 4))[:nt].copy()
+        opp = (np.ctypeslib.as_array(res.opp, shape=(max(nt, 1), 4))[:nt].copy() if res.opp else None)
+        toff = np.ctypeslib.as_array(res.tet_offsets, shape=(S + 1,)).copy()
+        gs = np.ctypeslib.as_array(res.grid_sizes, shape=(S,)).copy()
+        return tets, opp, toff, gs
+
+    def triangulate_batch_device(self, d_xyz, set_offsets, grids=None):
+        off = np.ascontiguousarray(set_offsets, dtype=np.int64)
+        gr = None if grids is None else np.ascontiguousarray(grids, dtype=np.int32)
+        nt = ctypes.c_int64(0)
+        self._check(self._L.g4d_triangulate_batch_device(self._ctx, _vp(d_xyz), _ptr(off, _lp), off.shape[0] - 1,
+                                                         None if gr is None else _ptr(gr, _ip), ctypes.byref(nt)))
+        return nt.value
 
     def upload(self, xyz):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)

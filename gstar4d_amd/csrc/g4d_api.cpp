@@ -60,12 +60,14 @@ struct g4d_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     g4d::Prims P;
-    DevBuf xyz_up, keys, pts, perm, vox, bounds, grid, gtmp, idgrid, pbaspill, pcount, pairs, ws_off, cnt;
+    DevBuf xyz_up, keys, pts, perm, vox, bounds, settab, ptset, grid, gtmp, idgrid, pbaspill, pcount, pairs, ws_off, cnt;
     Store store;
-    DevBuf drown, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, aflags, apos, active, mid, big, huge, xstars,
+    DevBuf settet, drown, dflag, pairbuf, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, aflags, apos, active, mid, big, huge, xstars,
         xtasks, xtris, gwork, tetcnt, tetoff, tets, opp, table;
-    std::vector<int32_t> h_tets, h_opp, h_perm;
+    std::vector<int32_t> h_tets, h_opp, h_perm, h_set_grid;
+    std::vector<int64_t> h_set_tet_off;
     const double *last_xyz = nullptr; // device points of the last triangulation (input order)
+    int last_nsets = 1;
     int64_t num_tets = 0;
     int last_n = 0;
     bool have_result = false;
@@ -230,14 +232,93 @@ static double ev_ms(g4d_ctx *ctx, int a, int b)
 }
 
 // The whole path on device-resident FP64 points.
-static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
+// Point sets of one call (host side): set k holds points [pt_off[k], pt_off[k+1]) with grid
+// width g[k].  A single triangulation is one set.
+struct SetSpec {
+    int nsets = 1;
+    std::vector<int64_t> pt_off;
+    std::vector<int> g;
+};
+
+static SetSpec single_set(int n, int g)
+{
+    SetSpec sp;
+    sp.nsets = 1;
+    sp.pt_off = {0, (int64_t)n};
+    sp.g = {g};
+    return sp;
+}
+
+struct SetTotals {
+    int64_t voxels = 0, pba_blocks = 0, spill = 0, pair_threads = 0;
+};
+
+// Uploads the set table (point ranges, grids, prefix sums of the grid stages' work) and, for
+// several sets, the per-point set ids; GS.bounds points at room for 2 per set (or the
+// single-set reduction's scratch).
+static int setup_sets(g4d_ctx *ctx, const SetSpec &spec, int n, g4d::GridSets &GS, SetTotals &tot)
+{
+    hipStream_t s = ctx->stream;
+    const int S = spec.nsets;
+    std::vector<int64_t> tab(5 * (size_t)(S + 1));
+    int64_t *h_pt = tab.data(), *h_vox = h_pt + (S + 1), *h_blk = h_vox + (S + 1), *h_spill = h_blk + (S + 1),
+            *h_thr = h_spill + (S + 1);
+    h_pt[0] = h_vox[0] = h_blk[0] = h_spill[0] = h_thr[0] = 0;
+    for (int k = 0; k < S; k++) {
+        const int64_t G = spec.g[k];
+        h_pt[k + 1] = spec.pt_off[k + 1];
+        h_vox[k + 1] = h_vox[k] + G * G * G;
+        h_blk[k + 1] = h_blk[k] + g4d::pba_blocks((int)G);
+        h_spill[k + 1] = h_spill[k] + g4d::pba_spill_ints((int)G);
+        h_thr[k + 1] = h_thr[k] + g4d::grid_pair_threads((int)G, g4d::grid_pair_chunk((int)G));
+    }
+    tot.voxels = h_vox[S];
+    tot.pba_blocks = h_blk[S];
+    tot.spill = h_spill[S];
+    tot.pair_threads = h_thr[S];
+    const size_t tab_bytes = sizeof(int64_t) * tab.size();
+    const size_t bounds_at = (tab_bytes + sizeof(int) * (size_t)S + 15) & ~(size_t)15;
+    HCK(ctx->settab.ensure(bounds_at + sizeof(double) * (size_t)std::max(2 * S, 768)));
+    char *dtab = buf<char>(ctx->settab);
+    HCK(hipMemcpyAsync(dtab, tab.data(), tab_bytes, hipMemcpyHostToDevice, s));
+    HCK(hipMemcpyAsync(dtab + tab_bytes, spec.g.data(), sizeof(int) * (size_t)S, hipMemcpyHostToDevice, s));
+    GS.nsets = S;
+    GS.pt_off = reinterpret_cast<const int64_t *>(dtab);
+    GS.vox_off = GS.pt_off + (S + 1);
+    GS.pba_blk = GS.vox_off + (S + 1);
+    GS.spill = GS.pba_blk + (S + 1);
+    GS.pair_thr = GS.spill + (S + 1);
+    GS.g = reinterpret_cast<const int *>(dtab + tab_bytes);
+    GS.bounds = reinterpret_cast<const double *>(dtab + bounds_at);
+    GS.pt_set = nullptr;
+    if (S > 1) {
+        HCK(ctx->ptset.ensure(sizeof(int) * (size_t)n));
+        HCK(g4d::launch_set_ids(GS.pt_off, S, buf<int>(ctx->ptset), s));
+        GS.pt_set = buf<int>(ctx->ptset);
+    }
+    return G4D_OK;
+}
+
+static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &spec)
 {
     ctx->last_xyz = d_xyz;
+    ctx->last_nsets = spec.nsets;
     hipStream_t s = ctx->stream;
     g4d::Prims &P = ctx->P;
-    const int G = ctx->opt.grid_size;
-    if (G < 4 || G > 1024) return set_err(ctx, G4D_ERR_GRID_LIMIT, "grid_size %d outside [4, 1024]", G);
-    if (n < 5) return set_err(ctx, G4D_ERR_INVALID_INPUT, "need at least 5 points, got %d", n);
+    const int S = spec.nsets;
+    const bool batched = S > 1;
+    int gmax = 0;
+    for (int k = 0; k < S; k++) {
+        const int G = spec.g[k];
+        const int64_t nk = spec.pt_off[k + 1] - spec.pt_off[k];
+        if (G < 4 || G > 1024) return set_err(ctx, G4D_ERR_GRID_LIMIT, "grid_size %d outside [4, 1024]", G);
+        if (batched && G > g4d::BATCH_MAX_GRID)
+            return set_err(ctx, G4D_ERR_GRID_LIMIT, "set %d: batched grid %d above %d", k, G, g4d::BATCH_MAX_GRID);
+        if (nk < 5) return set_err(ctx, G4D_ERR_INVALID_INPUT, "set %d: need at least 5 points, got %lld", k, (long long)nk);
+        if (batched && nk > g4d::BATCH_MAX_SET_POINTS)
+            return set_err(ctx, G4D_ERR_INVALID_INPUT, "set %d: %lld points, above the batched limit", k, (long long)nk);
+        gmax = std::max(gmax, G);
+    }
     if (n > (1 << 29)) return set_err(ctx, G4D_ERR_INVALID_INPUT, "too many points (%d)", n);
     if (ctx->opt.lds_star_limit != 0 && ctx->opt.lds_star_limit < 4)
         return set_err(ctx, G4D_ERR_INVALID_INPUT, "lds_star_limit %d below 4", ctx->opt.lds_star_limit);
@@ -246,7 +327,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
     const int nb = bits_for(n);
     memset(&ctx->st, 0, sizeof(ctx->st));
     ctx->st.num_points = n;
-    ctx->st.grid_size = G;
+    ctx->st.grid_size = gmax;
     ctx->have_result = false;
     unsigned long long hc[G4D_CNT_NUM];
 
@@ -255,54 +336,63 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
     HCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * G4D_CNT_NUM, s));
     ev_record(ctx, EV_START);
 
+    // ---- set tables: points, grids and the work decompositions of the grid stages ----
+    g4d::GridSets GS;
+    SetTotals tot;
+    CK(setup_sets(ctx, spec, n, GS, tot));
+    double *dbounds = const_cast<double *>(GS.bounds);
+    const int64_t G3 = tot.voxels;
+
     // ---- B: Morton order (GD/GDelaunay/GDelData.cu:91-131) ----
-    HCK(ctx->bounds.ensure(sizeof(double) * 768));
-    HCK(g4d::launch_bounds(d_xyz, n, buf<double>(ctx->bounds), s));
-    double hb[2];
-    HCK(hipMemcpyAsync(hb, ctx->bounds.ptr, sizeof(hb), hipMemcpyDeviceToHost, s));
-    HCK(hipStreamSynchronize(s));
-    if (!std::isfinite(hb[0]) || !std::isfinite(hb[1]) || !(hb[1] > hb[0]))
-        return set_err(ctx, G4D_ERR_INVALID_INPUT, "coordinates must be finite and not all equal");
+    if (batched) HCK(g4d::launch_set_bounds(d_xyz, GS, dbounds, s));
+    else HCK(g4d::launch_bounds(d_xyz, n, dbounds, s));
+    {
+        std::vector<double> hb(2 * (size_t)S);
+        HCK(hipMemcpyAsync(hb.data(), dbounds, sizeof(double) * 2 * (size_t)S, hipMemcpyDeviceToHost, s));
+        HCK(hipStreamSynchronize(s));
+        for (int k = 0; k < S; k++)
+            if (!std::isfinite(hb[2 * k]) || !std::isfinite(hb[2 * k + 1]) || !(hb[2 * k + 1] > hb[2 * k]))
+                return set_err(ctx, G4D_ERR_INVALID_INPUT, "set %d: coordinates must be finite and not all equal", k);
+    }
     HCK(ctx->keys.ensure(sizeof(uint64_t) * (size_t)n));
     HCK(ctx->pts.ensure(sizeof(double) * 3 * (size_t)n));
     HCK(ctx->perm.ensure(sizeof(int) * (size_t)n));
     HCK(ctx->vox.ensure(sizeof(int) * (size_t)n));
     uint64_t *keys = buf<uint64_t>(ctx->keys);
-    HCK(g4d::launch_morton_keys(d_xyz, n, G, buf<double>(ctx->bounds), keys, s));
-    HCK(g4d::radix_sort_u64(P, keys, n, 32, 62, s));
+    HCK(g4d::launch_morton_keys(d_xyz, n, GS, keys, s));
+    if (batched) HCK(g4d::radix_sort_u64(P, keys, n, 24, 64, s));
+    else HCK(g4d::radix_sort_u64(P, keys, n, 32, 62, s));
     double *pts = buf<double>(ctx->pts);
     int *perm = buf<int>(ctx->perm), *vox = buf<int>(ctx->vox);
-    HCK(g4d::launch_gather_sorted(d_xyz, keys, n, G, buf<double>(ctx->bounds), pts, perm, vox, s));
-    HCK(g4d::launch_dup_check(pts, vox, n, cnt, s));
+    HCK(g4d::launch_gather_sorted(d_xyz, keys, n, GS, pts, perm, vox, s));
+    HCK(g4d::launch_dup_check(pts, vox, n, GS, cnt, s));
     ev_record(ctx, EV_MORTON);
 
-    // ---- C: PBA (GD/PBA/Pba.cu:105-122) ----
-    const int64_t G3 = (int64_t)G * G * G;
+    // ---- C: PBA (GD/PBA/Pba.cu:105-122), every set's grid in one launch per pass ----
     HCK(ctx->grid.ensure(sizeof(int) * (size_t)G3));
     HCK(ctx->gtmp.ensure(sizeof(int) * (size_t)G3));
     HCK(ctx->idgrid.ensure(sizeof(int) * (size_t)G3));
     int *grid = buf<int>(ctx->grid), *gtmp = buf<int>(ctx->gtmp), *idgrid = buf<int>(ctx->idgrid);
     HCK(g4d::launch_fill_i32(grid, G3, G4D_MARKER, s));
-    HCK(g4d::launch_scatter_sites(vox, n, G, grid, idgrid, s));
+    HCK(g4d::launch_scatter_sites(vox, n, GS, grid, idgrid, s));
     ev_record(ctx, EV_PBA0);
-    HCK(ctx->pbaspill.ensure(sizeof(int) * (size_t)g4d::pba_spill_ints(G)));
+    HCK(ctx->pbaspill.ensure(sizeof(int) * (size_t)tot.spill));
     int *spill = buf<int>(ctx->pbaspill);
-    HCK(g4d::pba_pass(2, G, grid, gtmp, nullptr, spill, s));
+    HCK(g4d::pba_pass(2, GS, tot.pba_blocks, grid, gtmp, nullptr, spill, s));
     ev_record(ctx, EV_PBA1);
-    HCK(g4d::pba_pass(1, G, gtmp, grid, nullptr, spill, s));
+    HCK(g4d::pba_pass(1, GS, tot.pba_blocks, gtmp, grid, nullptr, spill, s));
     ev_record(ctx, EV_PBA2);
-    HCK(g4d::pba_pass(0, G, grid, gtmp, idgrid, spill, s)); // point ids in gtmp
+    HCK(g4d::pba_pass(0, GS, tot.pba_blocks, grid, gtmp, idgrid, spill, s)); // point ids in gtmp
     ev_record(ctx, EV_PBA3);
     const int *idg = gtmp;
 
     // ---- D: missing points, grid pairs, worksets, pumping (GD/GDelaunay/GDelHost.cu:259-529) ----
     // Pairs are extracted once per unordered pair (min<<32 | max), deduplicated, then
     // reciprocated into directed workset keys (star<<32 | vertex) and grouped by star.
-    const int zc = G > 512 ? 32 : 16;
-    const int64_t nthr = g4d::grid_pair_threads(G, zc);
+    const int64_t nthr = tot.pair_threads;
     HCK(ctx->pcount.ensure(sizeof(int64_t) * (size_t)(nthr + 1)));
     int64_t *pcnt = buf<int64_t>(ctx->pcount);
-    HCK(g4d::launch_grid_pairs(idg, G, zc, pcnt, nullptr, false, s));
+    HCK(g4d::launch_grid_pairs(idg, GS, nthr, pcnt, nullptr, false, s));
     HCK(g4d::scan_excl_i64(P, pcnt, pcnt, nthr, pcnt + nthr, s));
     int64_t ngp = 0;
     HCK(hipMemcpyAsync(&ngp, pcnt + nthr, 8, hipMemcpyDeviceToHost, s));
@@ -310,13 +400,13 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
     const int64_t ucap = ngp + n;
     HCK(ctx->pairs.ensure(sizeof(uint64_t) * (size_t)(2 * ucap + 2 * (int64_t)n * min_ws + 64)));
     uint64_t *pr = buf<uint64_t>(ctx->pairs);
-    HCK(g4d::launch_grid_pairs(idg, G, zc, pcnt, pr, true, s));
+    HCK(g4d::launch_grid_pairs(idg, GS, nthr, pcnt, pr, true, s));
     HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8, s));
-    HCK(g4d::launch_missing(vox, n, G, idg, pr + ngp, cnt + G4D_CNT_APPEND0, n, s));
+    HCK(g4d::launch_missing(vox, n, GS, idg, pr + ngp, cnt + G4D_CNT_APPEND0, n, s));
     CK(read_counters(ctx, hc));
     CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], "morton"));
     const int64_t nmiss = (int64_t)hc[G4D_CNT_APPEND0];
-    trace("n %d grid %d: grid pairs %lld, missing %lld", n, G, (long long)ngp, (long long)nmiss);
+    trace("n %d in %d set(s), grid %d: grid pairs %lld, missing %lld", n, S, gmax, (long long)ngp, (long long)nmiss);
     int64_t u = ngp + nmiss;
     HCK(sort_pairs(P, pr, u, nb, s));
     HCK(g4d::unique_u64(P, pr, u, &u, s));
@@ -329,7 +419,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
     int64_t *ws_off = buf<int64_t>(ctx->ws_off);
     HCK(g4d::launch_csr_offsets(pr, m, n, ws_off, s));
     HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8, s));
-    HCK(g4d::launch_pump(pr, ws_off, n, min_ws, pr + m, cnt + G4D_CNT_APPEND0, 2 * (int64_t)n * min_ws + 64, s));
+    HCK(g4d::launch_pump(pr, ws_off, n, GS, min_ws, pr + m, cnt + G4D_CNT_APPEND0, 2 * (int64_t)n * min_ws + 64, s));
     CK(read_counters(ctx, hc));
     const int64_t npump = (int64_t)hc[G4D_CNT_APPEND0];
     if (npump > 0) {
@@ -415,10 +505,14 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
         HCK(g4d::launch_assign_regions(list, nitems, vcap, tcap, hcap, store.top[0], store.top[1], store.top[2],
                                        buf<int64_t>(store.nvoff), buf<int64_t>(store.ntoff), buf<int64_t>(store.nhoff),
                                        s));
+        const int64_t tbase = store.top[1];
         for (int k = 0; k < 3; k++) store.top[k] += need[k];
         const int64_t dcap = std::max<int64_t>(nins, 1);
-        const int64_t tpcap = 2 * need[1] + 4 * dcap + 64;
+        const int64_t npb = 2 * need[1];
+        const int64_t tpcap = npb + 4 * dcap + 64;
         HCK(ctx->drown.ensure(sizeof(uint64_t) * (size_t)dcap));
+        HCK(ctx->dflag.ensure((size_t)dcap));
+        HCK(ctx->pairbuf.ensure(sizeof(uint64_t) * (size_t)std::max<int64_t>(npb, 1)));
         HCK(ctx->cand.ensure(sizeof(uint64_t) * (size_t)tpcap));
         g4d::StarParams sp;
         memset(&sp, 0, sizeof(sp));
@@ -439,10 +533,9 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
         sp.nvoff = buf<int64_t>(store.nvoff);
         sp.ntoff = buf<int64_t>(store.ntoff);
         sp.nhoff = buf<int64_t>(store.nhoff);
-        sp.drown = buf<uint64_t>(ctx->drown);
-        sp.drown_cap = dcap;
-        sp.tpairs = buf<uint64_t>(ctx->cand);
-        sp.tpairs_cap = tpcap;
+        sp.dflag = buf<uint8_t>(ctx->dflag);
+        sp.pairbuf = buf<uint64_t>(ctx->pairbuf);
+        sp.tbase = tbase;
         sp.cnt = cnt;
         sp.lds_limit = lds_limit;
         sp.defer_list = buf<int>(ctx->mid);
@@ -451,7 +544,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
         sp.exact_list = buf<int>(ctx->xstars);
         sp.exact_cap = n;
         sp.exact_counter = G4D_CNT_APPEND4;
-        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8 * 3, s));
+        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND2, 0, 8, s));
         HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND3, 0, 8 * 2, s));
         HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND6, 0, 8, s));
         HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
@@ -532,14 +625,17 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
             HCK(g4d::launch_star_global(sp, buf<int>(ctx->huge), (int)nhuge, buf<char>(ctx->gwork), stride, vcap_g,
                                         nblk, s));
             CK(read_counters(ctx, hc));
+            trace("global-memory star kernel: %lld stars, largest link plus insertions %d", (long long)nhuge, vcap_g);
         }
-        trace("star pass mode %d: %lld insertions into %d stars, over 32: %lld, over 64: %lld, exact %lld, "
-              "global-memory %lld, drowned %llu, pairs %llu, err 0x%llx",
-              mode, (long long)nins, nitems, (long long)nmid, (long long)nbig, (long long)nexact, (long long)nhuge,
-              hc[G4D_CNT_APPEND0], hc[G4D_CNT_APPEND1], hc[G4D_CNT_ERROR]);
         CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], mode ? "splaying" : "star creation"));
-        *drown_n = (int64_t)hc[G4D_CNT_APPEND0];
-        *tp_n = (int64_t)hc[G4D_CNT_APPEND1];
+        // drowned insertions and triangle pairs, in star order
+        HCK(g4d::compact_u64(P, ins_keys, buf<uint8_t>(ctx->dflag), buf<uint64_t>(ctx->drown), nins, drown_n, s));
+        HCK(g4d::compact_valid_u64(P, buf<uint64_t>(ctx->pairbuf), buf<uint64_t>(ctx->cand), npb, g4d::STAR_PAIR_NONE,
+                                   tp_n, s));
+        trace("star pass mode %d: %lld insertions into %d stars, over 32: %lld, over 64: %lld, exact %lld, "
+              "global-memory %lld, drowned %lld, pairs %lld",
+              mode, (long long)nins, nitems, (long long)nmid, (long long)nbig, (long long)nexact, (long long)nhuge,
+              (long long)*drown_n, (long long)*tp_n);
         return G4D_OK;
     };
 
@@ -673,7 +769,21 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n)
         HCK(g4d::launch_adjacency(buf<int>(ctx->tets), nt, buf<int>(ctx->opp), buf<uint32_t>(ctx->table), (int64_t)tsz,
                                   cnt, s));
     }
-    if (ctx->opt.index_space == G4D_INDEX_INPUT) HCK(g4d::launch_map_perm(buf<int>(ctx->tets), 4 * nt, perm, s));
+    if (batched) {
+        // per-set tetrahedron ranges (tetrahedra are grouped by owner star, stars by set) and
+        // set-local vertex / tetrahedron ids in each set's input order
+        HCK(ctx->settet.ensure(sizeof(int64_t) * (size_t)(S + 1)));
+        int64_t *tet_off = buf<int64_t>(ctx->settet);
+        HCK(g4d::launch_gather_i64(toffs, GS.pt_off, S + 1, tet_off, s));
+        HCK(g4d::launch_batch_localize(buf<int>(ctx->tets), ctx->opt.compute_adjacency ? buf<int>(ctx->opp) : nullptr,
+                                       nt, GS.pt_set, GS.pt_off, tet_off, perm, s));
+        ctx->h_set_tet_off.resize((size_t)S + 1);
+        HCK(hipMemcpyAsync(ctx->h_set_tet_off.data(), tet_off, sizeof(int64_t) * (size_t)(S + 1), hipMemcpyDeviceToHost,
+                           s));
+        ctx->h_set_grid = spec.g;
+    } else if (ctx->opt.index_space == G4D_INDEX_INPUT) {
+        HCK(g4d::launch_map_perm(buf<int>(ctx->tets), 4 * nt, perm, s));
+    }
     ev_record(ctx, EV_ADJ);
     CK(read_counters(ctx, hc));
     CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], "output"));
@@ -760,8 +870,8 @@ void g4d_destroy(g4d_ctx *ctx)
     if (!ctx) return;
     (void)hipSetDevice(ctx->opt.device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    DevBuf *bufs[] = {&ctx->xyz_up, &ctx->keys, &ctx->pts, &ctx->perm, &ctx->vox, &ctx->bounds, &ctx->grid,
-                      &ctx->gtmp, &ctx->idgrid, &ctx->pbaspill, &ctx->pcount, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->drown, &ctx->dkeys,
+    DevBuf *bufs[] = {&ctx->xyz_up, &ctx->keys, &ctx->pts, &ctx->perm, &ctx->vox, &ctx->bounds, &ctx->settab, &ctx->ptset, &ctx->grid,
+                      &ctx->gtmp, &ctx->idgrid, &ctx->pbaspill, &ctx->pcount, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->settet, &ctx->drown, &ctx->dflag, &ctx->pairbuf, &ctx->dkeys,
                       &ctx->tasks, &ctx->cand, &ctx->keep, &ctx->ins, &ctx->ins_off, &ctx->vcap, &ctx->tcap,
                       &ctx->hcap, &ctx->aflags, &ctx->apos, &ctx->active, &ctx->mid, &ctx->big, &ctx->huge, &ctx->xstars, &ctx->xtasks, &ctx->xtris, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table};
     for (DevBuf *b : bufs) b->release();
@@ -801,9 +911,89 @@ int g4d_triangulate_device(g4d_ctx *ctx, const double *d_xyz, int32_t n, int64_t
 {
     if (!ctx || !d_xyz) return G4D_ERR_INVALID_INPUT;
     HCK(hipSetDevice(ctx->opt.device));
-    const int r = run_device(ctx, d_xyz, n);
+    const int r = run_device(ctx, d_xyz, n, single_set(n, ctx->opt.grid_size));
     if (r != G4D_OK) return r;
     if (num_tets) *num_tets = ctx->num_tets;
+    return G4D_OK;
+}
+
+int g4d_batch_grid_size(int64_t n)
+{
+    // about 24 voxels per point (C1 has 21, C2 32), in steps of 16, within [32, 256]
+    int g = 32;
+    while (g < g4d::BATCH_MAX_GRID && (int64_t)g * g * g < 24 * n) g += 16;
+    return g;
+}
+
+static int batch_spec(g4d_ctx *ctx, const int64_t *set_offsets, int32_t nsets, const int32_t *grids, int64_t *n_out,
+                      SetSpec &spec)
+{
+    if (!set_offsets || nsets < 1 || nsets > g4d::BATCH_MAX_SETS)
+        return set_err(ctx, G4D_ERR_INVALID_INPUT, "batch needs 1..%d sets", g4d::BATCH_MAX_SETS);
+    if (set_offsets[0] != 0) return set_err(ctx, G4D_ERR_INVALID_INPUT, "set_offsets[0] must be 0");
+    spec.nsets = nsets;
+    spec.pt_off.assign(set_offsets, set_offsets + nsets + 1);
+    spec.g.resize((size_t)nsets);
+    for (int k = 0; k < nsets; k++) {
+        if (set_offsets[k + 1] <= set_offsets[k]) return set_err(ctx, G4D_ERR_INVALID_INPUT, "set %d is empty", k);
+        spec.g[(size_t)k] = grids ? grids[k] : g4d_batch_grid_size(set_offsets[k + 1] - set_offsets[k]);
+    }
+    if (set_offsets[nsets] > (1 << 29)) return set_err(ctx, G4D_ERR_INVALID_INPUT, "too many points in the batch");
+    *n_out = set_offsets[nsets];
+    return G4D_OK;
+}
+
+int g4d_triangulate_batch_device(g4d_ctx *ctx, const double *d_xyz, const int64_t *set_offsets, int32_t nsets,
+                                 const int32_t *grids, int64_t *num_tets)
+{
+    if (!ctx || !d_xyz) return G4D_ERR_INVALID_INPUT;
+    HCK(hipSetDevice(ctx->opt.device));
+    SetSpec spec;
+    int64_t n = 0;
+    CK(batch_spec(ctx, set_offsets, nsets, grids, &n, spec));
+    if (nsets == 1) {
+        // one set: the plain path in the input index space
+        const int saved = ctx->opt.index_space;
+        ctx->opt.index_space = G4D_INDEX_INPUT;
+        const int r = run_device(ctx, d_xyz, (int)n, spec);
+        ctx->opt.index_space = saved;
+        CK(r);
+        ctx->h_set_tet_off = {0, ctx->num_tets};
+        ctx->h_set_grid = spec.g;
+    } else {
+        CK(run_device(ctx, d_xyz, (int)n, spec));
+    }
+    if (num_tets) *num_tets = ctx->num_tets;
+    return G4D_OK;
+}
+
+int g4d_triangulate_batch(g4d_ctx *ctx, const double *xyz, const int64_t *set_offsets, int32_t nsets,
+                          const int32_t *grids, g4d_batch_result *out)
+{
+    if (!ctx || !xyz || !out || !set_offsets || nsets < 1) return G4D_ERR_INVALID_INPUT;
+    const int64_t n = set_offsets[nsets];
+    if (n <= 0 || n > (1 << 29)) return set_err(ctx, G4D_ERR_INVALID_INPUT, "bad batch size");
+    const double *d_xyz = nullptr;
+    CK(g4d_upload_points(ctx, xyz, (int32_t)n, &d_xyz));
+    CK(g4d_triangulate_batch_device(ctx, d_xyz, set_offsets, nsets, grids, nullptr));
+    const int64_t nt = ctx->num_tets;
+    ctx->h_tets.resize((size_t)std::max<int64_t>(4 * nt, 1));
+    ctx->h_opp.resize(ctx->opt.compute_adjacency ? (size_t)std::max<int64_t>(4 * nt, 1) : 0);
+    if (nt > 0)
+        HCK(hipMemcpyAsync(ctx->h_tets.data(), ctx->tets.ptr, sizeof(int) * 4 * (size_t)nt, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    if (nt > 0 && ctx->opt.compute_adjacency)
+        HCK(hipMemcpyAsync(ctx->h_opp.data(), ctx->opp.ptr, sizeof(int) * 4 * (size_t)nt, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HCK(hipStreamSynchronize(ctx->stream));
+    out->num_sets = nsets;
+    out->pad = 0;
+    out->num_points = n;
+    out->num_tets = nt;
+    out->tet_offsets = ctx->h_set_tet_off.data();
+    out->tets = ctx->h_tets.data();
+    out->opp = ctx->opt.compute_adjacency ? ctx->h_opp.data() : nullptr;
+    out->grid_sizes = ctx->h_set_grid.data();
     return G4D_OK;
 }
 
@@ -827,7 +1017,7 @@ int g4d_triangulate(g4d_ctx *ctx, const double *xyz, int32_t n, g4d_result *out)
     const double *d_xyz = nullptr;
     int r = g4d_upload_points(ctx, xyz, n, &d_xyz);
     if (r != G4D_OK) return r;
-    r = run_device(ctx, d_xyz, n);
+    r = run_device(ctx, d_xyz, n, single_set(n, ctx->opt.grid_size));
     if (r != G4D_OK) return r;
     const int64_t nt = ctx->num_tets;
     ctx->h_tets.resize((size_t)std::max<int64_t>(4 * nt, 1));
@@ -904,16 +1094,18 @@ int g4d_diag_morton(g4d_ctx *ctx, const double *xyz, int32_t n, int32_t grid, in
     hipStream_t s = ctx->stream;
     const double *d_xyz = nullptr;
     CK(g4d_upload_points(ctx, xyz, n, &d_xyz));
-    HCK(ctx->bounds.ensure(sizeof(double) * 768));
     HCK(ctx->keys.ensure(sizeof(uint64_t) * (size_t)n));
     HCK(ctx->pts.ensure(sizeof(double) * 3 * (size_t)n));
     HCK(ctx->perm.ensure(sizeof(int) * (size_t)n));
     HCK(ctx->vox.ensure(sizeof(int) * (size_t)n));
-    HCK(g4d::launch_bounds(d_xyz, n, buf<double>(ctx->bounds), s));
-    HCK(g4d::launch_morton_keys(d_xyz, n, grid, buf<double>(ctx->bounds), buf<uint64_t>(ctx->keys), s));
+    g4d::GridSets GS;
+    SetTotals tot;
+    CK(setup_sets(ctx, single_set(n, grid), n, GS, tot));
+    HCK(g4d::launch_bounds(d_xyz, n, const_cast<double *>(GS.bounds), s));
+    HCK(g4d::launch_morton_keys(d_xyz, n, GS, buf<uint64_t>(ctx->keys), s));
     HCK(g4d::radix_sort_u64(ctx->P, buf<uint64_t>(ctx->keys), n, 32, 62, s));
-    HCK(g4d::launch_gather_sorted(d_xyz, buf<uint64_t>(ctx->keys), n, grid, buf<double>(ctx->bounds),
-                                  buf<double>(ctx->pts), buf<int>(ctx->perm), buf<int>(ctx->vox), s));
+    HCK(g4d::launch_gather_sorted(d_xyz, buf<uint64_t>(ctx->keys), n, GS, buf<double>(ctx->pts), buf<int>(ctx->perm),
+                                  buf<int>(ctx->vox), s));
     std::vector<int> vs((size_t)n);
     HCK(hipMemcpyAsync(perm_out, ctx->perm.ptr, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
     HCK(hipMemcpyAsync(vs.data(), ctx->vox.ptr, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
@@ -945,11 +1137,17 @@ int g4d_diag_pba(g4d_ctx *ctx, const int32_t *vox_sorted, int32_t n, int32_t gri
     HCK(ctx->gtmp.ensure(sizeof(int) * (size_t)G3));
     HCK(ctx->idgrid.ensure(sizeof(int) * (size_t)G3));
     HCK(hipMemcpyAsync(ctx->vox.ptr, enc.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s));
+    g4d::GridSets GS;
+    SetTotals tot;
+    CK(setup_sets(ctx, single_set(n, grid), n, GS, tot));
     HCK(g4d::launch_fill_i32(buf<int>(ctx->grid), G3, G4D_MARKER, s));
-    HCK(g4d::launch_scatter_sites(buf<int>(ctx->vox), n, grid, buf<int>(ctx->grid), buf<int>(ctx->idgrid), s));
-    HCK(ctx->pbaspill.ensure(sizeof(int) * (size_t)g4d::pba_spill_ints(grid)));
-    HCK(g4d::pba_run(grid, buf<int>(ctx->grid), buf<int>(ctx->gtmp), codes ? nullptr : buf<int>(ctx->idgrid),
-                     buf<int>(ctx->pbaspill), s));
+    HCK(g4d::launch_scatter_sites(buf<int>(ctx->vox), n, GS, buf<int>(ctx->grid), buf<int>(ctx->idgrid), s));
+    HCK(ctx->pbaspill.ensure(sizeof(int) * (size_t)tot.spill));
+    int *spill = buf<int>(ctx->pbaspill);
+    HCK(g4d::pba_pass(2, GS, tot.pba_blocks, buf<int>(ctx->grid), buf<int>(ctx->gtmp), nullptr, spill, s));
+    HCK(g4d::pba_pass(1, GS, tot.pba_blocks, buf<int>(ctx->gtmp), buf<int>(ctx->grid), nullptr, spill, s));
+    HCK(g4d::pba_pass(0, GS, tot.pba_blocks, buf<int>(ctx->grid), buf<int>(ctx->gtmp),
+                      codes ? nullptr : buf<int>(ctx->idgrid), spill, s));
     HCK(hipMemcpyAsync(out, ctx->gtmp.ptr, sizeof(int) * (size_t)G3, hipMemcpyDeviceToHost, s));
     HCK(hipStreamSynchronize(s));
     return G4D_OK;
@@ -1061,6 +1259,8 @@ extern "C" {
 int g4d_check_result(g4d_ctx *ctx, int32_t full_empty_sphere, g4d_check_report *rep)
 {
     if (!ctx || !rep || !ctx->have_result) return G4D_ERR_INVALID_INPUT;
+    if (ctx->last_nsets > 1)
+        return set_err(ctx, G4D_ERR_INVALID_INPUT, "the last result is a batch; check each set with g4d_check_mesh");
     HCK(hipSetDevice(ctx->opt.device));
     const double *pts = ctx->opt.index_space == G4D_INDEX_MORTON ? buf<double>(ctx->pts) : ctx->last_xyz;
     const int *opp = ctx->opt.compute_adjacency ? buf<int>(ctx->opp) : nullptr;
@@ -1122,18 +1322,20 @@ int g4d_diag_grid_pairs(g4d_ctx *ctx, const int32_t *id_grid, int32_t grid, uint
     HCK(hipMemcpyAsync(ctx->gtmp.ptr, id_grid, sizeof(int) * (size_t)G3, hipMemcpyHostToDevice, s));
     HCK(ctx->cnt.ensure(sizeof(unsigned long long) * G4D_CNT_NUM));
     unsigned long long *cnt = buf<unsigned long long>(ctx->cnt);
-    const int zc = 16;
-    const int64_t nthr = g4d::grid_pair_threads(grid, zc);
+    g4d::GridSets GS;
+    SetTotals tot;
+    CK(setup_sets(ctx, single_set(1, grid), 1, GS, tot));
+    const int64_t nthr = tot.pair_threads;
     HCK(ctx->pcount.ensure(sizeof(int64_t) * (size_t)(nthr + 1)));
     int64_t *pcnt = buf<int64_t>(ctx->pcount);
-    HCK(g4d::launch_grid_pairs(buf<int>(ctx->gtmp), grid, zc, pcnt, nullptr, false, s));
+    HCK(g4d::launch_grid_pairs(buf<int>(ctx->gtmp), GS, nthr, pcnt, nullptr, false, s));
     HCK(g4d::scan_excl_i64(ctx->P, pcnt, pcnt, nthr, pcnt + nthr, s));
     int64_t m = 0;
     HCK(hipMemcpyAsync(&m, pcnt + nthr, 8, hipMemcpyDeviceToHost, s));
     HCK(hipStreamSynchronize(s));
     HCK(ctx->pairs.ensure(sizeof(uint64_t) * (size_t)std::max<int64_t>(m, 1)));
     uint64_t *pr = buf<uint64_t>(ctx->pairs);
-    HCK(g4d::launch_grid_pairs(buf<int>(ctx->gtmp), grid, zc, pcnt, pr, true, s));
+    HCK(g4d::launch_grid_pairs(buf<int>(ctx->gtmp), GS, nthr, pcnt, pr, true, s));
     int maxid = 0;
     for (int64_t i = 0; i < G3; i++) maxid = std::max(maxid, id_grid[i]);
     const int nb = bits_for((int64_t)maxid + 1);

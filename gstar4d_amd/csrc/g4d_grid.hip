@@ -69,6 +69,60 @@ __global__ void k_bounds_final(double *part, int nb)
     part[1] = hi;
 }
 
+__device__ __forceinline__ int set_of(const GridSets &S, int64_t k) { return S.pt_set ? S.pt_set[k] : 0; }
+
+// First set index whose prefix entry exceeds x (sets of a work decomposition).
+__device__ __forceinline__ int set_of_work(const int64_t *off, int nsets, int64_t x)
+{
+    int lo = 0, hi = nsets - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// One work-group per set: min and max over the set's coordinates (NaN minimum on a
+// non-finite coordinate).
+__global__ void __launch_bounds__(256) k_set_bounds(const double *xyz, GridSets S, double *bounds)
+{
+    __shared__ double smin[256], smax[256];
+    __shared__ int sbad;
+    const int k = blockIdx.x;
+    if (threadIdx.x == 0) sbad = 0;
+    __syncthreads();
+    double lo = 1e308, hi = -1e308;
+    bool bad = false;
+    for (int64_t i = 3 * S.pt_off[k] + threadIdx.x; i < 3 * S.pt_off[k + 1]; i += 256) {
+        const double v = xyz[i];
+        bad |= !isfinite(v);
+        lo = fmin(lo, v);
+        hi = fmax(hi, v);
+    }
+    if (bad) atomicOr(&sbad, 1);
+    smin[threadIdx.x] = lo;
+    smax[threadIdx.x] = hi;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            smin[threadIdx.x] = fmin(smin[threadIdx.x], smin[threadIdx.x + off]);
+            smax[threadIdx.x] = fmax(smax[threadIdx.x], smax[threadIdx.x + off]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        bounds[2 * k] = sbad ? __longlong_as_double(0x7ff8000000000000ll) : smin[0];
+        bounds[2 * k + 1] = smax[0];
+    }
+}
+
+__global__ void k_set_ids(const int64_t *pt_off, int *pt_set)
+{
+    const int k = blockIdx.x;
+    for (int64_t i = pt_off[k] + threadIdx.x; i < pt_off[k + 1]; i += blockDim.x) pt_set[i] = k;
+}
+
 __device__ __forceinline__ uint32_t spread10(uint32_t v)
 {
     v = (v | (v << 16)) & 0x030000FFu;
@@ -91,24 +145,37 @@ __device__ __forceinline__ int voxel_of(double x, double mn, double range, doubl
     return iv;
 }
 
-__global__ void k_morton_keys(const double *xyz, int n, int g, const double *bounds, uint64_t *keys)
+__global__ void k_morton_keys(const double *xyz, int n, GridSets S, uint64_t *keys)
 {
-    const double mn = bounds[0], range = bounds[1] - bounds[0], span = (double)(g - 3);
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const int st = set_of(S, i);
+        const int g = S.g[st];
+        const double mn = S.bounds[2 * st], range = S.bounds[2 * st + 1] - mn, span = (double)(g - 3);
         const int x = voxel_of(xyz[3 * (int64_t)i], mn, range, span, g);
         const int y = voxel_of(xyz[3 * (int64_t)i + 1], mn, range, span, g);
         const int z = voxel_of(xyz[3 * (int64_t)i + 2], mn, range, span, g);
         const uint32_t code = spread10(x) | (spread10(y) << 1) | (spread10(z) << 2);
-        keys[i] = ((uint64_t)code << 32) | (uint32_t)i;
+        if (S.nsets == 1)
+            keys[i] = ((uint64_t)code << 32) | (uint32_t)i;
+        else
+            keys[i] = ((uint64_t)st << 48) | ((uint64_t)code << 24) | (uint64_t)(i - S.pt_off[st]);
     }
 }
 
-__global__ void k_gather_sorted(const double *xyz, const uint64_t *keys, int n, int g, const double *bounds,
-                                double *pts, int *perm, int *vox)
+__global__ void k_gather_sorted(const double *xyz, const uint64_t *keys, int n, GridSets S, double *pts, int *perm,
+                                int *vox)
 {
-    const double mn = bounds[0], range = bounds[1] - bounds[0], span = (double)(g - 3);
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const int i = (int)(uint32_t)keys[k];
+        const uint64_t key = keys[k];
+        int st = 0, i;
+        if (S.nsets == 1) {
+            i = (int)(uint32_t)key;
+        } else {
+            st = (int)(key >> 48);
+            i = (int)(S.pt_off[st] + (int64_t)(key & 0xffffffull));
+        }
+        const int g = S.g[st];
+        const double mn = S.bounds[2 * st], range = S.bounds[2 * st + 1] - mn, span = (double)(g - 3);
         const double x = xyz[3 * (int64_t)i], y = xyz[3 * (int64_t)i + 1], z = xyz[3 * (int64_t)i + 2];
         pts[3 * (int64_t)k] = x;
         pts[3 * (int64_t)k + 1] = y;
@@ -120,10 +187,11 @@ __global__ void k_gather_sorted(const double *xyz, const uint64_t *keys, int n, 
 }
 
 // Equal points share a voxel, and equal voxels are adjacent in Morton order.
-__global__ void k_dup_check(const double *pts, const int *vox, int n, unsigned long long *cnt)
+__global__ void k_dup_check(const double *pts, const int *vox, int n, GridSets S, unsigned long long *cnt)
 {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        for (int j = k - 1; j >= 0 && vox[j] == vox[k]; j--) {
+        const int64_t first = S.pt_off[set_of(S, k)];
+        for (int j = k - 1; j >= first && vox[j] == vox[k]; j--) {
             if (pts[3 * (int64_t)j] == pts[3 * (int64_t)k] && pts[3 * (int64_t)j + 1] == pts[3 * (int64_t)k + 1] &&
                 pts[3 * (int64_t)j + 2] == pts[3 * (int64_t)k + 2]) {
                 atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_DUPLICATE);
@@ -150,12 +218,14 @@ __global__ void k_fill_i32(int *a, int64_t n, int v)
 
 // Site scatter: the first point of each equal-voxel run (lowest Morton index) is the voxel's
 // leader; the reference picks a racy last writer (GD/PBA/pba3DKernel.h:518-542).
-__global__ void k_scatter_sites(const int *vox, int n, int g, int *grid, int *idgrid)
+__global__ void k_scatter_sites(const int *vox, int n, GridSets S, int *grid, int *idgrid)
 {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const int c = vox[k];
-        if (k > 0 && vox[k - 1] == c) continue;
-        const int64_t idx = ((int64_t)G4D_DEC_Z(c) * g + G4D_DEC_Y(c)) * g + G4D_DEC_X(c);
+        const int st = set_of(S, k);
+        if (k > S.pt_off[st] && vox[k - 1] == c) continue;
+        const int64_t g = S.g[st];
+        const int64_t idx = S.vox_off[st] + ((int64_t)G4D_DEC_Z(c) * g + G4D_DEC_Y(c)) * g + G4D_DEC_X(c);
         grid[idx] = c;
         idgrid[idx] = k;
     }
@@ -226,18 +296,25 @@ __device__ __forceinline__ int64_t pba_gidx64(int g, int fixed, int c, int q)
 }
 
 template <int AXIS>
-__global__ void __launch_bounds__(64) k_pba_pass(const int *in, int *out, int g, const int *idgrid, int *spill)
+__global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all, GridSets S, const int *idgrid_all,
+                                                 int *spill_all)
 {
     __shared__ PbaLds L;
     const int lane = threadIdx.x;
+    const int st = S.nsets == 1 ? 0 : set_of_work(S.pba_blk, S.nsets, blockIdx.x);
+    const int64_t lb = blockIdx.x - S.pba_blk[st];
+    const int g = S.g[st];
+    const int *in = in_all + S.vox_off[st];
+    int *out = out_all + S.vox_off[st];
+    const int *idgrid = idgrid_all ? idgrid_all + S.vox_off[st] : nullptr;
     const int ntiles = (g + PBA_COLS - 1) / PBA_COLS;
-    const int fixed = blockIdx.x / ntiles;
-    const int c0 = (blockIdx.x % ntiles) * PBA_COLS;
+    const int fixed = (int)(lb / ntiles);
+    const int c0 = (int)(lb % ntiles) * PBA_COLS;
     const int ncols = min(PBA_COLS, g - c0);
     const bool active = lane < ncols;
     const int c = c0 + lane;
     const int ca = c, cb = fixed; // x (Z, Y passes) or y (X pass); y (Z), z (Y), z (X)
-    int *sp = spill + ((int64_t)blockIdx.x * PBA_COLS + lane) * g;
+    int *sp = spill_all + S.spill[st] + (lb * PBA_COLS + lane) * g;
 
     // ---- lower envelope ----
     int m = 0;
@@ -328,29 +405,21 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in, int *out, int g,
     }
 }
 
-int64_t pba_spill_ints(int g)
-{
-    const int64_t ntiles = (g + PBA_COLS - 1) / PBA_COLS;
-    return (int64_t)g * ntiles * PBA_COLS * g;
-}
+int64_t pba_blocks(int g) { return (int64_t)g * ((g + PBA_COLS - 1) / PBA_COLS); }
 
-hipError_t pba_pass(int axis, int g, const int *in, int *out, const int *idgrid, int *spill, hipStream_t s)
+int64_t pba_spill_ints(int g) { return pba_blocks(g) * PBA_COLS * g; }
+
+hipError_t pba_pass(int axis, const GridSets &S, int64_t nblocks, const int *in, int *out, const int *idgrid,
+                    int *spill, hipStream_t s)
 {
-    const int nblk = g * ((g + PBA_COLS - 1) / PBA_COLS);
+    if (nblocks <= 0 || nblocks > INT32_MAX) return hipErrorInvalidValue;
     if (axis == 2)
-        hipLaunchKernelGGL(k_pba_pass<2>, dim3(nblk), dim3(64), 0, s, in, out, g, idgrid, spill);
+        hipLaunchKernelGGL(k_pba_pass<2>, dim3((unsigned)nblocks), dim3(64), 0, s, in, out, S, idgrid, spill);
     else if (axis == 1)
-        hipLaunchKernelGGL(k_pba_pass<1>, dim3(nblk), dim3(64), 0, s, in, out, g, idgrid, spill);
+        hipLaunchKernelGGL(k_pba_pass<1>, dim3((unsigned)nblocks), dim3(64), 0, s, in, out, S, idgrid, spill);
     else
-        hipLaunchKernelGGL(k_pba_pass<0>, dim3(nblk), dim3(64), 0, s, in, out, g, idgrid, spill);
+        hipLaunchKernelGGL(k_pba_pass<0>, dim3((unsigned)nblocks), dim3(64), 0, s, in, out, S, idgrid, spill);
     return hipGetLastError();
-}
-
-hipError_t pba_run(int g, int *grid, int *tmp, const int *idgrid, int *spill, hipStream_t s)
-{
-    HIP_CHECK_RET(pba_pass(2, g, grid, tmp, nullptr, spill, s));
-    HIP_CHECK_RET(pba_pass(1, g, tmp, grid, nullptr, spill, s));
-    return pba_pass(0, g, grid, tmp, idgrid, spill, s);
 }
 
 // ------------------------------------------------------------------------------------
@@ -365,8 +434,8 @@ __device__ __forceinline__ uint64_t upair(int a, int b)
 
 // Missing points (GD/GDelaunay/GDelKernels.cu:109-135): a point whose voxel is led by another
 // point pairs with that leader.  Emits ordered pairs (min<<32 | max).
-__global__ void k_missing(const int *vox, int n, int g, const int *grid, uint64_t *out, unsigned long long *counter,
-                          int64_t cap)
+__global__ void k_missing(const int *vox, int n, GridSets S, const int *grid, uint64_t *out,
+                          unsigned long long *counter, int64_t cap)
 {
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
         const int k = base + threadIdx.x;
@@ -374,7 +443,9 @@ __global__ void k_missing(const int *vox, int n, int g, const int *grid, uint64_
         int leader = 0;
         if (k < n) {
             const int c = vox[k];
-            leader = grid[((int64_t)G4D_DEC_Z(c) * g + G4D_DEC_Y(c)) * g + G4D_DEC_X(c)];
+            const int st = set_of(S, k);
+            const int64_t g = S.g[st];
+            leader = grid[S.vox_off[st] + ((int64_t)G4D_DEC_Z(c) * g + G4D_DEC_Y(c)) * g + G4D_DEC_X(c)];
             miss = leader != k;
         }
         const long long slot = g4d_wave_append(miss, counter);
@@ -419,18 +490,24 @@ __device__ __forceinline__ uint32_t cube_pairs(const int *v, uint64_t *pr)
     return mask;
 }
 
+__host__ __device__ inline int grid_pair_chunk_dev(int g) { return g > 512 ? 32 : 16; }
+
 // One thread per (x, y) column of cubes and a chunk of zc cubes along z; pass 0 counts the
 // thread's pairs, pass 1 writes them at the scanned offset (deterministic order, no atomics).
 template <bool WRITE>
-__global__ void __launch_bounds__(256) k_grid_pairs(const int *grid, int g, int zc, int64_t *count, uint64_t *out)
+__global__ void __launch_bounds__(256) k_grid_pairs(const int *grid_all, GridSets S, int64_t nthr, int64_t *count,
+                                                    uint64_t *out)
 {
-    const int w = g + 1;
-    const int64_t ncol = (int64_t)w * w;
-    const int nzc = (w + zc - 1) / zc;
-    const int64_t nthr = ncol * nzc;
     for (int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; tid < nthr; tid += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t col = tid % ncol;
-        const int zb = (int)(tid / ncol);
+        const int st = S.nsets == 1 ? 0 : set_of_work(S.pair_thr, S.nsets, tid);
+        const int64_t lt = tid - S.pair_thr[st];
+        const int g = S.g[st];
+        const int *grid = grid_all + S.vox_off[st];
+        const int zc = grid_pair_chunk_dev(g);
+        const int w = g + 1;
+        const int64_t ncol = (int64_t)w * w;
+        const int64_t col = lt % ncol;
+        const int zb = (int)(lt / ncol);
         const int x = (int)(col % w) - 1, y = (int)(col / w) - 1;
         const int z0 = zb * zc - 1, z1 = min(z0 + zc, g);
         int v[8];
@@ -486,7 +563,7 @@ __device__ __forceinline__ bool list_has(const uint64_t *keys, int64_t b, int64_
 
 // Pumping of stars with fewer than min_ws candidates (GD/GDelaunay/GDelKernels.cu:347-477):
 // neighbours' candidates in ascending order, then ids 0,1,2,...  Emits both directions.
-__global__ void k_pump(const uint64_t *keys, const int64_t *off, int n, int min_ws, uint64_t *out,
+__global__ void k_pump(const uint64_t *keys, const int64_t *off, int n, GridSets S, int min_ws, uint64_t *out,
                        unsigned long long *counter, int64_t cap)
 {
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
@@ -506,7 +583,8 @@ __global__ void k_pump(const uint64_t *keys, const int64_t *off, int n, int min_
                 if (!dup) got[ng++] = cv;
             }
         }
-        for (int cv = 0; ng < need && cv < n; cv++) {
+        const int st = set_of(S, s);
+        for (int cv = (int)S.pt_off[st]; ng < need && cv < (int)S.pt_off[st + 1]; cv++) {
             if (cv == s || list_has(keys, b, e, cv)) continue;
             bool dup = false;
             for (int r = 0; r < ng; r++) dup |= (got[r] == cv);
@@ -536,23 +614,36 @@ hipError_t launch_bounds(const double *xyz, int n, double *part, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_morton_keys(const double *xyz, int n, int g, const double *bounds, uint64_t *keys, hipStream_t s)
+hipError_t launch_set_bounds(const double *xyz, const GridSets &S, double *bounds, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_morton_keys, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, xyz, n, g, bounds, keys);
+    hipLaunchKernelGGL(k_set_bounds, dim3(S.nsets), dim3(256), 0, s, xyz, S, bounds);
     return hipGetLastError();
 }
 
-hipError_t launch_gather_sorted(const double *xyz, const uint64_t *keys, int n, int g, const double *bounds,
-                                double *pts, int *perm, int *vox, hipStream_t s)
+hipError_t launch_set_ids(const int64_t *pt_off, int nsets, int *pt_set, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gather_sorted, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, xyz, keys, n, g, bounds, pts,
-                       perm, vox);
+    hipLaunchKernelGGL(k_set_ids, dim3(nsets), dim3(256), 0, s, pt_off, pt_set);
     return hipGetLastError();
 }
 
-hipError_t launch_dup_check(const double *pts, const int *vox, int n, unsigned long long *cnt, hipStream_t s)
+hipError_t launch_morton_keys(const double *xyz, int n, const GridSets &S, uint64_t *keys, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dup_check, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, pts, vox, n, cnt);
+    hipLaunchKernelGGL(k_morton_keys, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, xyz, n, S, keys);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_sorted(const double *xyz, const uint64_t *keys, int n, const GridSets &S, double *pts,
+                                int *perm, int *vox, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gather_sorted, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, xyz, keys, n, S, pts, perm,
+                       vox);
+    return hipGetLastError();
+}
+
+hipError_t launch_dup_check(const double *pts, const int *vox, int n, const GridSets &S, unsigned long long *cnt,
+                            hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dup_check, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, pts, vox, n, S, cnt);
     return hipGetLastError();
 }
 
@@ -562,18 +653,20 @@ hipError_t launch_fill_i32(int *a, int64_t n, int v, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_scatter_sites(const int *vox, int n, int g, int *grid, int *idgrid, hipStream_t s)
+hipError_t launch_scatter_sites(const int *vox, int n, const GridSets &S, int *grid, int *idgrid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_scatter_sites, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, vox, n, g, grid, idgrid);
+    hipLaunchKernelGGL(k_scatter_sites, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, vox, n, S, grid, idgrid);
     return hipGetLastError();
 }
 
-hipError_t launch_missing(const int *vox, int n, int g, const int *grid, uint64_t *out, unsigned long long *counter,
-                          int64_t cap, hipStream_t s)
+hipError_t launch_missing(const int *vox, int n, const GridSets &S, const int *grid, uint64_t *out,
+                          unsigned long long *counter, int64_t cap, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_missing, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, vox, n, g, grid, out, counter, cap);
+    hipLaunchKernelGGL(k_missing, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, vox, n, S, grid, out, counter, cap);
     return hipGetLastError();
 }
+
+int grid_pair_chunk(int g) { return grid_pair_chunk_dev(g); }
 
 int64_t grid_pair_threads(int g, int zc)
 {
@@ -581,13 +674,16 @@ int64_t grid_pair_threads(int g, int zc)
     return w * w * ((w + zc - 1) / zc);
 }
 
-hipError_t launch_grid_pairs(const int *grid, int g, int zc, int64_t *count, uint64_t *out, bool write, hipStream_t s)
+hipError_t launch_grid_pairs(const int *grid, const GridSets &S, int64_t nthr, int64_t *count, uint64_t *out,
+                             bool write, hipStream_t s)
 {
-    const int64_t nthr = grid_pair_threads(g, zc);
+    if (nthr <= 0) return hipSuccess;
     if (write)
-        hipLaunchKernelGGL(k_grid_pairs<true>, dim3(g4d_blocks(nthr, 256, 65536)), dim3(256), 0, s, grid, g, zc, count, out);
+        hipLaunchKernelGGL(k_grid_pairs<true>, dim3(g4d_blocks(nthr, 256, 65536)), dim3(256), 0, s, grid, S, nthr, count,
+                           out);
     else
-        hipLaunchKernelGGL(k_grid_pairs<false>, dim3(g4d_blocks(nthr, 256, 65536)), dim3(256), 0, s, grid, g, zc, count, out);
+        hipLaunchKernelGGL(k_grid_pairs<false>, dim3(g4d_blocks(nthr, 256, 65536)), dim3(256), 0, s, grid, S, nthr, count,
+                           out);
     return hipGetLastError();
 }
 
@@ -597,10 +693,10 @@ hipError_t launch_csr_offsets(const uint64_t *keys, int64_t m, int n, int64_t *o
     return hipGetLastError();
 }
 
-hipError_t launch_pump(const uint64_t *keys, const int64_t *off, int n, int min_ws, uint64_t *out,
+hipError_t launch_pump(const uint64_t *keys, const int64_t *off, int n, const GridSets &S, int min_ws, uint64_t *out,
                        unsigned long long *counter, int64_t cap, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pump, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, keys, off, n, min_ws, out, counter,
+    hipLaunchKernelGGL(k_pump, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, keys, off, n, S, min_ws, out, counter,
                        cap);
     return hipGetLastError();
 }

@@ -71,6 +71,10 @@ hipError_t unique_u64(Prims &P, uint64_t *keys, int64_t n, int64_t *n_out, hipSt
 hipError_t compact_u64(Prims &P, const uint64_t *in, const uint8_t *keep, uint64_t *out, int64_t n,
                        int64_t *n_out, hipStream_t s);
 
+// Order-preserving compaction of the entries different from `none`.  Count via host sync.
+hipError_t compact_valid_u64(Prims &P, const uint64_t *in, uint64_t *out, int64_t n, uint64_t none,
+                             int64_t *n_out, hipStream_t s);
+
 // Kernel launch helpers
 inline int g4d_blocks(int64_t n, int per_block, int cap = 1 << 20)
 {

@@ -245,6 +245,12 @@ __global__ void k_keep_flags(const uint8_t *keep, int64_t n, uint32_t *flags)
         flags[i] = keep[i] ? 1u : 0u;
 }
 
+__global__ void k_valid_flags(const uint64_t *in, int64_t n, uint64_t none, uint32_t *flags)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        flags[i] = in[i] != none ? 1u : 0u;
+}
+
 __global__ void k_scatter_flagged(const uint64_t *in, const uint32_t *flags, const uint32_t *pos, uint64_t *out,
                                   int64_t n)
 {
@@ -303,6 +309,21 @@ hipError_t compact_u64(Prims &P, const uint64_t *in, const uint8_t *keep, uint64
     HIP_CHECK_RET(P.flags.ensure(sizeof(uint32_t) * (size_t)n));
     HIP_CHECK_RET(P.pos.ensure(sizeof(uint32_t) * (size_t)n));
     hipLaunchKernelGGL(k_keep_flags, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, keep, n, P.flags.as<uint32_t>());
+    HIP_CHECK_RET(hipGetLastError());
+    return compact_with_flags(P, in, out, n, n_out, s);
+}
+
+hipError_t compact_valid_u64(Prims &P, const uint64_t *in, uint64_t *out, int64_t n, uint64_t none, int64_t *n_out,
+                             hipStream_t s)
+{
+    if (n <= 0) {
+        *n_out = 0;
+        return hipSuccess;
+    }
+    HIP_CHECK_RET(P.flags.ensure(sizeof(uint32_t) * (size_t)n));
+    HIP_CHECK_RET(P.pos.ensure(sizeof(uint32_t) * (size_t)n));
+    hipLaunchKernelGGL(k_valid_flags, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, in, n, none,
+                       P.flags.as<uint32_t>());
     HIP_CHECK_RET(hipGetLastError());
     return compact_with_flags(P, in, out, n, n_out, s);
 }

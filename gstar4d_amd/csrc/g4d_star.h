@@ -5,18 +5,29 @@
 
 namespace g4d {
 
-// Star triangles in HBM.  A star with fewer than STAR_WIDE_MIN link vertices stores each
-// triangle in one word: local vertex indices (9 bits each), adjacency slots (10 bits each),
-// checked flag (bit 57).  A larger star stores two words per triangle: local vertices
-// (21 bits each, checked flag in bit 63), then adjacency slots (21 bits each).
 // LDS star kernel tiers by link vertices (plus this pass's insertions).
 static constexpr int STAR_VMAX_T0 = 32;
 static constexpr int STAR_VMAX_T1 = 64;
 static constexpr int STAR_VMAX_T2 = 256;
+// Stars taken per work-queue atomic by the filter-only LDS kernels.
+static constexpr int STAR_CHUNK = 8;
+// Star triangles in HBM.  A star with fewer than STAR_WIDE_MIN link vertices stores each
+// triangle in one word: local vertex indices (9 bits each), adjacency slots (10 bits each),
+// checked flag (bit 57).  A larger star stores two words per triangle: local vertices
+// (21 bits each, checked flag in bit 63), then adjacency slots (21 bits each).
 static constexpr int STAR_WIDE_MIN = 512;
 // Stars whose link (plus this pass's insertions) exceeds the LDS kernels' limit run in the
 // global-memory star kernel, which has no fixed capacity.
 static constexpr int STAR_LDS_LIMIT = STAR_WIDE_MIN - 1;
+
+// Triangle words reserved for a star of v link vertices (including this pass's insertions):
+// 2v - 4 triangles, two words each in the wide format.
+__host__ __device__ inline int64_t star_tcap_words(int64_t v)
+{
+    const int64_t t = v >= 4 ? 2 * v - 4 : 4;
+    return v >= STAR_WIDE_MIN ? 2 * t : t;
+}
+static constexpr uint64_t STAR_PAIR_NONE = ~0ull;
 
 // Star storage: per-star records (vertex count/offset, triangle count/offset, history
 // count/offset) over three append-only arenas (link vertices, triangle words, history
@@ -45,11 +56,10 @@ struct StarParams {
     const int64_t *nvoff;
     const int64_t *ntoff;
     const int64_t *nhoff;
-    // appends
-    uint64_t *drown;      // (star<<32 | vertex)       counter G4D_CNT_APPEND0
-    int64_t drown_cap;
-    uint64_t *tpairs;     // (a<<32 | b), a < b         counter G4D_CNT_APPEND1
-    int64_t tpairs_cap;
+    // outputs without atomics (compacted by the host code after the pass)
+    uint8_t *dflag;       // per insertion, indexed like ins_keys: 1 when the vertex drowned
+    uint64_t *pairbuf;    // per triangle word of this pass's regions two entries (a<<32 | b,
+    int64_t tbase;        // a < b, or STAR_PAIR_NONE), from 2 * (ntoff[s] - tbase)
     int *defer_list;      // stars that exceed this kernel's capacity, counter `defer_counter`
     int64_t defer_cap;
     int defer_counter;
@@ -152,5 +162,11 @@ hipError_t launch_out_write(const OutParams &p, hipStream_t s);
 hipError_t launch_adjacency(const int *tets, int64_t nt, int *opp, uint32_t *table, int64_t table_size,
                             unsigned long long *cnt, hipStream_t s);
 hipError_t launch_map_perm(int *tets, int64_t m4, const int *perm, hipStream_t s);
+// out[i] = a[idx[i]] for i < m.
+hipError_t launch_gather_i64(const int64_t *a, const int64_t *idx, int64_t m, int64_t *out, hipStream_t s);
+// Batched output: vertex ids (sorted positions) -> input index within the set; opp (global
+// tetrahedron ids) -> index within the set's range of tetrahedra.
+hipError_t launch_batch_localize(int *tets, int *opp, int64_t nt, const int *pt_set, const int64_t *pt_off,
+                                 const int64_t *tet_off, const int *perm, hipStream_t s);
 
 } // namespace g4d

@@ -102,6 +102,40 @@ __device__ __forceinline__ int wave_excl(int v, int *tot)
     return inc - v;
 }
 
+// Insertion pairs of a link triangle with global vertices (a, b, c) (GD/GDelaunay/GDelKernels.cu:
+// 818-939): when it is unchecked, each cyclic edge (u, w) with u < w gives (u<<32 | w); there
+// are one or two such edges, the unused entry is STAR_PAIR_NONE.
+struct TriPairs {
+    uint64_t x, y;
+};
+__device__ __forceinline__ TriPairs star_tri_pairs(int a, int b, int c, bool unchecked)
+{
+    uint64_t e0 = STAR_PAIR_NONE, e1 = STAR_PAIR_NONE;
+    if (unchecked) {
+        const int u[3] = {a, b, c};
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+            const int x = u[i], y = u[(i + 1) % 3];
+            if (x < y) {
+                const uint64_t k = ((uint64_t)(uint32_t)x << 32) | (uint32_t)y;
+                if (e0 == STAR_PAIR_NONE) e0 = k;
+                else e1 = k;
+            }
+        }
+    }
+    return TriPairs{e0, e1};
+}
+
+// The LDS star kernels run one wave per work-group.  A wave's LDS operations complete in
+// order, so ordering its LDS traffic needs only a compiler-level barrier at wavefront scope;
+// unlike __syncthreads() this does not wait for the wave's outstanding global stores.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Appends star s to a list (lane 0), flagging a buffer error when the list is full.
 __device__ __forceinline__ void star_list_push(int *list, int64_t cap, unsigned long long *counter, int s,
                                                unsigned long long *cnt)
@@ -173,334 +207,321 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
     const int lane = threadIdx.x;
     unsigned long long *cnt = p.cnt;
 
+    // Work queue in chunks: one atomic per STAR_CHUNK stars for the filter-only kernels (many
+    // small stars), per star for the exact ones (uneven cost).
+    constexpr int CHUNK = EXACT ? 1 : STAR_CHUNK;
     for (;;) {
-        if (lane == 0) L.sc[0] = (int)atomicAdd(&cnt[G4D_CNT_WORK], 1ull);
-        __syncthreads();
-        const int item = L.sc[0];
-        __syncthreads();
-        if (item >= nitems) break;
-        const int s = p.star_list ? p.star_list[item] : item;
-        const int64_t ib = p.ins_off[s];
-        const int m = (int)(p.ins_off[s + 1] - ib);
-        const int vn = p.mode ? p.vnum[s] : 0;
-        const int need = vn + m;
-        const int lim = VMAX < p.lds_limit ? VMAX : p.lds_limit;
-        if (need > lim) {
-            star_list_push(p.defer_list, p.defer_cap, &cnt[p.defer_counter], s, cnt);
-            continue;
-        }
-        if (p.mode == 0 && m < 4) {
-            if (lane == 0) {
-                atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_CONE);
-                p.vnum[s] = 0;
-                p.tnum[s] = 0;
-            }
-            continue;
-        }
-        if (p.mode == 1 && m == 0) continue; // untouched star: stays where it is
-        const double sx = p.pts[3 * (int64_t)s], sy = p.pts[3 * (int64_t)s + 1], sz = p.pts[3 * (int64_t)s + 2];
-        // Insertion q takes link-vertex slot vn + q; every insertion's coordinates are fetched
-        // here at once instead of one dependent load per insertion step.  (In mode 0 the
-        // first four form the initial simplex, set up below.)
-        const int q0 = p.mode ? 0 : 4;
-        for (int q = lane; q < m; q += 64) {
-            const int g = (int)(uint32_t)p.ins_keys[ib + q];
-            L.ins[q] = g;
-            L.ins_local[q] = -1;
-            if (q >= q0) {
-                L.vid[vn + q] = g;
-                rel_coords(p.pts, g, sx, sy, sz, L.vc[vn + q]);
-            }
-        }
-        int T, nfree = 0;
-        if (p.mode == 1) {
-            const int tn = p.tnum[s];
-            const int64_t vi = p.voff[s], ti = p.toff[s];
-            for (int v = lane; v < vn; v += 64) {
-                const int g = p.vpool[vi + v];
-                L.vid[v] = g;
-                rel_coords(p.pts, g, sx, sy, sz, L.vc[v]);
-            }
-            for (int t = lane; t < tn; t += 64) {
-                const uint64_t h = p.tpool[ti + t];
-                L.tv[t] = tpack(hvert(h, 0), hvert(h, 1), hvert(h, 2), TF_ALIVE | (hchecked(h) ? TF_CHECKED : 0u));
-                L.to[t][0] = (uint16_t)hopp(h, 0);
-                L.to[t][1] = (uint16_t)hopp(h, 1);
-                L.to[t][2] = (uint16_t)hopp(h, 2);
-            }
-            T = tn;
-        } else {
-            // initial 4-simplex (GD/GDelaunay/GDelPredKernels.cu:45-158)
-            int l[4];
-            for (int i = 0; i < 4; i++) l[i] = (int)(uint32_t)p.ins_keys[ib + i];
-            const double *pp[5] = {p.pts + 3 * (int64_t)l[0], p.pts + 3 * (int64_t)l[1], p.pts + 3 * (int64_t)l[2],
-                                   p.pts + 3 * (int64_t)l[3], p.pts + 3 * (int64_t)s};
-            const int idx[5] = {l[0], l[1], l[2], l[3], s};
-            const int o = EXACT ? orient4_full(pp, idx, cnt) : filter_insphere(pp);
-            if (!EXACT && o == 0) {
-                star_list_push(p.exact_list, p.exact_cap, &cnt[p.exact_counter], s, cnt);
+        int base = 0;
+        if (lane == 0) base = (int)atomicAdd(&cnt[G4D_CNT_WORK], (unsigned long long)CHUNK);
+        base = __shfl(base, 0, 64);
+        if (base >= nitems) break;
+        const int bend = base + CHUNK < nitems ? base + CHUNK : nitems;
+#pragma unroll 1
+        for (int item = base; item < bend; item++) {
+            const int s = p.star_list ? p.star_list[item] : item;
+            const int64_t ib = p.ins_off[s];
+            const int m = (int)(p.ins_off[s + 1] - ib);
+            const int vn = p.mode ? p.vnum[s] : 0;
+            const int need = vn + m;
+            const int lim = VMAX < p.lds_limit ? VMAX : p.lds_limit;
+            if (need > lim) {
+                star_list_push(p.defer_list, p.defer_cap, &cnt[p.defer_counter], s, cnt);
                 continue;
             }
-            const bool sw = o < 0;
-            if (sw) {
-                const int t = l[0];
-                l[0] = l[1];
-                l[1] = t;
+            if (p.mode == 0 && m < 4) {
+                if (lane == 0) {
+                    atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_CONE);
+                    p.vnum[s] = 0;
+                    p.tnum[s] = 0;
+                }
+                continue;
             }
-            if (lane < 4) {
-                L.vid[lane] = l[lane];
-                rel_coords(p.pts, l[lane], sx, sy, sz, L.vc[lane]);
-                // cone triangles (link triangles seen from the star point) and adjacency
-                const int lt[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
-                L.tv[lane] = tpack(lt[lane][0], lt[lane][1], lt[lane][2], TF_ALIVE);
-                L.to[lane][0] = (uint16_t)lt[lane][0];
-                L.to[lane][1] = (uint16_t)lt[lane][1];
-                L.to[lane][2] = (uint16_t)lt[lane][2];
-                const int local = (lane < 2 && sw) ? 1 - lane : lane;
-                L.ins_local[lane] = (int16_t)local;
+            if (p.mode == 1 && m == 0) continue; // untouched star: stays where it is
+            const double sx = p.pts[3 * (int64_t)s], sy = p.pts[3 * (int64_t)s + 1], sz = p.pts[3 * (int64_t)s + 2];
+            // Insertion q takes link-vertex slot vn + q; every insertion's coordinates are fetched
+            // here at once instead of one dependent load per insertion step.  (In mode 0 the
+            // first four form the initial simplex, set up below.)
+            const int q0 = p.mode ? 0 : 4;
+            for (int q = lane; q < m; q += 64) {
+                const int g = (int)(uint32_t)p.ins_keys[ib + q];
+                L.ins[q] = g;
+                L.ins_local[q] = -1;
+                if (q >= q0) {
+                    L.vid[vn + q] = g;
+                    rel_coords(p.pts, g, sx, sy, sz, L.vc[vn + q]);
+                }
             }
-            T = 4;
-        }
-        __syncthreads();
+            int T, nfree = 0;
+            if (p.mode == 1) {
+                const int tn = p.tnum[s];
+                const int64_t vi = p.voff[s], ti = p.toff[s];
+                for (int v = lane; v < vn; v += 64) {
+                    const int g = p.vpool[vi + v];
+                    L.vid[v] = g;
+                    rel_coords(p.pts, g, sx, sy, sz, L.vc[v]);
+                }
+                for (int t = lane; t < tn; t += 64) {
+                    const uint64_t h = p.tpool[ti + t];
+                    L.tv[t] = tpack(hvert(h, 0), hvert(h, 1), hvert(h, 2), TF_ALIVE | (hchecked(h) ? TF_CHECKED : 0u));
+                    L.to[t][0] = (uint16_t)hopp(h, 0);
+                    L.to[t][1] = (uint16_t)hopp(h, 1);
+                    L.to[t][2] = (uint16_t)hopp(h, 2);
+                }
+                T = tn;
+            } else {
+                // initial 4-simplex (GD/GDelaunay/GDelPredKernels.cu:45-158)
+                int l[4];
+                for (int i = 0; i < 4; i++) l[i] = (int)(uint32_t)p.ins_keys[ib + i];
+                const double *pp[5] = {p.pts + 3 * (int64_t)l[0], p.pts + 3 * (int64_t)l[1], p.pts + 3 * (int64_t)l[2],
+                                       p.pts + 3 * (int64_t)l[3], p.pts + 3 * (int64_t)s};
+                const int idx[5] = {l[0], l[1], l[2], l[3], s};
+                const int o = EXACT ? orient4_full(pp, idx, cnt) : filter_insphere(pp);
+                if (!EXACT && o == 0) {
+                    star_list_push(p.exact_list, p.exact_cap, &cnt[p.exact_counter], s, cnt);
+                    continue;
+                }
+                const bool sw = o < 0;
+                if (sw) {
+                    const int t = l[0];
+                    l[0] = l[1];
+                    l[1] = t;
+                }
+                if (lane < 4) {
+                    L.vid[lane] = l[lane];
+                    rel_coords(p.pts, l[lane], sx, sy, sz, L.vc[lane]);
+                    // cone triangles (link triangles seen from the star point) and adjacency
+                    const int lt[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+                    L.tv[lane] = tpack(lt[lane][0], lt[lane][1], lt[lane][2], TF_ALIVE);
+                    L.to[lane][0] = (uint16_t)lt[lane][0];
+                    L.to[lane][1] = (uint16_t)lt[lane][1];
+                    L.to[lane][2] = (uint16_t)lt[lane][2];
+                    const int local = (lane < 2 && sw) ? 1 - lane : lane;
+                    L.ins_local[lane] = (int16_t)local;
+                }
+                T = 4;
+            }
+            wave_sync();
 #pragma unroll 1
-        for (int k = 0; k < KPL; k++) {
-            const int slot = lane + 64 * k;
-            if (slot < T) {
-                const uint32_t t = L.tv[slot];
-                tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], L.cof[slot], L.cof[slot] + 4);
+            for (int k = 0; k < KPL; k++) {
+                const int slot = lane + 64 * k;
+                if (slot < T) {
+                    const uint32_t t = L.tv[slot];
+                    tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], L.cof[slot], L.cof[slot] + 4);
+                }
             }
-        }
-        bool failed = false, unsure = false;
-        for (int q = q0; q < m; q++) {
-            const int gv = L.ins[q];
-            const int Vn = vn + q;
-            const double r[4] = {L.vc[Vn][0], L.vc[Vn][1], L.vc[Vn][2], L.vc[Vn][3]};
-            int nv = 0;
-            bool unc = false;
+            bool failed = false, unsure = false;
+            for (int q = q0; q < m; q++) {
+                const int gv = L.ins[q];
+                const int Vn = vn + q;
+                const double r[4] = {L.vc[Vn][0], L.vc[Vn][1], L.vc[Vn][2], L.vc[Vn][3]};
+                int nv = 0;
+                bool unc = false;
+#pragma unroll 1
+                for (int k = 0; k < KPL; k++) {
+                    const int slot = lane + 64 * k;
+                    if (slot < T) {
+                        const uint32_t t = L.tv[slot];
+                        if (t & TF_ALIVE) {
+                            int sg = cofactor_filter_sign(r, L.cof[slot], L.cof[slot] + 4);
+                            if (sg == 0) {
+                                if (EXACT) {
+                                    const int a = L.vid[tvert(t, 0)], b = L.vid[tvert(t, 1)], c = L.vid[tvert(t, 2)];
+                                    const double *pp[5] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b,
+                                                           p.pts + 3 * (int64_t)c, p.pts + 3 * (int64_t)s,
+                                                           p.pts + 3 * (int64_t)gv};
+                                    const int idx[5] = {a, b, c, s, gv};
+                                    sg = orient4_full(pp, idx, cnt);
+                                } else {
+                                    unc = true;
+                                }
+                            }
+                            if (sg < 0) {
+                                L.tv[slot] = t | TF_VIS;
+                                nv++;
+                            }
+                        }
+                    }
+                }
+                if (!EXACT && __ballot(unc)) {
+                    unsure = true;
+                    break;
+                }
+                int nvis;
+                const int vbase = wave_excl(nv, &nvis);
+                if (nvis == 0) continue; // drowned at insertion
+                if (lane == 0) L.ins_local[q] = (int16_t)Vn;
+                wave_sync();
+                // hole boundary: edges of freed triangles whose neighbour survives
+                int nb = 0;
+#pragma unroll 1
+                for (int k = 0; k < KPL; k++) {
+                    const int slot = lane + 64 * k;
+                    if (slot < T && (L.tv[slot] & TF_VIS))
+                        for (int i = 0; i < 3; i++) nb += (L.tv[L.to[slot][i]] & TF_VIS) ? 0 : 1;
+                }
+                int nbtot;
+                int e = wave_excl(nb, &nbtot);
+                if (nbtot > TMAX || nbtot < 3) {
+                    failed = true;
+                    break;
+                }
+                int fpos = nfree + vbase;
+#pragma unroll 1
+                for (int k = 0; k < KPL; k++) {
+                    const int slot = lane + 64 * k;
+                    if (slot < T && (L.tv[slot] & TF_VIS)) {
+                        const uint32_t t = L.tv[slot];
+                        for (int i = 0; i < 3; i++) {
+                            const int u = L.to[slot][i];
+                            if (L.tv[u] & TF_VIS) continue;
+                            int j = 0;
+                            while (j < 2 && L.to[u][j] != slot) j++;
+                            L.be_a[e] = (uint16_t)tvert(t, (i + 1) % 3);
+                            L.be_b[e] = (uint16_t)tvert(t, (i + 2) % 3);
+                            L.be_u[e] = (uint16_t)u;
+                            L.be_j[e] = (uint8_t)j;
+                            e++;
+                        }
+                        L.freel[fpos++] = (uint16_t)slot;
+                    }
+                }
+                wave_sync();
+                const int nfree_tot = nfree + nvis;
+#pragma unroll 1
+                for (int k = 0; k < KPL; k++) {
+                    const int slot = lane + 64 * k;
+                    if (slot < T && (L.tv[slot] & TF_VIS)) L.tv[slot] = 0u;
+                }
+                for (int k = lane; k < nbtot; k += 64)
+                    L.be_slot[k] = (uint16_t)(k < nfree_tot ? L.freel[nfree_tot - 1 - k] : T + (k - nfree_tot));
+                const int newT = T + (nbtot > nfree_tot ? nbtot - nfree_tot : 0);
+                if (newT > TMAX || Vn + 1 > VMAX) {
+                    failed = true;
+                    break;
+                }
+                wave_sync();
+                for (int k = lane; k < nbtot; k += 64) {
+                    const int slot = L.be_slot[k], a = L.be_a[k], b = L.be_b[k], u = L.be_u[k];
+                    L.tv[slot] = tpack(a, b, Vn, TF_ALIVE | TF_FRESH);
+                    L.to[slot][2] = (uint16_t)u;
+                    L.to[u][L.be_j[k]] = (uint16_t)slot;
+                    L.vstart[a] = (int16_t)slot;
+                    L.vend[b] = (int16_t)slot;
+                }
+                wave_sync();
+                for (int k = lane; k < nbtot; k += 64) {
+                    const int slot = L.be_slot[k];
+                    L.to[slot][0] = (uint16_t)L.vstart[L.be_b[k]];
+                    L.to[slot][1] = (uint16_t)L.vend[L.be_a[k]];
+                }
+                nfree = nfree_tot - (nbtot < nfree_tot ? nbtot : nfree_tot);
+                T = newT;
+                wave_sync();
+#pragma unroll 1
+                for (int k = 0; k < KPL; k++) {
+                    const int slot = lane + 64 * k;
+                    if (slot < T) {
+                        const uint32_t t = L.tv[slot];
+                        if (t & TF_FRESH) {
+                            tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], L.cof[slot],
+                                          L.cof[slot] + 4);
+                            L.tv[slot] = t & ~TF_FRESH;
+                        }
+                    }
+                }
+                wave_sync();
+            }
+            if (unsure) {
+                star_list_push(p.exact_list, p.exact_cap, &cnt[p.exact_counter], s, cnt);
+                wave_sync();
+                continue;
+            }
+            if (failed) {
+                if (lane == 0) {
+                    atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
+                    p.vnum[s] = 0;
+                    p.tnum[s] = 0;
+                }
+                wave_sync();
+                continue;
+            }
+            // ---- finalize: link set, drowned insertions, compaction, pairs, history ----
+            const int V = need; // drowned insertions' slots are never referenced and drop out
+            for (int v = lane; v < V; v += 64) L.present[v] = 0;
+            wave_sync();
 #pragma unroll 1
             for (int k = 0; k < KPL; k++) {
                 const int slot = lane + 64 * k;
                 if (slot < T) {
                     const uint32_t t = L.tv[slot];
                     if (t & TF_ALIVE) {
-                        int sg = cofactor_filter_sign(r, L.cof[slot], L.cof[slot] + 4);
-                        if (sg == 0) {
-                            if (EXACT) {
-                                const int a = L.vid[tvert(t, 0)], b = L.vid[tvert(t, 1)], c = L.vid[tvert(t, 2)];
-                                const double *pp[5] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b,
-                                                       p.pts + 3 * (int64_t)c, p.pts + 3 * (int64_t)s,
-                                                       p.pts + 3 * (int64_t)gv};
-                                const int idx[5] = {a, b, c, s, gv};
-                                sg = orient4_full(pp, idx, cnt);
-                            } else {
-                                unc = true;
-                            }
-                        }
-                        if (sg < 0) {
-                            L.tv[slot] = t | TF_VIS;
-                            nv++;
-                        }
+                        L.present[tvert(t, 0)] = 1;
+                        L.present[tvert(t, 1)] = 1;
+                        L.present[tvert(t, 2)] = 1;
                     }
                 }
             }
-            if (!EXACT && __ballot(unc)) {
-                unsure = true;
-                break;
-            }
-            int nvis;
-            const int vbase = wave_excl(nv, &nvis);
-            if (nvis == 0) continue; // drowned at insertion
-            if (lane == 0) L.ins_local[q] = (int16_t)Vn;
-            __syncthreads();
-            // hole boundary: edges of freed triangles whose neighbour survives
-            int nb = 0;
-#pragma unroll 1
-            for (int k = 0; k < KPL; k++) {
-                const int slot = lane + 64 * k;
-                if (slot < T && (L.tv[slot] & TF_VIS))
-                    for (int i = 0; i < 3; i++) nb += (L.tv[L.to[slot][i]] & TF_VIS) ? 0 : 1;
-            }
-            int nbtot;
-            int e = wave_excl(nb, &nbtot);
-            if (nbtot > TMAX || nbtot < 3) {
-                failed = true;
-                break;
-            }
-            int fpos = nfree + vbase;
-#pragma unroll 1
-            for (int k = 0; k < KPL; k++) {
-                const int slot = lane + 64 * k;
-                if (slot < T && (L.tv[slot] & TF_VIS)) {
-                    const uint32_t t = L.tv[slot];
-                    for (int i = 0; i < 3; i++) {
-                        const int u = L.to[slot][i];
-                        if (L.tv[u] & TF_VIS) continue;
-                        int j = 0;
-                        while (j < 2 && L.to[u][j] != slot) j++;
-                        L.be_a[e] = (uint16_t)tvert(t, (i + 1) % 3);
-                        L.be_b[e] = (uint16_t)tvert(t, (i + 2) % 3);
-                        L.be_u[e] = (uint16_t)u;
-                        L.be_j[e] = (uint8_t)j;
-                        e++;
-                    }
-                    L.freel[fpos++] = (uint16_t)slot;
-                }
-            }
-            __syncthreads();
-            const int nfree_tot = nfree + nvis;
-#pragma unroll 1
-            for (int k = 0; k < KPL; k++) {
-                const int slot = lane + 64 * k;
-                if (slot < T && (L.tv[slot] & TF_VIS)) L.tv[slot] = 0u;
-            }
-            for (int k = lane; k < nbtot; k += 64)
-                L.be_slot[k] = (uint16_t)(k < nfree_tot ? L.freel[nfree_tot - 1 - k] : T + (k - nfree_tot));
-            const int newT = T + (nbtot > nfree_tot ? nbtot - nfree_tot : 0);
-            if (newT > TMAX || Vn + 1 > VMAX) {
-                failed = true;
-                break;
-            }
-            __syncthreads();
-            for (int k = lane; k < nbtot; k += 64) {
-                const int slot = L.be_slot[k], a = L.be_a[k], b = L.be_b[k], u = L.be_u[k];
-                L.tv[slot] = tpack(a, b, Vn, TF_ALIVE | TF_FRESH);
-                L.to[slot][2] = (uint16_t)u;
-                L.to[u][L.be_j[k]] = (uint16_t)slot;
-                L.vstart[a] = (int16_t)slot;
-                L.vend[b] = (int16_t)slot;
-            }
-            __syncthreads();
-            for (int k = lane; k < nbtot; k += 64) {
-                const int slot = L.be_slot[k];
-                L.to[slot][0] = (uint16_t)L.vstart[L.be_b[k]];
-                L.to[slot][1] = (uint16_t)L.vend[L.be_a[k]];
-            }
-            nfree = nfree_tot - (nbtot < nfree_tot ? nbtot : nfree_tot);
-            T = newT;
-            __syncthreads();
-#pragma unroll 1
-            for (int k = 0; k < KPL; k++) {
-                const int slot = lane + 64 * k;
-                if (slot < T) {
-                    const uint32_t t = L.tv[slot];
-                    if (t & TF_FRESH) {
-                        tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], L.cof[slot],
-                                      L.cof[slot] + 4);
-                        L.tv[slot] = t & ~TF_FRESH;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-        if (unsure) {
-            star_list_push(p.exact_list, p.exact_cap, &cnt[p.exact_counter], s, cnt);
-            __syncthreads();
-            continue;
-        }
-        if (failed) {
-            if (lane == 0) {
-                atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
-                p.vnum[s] = 0;
-                p.tnum[s] = 0;
-            }
-            __syncthreads();
-            continue;
-        }
-        // ---- finalize: link set, drowned insertions, compaction, pairs, history ----
-        const int V = need; // drowned insertions' slots are never referenced and drop out
-        for (int v = lane; v < V; v += 64) L.present[v] = 0;
-        __syncthreads();
-#pragma unroll 1
-        for (int k = 0; k < KPL; k++) {
-            const int slot = lane + 64 * k;
-            if (slot < T) {
-                const uint32_t t = L.tv[slot];
-                if (t & TF_ALIVE) {
-                    L.present[tvert(t, 0)] = 1;
-                    L.present[tvert(t, 1)] = 1;
-                    L.present[tvert(t, 2)] = 1;
-                }
-            }
-        }
-        __syncthreads();
-        for (int base = 0; base < m; base += 64) {
-            const int q = base + lane;
-            bool dr = false;
-            if (q < m) {
+            wave_sync();
+            for (int q = lane; q < m; q += 64) {
                 const int lv = L.ins_local[q];
-                dr = lv < 0 || !L.present[lv];
+                p.dflag[ib + q] = (lv < 0 || !L.present[lv]) ? 1 : 0;
             }
-            const long long slot = g4d_wave_append(dr, &cnt[G4D_CNT_APPEND0]);
-            if (dr) {
-                if (slot < p.drown_cap) p.drown[slot] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)L.ins[q];
-                else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
-            }
-        }
-        // vertex compaction
-        const int64_t vo = p.nvoff[s], to = p.ntoff[s];
-        int vcount = 0;
-        for (int base = 0; base < V; base += 64) {
-            const int v = base + lane;
-            const bool pr = v < V && L.present[v];
-            int tot;
-            const int rk = g4d_wave_excl_count(pr, &tot);
-            if (pr) {
-                L.vremap[v] = (uint16_t)(vcount + rk);
-                p.vpool[vo + vcount + rk] = L.vid[v];
-            }
-            vcount += tot;
-        }
-        int tcount = 0;
-        for (int base = 0; base < T; base += 64) {
-            const int t = base + lane;
-            const bool al = t < T && (L.tv[t] & TF_ALIVE);
-            int tot;
-            const int rk = g4d_wave_excl_count(al, &tot);
-            if (al) L.remap[t] = (uint16_t)(tcount + rk);
-            tcount += tot;
-        }
-        if (tcount > 2 * need - 4 || tcount != 2 * vcount - 4) {
-            // not a triangulated 2-sphere: report instead of writing past the star's region
-            if (lane == 0) {
-                atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
-                p.vnum[s] = 0;
-                p.tnum[s] = 0;
-            }
-            __syncthreads();
-            continue;
-        }
-        __syncthreads();
-        for (int base = 0; base < T; base += 64) {
-            const int t = base + lane;
-            uint32_t tt = 0;
-            if (t < T) tt = L.tv[t];
-            const bool al = t < T && (tt & TF_ALIVE);
-            if (al) {
-                const int a = tvert(tt, 0), b = tvert(tt, 1), c = tvert(tt, 2);
-                p.tpool[to + L.remap[t]] = hpack(L.vremap[a], L.vremap[b], L.vremap[c], L.remap[L.to[t][0]],
-                                                     L.remap[L.to[t][1]], L.remap[L.to[t][2]], true);
-            }
-            const bool unchecked = al && !(tt & TF_CHECKED);
-            // pairs from unchecked triangles: each edge (v_i, v_i+1) with v_i < v_i+1
-#pragma unroll
-            for (int i = 0; i < 3; i++) {
-                int ga = 0, gb = 0;
-                if (unchecked) {
-                    ga = L.vid[tvert(tt, i)];
-                    gb = L.vid[tvert(tt, (i + 1) % 3)];
+            // vertex compaction
+            const int64_t vo = p.nvoff[s], to = p.ntoff[s];
+            int vcount = 0;
+            for (int base = 0; base < V; base += 64) {
+                const int v = base + lane;
+                const bool pr = v < V && L.present[v];
+                int tot;
+                const int rk = g4d_wave_excl_count(pr, &tot);
+                if (pr) {
+                    L.vremap[v] = (uint16_t)(vcount + rk);
+                    p.vpool[vo + vcount + rk] = L.vid[v];
                 }
-                const bool em = unchecked && ga < gb;
-                const long long slot = g4d_wave_append(em, &cnt[G4D_CNT_APPEND1]);
-                if (em) {
-                    if (slot < p.tpairs_cap) p.tpairs[slot] = ((uint64_t)(uint32_t)ga << 32) | (uint32_t)gb;
-                    else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
+                vcount += tot;
+            }
+            int tcount = 0;
+            for (int base = 0; base < T; base += 64) {
+                const int t = base + lane;
+                const bool al = t < T && (L.tv[t] & TF_ALIVE);
+                int tot;
+                const int rk = g4d_wave_excl_count(al, &tot);
+                if (al) L.remap[t] = (uint16_t)(tcount + rk);
+                tcount += tot;
+            }
+            if (tcount > 2 * need - 4 || tcount != 2 * vcount - 4) {
+                // not a triangulated 2-sphere: report instead of writing past the star's region
+                if (lane == 0) {
+                    atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
+                    p.vnum[s] = 0;
+                    p.tnum[s] = 0;
+                }
+                wave_sync();
+                continue;
+            }
+            wave_sync();
+            const int64_t pb = 2 * (to - p.tbase);
+            for (int base = 0; base < T; base += 64) {
+                const int t = base + lane;
+                uint32_t tt = 0;
+                if (t < T) tt = L.tv[t];
+                if (t < T && (tt & TF_ALIVE)) {
+                    const int a = tvert(tt, 0), b = tvert(tt, 1), c = tvert(tt, 2);
+                    const int d = L.remap[t];
+                    p.tpool[to + d] = hpack(L.vremap[a], L.vremap[b], L.vremap[c], L.remap[L.to[t][0]],
+                                            L.remap[L.to[t][1]], L.remap[L.to[t][2]], true);
+                    const TriPairs pr = star_tri_pairs(L.vid[a], L.vid[b], L.vid[c], !(tt & TF_CHECKED));
+                    p.pairbuf[pb + 2 * d] = pr.x;
+                    p.pairbuf[pb + 2 * d + 1] = pr.y;
                 }
             }
+            for (int64_t j = 2 * (int64_t)tcount + lane; j < 2 * star_tcap_words(need); j += 64)
+                p.pairbuf[pb + j] = STAR_PAIR_NONE;
+            const int hn = star_history_merge(p, s, L.ins, m, lane, 64);
+            wave_sync();
+            if (lane == 0) star_commit(p, s, vcount, vo, tcount, to, hn + m);
+            wave_sync();
         }
-        const int hn = star_history_merge(p, s, L.ins, m, lane, 64);
-        __syncthreads();
-        if (lane == 0) star_commit(p, s, vcount, vo, tcount, to, hn + m);
-        __syncthreads();
     }
 }
 
@@ -839,18 +860,9 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
             if (L.tfl[slot] & TF_ALIVE)
                 for (int i = 0; i < 3; i++) L.present[L.tva[3 * slot + i]] = 1;
         __syncthreads();
-        for (int base = 0; base < m; base += GSTAR_NT) {
-            const int q = base + tid;
-            bool dr = false;
-            if (q < m) {
-                const int lv = L.ins_local[q];
-                dr = lv < 0 || !L.present[lv];
-            }
-            const long long slot = g4d_wave_append(dr, &cnt[G4D_CNT_APPEND0]);
-            if (dr) {
-                if (slot < p.drown_cap) p.drown[slot] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)L.ins[q];
-                else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
-            }
+        for (int q = tid; q < m; q += GSTAR_NT) {
+            const int lv = L.ins_local[q];
+            p.dflag[ib + q] = (lv < 0 || !L.present[lv]) ? 1 : 0;
         }
         const int64_t vo = p.nvoff[s], to = p.ntoff[s];
         int vcount = 0;
@@ -876,8 +888,7 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
         }
         const bool wide = vcount >= STAR_WIDE_MIN;
         const int64_t words = wide ? 2 * (int64_t)tcount : tcount;
-        const int64_t tneed = 2 * (int64_t)(vn + m) - 4;
-        if (words > (vn + m >= STAR_WIDE_MIN ? 2 * tneed : tneed) || tcount != 2 * vcount - 4) {
+        if (words > star_tcap_words(vn + m) || tcount != 2 * vcount - 4) {
             if (tid == 0) {
                 atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
                 p.vnum[s] = 0;
@@ -887,42 +898,26 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
             continue;
         }
         __syncthreads();
-        for (int base = 0; base < T; base += GSTAR_NT) {
-            const int t = base + tid;
-            const bool al = t < T && (L.tfl[t] & TF_ALIVE);
-            int a = 0, b = 0, c = 0;
-            uint32_t f = 0;
-            if (al) {
-                f = L.tfl[t];
-                a = L.tva[3 * t];
-                b = L.tva[3 * t + 1];
-                c = L.tva[3 * t + 2];
-                const int ra = L.vremap[a], rb = L.vremap[b], rc = L.vremap[c];
-                const int o0 = L.remap[L.to[3 * t]], o1 = L.remap[L.to[3 * t + 1]], o2 = L.remap[L.to[3 * t + 2]];
-                const int64_t d = L.remap[t];
-                if (wide) {
-                    p.tpool[to + 2 * d] = wpack3(ra, rb, rc) | (1ull << 63);
-                    p.tpool[to + 2 * d + 1] = wpack3(o0, o1, o2);
-                } else {
-                    p.tpool[to + d] = hpack(ra, rb, rc, o0, o1, o2, true);
-                }
+        const int64_t pb = 2 * (to - p.tbase);
+        for (int t = tid; t < T; t += GSTAR_NT) {
+            const uint32_t f = L.tfl[t];
+            if (!(f & TF_ALIVE)) continue;
+            const int a = L.tva[3 * t], b = L.tva[3 * t + 1], c = L.tva[3 * t + 2];
+            const int ra = L.vremap[a], rb = L.vremap[b], rc = L.vremap[c];
+            const int o0 = L.remap[L.to[3 * t]], o1 = L.remap[L.to[3 * t + 1]], o2 = L.remap[L.to[3 * t + 2]];
+            const int64_t d = L.remap[t];
+            if (wide) {
+                p.tpool[to + 2 * d] = wpack3(ra, rb, rc) | (1ull << 63);
+                p.tpool[to + 2 * d + 1] = wpack3(o0, o1, o2);
+            } else {
+                p.tpool[to + d] = hpack(ra, rb, rc, o0, o1, o2, true);
             }
-            const bool unchecked = al && !(f & TF_CHECKED);
-            const int tv3[3] = {a, b, c};
-            for (int i = 0; i < 3; i++) {
-                int ga = 0, gb = 0;
-                if (unchecked) {
-                    ga = L.vid[tv3[i]];
-                    gb = L.vid[tv3[(i + 1) % 3]];
-                }
-                const bool em = unchecked && ga < gb;
-                const long long slot = g4d_wave_append(em, &cnt[G4D_CNT_APPEND1]);
-                if (em) {
-                    if (slot < p.tpairs_cap) p.tpairs[slot] = ((uint64_t)(uint32_t)ga << 32) | (uint32_t)gb;
-                    else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
-                }
-            }
+            const TriPairs pr = star_tri_pairs(L.vid[a], L.vid[b], L.vid[c], !(f & TF_CHECKED));
+            p.pairbuf[pb + 2 * d] = pr.x;
+            p.pairbuf[pb + 2 * d + 1] = pr.y;
         }
+        for (int64_t j = 2 * (int64_t)tcount + tid; j < 2 * star_tcap_words(vn + m); j += GSTAR_NT)
+            p.pairbuf[pb + j] = STAR_PAIR_NONE;
         const int hn = star_history_merge(p, s, L.ins, m, tid, GSTAR_NT);
         __syncthreads();
         if (tid == 0) star_commit(p, s, vcount, vo, tcount, to, hn + m);
@@ -1181,8 +1176,7 @@ __global__ void k_capacities(const int *list, int nitems, const int *vnum, const
         const int64_t m = ins_off[s + 1] - ins_off[s];
         const int64_t v = (vnum ? vnum[s] : 0) + m;
         vcap[i] = v;
-        const int64_t t = v >= 4 ? 2 * v - 4 : 4;
-        tcap[i] = v >= STAR_WIDE_MIN ? 2 * t : t;
+        tcap[i] = star_tcap_words(v);
         hcap[i] = (hnum ? hnum[s] : 0) + m;
     }
 }
@@ -1434,6 +1428,43 @@ __global__ void k_map_perm(int *a, int64_t m, const int *perm)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
         a[i] = perm[a[i]];
+}
+
+__global__ void k_gather_i64(const int64_t *a, const int64_t *idx, int64_t m, int64_t *out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = a[idx[i]];
+}
+
+hipError_t launch_gather_i64(const int64_t *a, const int64_t *idx, int64_t m, int64_t *out, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_gather_i64, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, a, idx, m, out);
+    return hipGetLastError();
+}
+
+__global__ void k_batch_localize(int *tets, int *opp, int64_t nt, const int *pt_set, const int64_t *pt_off,
+                                 const int64_t *tet_off, const int *perm)
+{
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += (int64_t)gridDim.x * blockDim.x) {
+        const int st = pt_set[tets[4 * t]];
+        const int base = (int)pt_off[st];
+        for (int i = 0; i < 4; i++) tets[4 * t + i] = perm[tets[4 * t + i]] - base;
+        if (opp)
+            for (int i = 0; i < 4; i++) {
+                const int o = opp[4 * t + i];
+                opp[4 * t + i] = o < 0 ? -1 : (int)(o - tet_off[st]);
+            }
+    }
+}
+
+hipError_t launch_batch_localize(int *tets, int *opp, int64_t nt, const int *pt_set, const int64_t *pt_off,
+                                 const int64_t *tet_off, const int *perm, hipStream_t s)
+{
+    if (nt <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_batch_localize, dim3(g4d_blocks(nt, 256, 65536)), dim3(256), 0, s, tets, opp, nt, pt_set,
+                       pt_off, tet_off, perm);
+    return hipGetLastError();
 }
 
 hipError_t launch_map_perm(int *tets, int64_t m4, const int *perm, hipStream_t s)

@@ -125,6 +125,34 @@ int g4d_upload_points(g4d_ctx *ctx, const double *xyz, int32_t n, const double *
 int g4d_get_stats(const g4d_ctx *ctx, g4d_stats *st);
 int g4d_set_options(g4d_ctx *ctx, const g4d_options *opt);
 
+/* ---- Batched mode (no reference counterpart): many independent point sets, one launch per
+   stage for all of them ---- */
+typedef struct {
+    int32_t num_sets;
+    int32_t pad;
+    int64_t num_points;          /* total over the sets */
+    int64_t num_tets;            /* total over the sets */
+    const int64_t *tet_offsets;  /* num_sets + 1: set k's tetrahedra are [tet_offsets[k], tet_offsets[k+1]) */
+    const int32_t *tets;         /* 4 * num_tets; vertex ids index the set's own points (input order) */
+    const int32_t *opp;          /* 4 * num_tets; tetrahedron ids within the set's range, -1 on the hull;
+                                    NULL without adjacency */
+    const int32_t *grid_sizes;   /* num_sets: the PBA grid width used for each set */
+} g4d_batch_result;
+
+/* Triangulates num_sets independent point sets: set k is xyz[set_offsets[k] .. set_offsets[k+1])
+   (set_offsets[0] = 0, every set at least 5 points, at most 2^24 with several sets).  Each set's
+   tetrahedra are those g4d_triangulate returns for that set alone with grid_size = grids[k];
+   grids NULL picks them with g4d_batch_grid_size (several sets: at most 256).  All sets go
+   through every stage together (one set table, one launch per stage).  Arrays belong to ctx
+   until its next call. */
+int g4d_triangulate_batch(g4d_ctx *ctx, const double *xyz, const int64_t *set_offsets, int32_t num_sets,
+                          const int32_t *grids, g4d_batch_result *out);
+/* Device-resident points; the result stays in ctx. */
+int g4d_triangulate_batch_device(g4d_ctx *ctx, const double *d_xyz, const int64_t *set_offsets, int32_t num_sets,
+                                 const int32_t *grids, int64_t *num_tets);
+/* The grid width chosen for a set of n points: about 24 voxels per point, in steps of 16, 32..256. */
+int g4d_batch_grid_size(int64_t n);
+
 /* Validate the last result of ctx on the GPU (TetraMesh::check, GD/Common/Geometry.cu:200-393);
    full_empty_sphere != 0 adds the empty-sphere test of every point near each tetrahedron. */
 int g4d_check_result(g4d_ctx *ctx, int32_t full_empty_sphere, g4d_check_report *rep);
