@@ -56,7 +56,9 @@ class Stats(ctypes.Structure):
                 ("exact_stars", ctypes.c_int64),
                 ("pba_bytes", ctypes.c_int64), ("t_pba_pass_ms", ctypes.c_double * 3),
                 ("star_store_bytes", ctypes.c_int64), ("store_repacks", ctypes.c_int32),
-                ("star_passes", ctypes.c_int32)]
+                ("star_passes", ctypes.c_int32), ("star_t0_launches", ctypes.c_int32), ("pad1", ctypes.c_int32),
+                ("t_star_t0_ms", ctypes.c_double), ("star_t0_filter_evals", ctypes.c_int64),
+                ("star_t0_cofactors", ctypes.c_int64)]
 
     def as_dict(self):
         d = {}
@@ -104,7 +106,8 @@ EXPORTS = ["g4d_default_options", "g4d_create", "g4d_destroy", "g4d_last_error",
            "g4d_get_stats", "g4d_set_options", "g4d_diag_orient4", "g4d_diag_orient3d",
            "g4d_diag_morton", "g4d_diag_pba", "g4d_diag_grid_pairs", "g4d_diag_copy",
            "g4d_check_result", "g4d_check_mesh", "g4d_read_asc", "g4d_free_point_file", "g4d_write_asc",
-           "g4d_write_ply", "g4d_triangulate_batch", "g4d_triangulate_batch_device", "g4d_batch_grid_size"]
+           "g4d_write_ply", "g4d_triangulate_batch", "g4d_triangulate_batch_device", "g4d_batch_grid_size",
+           "g4d_struct_sizes"]
 
 
 def load_library(path=None):
@@ -140,6 +143,7 @@ def load_library(path=None):
     L.g4d_triangulate_batch.argtypes = [_vp, _dp, _lp, ctypes.c_int32, _ip, ctypes.POINTER(BatchResult)]
     L.g4d_triangulate_batch_device.argtypes = [_vp, _vp, _lp, ctypes.c_int32, _ip, _lp]
     L.g4d_batch_grid_size.argtypes = [ctypes.c_int64]
+    L.g4d_struct_sizes.argtypes = [_lp, ctypes.c_int32]
     L.g4d_read_asc.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(PointFile)]
     L.g4d_free_point_file.argtypes = [ctypes.POINTER(PointFile)]
     L.g4d_free_point_file.restype = None

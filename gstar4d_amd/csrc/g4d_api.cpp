@@ -73,6 +73,7 @@ struct g4d_ctx {
     bool have_result = false;
     g4d_stats st;
     hipEvent_t ev[EV_NUM];
+    hipEvent_t ev_t0[2]; // around each launch of the 32-vertex star kernel
     bool ev_ok = false;
 };
 
@@ -553,8 +554,20 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         // Filter-only LDS kernels by size tier (32 -> 64 -> 256 link vertices); each passes the
         // stars it cannot hold to the next tier's list and the ones its filters cannot decide
         // to the exact list.
+        HCK(hipMemsetAsync(cnt + G4D_CNT_ORIENT4, 0, 8, s));
+        HCK(hipMemsetAsync(cnt + G4D_CNT_COFACTOR, 0, 8, s));
+        if (ctx->ev_ok) (void)hipEventRecord(ctx->ev_t0[0], s);
         HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T0, false, nblk_star, s));
+        if (ctx->ev_ok) (void)hipEventRecord(ctx->ev_t0[1], s);
         CK(read_counters(ctx, hc));
+        {
+            float ms = 0.f;
+            if (ctx->ev_ok && hipEventElapsedTime(&ms, ctx->ev_t0[0], ctx->ev_t0[1]) == hipSuccess)
+                ctx->st.t_star_t0_ms += ms;
+            ctx->st.star_t0_filter_evals += (int64_t)hc[G4D_CNT_ORIENT4];
+            ctx->st.star_t0_cofactors += (int64_t)hc[G4D_CNT_COFACTOR];
+            ctx->st.star_t0_launches++;
+        }
         const int64_t nmid = (int64_t)hc[G4D_CNT_APPEND6];
         if (nmid > 0) {
             sp.star_list = buf<int>(ctx->mid);
@@ -861,6 +874,8 @@ int g4d_create(g4d_ctx **out, const g4d_options *opt)
     ctx->ev_ok = true;
     for (int i = 0; i < EV_NUM; i++)
         if (hipEventCreate(&ctx->ev[i]) != hipSuccess) ctx->ev_ok = false;
+    for (int i = 0; i < 2; i++)
+        if (hipEventCreate(&ctx->ev_t0[i]) != hipSuccess) ctx->ev_ok = false;
     *out = ctx;
     return G4D_OK;
 }
@@ -879,6 +894,8 @@ void g4d_destroy(g4d_ctx *ctx)
     ctx->P.release();
     if (ctx->ev_ok)
         for (int i = 0; i < EV_NUM; i++) (void)hipEventDestroy(ctx->ev[i]);
+    if (ctx->ev_ok)
+        for (int i = 0; i < 2; i++) (void)hipEventDestroy(ctx->ev_t0[i]);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }

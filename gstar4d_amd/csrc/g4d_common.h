@@ -33,7 +33,7 @@ enum g4d_counter : int {
     G4D_CNT_APPEND0 = 1,  // generic append counters
     G4D_CNT_APPEND1 = 2,
     G4D_CNT_APPEND2 = 3,
-    G4D_CNT_ORIENT4 = 4,  // filtered orient4 evaluations
+    G4D_CNT_ORIENT4 = 4,  // cofactor-filter orient4 evaluations in the LDS star kernels
     G4D_CNT_EXACT = 5,    // exact insphere evaluations
     G4D_CNT_SOS = 6,      // SoS resolutions
     G4D_CNT_ORIENT3 = 7,
@@ -43,6 +43,7 @@ enum g4d_counter : int {
     G4D_CNT_APPEND4 = 11,
     G4D_CNT_APPEND5 = 12,
     G4D_CNT_APPEND6 = 13,
+    G4D_CNT_COFACTOR = 14, // cached-cofactor computations in the LDS star kernels
     G4D_CNT_NUM = 16
 };
 

@@ -139,3 +139,12 @@ extern "C" int g4d_write_ply(const char *path, const double *xyz, int32_t n, con
     }
     return fclose(f) == 0 ? G4D_OK : G4D_ERR_IO;
 }
+
+extern "C" int g4d_struct_sizes(int64_t *out, int32_t max_out)
+{
+    const int64_t sz[6] = {(int64_t)sizeof(g4d_options), (int64_t)sizeof(g4d_result), (int64_t)sizeof(g4d_stats),
+                           (int64_t)sizeof(g4d_check_report), (int64_t)sizeof(g4d_point_file),
+                           (int64_t)sizeof(g4d_batch_result)};
+    for (int i = 0; i < 6 && out && i < max_out; i++) out[i] = sz[i];
+    return 6;
+}

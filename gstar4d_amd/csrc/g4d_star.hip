@@ -210,6 +210,7 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
     // Work queue in chunks: one atomic per STAR_CHUNK stars for the filter-only kernels (many
     // small stars), per star for the exact ones (uneven cost).
     constexpr int CHUNK = EXACT ? 1 : STAR_CHUNK;
+    int nfilt = 0, ncof = 0; // this lane's cofactor-filter evaluations and cofactor computations
     for (;;) {
         int base = 0;
         if (lane == 0) base = (int)atomicAdd(&cnt[G4D_CNT_WORK], (unsigned long long)CHUNK);
@@ -307,6 +308,7 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                 if (slot < T) {
                     const uint32_t t = L.tv[slot];
                     tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], L.cof[slot], L.cof[slot] + 4);
+                    ncof++;
                 }
             }
             bool failed = false, unsure = false;
@@ -322,6 +324,7 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                     if (slot < T) {
                         const uint32_t t = L.tv[slot];
                         if (t & TF_ALIVE) {
+                            nfilt++;
                             int sg = cofactor_filter_sign(r, L.cof[slot], L.cof[slot] + 4);
                             if (sg == 0) {
                                 if (EXACT) {
@@ -425,6 +428,7 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                         if (t & TF_FRESH) {
                             tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], L.cof[slot],
                                           L.cof[slot] + 4);
+                            ncof++;
                             L.tv[slot] = t & ~TF_FRESH;
                         }
                     }
@@ -522,6 +526,11 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
             if (lane == 0) star_commit(p, s, vcount, vo, tcount, to, hn + m);
             wave_sync();
         }
+    }
+    const int wf = g4d_wave_sum(nfilt), wc = g4d_wave_sum(ncof);
+    if (lane == 0) {
+        atomicAdd(&cnt[G4D_CNT_ORIENT4], (unsigned long long)wf);
+        atomicAdd(&cnt[G4D_CNT_COFACTOR], (unsigned long long)wc);
     }
 }
 

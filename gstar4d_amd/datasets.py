@@ -38,6 +38,11 @@ def gaussians(seed, count):
 
 
 def _dedup(p):
+    """Drops exact duplicate rows, keeping first occurrences in order.  Equal rows share x, so
+    the full row comparison runs only when some x value repeats."""
+    xs = np.sort(p[:, 0])
+    if not (xs[1:] == xs[:-1]).any():
+        return p
     _, keep = np.unique(p, axis=0, return_index=True)
     return p[np.sort(keep)]
 

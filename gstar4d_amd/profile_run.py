@@ -98,5 +98,22 @@ def pba_traffic(fetch_dir, write_dir, copy_bytes, grid):
             "hbm_bytes_per_triangulation": total, "algorithmic_bytes": 24 * grid ** 3}
 
 
+
+def kernel_traffic(fetch_dir, write_dir, copy_bytes, grid, kernel, workload):
+    """Mean HBM bytes per launch of `kernel` (a name substring) from FETCH_SIZE and WRITE_SIZE,
+    scaled by the copy kernel as in pba_traffic."""
+    f = pmc_per_kernel(fetch_dir, "FETCH_SIZE")
+    w = pmc_per_kernel(write_dir, "WRITE_SIZE")
+    ck = [k for k in f if "k_diag_copy" in k][0]
+    sf = copy_bytes / (f[ck] * 1024.0)
+    sw = copy_bytes / (w[ck] * 1024.0)
+    ks = [k for k in f if kernel in k and "k_diag_copy" not in k]
+    fetch = sum(f[k] for k in ks) * 1024.0 * sf
+    write = sum(w.get(k, 0.0) for k in ks) * 1024.0 * sw
+    return {"workload": workload, "grid": grid, "kernel": kernel, "copy_bytes": copy_bytes, "fetch_scale": sf,
+            "write_scale": sw, "fetch_bytes_per_launch": fetch, "write_bytes_per_launch": write,
+            "hbm_bytes_per_launch": fetch + write}
+
+
 if __name__ == "__main__":
     sys.exit(main())

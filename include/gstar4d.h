@@ -91,6 +91,13 @@ typedef struct {
     int64_t star_store_bytes;  /* star arenas in use at the end (vertices, triangles, history) */
     int32_t store_repacks;     /* times the star arenas were compacted into larger ones */
     int32_t star_passes;       /* star kernel passes: creation plus one per splaying round */
+    /* The 32-vertex LDS star kernel (the dominant kernel): launches, summed time (hipEvents on
+       the context's stream), cofactor-filter orient4 evaluations and cofactor computations. */
+    int32_t star_t0_launches;
+    int32_t pad1;
+    double t_star_t0_ms;
+    int64_t star_t0_filter_evals;
+    int64_t star_t0_cofactors;
 } g4d_stats;
 
 /* Mesh validation report (GDelaunay/Common/Geometry.cu:200-393 plus adjacency and a full
@@ -184,6 +191,10 @@ int g4d_write_asc(const char *path, const double *xyz, int32_t n);
 /* ASCII PLY of the points and three faces per tetrahedron (TetraMesh::writeToFile,
    GD/Common/Geometry.cu:395-466). */
 int g4d_write_ply(const char *path, const double *xyz, int32_t n, const int32_t *tets, int64_t num_tets);
+
+/* sizeof of g4d_options, g4d_result, g4d_stats, g4d_check_report, g4d_point_file and
+   g4d_batch_result, in that order (for bindings that mirror the structs); returns the count. */
+int g4d_struct_sizes(int64_t *out, int32_t max_out);
 
 /* ---- Kernel-level diagnostics (used by parity tests; no reference counterpart) ---- */
 /* orient4 of T tuples (15 doubles, 5 indices each).  mode 0: filter + exact + SoS;

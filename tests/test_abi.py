@@ -56,3 +56,12 @@ def test_cpp_shim_compiles(tmp_path):
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-I", os.path.join(ROOT, "include"), str(src)],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_python_structs_match_c_sizes():
+    import gstar4d_amd as g
+    L = g.load_library()
+    out = (ctypes.c_int64 * 6)()
+    assert L.g4d_struct_sizes(out, 6) == 6
+    mine = [ctypes.sizeof(c) for c in (g.Options, g.Result, g.Stats, g.CheckReport, g.PointFile, g.BatchResult)]
+    assert list(out) == mine
