@@ -765,7 +765,6 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     const int64_t nxt = (int64_t)hc[G4D_CNT_APPEND5];
     if (nxt > xcap) return set_err(ctx, G4D_ERR_INTERNAL, "output exact list sizing failed");
     HCK(g4d::launch_out_exact(op, buf<uint64_t>(ctx->xtris), nxt, s));
-    HCK(g4d::launch_out_count(op, s));
     int64_t *toffs = buf<int64_t>(ctx->tetoff);
     HCK(g4d::scan_excl_i64(P, buf<int64_t>(ctx->tetcnt), toffs, n, toffs + n, s));
     int64_t nt = 0;

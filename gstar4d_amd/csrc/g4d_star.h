@@ -152,11 +152,12 @@ struct OutParams {
     int *tets;               // 4 per tetra
     unsigned long long *cnt;
 };
-// Owner triangles whose orient3d filter is inconclusive -> list (star<<32 | triangle),
-// counter G4D_CNT_APPEND5; then the exact pass marks the ones to keep in their triangle word.
+// Decision pass over the owner triangles: filter-decided keeps get the keep bit in their word
+// and are counted in star_tet_count; inconclusive ones -> list (star<<32 | triangle), counter
+// G4D_CNT_APPEND5.  The exact pass keeps and counts those with exact orient3d < 0.  After the
+// caller scans star_tet_count into star_tet_off, the write pass emits the kept triangles.
 hipError_t launch_out_flag(const OutParams &p, uint64_t *list, int64_t cap, hipStream_t s);
 hipError_t launch_out_exact(const OutParams &p, const uint64_t *list, int64_t m, hipStream_t s);
-hipError_t launch_out_count(const OutParams &p, hipStream_t s);
 hipError_t launch_out_write(const OutParams &p, hipStream_t s);
 // Face adjacency via an open-addressing hash of face triples.
 hipError_t launch_adjacency(const int *tets, int64_t nt, int *opp, uint32_t *table, int64_t table_size,

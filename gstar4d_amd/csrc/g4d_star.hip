@@ -94,13 +94,24 @@ struct StarLds {
     int sc[4];
 };
 
-// Exclusive scan of small per-lane counts across the wave; returns prefix, total in *tot.
+// Exclusive scan across the wave of per-lane counts below 2^BITS; returns the prefix, the total
+// in *tot.  One ballot and a lane-masked bit count per bit: no cross-lane data movement.
+template <int BITS>
 __device__ __forceinline__ int wave_excl(int v, int *tot)
 {
-    const int inc = g4d_wave_incl_sum(v);
-    *tot = __shfl(inc, 63, 64);
-    return inc - v;
+    int pre = 0, all = 0;
+#pragma unroll
+    for (int b = 0; b < BITS; b++) {
+        const uint64_t m = __ballot((v >> b) & 1);
+        pre += (int)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << b;
+        all += __popcll(m) << b;
+    }
+    *tot = all;
+    return pre;
 }
+
+// Bits needed for counts up to x.
+__host__ __device__ constexpr int count_bits(int x) { return x < 2 ? 1 : 1 + count_bits(x >> 1); }
 
 // Insertion pairs of a link triangle with global vertices (a, b, c) (GD/GDelaunay/GDelKernels.cu:
 // 818-939): when it is unchecked, each cyclic edge (u, w) with u < w gives (u<<32 | w); there
@@ -350,7 +361,7 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                     break;
                 }
                 int nvis;
-                const int vbase = wave_excl(nv, &nvis);
+                const int vbase = wave_excl<count_bits(KPL)>(nv, &nvis);
                 if (nvis == 0) continue; // drowned at insertion
                 if (lane == 0) L.ins_local[q] = (int16_t)Vn;
                 wave_sync();
@@ -363,7 +374,7 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                         for (int i = 0; i < 3; i++) nb += (L.tv[L.to[slot][i]] & TF_VIS) ? 0 : 1;
                 }
                 int nbtot;
-                int e = wave_excl(nb, &nbtot);
+                int e = wave_excl<count_bits(3 * KPL)>(nb, &nbtot);
                 if (nbtot > TMAX || nbtot < 3) {
                     failed = true;
                     break;
@@ -1282,13 +1293,15 @@ __device__ __forceinline__ uint64_t *out_keep_word(const OutParams &p, int64_t t
 }
 __device__ __forceinline__ uint64_t out_keep_bit(bool wide) { return wide ? (1ull << 63) : (1ull << 58); }
 
-// Filter pass: owner triangles whose orient3d filter is inconclusive go to the list.
+// Decision pass: every owner triangle whose orient3d filter says "lower hull" gets its keep
+// bit and is counted for its star; inconclusive ones go to the list for the exact pass.
 __global__ void k_out_flag(OutParams p, uint64_t *list, int64_t cap)
 {
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < p.n; s += gridDim.x * blockDim.x) {
         const int tn = p.tnum[s];
         const bool wide = p.vnum[s] >= STAR_WIDE_MIN;
         const int64_t vo = p.voff[s], to = p.toff[s];
+        int64_t c = 0;
         for (int t = 0; t < tn; t++) {
             int lv[3];
             star_tri_verts(p.tpool, to, t, wide, lv);
@@ -1296,14 +1309,20 @@ __global__ void k_out_flag(OutParams p, uint64_t *list, int64_t cap)
             if (!(s < a && s < b && s < cc)) continue;
             const double *pp[4] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b, p.pts + 3 * (int64_t)cc,
                                    p.pts + 3 * (int64_t)s};
-            if (filter_orient3d(pp) != 0) continue;
-            const unsigned long long k = atomicAdd(&p.cnt[G4D_CNT_APPEND5], 1ull);
-            if ((int64_t)k < cap) list[k] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)t;
+            const int o = filter_orient3d(pp);
+            if (o < 0) {
+                *out_keep_word(p, to, t, wide) |= out_keep_bit(wide);
+                c++;
+            } else if (o == 0) {
+                const unsigned long long k = atomicAdd(&p.cnt[G4D_CNT_APPEND5], 1ull);
+                if ((int64_t)k < cap) list[k] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)t;
+            }
         }
+        p.star_tet_count[s] = c;
     }
 }
 
-// Exact pass over the inconclusive owner triangles: mark those with orient3d < 0.
+// Exact pass over the inconclusive owner triangles: keep (and count) those with orient3d < 0.
 __global__ void k_out_exact(OutParams p, const uint64_t *list, int64_t m)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
@@ -1315,39 +1334,31 @@ __global__ void k_out_exact(OutParams p, const uint64_t *list, int64_t m)
         const int a = p.vpool[vo + lv[0]], b = p.vpool[vo + lv[1]], cc = p.vpool[vo + lv[2]];
         const double *pp[4] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b, p.pts + 3 * (int64_t)cc,
                                p.pts + 3 * (int64_t)s};
-        if (exact_orient3d(pp, p.cnt) < 0) *out_keep_word(p, to, t, wide) |= out_keep_bit(wide);
+        if (exact_orient3d(pp, p.cnt) < 0) {
+            *out_keep_word(p, to, t, wide) |= out_keep_bit(wide);
+            atomicAdd(reinterpret_cast<unsigned long long *>(&p.star_tet_count[s]), 1ull);
+        }
     }
 }
 
-template <bool WRITE>
-__global__ void k_out(OutParams p)
+// Write pass: the kept triangles of each star at its scanned offset.
+__global__ void k_out_write(OutParams p)
 {
     for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < p.n; s += gridDim.x * blockDim.x) {
         const int tn = p.tnum[s];
         const bool wide = p.vnum[s] >= STAR_WIDE_MIN;
         const int64_t vo = p.voff[s], to = p.toff[s];
-        int64_t k = WRITE ? p.star_tet_off[s] : 0;
-        int64_t c = 0;
+        int64_t k = p.star_tet_off[s];
         for (int t = 0; t < tn; t++) {
+            if (!(*out_keep_word(p, to, t, wide) & out_keep_bit(wide))) continue;
             int lv[3];
             star_tri_verts(p.tpool, to, t, wide, lv);
-            const int a = p.vpool[vo + lv[0]], b = p.vpool[vo + lv[1]], cc = p.vpool[vo + lv[2]];
-            if (!(s < a && s < b && s < cc)) continue;
-            const double *pp[4] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b, p.pts + 3 * (int64_t)cc,
-                                   p.pts + 3 * (int64_t)s};
-            const int o = filter_orient3d(pp);
-            const bool keep = o < 0 || (o == 0 && (*out_keep_word(p, to, t, wide) & out_keep_bit(wide)));
-            if (!keep) continue;
-            if (WRITE) {
-                int *out = p.tets + 4 * (k + c);
-                out[0] = a;
-                out[1] = b;
-                out[2] = cc;
-                out[3] = s;
-            }
-            c++;
+            int *out = p.tets + 4 * k++;
+            out[0] = p.vpool[vo + lv[0]];
+            out[1] = p.vpool[vo + lv[1]];
+            out[2] = p.vpool[vo + lv[2]];
+            out[3] = s;
         }
-        if (!WRITE) p.star_tet_count[s] = c;
     }
 }
 
@@ -1364,15 +1375,9 @@ hipError_t launch_out_exact(const OutParams &p, const uint64_t *list, int64_t m,
     return hipGetLastError();
 }
 
-hipError_t launch_out_count(const OutParams &p, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_out<false>, dim3(g4d_blocks(p.n, 64, 65536)), dim3(64), 0, s, p);
-    return hipGetLastError();
-}
-
 hipError_t launch_out_write(const OutParams &p, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_out<true>, dim3(g4d_blocks(p.n, 64, 65536)), dim3(64), 0, s, p);
+    hipLaunchKernelGGL(k_out_write, dim3(g4d_blocks(p.n, 64, 65536)), dim3(64), 0, s, p);
     return hipGetLastError();
 }
 
