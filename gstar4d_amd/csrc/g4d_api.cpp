@@ -551,14 +551,39 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
         sp.star_list = list;
         sp.nlist = nitems;
+        sp.nlist_dev = nullptr;
         // Filter-only LDS kernels by size tier (32 -> 64 -> 256 link vertices); each passes the
         // stars it cannot hold to the next tier's list and the ones its filters cannot decide
-        // to the exact list.
+        // to the exact list.  Then the exact-capable kernels of the same tiers over the exact
+        // list.  Every later launch reads its list length on the device, so the chain runs
+        // without host round trips; lists and counters are reused in stream order.
         HCK(hipMemsetAsync(cnt + G4D_CNT_ORIENT4, 0, 8, s));
         HCK(hipMemsetAsync(cnt + G4D_CNT_COFACTOR, 0, 8, s));
         if (ctx->ev_ok) (void)hipEventRecord(ctx->ev_t0[0], s);
         HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T0, false, nblk_star, s));
         if (ctx->ev_ok) (void)hipEventRecord(ctx->ev_t0[1], s);
+        auto chain = [&](int *lst, int counter, int *defer, int defer_counter, int vmax, bool exact) -> int {
+            sp.star_list = lst;
+            sp.nlist = n;
+            sp.nlist_dev = cnt + counter;
+            sp.defer_list = defer;
+            sp.defer_counter = defer_counter;
+            HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
+            HCK(g4d::launch_star_kernel(sp, vmax, exact, vmax == g4d::STAR_VMAX_T1 && !exact ? 4096 : 1024, s));
+            return G4D_OK;
+        };
+        // filter tiers: mid list (A6) -> 64, big list (A2) -> 256, beyond -> huge (A3)
+        CK(chain(buf<int>(ctx->mid), G4D_CNT_APPEND6, buf<int>(ctx->big), G4D_CNT_APPEND2, g4d::STAR_VMAX_T1, false));
+        CK(chain(buf<int>(ctx->big), G4D_CNT_APPEND2, buf<int>(ctx->huge), G4D_CNT_APPEND3, g4d::STAR_VMAX_T2, false));
+        // keep the tier counts for the statistics, then reuse mid/big for the exact chain
+        HCK(hipMemcpyAsync(cnt + G4D_CNT_SNAP0, cnt + G4D_CNT_APPEND6, 8, hipMemcpyDeviceToDevice, s));
+        HCK(hipMemcpyAsync(cnt + G4D_CNT_SNAP1, cnt + G4D_CNT_APPEND2, 8, hipMemcpyDeviceToDevice, s));
+        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND6, 0, 8, s));
+        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND2, 0, 8, s));
+        CK(chain(buf<int>(ctx->xstars), G4D_CNT_APPEND4, buf<int>(ctx->mid), G4D_CNT_APPEND6, g4d::STAR_VMAX_T0, true));
+        CK(chain(buf<int>(ctx->mid), G4D_CNT_APPEND6, buf<int>(ctx->big), G4D_CNT_APPEND2, g4d::STAR_VMAX_T1, true));
+        CK(chain(buf<int>(ctx->big), G4D_CNT_APPEND2, buf<int>(ctx->huge), G4D_CNT_APPEND3, g4d::STAR_VMAX_T2, true));
+        sp.nlist_dev = nullptr;
         CK(read_counters(ctx, hc));
         {
             float ms = 0.f;
@@ -568,62 +593,10 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
             ctx->st.star_t0_cofactors += (int64_t)hc[G4D_CNT_COFACTOR];
             ctx->st.star_t0_launches++;
         }
-        const int64_t nmid = (int64_t)hc[G4D_CNT_APPEND6];
-        if (nmid > 0) {
-            sp.star_list = buf<int>(ctx->mid);
-            sp.nlist = (int)nmid;
-            sp.defer_list = buf<int>(ctx->big);
-            sp.defer_counter = G4D_CNT_APPEND2;
-            HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
-            HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T1, false, 4096, s));
-            CK(read_counters(ctx, hc));
-        }
-        const int64_t nbig = (int64_t)hc[G4D_CNT_APPEND2];
-        if (nbig > 0) {
-            ctx->st.big_stars += (int32_t)nbig;
-            sp.star_list = buf<int>(ctx->big);
-            sp.nlist = (int)nbig;
-            sp.defer_list = buf<int>(ctx->huge);
-            sp.defer_counter = G4D_CNT_APPEND3;
-            HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
-            HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T2, false, 1024, s));
-            CK(read_counters(ctx, hc));
-        }
-        // Exact-capable kernels over the stars the filters could not decide, by the same size
-        // tiers (the filter tiers' lists are reused), at most 1,024 work-groups each.
+        const int64_t nmid = (int64_t)hc[G4D_CNT_SNAP0], nbig = (int64_t)hc[G4D_CNT_SNAP1];
         const int64_t nexact = (int64_t)hc[G4D_CNT_APPEND4];
-        if (nexact > 0) {
-            ctx->st.exact_stars += nexact;
-            sp.star_list = buf<int>(ctx->xstars);
-            sp.nlist = (int)nexact;
-            sp.defer_list = buf<int>(ctx->mid);
-            sp.defer_counter = G4D_CNT_APPEND6;
-            HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND6, 0, 8, s));
-            HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
-            HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T0, true, 1024, s));
-            CK(read_counters(ctx, hc));
-            const int64_t xmid = (int64_t)hc[G4D_CNT_APPEND6];
-            if (xmid > 0) {
-                sp.star_list = buf<int>(ctx->mid);
-                sp.nlist = (int)xmid;
-                sp.defer_list = buf<int>(ctx->big);
-                sp.defer_counter = G4D_CNT_APPEND2;
-                HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND2, 0, 8, s));
-                HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
-                HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T1, true, 1024, s));
-                CK(read_counters(ctx, hc));
-                const int64_t xbig = (int64_t)hc[G4D_CNT_APPEND2];
-                if (xbig > 0) {
-                    sp.star_list = buf<int>(ctx->big);
-                    sp.nlist = (int)xbig;
-                    sp.defer_list = buf<int>(ctx->huge);
-                    sp.defer_counter = G4D_CNT_APPEND3;
-                    HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
-                    HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T2, true, 1024, s));
-                    CK(read_counters(ctx, hc));
-                }
-            }
-        }
+        ctx->st.big_stars += (int32_t)nbig;
+        ctx->st.exact_stars += nexact;
         const int64_t nhuge = (int64_t)hc[G4D_CNT_APPEND3];
         if (nhuge > 0) {
             ctx->st.huge_stars += (int32_t)nhuge;

@@ -44,7 +44,9 @@ enum g4d_counter : int {
     G4D_CNT_APPEND5 = 12,
     G4D_CNT_APPEND6 = 13,
     G4D_CNT_COFACTOR = 14, // cached-cofactor computations in the LDS star kernels
-    G4D_CNT_NUM = 16
+    G4D_CNT_SNAP0 = 15,    // copies of list counters kept for the statistics
+    G4D_CNT_SNAP1 = 16,
+    G4D_CNT_NUM = 24
 };
 
 #define HIP_CHECK_RET(expr)                                                                    \

@@ -67,9 +67,11 @@ struct StarParams {
     int64_t exact_cap;    // counter `exact_counter`
     int exact_counter;
     int lds_limit;        // largest vn + m handled in LDS (<= STAR_LDS_LIMIT)
-    // work list (nullptr = all stars)
+    // work list (nullptr = all stars); with nlist_dev the list length is read on the device
+    // (the counter a previous kernel appended with), nlist being its capacity
     const int *star_list;
     int nlist;
+    const unsigned long long *nlist_dev;
     unsigned long long *cnt;
 };
 

@@ -221,6 +221,10 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
     // Work queue in chunks: one atomic per STAR_CHUNK stars for the filter-only kernels (many
     // small stars), per star for the exact ones (uneven cost).
     constexpr int CHUNK = EXACT ? 1 : STAR_CHUNK;
+    if (p.nlist_dev) {
+        const unsigned long long c = *p.nlist_dev;
+        nitems = c < (unsigned long long)nitems ? (int)c : nitems;
+    }
     int nfilt = 0, ncof = 0; // this lane's cofactor-filter evaluations and cofactor computations
     for (;;) {
         int base = 0;
@@ -562,6 +566,7 @@ hipError_t launch_star_kernel(const StarParams &p, int vmax, bool exact, int nbl
 {
     const int nitems = p.star_list ? p.nlist : p.n;
     if (nitems <= 0) return hipSuccess;
+    if (p.nlist_dev && !p.star_list) return hipErrorInvalidValue;
     if (p.lds_limit > STAR_LDS_LIMIT) return hipErrorInvalidValue; // local ids must fit 9 bits
     if (nblocks > nitems) nblocks = nitems;
     if (exact) {
