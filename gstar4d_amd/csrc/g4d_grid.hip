@@ -231,32 +231,34 @@ __global__ void k_scatter_sites(const int *vox, int n, GridSets S, int *grid, in
     }
 }
 
-__device__ __forceinline__ bool env_dominated(int qu, int64_t du, int qv, int64_t dv, int qw, int64_t dw)
+// Coordinates are below 1024, so squared distances stay below 2^22 and the numerators below
+// 2^23: 32-bit terms, 64-bit products.
+__device__ __forceinline__ bool env_dominated(int qu, int du, int qv, int dv, int qw, int dw)
 {
-    const int64_t n_uv = ((int64_t)qv * qv + dv) - ((int64_t)qu * qu + du);
-    const int64_t d_uv = qv - qu;
-    const int64_t n_vw = ((int64_t)qw * qw + dw) - ((int64_t)qv * qv + dv);
-    const int64_t d_vw = qw - qv;
-    return n_uv * d_vw >= n_vw * d_uv;
+    const int n_uv = (qv * qv + dv) - (qu * qu + du);
+    const int d_uv = qv - qu;
+    const int n_vw = (qw * qw + dw) - (qv * qv + dv);
+    const int d_vw = qw - qv;
+    return (int64_t)n_uv * d_vw >= (int64_t)n_vw * d_uv;
 }
 
 // Offset of a site from the column line (squared distance in the two non-sweep axes) and
 // its position along the sweep axis.
 template <int AXIS>
-__device__ __forceinline__ void site_pos_off(int s, int ca, int cb, int *q, int64_t *d)
+__device__ __forceinline__ void site_pos_off(int s, int ca, int cb, int *q, int *d)
 {
     const int sx = G4D_DEC_X(s), sy = G4D_DEC_Y(s), sz = G4D_DEC_Z(s);
     if (AXIS == 2) { // sweep z; column (x=ca, y=cb)
         *q = sz;
-        const int64_t dx = ca - sx, dy = cb - sy;
+        const int dx = ca - sx, dy = cb - sy;
         *d = dx * dx + dy * dy;
     } else if (AXIS == 1) { // sweep y; column (x=ca, z=cb)
         *q = sy;
-        const int64_t dx = ca - sx, dz = cb - sz;
+        const int dx = ca - sx, dz = cb - sz;
         *d = dx * dx + dz * dz;
     } else { // sweep x; column (y=ca, z=cb)
         *q = sx;
-        const int64_t dy = ca - sy, dz = cb - sz;
+        const int dy = ca - sy, dz = cb - sz;
         *d = dy * dy + dz * dz;
     }
 }
@@ -330,12 +332,10 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
             for (int qq = 0; qq < nq; qq++) {
                 const int s = L.tile[qq][lane];
                 if (s == G4D_MARKER) continue;
-                int qs;
-                int64_t ds;
+                int qs, ds;
                 site_pos_off<AXIS>(s, ca, cb, &qs, &ds);
                 while (m >= 2) {
-                    int qu, qv;
-                    int64_t du, dv;
+                    int qu, qv, du, dv;
                     const int su = m - 2 < PBA_STACK ? L.scode[m - 2][lane] : sp[m - 2];
                     const int sv = m - 1 < PBA_STACK ? L.scode[m - 1][lane] : sp[m - 1];
                     site_pos_off<AXIS>(su, ca, cb, &qu, &du);
@@ -352,7 +352,7 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
     }
     // ---- colour walk ----
     int idx = 0, qa = 0, qb = 0, sa = G4D_MARKER;
-    int64_t da = 0, db = 0;
+    int da = 0, db = 0;
     if (active && m > 0) {
         sa = L.scode[0][lane];
         site_pos_off<AXIS>(sa, ca, cb, &qa, &da);
@@ -371,8 +371,8 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
                     continue;
                 }
                 while (idx + 1 < m) {
-                    const int64_t fa = (int64_t)(t - qa) * (t - qa) + da;
-                    const int64_t fb = (int64_t)(t - qb) * (t - qb) + db;
+                    const int fa = (t - qa) * (t - qa) + da;
+                    const int fb = (t - qb) * (t - qb) + db;
                     if (fb <= fa) {
                         idx++;
                         qa = qb;

@@ -461,15 +461,36 @@ __device__ __forceinline__ int64_t fixed_coord64(const FixedPts &f, int i, int c
     return mm == 0 ? 0 : mm * ((int64_t)1 << (f.e[i][c] - f.emin));
 }
 
-__device__ __noinline__ int exact_insphere_fx(const FixedPts &f)
+// Translated coordinates d[i] = p_i - p_last (i < np - 1) as 64-bit integers when every one fits
+// 62 bits.  Coordinates of up to 125 bits are formed exactly in 128-bit integers, so clustered
+// points whose absolute coordinates span many binades still take the fixed-width path.
+__device__ __forceinline__ bool fixed_diffs64(const FixedPts &f, int np, int64_t (*d)[3])
+{
+    const __int128 lim = (__int128)1 << 62;
+#pragma unroll 1
+    for (int c = 0; c < 3; c++) {
+        const int64_t ml = f.m[np - 1][c];
+        const __int128 xl = ml == 0 ? 0 : (__int128)ml * ((__int128)1 << (f.e[np - 1][c] - f.emin));
+#pragma unroll 1
+        for (int i = 0; i < np - 1; i++) {
+            const int64_t mi = f.m[i][c];
+            const __int128 xi = mi == 0 ? 0 : (__int128)mi * ((__int128)1 << (f.e[i][c] - f.emin));
+            const __int128 x = xi - xl;
+            if (x >= lim || x <= -lim) return false;
+            d[i][c] = (int64_t)x;
+        }
+    }
+    return true;
+}
+
+// Lifted 4x4 determinant of translated coordinates of up to 62 bits (sign of insphere).
+__device__ __noinline__ int exact_insphere_fx(const int64_t (*dd)[3])
 {
     fx::I<1> d[4][3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const int64_t e = fixed_coord64(f, 4, c);
+    for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int i = 0; i < 4; i++) d[i][c] = fx::from64<1>(fixed_coord64(f, i, c) - e);
-    }
+        for (int c = 0; c < 3; c++) d[i][c] = fx::from64<1>(dd[i][c]);
     fx::I<5> acc = fx::from64<5>(0);
 #pragma unroll
     for (int out = 0; out < 4; out++) {
@@ -485,15 +506,14 @@ __device__ __noinline__ int exact_insphere_fx(const FixedPts &f)
     return fx::sign(acc);
 }
 
-__device__ __noinline__ int exact_orient3d_fx(const FixedPts &f)
+// 3x3 determinant of translated coordinates of up to 62 bits (sign of orient3d).
+__device__ __noinline__ int exact_orient3d_fx(const int64_t (*dd)[3])
 {
     fx::I<1> d[3][3];
 #pragma unroll
-    for (int c = 0; c < 3; c++) {
-        const int64_t e = fixed_coord64(f, 3, c);
+    for (int i = 0; i < 3; i++)
 #pragma unroll
-        for (int i = 0; i < 3; i++) d[i][c] = fx::from64<1>(fixed_coord64(f, i, c) - e);
-    }
+        for (int c = 0; c < 3; c++) d[i][c] = fx::from64<1>(dd[i][c]);
     const fx::I<3> acc = fx::add(fx::sub(fx::mul(d[0][2], fx::det2(d[1][0], d[2][1], d[2][0], d[1][1])),
                                          fx::mul(d[1][2], fx::det2(d[0][0], d[2][1], d[2][0], d[0][1]))),
                                  fx::mul(d[2][2], fx::det2(d[0][0], d[1][1], d[1][0], d[0][1])));
@@ -578,7 +598,8 @@ __device__ __noinline__ int exact_insphere(const double *const *p, unsigned long
 {
     FixedPts f;
     fixed_analyse(p, 5, f);
-    if (f.limbs == 2) return exact_insphere_fx(f);
+    int64_t d[4][3];
+    if (f.limbs && fixed_diffs64(f, 5, d)) return exact_insphere_fx(d);
     if (f.limbs) return exact_insphere_rt(f);
     pred_flag_error(cnt, G4D_DEV_ERR_PREDICATE_RANGE);
     return 0;
@@ -588,7 +609,8 @@ __device__ __noinline__ int exact_orient3d(const double *const *p, unsigned long
 {
     FixedPts f;
     fixed_analyse(p, 4, f);
-    if (f.limbs == 2) return exact_orient3d_fx(f);
+    int64_t d[3][3];
+    if (f.limbs && fixed_diffs64(f, 4, d)) return exact_orient3d_fx(d);
     if (f.limbs) return exact_orient3d_rt(f);
     pred_flag_error(cnt, G4D_DEV_ERR_PREDICATE_RANGE);
     return 0;
