@@ -357,10 +357,13 @@ class Triangulator:
         self._check(self._L.g4d_diag_morton(self._ctx, _ptr(xyz, _dp), n, grid, _ptr(perm, _ip), _ptr(vox, _ip)))
         return perm, vox
 
-    def diag_pba(self, vox_sorted, grid, codes=False):
+    def diag_pba(self, vox_sorted, grid, codes=False, id_grids=False):
+        """Nearest-site grid [z, y, x]: point ids, or site codes with codes=True; id_grids=True
+        runs the point-id grid kernels (used above 1024) at any grid width."""
         v = np.ascontiguousarray(vox_sorted, dtype=np.int32)
         out = np.empty(grid ** 3, np.int32)
-        self._check(self._L.g4d_diag_pba(self._ctx, _ptr(v, _ip), v.shape[0], grid, int(codes), _ptr(out, _ip)))
+        mode = 2 if id_grids else int(bool(codes))
+        self._check(self._L.g4d_diag_pba(self._ctx, _ptr(v, _ip), v.shape[0], grid, mode, _ptr(out, _ip)))
         return out.reshape(grid, grid, grid)
 
     def diag_grid_pairs(self, id_grid):

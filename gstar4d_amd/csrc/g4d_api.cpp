@@ -292,6 +292,7 @@ static int setup_sets(g4d_ctx *ctx, const SetSpec &spec, int n, g4d::GridSets &G
     GS.g = reinterpret_cast<const int *>(dtab + tab_bytes);
     GS.bounds = reinterpret_cast<const double *>(dtab + bounds_at);
     GS.pt_set = nullptr;
+    GS.key_shift = g4d::morton_key_shift(spec.g[0]);
     if (S > 1) {
         HCK(ctx->ptset.ensure(sizeof(int) * (size_t)n));
         HCK(g4d::launch_set_ids(GS.pt_off, S, buf<int>(ctx->ptset), s));
@@ -312,7 +313,8 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     for (int k = 0; k < S; k++) {
         const int G = spec.g[k];
         const int64_t nk = spec.pt_off[k + 1] - spec.pt_off[k];
-        if (G < 4 || G > 1024) return set_err(ctx, G4D_ERR_GRID_LIMIT, "grid_size %d outside [4, 1024]", G);
+        if (G < 4 || G > G4D_MAX_GRID)
+            return set_err(ctx, G4D_ERR_GRID_LIMIT, "grid_size %d outside [4, %d]", G, G4D_MAX_GRID);
         if (batched && G > g4d::BATCH_MAX_GRID)
             return set_err(ctx, G4D_ERR_GRID_LIMIT, "set %d: batched grid %d above %d", k, G, g4d::BATCH_MAX_GRID);
         if (nk < 5) return set_err(ctx, G4D_ERR_INVALID_INPUT, "set %d: need at least 5 points, got %lld", k, (long long)nk);
@@ -358,32 +360,37 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     HCK(ctx->keys.ensure(sizeof(uint64_t) * (size_t)n));
     HCK(ctx->pts.ensure(sizeof(double) * 3 * (size_t)n));
     HCK(ctx->perm.ensure(sizeof(int) * (size_t)n));
-    HCK(ctx->vox.ensure(sizeof(int) * (size_t)n));
+    HCK(ctx->vox.ensure(sizeof(uint64_t) * (size_t)n));
     uint64_t *keys = buf<uint64_t>(ctx->keys);
     HCK(g4d::launch_morton_keys(d_xyz, n, GS, keys, s));
     if (batched) HCK(g4d::radix_sort_u64(P, keys, n, 24, 64, s));
-    else HCK(g4d::radix_sort_u64(P, keys, n, 32, 62, s));
+    else HCK(g4d::radix_sort_u64(P, keys, n, GS.key_shift, 64, s));
     double *pts = buf<double>(ctx->pts);
-    int *perm = buf<int>(ctx->perm), *vox = buf<int>(ctx->vox);
+    int *perm = buf<int>(ctx->perm);
+    uint64_t *vox = buf<uint64_t>(ctx->vox);
     HCK(g4d::launch_gather_sorted(d_xyz, keys, n, GS, pts, perm, vox, s));
     HCK(g4d::launch_dup_check(pts, vox, n, GS, cnt, s));
     ev_record(ctx, EV_MORTON);
 
     // ---- C: PBA (GD/PBA/Pba.cu:105-122), every set's grid in one launch per pass ----
+    // Grids up to G4D_CODE_GRID carry packed site codes (and a point-id grid for the last
+    // pass); wider grids carry point ids throughout and read site voxels from `vox`.
+    const bool id_mode = gmax > G4D_CODE_GRID;
     HCK(ctx->grid.ensure(sizeof(int) * (size_t)G3));
     HCK(ctx->gtmp.ensure(sizeof(int) * (size_t)G3));
-    HCK(ctx->idgrid.ensure(sizeof(int) * (size_t)G3));
-    int *grid = buf<int>(ctx->grid), *gtmp = buf<int>(ctx->gtmp), *idgrid = buf<int>(ctx->idgrid);
+    if (!id_mode) HCK(ctx->idgrid.ensure(sizeof(int) * (size_t)G3));
+    int *grid = buf<int>(ctx->grid), *gtmp = buf<int>(ctx->gtmp), *idgrid = id_mode ? nullptr : buf<int>(ctx->idgrid);
+    const uint64_t *pvox = id_mode ? vox : nullptr;
     HCK(g4d::launch_fill_i32(grid, G3, G4D_MARKER, s));
     HCK(g4d::launch_scatter_sites(vox, n, GS, grid, idgrid, s));
     ev_record(ctx, EV_PBA0);
     HCK(ctx->pbaspill.ensure(sizeof(int) * (size_t)tot.spill));
     int *spill = buf<int>(ctx->pbaspill);
-    HCK(g4d::pba_pass(2, GS, tot.pba_blocks, grid, gtmp, nullptr, spill, s));
+    HCK(g4d::pba_pass(2, GS, tot.pba_blocks, grid, gtmp, nullptr, spill, pvox, s));
     ev_record(ctx, EV_PBA1);
-    HCK(g4d::pba_pass(1, GS, tot.pba_blocks, gtmp, grid, nullptr, spill, s));
+    HCK(g4d::pba_pass(1, GS, tot.pba_blocks, gtmp, grid, nullptr, spill, pvox, s));
     ev_record(ctx, EV_PBA2);
-    HCK(g4d::pba_pass(0, GS, tot.pba_blocks, grid, gtmp, idgrid, spill, s)); // point ids in gtmp
+    HCK(g4d::pba_pass(0, GS, tot.pba_blocks, grid, gtmp, idgrid, spill, pvox, s)); // point ids in gtmp
     ev_record(ctx, EV_PBA3);
     const int *idg = gtmp;
 
@@ -1078,7 +1085,7 @@ int g4d_diag_orient3d(g4d_ctx *ctx, const double *pts4, int64_t T, int32_t *sign
 
 int g4d_diag_morton(g4d_ctx *ctx, const double *xyz, int32_t n, int32_t grid, int32_t *perm_out, int32_t *vox_out)
 {
-    if (!ctx || !xyz || n <= 0 || !perm_out || !vox_out || grid < 4 || grid > 1024) return G4D_ERR_INVALID_INPUT;
+    if (!ctx || !xyz || n <= 0 || !perm_out || !vox_out || grid < 4 || grid > G4D_MAX_GRID) return G4D_ERR_INVALID_INPUT;
     HCK(hipSetDevice(ctx->opt.device));
     hipStream_t s = ctx->stream;
     const double *d_xyz = nullptr;
@@ -1086,57 +1093,64 @@ int g4d_diag_morton(g4d_ctx *ctx, const double *xyz, int32_t n, int32_t grid, in
     HCK(ctx->keys.ensure(sizeof(uint64_t) * (size_t)n));
     HCK(ctx->pts.ensure(sizeof(double) * 3 * (size_t)n));
     HCK(ctx->perm.ensure(sizeof(int) * (size_t)n));
-    HCK(ctx->vox.ensure(sizeof(int) * (size_t)n));
+    HCK(ctx->vox.ensure(sizeof(uint64_t) * (size_t)n));
     g4d::GridSets GS;
     SetTotals tot;
     CK(setup_sets(ctx, single_set(n, grid), n, GS, tot));
     HCK(g4d::launch_bounds(d_xyz, n, const_cast<double *>(GS.bounds), s));
     HCK(g4d::launch_morton_keys(d_xyz, n, GS, buf<uint64_t>(ctx->keys), s));
-    HCK(g4d::radix_sort_u64(ctx->P, buf<uint64_t>(ctx->keys), n, 32, 62, s));
+    HCK(g4d::radix_sort_u64(ctx->P, buf<uint64_t>(ctx->keys), n, GS.key_shift, 64, s));
     HCK(g4d::launch_gather_sorted(d_xyz, buf<uint64_t>(ctx->keys), n, GS, buf<double>(ctx->pts), buf<int>(ctx->perm),
-                                  buf<int>(ctx->vox), s));
-    std::vector<int> vs((size_t)n);
+                                  buf<uint64_t>(ctx->vox), s));
+    std::vector<uint64_t> vs((size_t)n);
     HCK(hipMemcpyAsync(perm_out, ctx->perm.ptr, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
-    HCK(hipMemcpyAsync(vs.data(), ctx->vox.ptr, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost, s));
+    HCK(hipMemcpyAsync(vs.data(), ctx->vox.ptr, sizeof(uint64_t) * (size_t)n, hipMemcpyDeviceToHost, s));
     HCK(hipStreamSynchronize(s));
     for (int k = 0; k < n; k++) {
-        const int i = perm_out[k], c = vs[(size_t)k];
-        vox_out[3 * (int64_t)i] = G4D_DEC_X(c);
-        vox_out[3 * (int64_t)i + 1] = G4D_DEC_Y(c);
-        vox_out[3 * (int64_t)i + 2] = G4D_DEC_Z(c);
+        const int i = perm_out[k];
+        const uint64_t c = vs[(size_t)k];
+        vox_out[3 * (int64_t)i] = G4D_VOX_X(c);
+        vox_out[3 * (int64_t)i + 1] = G4D_VOX_Y(c);
+        vox_out[3 * (int64_t)i + 2] = G4D_VOX_Z(c);
     }
     return G4D_OK;
 }
 
 int g4d_diag_pba(g4d_ctx *ctx, const int32_t *vox_sorted, int32_t n, int32_t grid, int32_t codes, int32_t *out)
 {
-    if (!ctx || !vox_sorted || n <= 0 || !out || grid < 4 || grid > 1024) return G4D_ERR_INVALID_INPUT;
+    if (!ctx || !vox_sorted || n <= 0 || !out || grid < 4 || grid > G4D_MAX_GRID) return G4D_ERR_INVALID_INPUT;
+    if (codes < 0 || codes > 2) return set_err(ctx, G4D_ERR_INVALID_INPUT, "codes %d outside [0, 2]", codes);
+    const bool id_mode = grid > G4D_CODE_GRID || codes == 2;
+    if (grid > G4D_CODE_GRID && codes == 1)
+        return set_err(ctx, G4D_ERR_INVALID_INPUT, "site codes exist only for grids up to %d", G4D_CODE_GRID);
     HCK(hipSetDevice(ctx->opt.device));
     hipStream_t s = ctx->stream;
-    std::vector<int> enc((size_t)n);
+    std::vector<uint64_t> enc((size_t)n);
     for (int k = 0; k < n; k++) {
         const int x = vox_sorted[3 * (int64_t)k], y = vox_sorted[3 * (int64_t)k + 1], z = vox_sorted[3 * (int64_t)k + 2];
         if (x < 0 || y < 0 || z < 0 || x >= grid || y >= grid || z >= grid)
             return set_err(ctx, G4D_ERR_INVALID_INPUT, "voxel outside the grid");
-        enc[(size_t)k] = G4D_ENC(x, y, z);
+        enc[(size_t)k] = G4D_VOX(x, y, z);
     }
     const int64_t G3 = (int64_t)grid * grid * grid;
-    HCK(ctx->vox.ensure(sizeof(int) * (size_t)n));
+    HCK(ctx->vox.ensure(sizeof(uint64_t) * (size_t)n));
     HCK(ctx->grid.ensure(sizeof(int) * (size_t)G3));
     HCK(ctx->gtmp.ensure(sizeof(int) * (size_t)G3));
-    HCK(ctx->idgrid.ensure(sizeof(int) * (size_t)G3));
-    HCK(hipMemcpyAsync(ctx->vox.ptr, enc.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s));
+    if (!id_mode) HCK(ctx->idgrid.ensure(sizeof(int) * (size_t)G3));
+    HCK(hipMemcpyAsync(ctx->vox.ptr, enc.data(), sizeof(uint64_t) * (size_t)n, hipMemcpyHostToDevice, s));
     g4d::GridSets GS;
     SetTotals tot;
     CK(setup_sets(ctx, single_set(n, grid), n, GS, tot));
+    const uint64_t *vox = buf<uint64_t>(ctx->vox), *pvox = id_mode ? vox : nullptr;
+    int *idgrid = id_mode ? nullptr : buf<int>(ctx->idgrid);
     HCK(g4d::launch_fill_i32(buf<int>(ctx->grid), G3, G4D_MARKER, s));
-    HCK(g4d::launch_scatter_sites(buf<int>(ctx->vox), n, GS, buf<int>(ctx->grid), buf<int>(ctx->idgrid), s));
+    HCK(g4d::launch_scatter_sites(vox, n, GS, buf<int>(ctx->grid), idgrid, s));
     HCK(ctx->pbaspill.ensure(sizeof(int) * (size_t)tot.spill));
     int *spill = buf<int>(ctx->pbaspill);
-    HCK(g4d::pba_pass(2, GS, tot.pba_blocks, buf<int>(ctx->grid), buf<int>(ctx->gtmp), nullptr, spill, s));
-    HCK(g4d::pba_pass(1, GS, tot.pba_blocks, buf<int>(ctx->gtmp), buf<int>(ctx->grid), nullptr, spill, s));
-    HCK(g4d::pba_pass(0, GS, tot.pba_blocks, buf<int>(ctx->grid), buf<int>(ctx->gtmp),
-                      codes ? nullptr : buf<int>(ctx->idgrid), spill, s));
+    HCK(g4d::pba_pass(2, GS, tot.pba_blocks, buf<int>(ctx->grid), buf<int>(ctx->gtmp), nullptr, spill, pvox, s));
+    HCK(g4d::pba_pass(1, GS, tot.pba_blocks, buf<int>(ctx->gtmp), buf<int>(ctx->grid), nullptr, spill, pvox, s));
+    HCK(g4d::pba_pass(0, GS, tot.pba_blocks, buf<int>(ctx->grid), buf<int>(ctx->gtmp), codes == 1 ? nullptr : idgrid,
+                      spill, pvox, s));
     HCK(hipMemcpyAsync(out, ctx->gtmp.ptr, sizeof(int) * (size_t)G3, hipMemcpyDeviceToHost, s));
     HCK(hipStreamSynchronize(s));
     return G4D_OK;
@@ -1303,7 +1317,7 @@ int g4d_diag_copy(g4d_ctx *ctx, int64_t bytes, double *ms)
 
 int g4d_diag_grid_pairs(g4d_ctx *ctx, const int32_t *id_grid, int32_t grid, uint64_t *out, int64_t cap, int64_t *count)
 {
-    if (!ctx || !id_grid || !count || grid < 4 || grid > 1024) return G4D_ERR_INVALID_INPUT;
+    if (!ctx || !id_grid || !count || grid < 4 || grid > G4D_MAX_GRID) return G4D_ERR_INVALID_INPUT;
     HCK(hipSetDevice(ctx->opt.device));
     hipStream_t s = ctx->stream;
     const int64_t G3 = (int64_t)grid * grid * grid;

@@ -13,6 +13,19 @@
 #define G4D_DEC_X(v) ((v) >> 20)
 #define G4D_DEC_Y(v) (((v) >> 10) & 0x3ff)
 #define G4D_DEC_Z(v) ((v)&0x3ff)
+// Voxel of a point: 21 bits per axis, x | y << 21 | z << 42.  Grids up to G4D_CODE_GRID wide
+// hold site codes in their cells; wider grids (up to G4D_MAX_GRID) hold point ids, and the site
+// coordinates come from the points' voxels.
+#define G4D_VOX(x, y, z) ((uint64_t)(x) | ((uint64_t)(y) << 21) | ((uint64_t)(z) << 42))
+#define G4D_VOX_X(v) ((int)((v)&0x1fffff))
+#define G4D_VOX_Y(v) ((int)(((v) >> 21) & 0x1fffff))
+#define G4D_VOX_Z(v) ((int)(((v) >> 42) & 0x1fffff))
+// A build may lower G4D_CODE_GRID (never raise it: codes have 10 bits per axis) to run the
+// point-id path on small grids.
+#ifndef G4D_CODE_GRID
+#define G4D_CODE_GRID 1024
+#endif
+#define G4D_MAX_GRID 2048
 
 // Device-side error flags (bitwise OR into a counter block word).
 enum g4d_dev_error : uint32_t {

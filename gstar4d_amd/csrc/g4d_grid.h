@@ -19,6 +19,7 @@ struct GridSets {
     const int64_t *pba_blk; // nsets + 1: PBA work-groups
     const int64_t *spill;   // nsets + 1: PBA spill ints
     const int64_t *pair_thr;// nsets + 1: grid-pair threads
+    int key_shift;          // one set: Morton keys are code << key_shift | index
 };
 // Batched Morton keys: set (16 bits) | code (24 bits, G <= 256) | index within the set (24 bits).
 static constexpr int BATCH_MAX_GRID = 256;
@@ -31,25 +32,29 @@ hipError_t launch_bounds(const double *xyz, int n, double *part /* >= 768 double
 hipError_t launch_set_bounds(const double *xyz, const GridSets &S, double *bounds, hipStream_t s);
 // pt_set[k] = set of point k.
 hipError_t launch_set_ids(const int64_t *pt_off, int nsets, int *pt_set, hipStream_t s);
-// Morton keys: one set: code << 32 | i (sort bits 32..62); several sets: the batched layout
-// (sort bits 24..64).
+// Morton keys: one set: code << key_shift | i, code 3 bits per level of the grid (sort bits
+// key_shift..64); several sets: the batched layout (sort bits 24..64).
+int morton_key_shift(int g);
 hipError_t launch_morton_keys(const double *xyz, int n, const GridSets &S, uint64_t *keys, hipStream_t s);
 hipError_t launch_gather_sorted(const double *xyz, const uint64_t *keys, int n, const GridSets &S, double *pts,
-                                int *perm, int *vox, hipStream_t s);
-hipError_t launch_dup_check(const double *pts, const int *vox, int n, const GridSets &S, unsigned long long *cnt,
+                                int *perm, uint64_t *vox, hipStream_t s);
+hipError_t launch_dup_check(const double *pts, const uint64_t *vox, int n, const GridSets &S, unsigned long long *cnt,
                             hipStream_t s);
 hipError_t launch_fill_i32(int *a, int64_t n, int v, hipStream_t s);
-hipError_t launch_scatter_sites(const int *vox, int n, const GridSets &S, int *grid, int *idgrid, hipStream_t s);
+// Leaders into the grids: site codes into grid and point ids into idgrid, or, with idgrid null
+// (required above G4D_CODE_GRID), point ids into grid.
+hipError_t launch_scatter_sites(const uint64_t *vox, int n, const GridSets &S, int *grid, int *idgrid, hipStream_t s);
 // PBA work-groups and stack-spill ints (per-column overflow of the in-LDS envelope stack) of a
 // G^3 grid.
 int64_t pba_blocks(int g);
 int64_t pba_spill_ints(int g);
 // One axis pass (2 = z, 1 = y, 0 = x) over every set's grid; with idgrid the output is point
-// ids.  nblocks = pba_blk[nsets].
+// ids.  nblocks = pba_blk[nsets].  With vox (one set wider than G4D_CODE_GRID) the cells hold
+// point ids throughout and the sites' coordinates are read from vox.
 hipError_t pba_pass(int axis, const GridSets &S, int64_t nblocks, const int *in, int *out, const int *idgrid,
-                    int *spill, hipStream_t s);
+                    int *spill, const uint64_t *vox, hipStream_t s);
 // Ordered pairs (min<<32 | max) of each non-leader point with its voxel's leader.
-hipError_t launch_missing(const int *vox, int n, const GridSets &S, const int *grid, uint64_t *out,
+hipError_t launch_missing(const uint64_t *vox, int n, const GridSets &S, const int *grid, uint64_t *out,
                           unsigned long long *counter, int64_t cap, hipStream_t s);
 // Voronoi-neighbour pairs (min<<32 | max): pass write=false fills count[thread], the caller
 // scans it to offsets (in place), pass write=true writes the pairs.  nthreads = pair_thr[nsets].

@@ -123,12 +123,15 @@ __global__ void k_set_ids(const int64_t *pt_off, int *pt_set)
     for (int64_t i = pt_off[k] + threadIdx.x; i < pt_off[k + 1]; i += blockDim.x) pt_set[i] = k;
 }
 
-__device__ __forceinline__ uint32_t spread10(uint32_t v)
+// Spreads the low 21 bits of v to every third bit.
+__device__ __forceinline__ uint64_t spread21(uint64_t v)
 {
-    v = (v | (v << 16)) & 0x030000FFu;
-    v = (v | (v << 8)) & 0x0300F00Fu;
-    v = (v | (v << 4)) & 0x030C30C3u;
-    v = (v | (v << 2)) & 0x09249249u;
+    v &= 0x1fffffull;
+    v = (v | (v << 32)) & 0x1f00000000ffffull;
+    v = (v | (v << 16)) & 0x1f0000ff0000ffull;
+    v = (v | (v << 8)) & 0x100f00f00f00f00full;
+    v = (v | (v << 4)) & 0x10c30c30c30c30c3ull;
+    v = (v | (v << 2)) & 0x1249249249249249ull;
     return v;
 }
 
@@ -154,22 +157,22 @@ __global__ void k_morton_keys(const double *xyz, int n, GridSets S, uint64_t *ke
         const int x = voxel_of(xyz[3 * (int64_t)i], mn, range, span, g);
         const int y = voxel_of(xyz[3 * (int64_t)i + 1], mn, range, span, g);
         const int z = voxel_of(xyz[3 * (int64_t)i + 2], mn, range, span, g);
-        const uint32_t code = spread10(x) | (spread10(y) << 1) | (spread10(z) << 2);
+        const uint64_t code = spread21(x) | (spread21(y) << 1) | (spread21(z) << 2);
         if (S.nsets == 1)
-            keys[i] = ((uint64_t)code << 32) | (uint32_t)i;
+            keys[i] = (code << S.key_shift) | (uint64_t)i;
         else
             keys[i] = ((uint64_t)st << 48) | ((uint64_t)code << 24) | (uint64_t)(i - S.pt_off[st]);
     }
 }
 
 __global__ void k_gather_sorted(const double *xyz, const uint64_t *keys, int n, GridSets S, double *pts, int *perm,
-                                int *vox)
+                                uint64_t *vox)
 {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const uint64_t key = keys[k];
         int st = 0, i;
         if (S.nsets == 1) {
-            i = (int)(uint32_t)key;
+            i = (int)(key & ((1ull << S.key_shift) - 1ull));
         } else {
             st = (int)(key >> 48);
             i = (int)(S.pt_off[st] + (int64_t)(key & 0xffffffull));
@@ -181,13 +184,13 @@ __global__ void k_gather_sorted(const double *xyz, const uint64_t *keys, int n, 
         pts[3 * (int64_t)k + 1] = y;
         pts[3 * (int64_t)k + 2] = z;
         perm[k] = i;
-        vox[k] = G4D_ENC(voxel_of(x, mn, range, span, g), voxel_of(y, mn, range, span, g),
+        vox[k] = G4D_VOX(voxel_of(x, mn, range, span, g), voxel_of(y, mn, range, span, g),
                          voxel_of(z, mn, range, span, g));
     }
 }
 
 // Equal points share a voxel, and equal voxels are adjacent in Morton order.
-__global__ void k_dup_check(const double *pts, const int *vox, int n, GridSets S, unsigned long long *cnt)
+__global__ void k_dup_check(const double *pts, const uint64_t *vox, int n, GridSets S, unsigned long long *cnt)
 {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const int64_t first = S.pt_off[set_of(S, k)];
@@ -217,22 +220,29 @@ __global__ void k_fill_i32(int *a, int64_t n, int v)
 }
 
 // Site scatter: the first point of each equal-voxel run (lowest Morton index) is the voxel's
-// leader; the reference picks a racy last writer (GD/PBA/pba3DKernel.h:518-542).
-__global__ void k_scatter_sites(const int *vox, int n, GridSets S, int *grid, int *idgrid)
+// leader; the reference picks a racy last writer (GD/PBA/pba3DKernel.h:518-542).  Without an
+// id grid the cells take the leader's point id instead of its site code.
+__global__ void k_scatter_sites(const uint64_t *vox, int n, GridSets S, int *grid, int *idgrid)
 {
     for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
-        const int c = vox[k];
+        const uint64_t v = vox[k];
         const int st = set_of(S, k);
-        if (k > S.pt_off[st] && vox[k - 1] == c) continue;
+        if (k > S.pt_off[st] && vox[k - 1] == v) continue;
         const int64_t g = S.g[st];
-        const int64_t idx = S.vox_off[st] + ((int64_t)G4D_DEC_Z(c) * g + G4D_DEC_Y(c)) * g + G4D_DEC_X(c);
+        const int x = G4D_VOX_X(v), y = G4D_VOX_Y(v), z = G4D_VOX_Z(v);
+        const int64_t idx = S.vox_off[st] + ((int64_t)z * g + y) * g + x;
+        if (!idgrid) { // point-id grid
+            grid[idx] = k;
+            continue;
+        }
+        const int c = G4D_ENC(x, y, z);
         grid[idx] = c;
         idgrid[idx] = k;
     }
 }
 
-// Coordinates are below 1024, so squared distances stay below 2^22 and the numerators below
-// 2^23: 32-bit terms, 64-bit products.
+// Coordinates are below 2048, so squared distances stay below 2^23 and the numerators below
+// 2^24: 32-bit terms, 64-bit products.
 __device__ __forceinline__ bool env_dominated(int qu, int du, int qv, int dv, int qw, int dw)
 {
     const int n_uv = (qv * qv + dv) - (qu * qu + du);
@@ -244,10 +254,20 @@ __device__ __forceinline__ bool env_dominated(int qu, int du, int qv, int dv, in
 
 // Offset of a site from the column line (squared distance in the two non-sweep axes) and
 // its position along the sweep axis.
-template <int AXIS>
-__device__ __forceinline__ void site_pos_off(int s, int ca, int cb, int *q, int *d)
+template <int AXIS, bool IDS>
+__device__ __forceinline__ void site_pos_off(int s, const uint64_t *vox, int ca, int cb, int *q, int *d)
 {
-    const int sx = G4D_DEC_X(s), sy = G4D_DEC_Y(s), sz = G4D_DEC_Z(s);
+    int sx, sy, sz;
+    if (IDS) {
+        const uint64_t v = vox[s];
+        sx = G4D_VOX_X(v);
+        sy = G4D_VOX_Y(v);
+        sz = G4D_VOX_Z(v);
+    } else {
+        sx = G4D_DEC_X(s);
+        sy = G4D_DEC_Y(s);
+        sz = G4D_DEC_Z(s);
+    }
     if (AXIS == 2) { // sweep z; column (x=ca, y=cb)
         *q = sz;
         const int dx = ca - sx, dy = cb - sy;
@@ -297,9 +317,9 @@ __device__ __forceinline__ int64_t pba_gidx64(int g, int fixed, int c, int q)
     return ((int64_t)fixed * G + c) * G + q;                  // z = fixed, y = c, x = q
 }
 
-template <int AXIS>
+template <int AXIS, bool IDS>
 __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all, GridSets S, const int *idgrid_all,
-                                                 int *spill_all)
+                                                 int *spill_all, const uint64_t *vox)
 {
     __shared__ PbaLds L;
     const int lane = threadIdx.x;
@@ -333,13 +353,13 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
                 const int s = L.tile[qq][lane];
                 if (s == G4D_MARKER) continue;
                 int qs, ds;
-                site_pos_off<AXIS>(s, ca, cb, &qs, &ds);
+                site_pos_off<AXIS, IDS>(s, vox, ca, cb, &qs, &ds);
                 while (m >= 2) {
                     int qu, qv, du, dv;
                     const int su = m - 2 < PBA_STACK ? L.scode[m - 2][lane] : sp[m - 2];
                     const int sv = m - 1 < PBA_STACK ? L.scode[m - 1][lane] : sp[m - 1];
-                    site_pos_off<AXIS>(su, ca, cb, &qu, &du);
-                    site_pos_off<AXIS>(sv, ca, cb, &qv, &dv);
+                    site_pos_off<AXIS, IDS>(su, vox, ca, cb, &qu, &du);
+                    site_pos_off<AXIS, IDS>(sv, vox, ca, cb, &qv, &dv);
                     if (env_dominated(qu, du, qv, dv, qs, ds)) m--;
                     else break;
                 }
@@ -355,11 +375,11 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
     int da = 0, db = 0;
     if (active && m > 0) {
         sa = L.scode[0][lane];
-        site_pos_off<AXIS>(sa, ca, cb, &qa, &da);
+        site_pos_off<AXIS, IDS>(sa, vox, ca, cb, &qa, &da);
     }
     if (active && m > 1) {
         const int sb = 1 < PBA_STACK ? L.scode[1][lane] : sp[1];
-        site_pos_off<AXIS>(sb, ca, cb, &qb, &db);
+        site_pos_off<AXIS, IDS>(sb, vox, ca, cb, &qb, &db);
     }
     for (int t0 = 0; t0 < g; t0 += PBA_CHUNK) {
         const int nt = min(PBA_CHUNK, g - t0);
@@ -380,7 +400,7 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
                         sa = idx < PBA_STACK ? L.scode[idx][lane] : sp[idx];
                         if (idx + 1 < m) {
                             const int sb = idx + 1 < PBA_STACK ? L.scode[idx + 1][lane] : sp[idx + 1];
-                            site_pos_off<AXIS>(sb, ca, cb, &qb, &db);
+                            site_pos_off<AXIS, IDS>(sb, vox, ca, cb, &qb, &db);
                         }
                     } else {
                         break;
@@ -395,7 +415,7 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
             if (AXIS == 0) { cc = i / PBA_CHUNK; qq = i % PBA_CHUNK; } else { qq = i / PBA_COLS; cc = i % PBA_COLS; }
             if (qq < nt && cc < ncols) {
                 int v = L.tile[qq][cc];
-                if (idgrid)
+                if (!IDS && idgrid)
                     v = (v == G4D_MARKER) ? -1
                                           : idgrid[((int64_t)G4D_DEC_Z(v) * g + G4D_DEC_Y(v)) * g + G4D_DEC_X(v)];
                 out[pba_gidx64<AXIS>(g, fixed, c0 + cc, t0 + qq)] = v;
@@ -410,16 +430,28 @@ int64_t pba_blocks(int g) { return (int64_t)g * ((g + PBA_COLS - 1) / PBA_COLS);
 int64_t pba_spill_ints(int g) { return pba_blocks(g) * PBA_COLS * g; }
 
 hipError_t pba_pass(int axis, const GridSets &S, int64_t nblocks, const int *in, int *out, const int *idgrid,
-                    int *spill, hipStream_t s)
+                    int *spill, const uint64_t *vox, hipStream_t s)
 {
     if (nblocks <= 0 || nblocks > INT32_MAX) return hipErrorInvalidValue;
-    if (axis == 2)
-        hipLaunchKernelGGL(k_pba_pass<2>, dim3((unsigned)nblocks), dim3(64), 0, s, in, out, S, idgrid, spill);
-    else if (axis == 1)
-        hipLaunchKernelGGL(k_pba_pass<1>, dim3((unsigned)nblocks), dim3(64), 0, s, in, out, S, idgrid, spill);
-    else
-        hipLaunchKernelGGL(k_pba_pass<0>, dim3((unsigned)nblocks), dim3(64), 0, s, in, out, S, idgrid, spill);
+    const dim3 gr((unsigned)nblocks), bl(64);
+    if (vox) {
+        if (S.nsets != 1) return hipErrorInvalidValue;
+        if (axis == 2) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pba_pass<2, true>), gr, bl, 0, s, in, out, S, idgrid, spill, vox);
+        else if (axis == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pba_pass<1, true>), gr, bl, 0, s, in, out, S, idgrid, spill, vox);
+        else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pba_pass<0, true>), gr, bl, 0, s, in, out, S, idgrid, spill, vox);
+    } else {
+        if (axis == 2) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pba_pass<2, false>), gr, bl, 0, s, in, out, S, idgrid, spill, vox);
+        else if (axis == 1) hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pba_pass<1, false>), gr, bl, 0, s, in, out, S, idgrid, spill, vox);
+        else hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pba_pass<0, false>), gr, bl, 0, s, in, out, S, idgrid, spill, vox);
+    }
     return hipGetLastError();
+}
+
+int morton_key_shift(int g)
+{
+    int b = 10;
+    while ((1 << b) < g) b++;
+    return 64 - 3 * b;
 }
 
 // ------------------------------------------------------------------------------------
@@ -434,7 +466,7 @@ __device__ __forceinline__ uint64_t upair(int a, int b)
 
 // Missing points (GD/GDelaunay/GDelKernels.cu:109-135): a point whose voxel is led by another
 // point pairs with that leader.  Emits ordered pairs (min<<32 | max).
-__global__ void k_missing(const int *vox, int n, GridSets S, const int *grid, uint64_t *out,
+__global__ void k_missing(const uint64_t *vox, int n, GridSets S, const int *grid, uint64_t *out,
                           unsigned long long *counter, int64_t cap)
 {
     for (int base = blockIdx.x * blockDim.x; base < n; base += gridDim.x * blockDim.x) {
@@ -442,10 +474,10 @@ __global__ void k_missing(const int *vox, int n, GridSets S, const int *grid, ui
         bool miss = false;
         int leader = 0;
         if (k < n) {
-            const int c = vox[k];
+            const uint64_t v = vox[k];
             const int st = set_of(S, k);
             const int64_t g = S.g[st];
-            leader = grid[S.vox_off[st] + ((int64_t)G4D_DEC_Z(c) * g + G4D_DEC_Y(c)) * g + G4D_DEC_X(c)];
+            leader = grid[S.vox_off[st] + ((int64_t)G4D_VOX_Z(v) * g + G4D_VOX_Y(v)) * g + G4D_VOX_X(v)];
             miss = leader != k;
         }
         const long long slot = g4d_wave_append(miss, counter);
@@ -633,14 +665,14 @@ hipError_t launch_morton_keys(const double *xyz, int n, const GridSets &S, uint6
 }
 
 hipError_t launch_gather_sorted(const double *xyz, const uint64_t *keys, int n, const GridSets &S, double *pts,
-                                int *perm, int *vox, hipStream_t s)
+                                int *perm, uint64_t *vox, hipStream_t s)
 {
     hipLaunchKernelGGL(k_gather_sorted, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, xyz, keys, n, S, pts, perm,
                        vox);
     return hipGetLastError();
 }
 
-hipError_t launch_dup_check(const double *pts, const int *vox, int n, const GridSets &S, unsigned long long *cnt,
+hipError_t launch_dup_check(const double *pts, const uint64_t *vox, int n, const GridSets &S, unsigned long long *cnt,
                             hipStream_t s)
 {
     hipLaunchKernelGGL(k_dup_check, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, pts, vox, n, S, cnt);
@@ -653,13 +685,13 @@ hipError_t launch_fill_i32(int *a, int64_t n, int v, hipStream_t s)
     return hipGetLastError();
 }
 
-hipError_t launch_scatter_sites(const int *vox, int n, const GridSets &S, int *grid, int *idgrid, hipStream_t s)
+hipError_t launch_scatter_sites(const uint64_t *vox, int n, const GridSets &S, int *grid, int *idgrid, hipStream_t s)
 {
     hipLaunchKernelGGL(k_scatter_sites, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, vox, n, S, grid, idgrid);
     return hipGetLastError();
 }
 
-hipError_t launch_missing(const int *vox, int n, const GridSets &S, const int *grid, uint64_t *out,
+hipError_t launch_missing(const uint64_t *vox, int n, const GridSets &S, const int *grid, uint64_t *out,
                           unsigned long long *counter, int64_t cap, hipStream_t s)
 {
     hipLaunchKernelGGL(k_missing, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, vox, n, S, grid, out, counter, cap);

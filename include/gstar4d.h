@@ -50,7 +50,7 @@ typedef enum {
 } g4d_index_space;
 
 typedef struct {
-    int32_t grid_size;          /* PBA grid width G, 4..1024 (Config::_gridSize) */
+    int32_t grid_size;          /* PBA grid width G, 4..2048 (Config::_gridSize) */
     int32_t min_workset;        /* MinWorksetSize (GD/GDelaunay/GDelCommon.h:49), 4..64 */
     int32_t max_rounds;         /* splaying round guard */
     int32_t index_space;        /* g4d_index_space */
@@ -206,7 +206,8 @@ int g4d_diag_orient3d(g4d_ctx *ctx, const double *pts4, int64_t T, int32_t *sign
 /* Morton order and voxel coordinates (3 per input point). */
 int g4d_diag_morton(g4d_ctx *ctx, const double *xyz, int32_t n, int32_t grid, int32_t *perm_out, int32_t *vox_out);
 /* PBA of sites given as Morton-sorted voxel coordinates (3n); out: G^3 nearest-site point ids,
-   index (z*G + y)*G + x.  codes != 0 returns site codes instead of ids. */
+   index (z*G + y)*G + x.  codes: 0 ids, 1 site codes x<<20|y<<10|z (grids up to 1024), 2 ids
+   through the point-id grids that wider grids use (any grid). */
 int g4d_diag_pba(g4d_ctx *ctx, const int32_t *vox_sorted, int32_t n, int32_t grid, int32_t codes, int32_t *out);
 /* Plain device-to-device copy of `bytes` (rounded down to 16) between two buffers the context
    allocates; *ms receives the kernel time.  Calibrates HBM traffic counters and bandwidth. */

@@ -25,6 +25,9 @@
 #ifndef GSTAR4D_ORC_H
 #define GSTAR4D_ORC_H
 
+/* Widest PBA grid: 11-bit voxel coordinates. */
+#define ORC_MAX_GRID 2048
+
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -51,7 +54,7 @@ void orc_pred_counter_reset(void);
 
 /* ---- pipeline (orc_pipeline.c) ---- */
 typedef struct {
-    int32_t grid_size;      /* PBA grid width G (<= 1024) */
+    int32_t grid_size;      /* PBA grid width G (<= ORC_MAX_GRID) */
     int32_t min_workset;    /* MinWorksetSize, 8 in the reference */
     int32_t max_rounds;     /* splaying round limit (guard), e.g. 1000 */
     int32_t verbose;

@@ -109,27 +109,29 @@ void orc_scale_to_voxels(const double *xyz, int32_t n, int32_t grid, int32_t *vo
     }
 }
 
-/* 30-bit Morton code: bit 3b = x_b, 3b+1 = y_b, 3b+2 = z_b (GD/GDelaunay/GDelData.cu:51-88). */
-static uint32_t morton30(int x, int y, int z)
+/* Morton code: bit 3b = x_b, 3b+1 = y_b, 3b+2 = z_b (GD/GDelaunay/GDelData.cu:51-88, which
+   interleaves 10 bits per axis); 11 bits per axis here so that grids up to 2048 order the same
+   way (the eleventh bits are zero below 1024). */
+static uint64_t morton33(int x, int y, int z)
 {
-    uint32_t m = 0;
-    for (int b = 0; b < 10; b++) {
-        m |= (uint32_t)((x >> b) & 1) << (3 * b);
-        m |= (uint32_t)((y >> b) & 1) << (3 * b + 1);
-        m |= (uint32_t)((z >> b) & 1) << (3 * b + 2);
+    uint64_t m = 0;
+    for (int b = 0; b < 11; b++) {
+        m |= (uint64_t)((x >> b) & 1) << (3 * b);
+        m |= (uint64_t)((y >> b) & 1) << (3 * b + 1);
+        m |= (uint64_t)((z >> b) & 1) << (3 * b + 2);
     }
     return m;
 }
 
 int orc_morton_order(const double *xyz, int32_t n, int32_t grid, int32_t *perm, int32_t *vox)
 {
-    if (n < 1 || grid < 4 || grid > 1024) return -1;
+    if (n < 1 || grid < 4 || grid > ORC_MAX_GRID) return -1;
     orc_scale_to_voxels(xyz, n, grid, vox);
     uint64_t *key = xmalloc(sizeof(uint64_t) * n);
     for (int32_t i = 0; i < n; i++)
-        key[i] = ((uint64_t)morton30(vox[3 * i], vox[3 * i + 1], vox[3 * i + 2]) << 32) | (uint32_t)i;
+        key[i] = (morton33(vox[3 * i], vox[3 * i + 1], vox[3 * i + 2]) << 31) | (uint32_t)i;
     radix_u64(key, n, 64); /* (code, input index): a stable order by code */
-    for (int32_t k = 0; k < n; k++) perm[k] = (int32_t)(key[k] & 0xffffffffu);
+    for (int32_t k = 0; k < n; k++) perm[k] = (int32_t)(key[k] & 0x7fffffffu);
     free(key);
     return 0;
 }
@@ -184,15 +186,19 @@ static void envelope_1d(int g, const int32_t *cand_site, const int64_t *cand_d, 
     }
 }
 
-int orc_pba_codes(const int32_t *vox_sorted, int32_t n, int32_t g, int32_t *codes)
+/* Site coordinate c (0 x, 1 y, 2 z) of a grid label: a site code, or (by_id) a point index
+   into vox_sorted. */
+static inline int64_t site_coord(int32_t s, int c, const int32_t *vox_sorted, int by_id)
 {
-    if (g < 4 || g > 1024) return -1;
-    const int64_t G = g, G2 = G * G, G3 = G2 * G;
-    for (int64_t i = 0; i < G3; i++) codes[i] = MARKER;
-    for (int32_t i = 0; i < n; i++) {
-        const int x = vox_sorted[3 * i], y = vox_sorted[3 * i + 1], z = vox_sorted[3 * i + 2];
-        codes[(z * G + y) * G + x] = ENC(x, y, z);
-    }
+    if (by_id) return vox_sorted[3 * (int64_t)s + c];
+    return c == 0 ? DEC_X(s) : c == 1 ? DEC_Y(s) : DEC_Z(s);
+}
+
+/* The three sweeps over a grid of site labels (MARKER where empty).  Only site positions enter
+   the envelope decisions, so the result does not depend on how sites are labelled. */
+static void pba_sweeps(int32_t g, int32_t *codes, const int32_t *vox_sorted, int by_id)
+{
+    const int64_t G = g;
     int32_t *cs = xmalloc(sizeof(int32_t) * g), *out = xmalloc(sizeof(int32_t) * g);
     int64_t *cd = xmalloc(sizeof(int64_t) * g);
     int32_t *sq = xmalloc(sizeof(int32_t) * g), *ss = xmalloc(sizeof(int32_t) * g);
@@ -214,7 +220,7 @@ int orc_pba_codes(const int32_t *vox_sorted, int32_t n, int32_t g, int32_t *code
                 const int32_t s = codes[(z * G + y) * G + x];
                 cs[y] = s;
                 if (s != MARKER) {
-                    const int64_t dz = z - DEC_Z(s);
+                    const int64_t dz = z - site_coord(s, 2, vox_sorted, by_id);
                     cd[y] = dz * dz;
                 }
             }
@@ -228,7 +234,7 @@ int orc_pba_codes(const int32_t *vox_sorted, int32_t n, int32_t g, int32_t *code
                 const int32_t s = codes[(z * G + y) * G + x];
                 cs[x] = s;
                 if (s != MARKER) {
-                    const int64_t dy = y - DEC_Y(s), dz = z - DEC_Z(s);
+                    const int64_t dy = y - site_coord(s, 1, vox_sorted, by_id), dz = z - site_coord(s, 2, vox_sorted, by_id);
                     cd[x] = dy * dy + dz * dz;
                 }
             }
@@ -236,26 +242,34 @@ int orc_pba_codes(const int32_t *vox_sorted, int32_t n, int32_t g, int32_t *code
             for (int64_t x = 0; x < G; x++) codes[(z * G + y) * G + x] = out[x];
         }
     free(cs); free(out); free(cd); free(sq); free(ss); free(sd);
+}
+
+int orc_pba_codes(const int32_t *vox_sorted, int32_t n, int32_t g, int32_t *codes)
+{
+    if (g < 4 || g > 1024) return -1;
+    const int64_t G = g, G3 = G * G * G;
+    for (int64_t i = 0; i < G3; i++) codes[i] = MARKER;
+    for (int32_t i = 0; i < n; i++) {
+        const int x = vox_sorted[3 * i], y = vox_sorted[3 * i + 1], z = vox_sorted[3 * i + 2];
+        codes[(z * G + y) * G + x] = ENC(x, y, z);
+    }
+    pba_sweeps(g, codes, vox_sorted, 0);
     return 0;
 }
 
 int orc_pba(const int32_t *vox_sorted, int32_t n, int32_t g, int32_t *out_grid)
 {
+    if (g < 4 || g > ORC_MAX_GRID) return -1;
     const int64_t G = g, G3 = G * G * G;
-    if (orc_pba_codes(vox_sorted, n, g, out_grid) != 0) return -1;
     /* leader of a voxel = lowest Morton-sorted index of the points in it (the reference's
-       racy last-writer scatter GD/PBA/pba3DKernel.h:518-542 made deterministic) */
-    int32_t *leader = xmalloc(sizeof(int32_t) * G3);
-    for (int64_t i = 0; i < G3; i++) leader[i] = -1;
+       racy last-writer scatter GD/PBA/pba3DKernel.h:518-542 made deterministic); the sweeps
+       carry leaders as labels */
+    for (int64_t i = 0; i < G3; i++) out_grid[i] = MARKER;
     for (int32_t i = n - 1; i >= 0; i--) {
         const int64_t x = vox_sorted[3 * i], y = vox_sorted[3 * i + 1], z = vox_sorted[3 * i + 2];
-        leader[(z * G + y) * G + x] = i;
+        out_grid[(z * G + y) * G + x] = i;
     }
-    for (int64_t v = 0; v < G3; v++) {
-        const int32_t c = out_grid[v];
-        out_grid[v] = (c == MARKER) ? -1 : leader[((int64_t)DEC_Z(c) * G + DEC_Y(c)) * G + DEC_X(c)];
-    }
-    free(leader);
+    pba_sweeps(g, out_grid, vox_sorted, 1);
     return 0;
 }
 
@@ -601,7 +615,7 @@ int64_t orc_triangulate(const double *xyz, int32_t n, const orc_options *opt, in
     st.num_points = n;
     const int g = opt->grid_size;
     const int min_ws = opt->min_workset > 0 ? opt->min_workset : 8;
-    if (n < 5 || g < 4 || g > 1024) return -1;
+    if (n < 5 || g < 4 || g > ORC_MAX_GRID) return -1;
     orc_pred_counter_reset();
     double t0 = now_s();
 

@@ -82,7 +82,7 @@ def test_cofactor_filter_never_wrong(tri, orc):
 
 # ---------------------------------------------------------------- grid stages
 
-@pytest.mark.parametrize("g_,n", [(64, 5000), (128, 20000), (1024, 3000)])
+@pytest.mark.parametrize("g_,n", [(64, 5000), (128, 20000), (1024, 3000), (1100, 5000), (2048, 20000)])
 def test_morton_matches_oracle(tri, orc, g_, n):
     x = datasets.hard_mix(n, seed=2)
     pg, vg = tri.diag_morton(x, g_)
@@ -98,6 +98,7 @@ def test_pba_matches_oracle(tri, orc, g_, n):
     assert (tri.diag_pba(vs, g_, codes=True) == orc.pba(vs, g_, codes=True)).all()
     ids = tri.diag_pba(vs, g_)
     assert (ids == orc.pba(vs, g_)).all()
+    assert (tri.diag_pba(vs, g_, id_grids=True) == ids).all()  # the kernels grids above 1024 use
     pairs = tri.diag_grid_pairs(ids)
     assert (pairs == orc.grid_pairs(ids)).all()
 
@@ -118,7 +119,9 @@ def test_pba_deep_envelope_matches_oracle(tri, orc):
             code |= ((vox[:, c] >> b) & 1).astype(np.int64) << (3 * b + c)
     vs = vox[np.argsort(code, kind="stable")]
     assert (tri.diag_pba(vs, g_, codes=True) == orc.pba(vs, g_, codes=True)).all()
-    assert (tri.diag_pba(vs, g_) == orc.pba(vs, g_)).all()
+    ids = orc.pba(vs, g_)
+    assert (tri.diag_pba(vs, g_) == ids).all()
+    assert (tri.diag_pba(vs, g_, id_grids=True) == ids).all()
 
 
 # ---------------------------------------------------------------- full pipeline
@@ -148,6 +151,24 @@ def test_uniform_small(orc, n, grid):
     x = datasets.uniform_cube(n, seed=n)
     tets, opp, perm, st, rst = _parity(orc, x, grid)
     assert st["splay_rounds"] == rst["splay_rounds"]
+
+
+@pytest.mark.parametrize("n,grid", [(30000, 1100), (60000, 2048)])
+def test_wide_grid_same_triangulation(orc, n, grid):
+    """Grids above 1024 (point-id PBA grids, 33-bit Morton codes): points in general position
+    have one Delaunay triangulation, so the result equals the oracle's at a small grid."""
+    x = datasets.uniform_cube(n, seed=grid)
+    with g.Triangulator(grid_size=grid) as t:
+        tets, opp, perm = t.triangulate(x)
+        st = t.stats()
+        rep = t.check(full_empty_sphere=True)
+    assert st["grid_size"] == grid and st["sos_calls"] == 0
+    assert rep["euler"] == 1 and rep["num_tets"] == tets.shape[0]
+    assert rep["orientation_failures"] == rep["flat_tets"] == rep["insphere_failures"] == 0
+    assert rep["adjacency_failures"] == rep["empty_sphere_failures"] == 0
+    ref, _, rst = orc.triangulate(x, 32)
+    assert rst["sos_calls"] == 0
+    assert (g.canonical(tets) == ref).all()
 
 
 def test_orientation_and_adjacency(orc):
@@ -317,7 +338,7 @@ def test_errors_are_reported(orc):
         y[3, 1] = np.nan
         with pytest.raises(g.G4DError):
             t.triangulate(y)
-        t.set_grid(2000)
+        t.set_grid(4096)
         with pytest.raises(g.G4DError) as e:
             t.triangulate(datasets.uniform_cube(100, seed=3))
         assert e.value.status == "GRID_LIMIT"

@@ -12,9 +12,9 @@ from gstar4d_amd import datasets
 GOLD = os.path.join(os.path.dirname(__file__), "golden", "pipeline_golden.json")
 
 
-def _morton_np(vox):
+def _morton_np(vox, bits=10):
     code = np.zeros(vox.shape[0], np.int64)
-    for b in range(10):
+    for b in range(bits):
         for c in range(3):
             code |= ((vox[:, c] >> b) & 1).astype(np.int64) << (3 * b + c)
     return code
@@ -32,6 +32,38 @@ def test_morton_order_is_stable_sort_of_codes(orc):
     assert (vox == want_vox).all()
     code = _morton_np(want_vox)
     assert (perm == np.argsort(code, kind="stable")).all()
+
+
+def test_morton_order_on_wide_grids(orc):
+    """Grids above 1024 need an eleventh bit per axis; the order is still the stable sort of the
+    interleaved codes, and below 1024 the wider code orders exactly as the 10-bit one."""
+    rng = np.random.default_rng(4)
+    x = rng.random((3000, 3))
+    for g in (1100, 2048):
+        perm, vox = orc.morton_order(x, g)
+        mn, mx = x.min(), x.max()
+        want_vox = np.clip(((float(g - 3) * (x - mn)) / (mx - mn) + 1.0).astype(np.int64), 1, g - 2)
+        assert (vox == want_vox).all() and vox.max() >= 1024
+        assert (perm == np.argsort(_morton_np(want_vox, 11), kind="stable")).all()
+        assert not (perm == np.argsort(_morton_np(want_vox & 1023, 10), kind="stable")).all()
+    perm, vox = orc.morton_order(x, 1000)
+    assert (perm == np.argsort(_morton_np(vox, 10), kind="stable")).all()
+
+
+def test_pba_labels_do_not_change_the_diagram(orc):
+    """orc.pba sweeps point-index labels, orc.pba(codes=True) sweeps site codes; the id grid is
+    the code grid mapped through the voxel leaders (lowest Morton index in a voxel)."""
+    rng = np.random.default_rng(6)
+    for g, ns in ((24, 300), (40, 60)):
+        vox = rng.integers(0, g, size=(ns, 3))  # repeated voxels: several points per site
+        vs = vox[np.argsort(_morton_np(vox), kind="stable")]
+        ids = orc.pba(vs, g)
+        codes = orc.pba(vs, g, codes=True)
+        leader = {}
+        for i in range(vs.shape[0] - 1, -1, -1):
+            leader[(int(vs[i, 0]) << 20) | (int(vs[i, 1]) << 10) | int(vs[i, 2])] = i
+        want = np.vectorize(lambda c: leader[int(c)])(codes)
+        assert (ids == want).all()
 
 
 def test_pba_is_exact_nearest_site(orc):
