@@ -397,9 +397,13 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     // ---- D: missing points, grid pairs, worksets, pumping (GD/GDelaunay/GDelHost.cu:259-529) ----
     // Pairs are extracted once per unordered pair (min<<32 | max), deduplicated, then
     // reciprocated into directed workset keys (star<<32 | vertex) and grouped by star.
-    const int64_t nthr = tot.pair_threads;
-    HCK(ctx->pcount.ensure(sizeof(int64_t) * (size_t)(nthr + 1)));
+    // Raw pairs are counted per thread; each 256-thread tile then writes its pairs, deduplicated
+    // on chip, into its raw region at pr[ucap...], and the tiles are packed into pr[0...).
+    const int64_t nthr = tot.pair_threads, ntiles = nthr / 256;
+    HCK(ctx->pcount.ensure(sizeof(int64_t) * (size_t)(nthr + 1 + 2 * ntiles + 1)));
     int64_t *pcnt = buf<int64_t>(ctx->pcount);
+    int64_t *ttot = pcnt + nthr + 1;
+    uint2 *tcnt = reinterpret_cast<uint2 *>(ttot + ntiles + 1);
     HCK(g4d::launch_grid_pairs(idg, GS, nthr, pcnt, nullptr, false, s));
     HCK(g4d::scan_excl_i64(P, pcnt, pcnt, nthr, pcnt + nthr, s));
     int64_t ngp = 0;
@@ -408,14 +412,22 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     const int64_t ucap = ngp + n;
     HCK(ctx->pairs.ensure(sizeof(uint64_t) * (size_t)(2 * ucap + 2 * (int64_t)n * min_ws + 64)));
     uint64_t *pr = buf<uint64_t>(ctx->pairs);
-    HCK(g4d::launch_grid_pairs(idg, GS, nthr, pcnt, pr, true, s));
+    HCK(g4d::launch_grid_pairs_dedup(idg, GS, nthr, pcnt, pr + ucap, tcnt, s));
+    HCK(g4d::launch_tile_totals(tcnt, ntiles, ttot, s));
+    HCK(g4d::scan_excl_i64(P, ttot, ttot, ntiles, ttot + ntiles, s));
+    HCK(g4d::launch_gather_tiles(pr + ucap, pcnt, tcnt, ntiles, ttot, pr, s));
     HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8, s));
-    HCK(g4d::launch_missing(vox, n, GS, idg, pr + ngp, cnt + G4D_CNT_APPEND0, n, s));
+    HCK(g4d::launch_missing(vox, n, GS, idg, pr + ucap + ngp, cnt + G4D_CNT_APPEND0, n, s));
+    int64_t ngd = 0;
+    HCK(hipMemcpyAsync(&ngd, ttot + ntiles, 8, hipMemcpyDeviceToHost, s));
     CK(read_counters(ctx, hc));
     CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], "morton"));
     const int64_t nmiss = (int64_t)hc[G4D_CNT_APPEND0];
-    trace("n %d in %d set(s), grid %d: grid pairs %lld, missing %lld", n, S, gmax, (long long)ngp, (long long)nmiss);
-    int64_t u = ngp + nmiss;
+    if (nmiss > 0)
+        HCK(hipMemcpyAsync(pr + ngd, pr + ucap + ngp, sizeof(uint64_t) * (size_t)nmiss, hipMemcpyDeviceToDevice, s));
+    trace("n %d in %d set(s), grid %d: grid pairs %lld raw, %lld after tile dedup, missing %lld", n, S, gmax,
+          (long long)ngp, (long long)ngd, (long long)nmiss);
+    int64_t u = ngd + nmiss;
     HCK(sort_pairs(P, pr, u, nb, s));
     HCK(g4d::unique_u64(P, pr, u, &u, s));
     ctx->st.num_missing = nmiss;

@@ -140,3 +140,21 @@ __device__ __forceinline__ long long g4d_wave_append(bool pred, unsigned long lo
     base = __shfl(base, leader, 64);
     return pred ? (long long)(base + rank) : -1;
 }
+
+// Wave-aggregated append of cnt >= 0 items per lane: one atomic per wave; returns the first
+// slot of this lane's items.  Every lane of the wave must call it.
+__device__ __forceinline__ unsigned long long g4d_wave_append_n(unsigned cnt, unsigned long long *counter)
+{
+    const int lane = __lane_id();
+    unsigned inc = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const unsigned t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    const unsigned tot = __shfl(inc, 63, 64);
+    unsigned long long base = 0;
+    if (lane == 0 && tot > 0) base = atomicAdd(counter, (unsigned long long)tot);
+    base = __shfl(base, 0, 64);
+    return base + (inc - cnt);
+}

@@ -62,6 +62,14 @@ int grid_pair_chunk(int g);
 int64_t grid_pair_threads(int g, int zc);
 hipError_t launch_grid_pairs(const int *grid, const GridSets &S, int64_t nthreads, int64_t *count, uint64_t *out,
                              bool write, hipStream_t s);
+// Pass 1 with each 256-thread tile's pairs deduplicated on chip: tile b writes into its raw
+// region [off[256 b], off[256 (b + 1)]) and reports (distinct, overflow) counts in cnt[b];
+// launch_tile_totals + an exclusive scan give the dense offsets for launch_gather_tiles.
+hipError_t launch_grid_pairs_dedup(const int *grid, const GridSets &S, int64_t nthreads, const int64_t *off,
+                                   uint64_t *out, uint2 *cnt, hipStream_t s);
+hipError_t launch_tile_totals(const uint2 *cnt, int64_t ntiles, int64_t *tot, hipStream_t s);
+hipError_t launch_gather_tiles(const uint64_t *src, const int64_t *off, const uint2 *cnt, int64_t ntiles,
+                               const int64_t *doff, uint64_t *dst, hipStream_t s);
 hipError_t launch_csr_offsets(const uint64_t *keys, int64_t m, int n, int64_t *off, hipStream_t s);
 // Pumping; a star short of candidates after its neighbours' lists takes the lowest ids of its
 // own set.

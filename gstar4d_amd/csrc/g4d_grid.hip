@@ -283,25 +283,18 @@ __device__ __forceinline__ void site_pos_off(int s, const uint64_t *vox, int ca,
     }
 }
 
-// Global index of element q of column c in this workgroup's tile.
-template <int AXIS>
-__device__ __forceinline__ int64_t pba_gidx(int g, int fixed, int c0, int c, int q)
-{
-    const int64_t G = g;
-    if (AXIS == 2) return ((int64_t)q * G + fixed) * G + c0 + c;   // z=q, y=fixed, x=c0+c
-    if (AXIS == 1) return ((int64_t)fixed * G + q) * G + c0 + c;   // z=fixed, y=q, x=c0+c
-    return ((int64_t)fixed * G + c0 + c) * G + q;                  // z=fixed, y=c0+c, x=q
-}
-
 // One wave per tile of 64 columns along the sweep axis, one lane per column.  The column is
-// streamed through an LDS tile of 64 positions x 64 columns (coalesced loads and stores for
-// every axis), the lower envelope of the candidate sites (Maurer stack, exact integer
-// dominance test) is kept per lane in LDS -- entries beyond PBA_STACK spill to a per-column
-// region of `spill` -- and the colour walk writes the nearest site of every voxel back through
-// the same tile.  Ties: a later site along the axis wins (colour walk advances on <=).
+// streamed through an LDS tile of PBA_CHUNK positions x 64 columns (coalesced loads and stores
+// for every axis).  The lower envelope of the candidate sites (Maurer stack, exact integer
+// dominance test) keeps its two top entries in registers, so a candidate costs one stack read
+// per pop; the stack itself lives in LDS up to PBA_STACK entries, deeper entries in a per-block
+// region of `spill` laid out [depth][lane] so that a wave's accesses share cache lines.  At
+// 32 voxels per site a 512-long column's envelope is about 115 entries deep (at most ~140).
+// The colour walk writes the nearest site of every voxel back through the same tile.  Ties: a
+// later site along the axis wins (the walk advances on <=).
 static constexpr int PBA_COLS = 64;
-static constexpr int PBA_CHUNK = 64;
-static constexpr int PBA_STACK = 64;
+static constexpr int PBA_CHUNK = 32;
+static constexpr int PBA_STACK = 120;
 
 struct PbaLds {
     int tile[PBA_CHUNK][PBA_COLS + 1];
@@ -315,6 +308,42 @@ __device__ __forceinline__ int64_t pba_gidx64(int g, int fixed, int c, int q)
     if (AXIS == 2) return ((int64_t)q * G + fixed) * G + c;   // z = q, y = fixed, x = c
     if (AXIS == 1) return ((int64_t)fixed * G + q) * G + c;   // z = fixed, y = q, x = c
     return ((int64_t)fixed * G + c) * G + q;                  // z = fixed, y = c, x = q
+}
+
+// Tile element k of a lane: column cc and position qq (position fastest for the X pass, whose
+// positions are contiguous in memory; column fastest otherwise).
+static constexpr int PBA_PER_LANE = PBA_CHUNK * PBA_COLS / 64;
+template <int AXIS>
+__device__ __forceinline__ int pba_tile_c(int k)
+{
+    const int i = (int)threadIdx.x + 64 * k;
+    return AXIS == 0 ? i / PBA_CHUNK : i % PBA_COLS;
+}
+template <int AXIS>
+__device__ __forceinline__ int pba_tile_q(int k)
+{
+    const int i = (int)threadIdx.x + 64 * k;
+    return AXIS == 0 ? i % PBA_CHUNK : i / PBA_COLS;
+}
+
+// Issues every load of a chunk before any is consumed.
+template <int AXIS>
+__device__ __forceinline__ void pba_load_chunk(int (&pre)[PBA_PER_LANE], const int *in, int g, int fixed, int c0,
+                                               int ncols, int r0)
+{
+    const int nq = min(PBA_CHUNK, g - r0);
+#pragma unroll
+    for (int k = 0; k < PBA_PER_LANE; k++) {
+        const int cc = pba_tile_c<AXIS>(k), qq = pba_tile_q<AXIS>(k);
+        pre[k] = (qq < nq && cc < ncols) ? in[pba_gidx64<AXIS>(g, fixed, c0 + cc, r0 + qq)] : G4D_MARKER;
+    }
+}
+
+// Stack entry below the LDS part.  Kept out of line: inlined next to the LDS read, the
+// compiler (ROCm 7.2, gfx950) emits an illegal operand for the unrolled tile loops.
+__device__ __attribute__((noinline)) int pba_spill_get(const int *sp, int k)
+{
+    return sp[(int64_t)(k - PBA_STACK) * PBA_COLS];
 }
 
 template <int AXIS, bool IDS>
@@ -336,35 +365,44 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
     const bool active = lane < ncols;
     const int c = c0 + lane;
     const int ca = c, cb = fixed; // x (Z, Y passes) or y (X pass); y (Z), z (Y), z (X)
-    int *sp = spill_all + S.spill[st] + (lb * PBA_COLS + lane) * g;
+    int *sp = spill_all + S.spill[st] + lb * PBA_COLS * g + lane; // entry k >= PBA_STACK at (k - PBA_STACK) * 64
+#define get(k) ((k) < PBA_STACK ? L.scode[(k)][lane] : pba_spill_get(sp, (k)))
 
-    // ---- lower envelope ----
+    // Tile element k of this lane: (cc, qq) with every load of a chunk issued before the first
+    // is consumed, and the next chunk's loads in flight while the current one is processed.
+    int pre[PBA_PER_LANE];
+
+    // ---- lower envelope; (q0, d0) = entry m-2, (q1, d1) = entry m-1 ----
     int m = 0;
-    for (int q0 = 0; q0 < g; q0 += PBA_CHUNK) {
-        const int nq = min(PBA_CHUNK, g - q0);
-        for (int i = lane; i < PBA_CHUNK * PBA_COLS; i += 64) {
-            int cc, qq;
-            if (AXIS == 0) { cc = i / PBA_CHUNK; qq = i % PBA_CHUNK; } else { qq = i / PBA_COLS; cc = i % PBA_COLS; }
-            if (qq < nq && cc < ncols) L.tile[qq][cc] = in[pba_gidx64<AXIS>(g, fixed, c0 + cc, q0 + qq)];
+    int q0 = 0, d0 = 0, q1 = 0, d1 = 0;
+    pba_load_chunk<AXIS>(pre, in, g, fixed, c0, ncols, 0);
+    for (int r0 = 0; r0 < g; r0 += PBA_CHUNK) {
+        const int nq = min(PBA_CHUNK, g - r0);
+#pragma unroll
+        for (int k = 0; k < PBA_PER_LANE; k++) {
+            const int cc = pba_tile_c<AXIS>(k), qq = pba_tile_q<AXIS>(k);
+            L.tile[qq][cc] = pre[k];
         }
         __syncthreads();
+        if (r0 + PBA_CHUNK < g) pba_load_chunk<AXIS>(pre, in, g, fixed, c0, ncols, r0 + PBA_CHUNK);
         if (active) {
             for (int qq = 0; qq < nq; qq++) {
                 const int s = L.tile[qq][lane];
                 if (s == G4D_MARKER) continue;
                 int qs, ds;
                 site_pos_off<AXIS, IDS>(s, vox, ca, cb, &qs, &ds);
-                while (m >= 2) {
-                    int qu, qv, du, dv;
-                    const int su = m - 2 < PBA_STACK ? L.scode[m - 2][lane] : sp[m - 2];
-                    const int sv = m - 1 < PBA_STACK ? L.scode[m - 1][lane] : sp[m - 1];
-                    site_pos_off<AXIS, IDS>(su, vox, ca, cb, &qu, &du);
-                    site_pos_off<AXIS, IDS>(sv, vox, ca, cb, &qv, &dv);
-                    if (env_dominated(qu, du, qv, dv, qs, ds)) m--;
-                    else break;
+                while (m >= 2 && env_dominated(q0, d0, q1, d1, qs, ds)) {
+                    m--;
+                    q1 = q0;
+                    d1 = d0;
+                    if (m >= 2) site_pos_off<AXIS, IDS>(get(m - 2), vox, ca, cb, &q0, &d0);
                 }
                 if (m < PBA_STACK) L.scode[m][lane] = s;
-                else sp[m] = s;
+                else sp[(int64_t)(m - PBA_STACK) * PBA_COLS] = s;
+                q0 = q1;
+                d0 = d1;
+                q1 = qs;
+                d1 = ds;
                 m++;
             }
         }
@@ -377,10 +415,7 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
         sa = L.scode[0][lane];
         site_pos_off<AXIS, IDS>(sa, vox, ca, cb, &qa, &da);
     }
-    if (active && m > 1) {
-        const int sb = 1 < PBA_STACK ? L.scode[1][lane] : sp[1];
-        site_pos_off<AXIS, IDS>(sb, vox, ca, cb, &qb, &db);
-    }
+    if (active && m > 1) site_pos_off<AXIS, IDS>(get(1), vox, ca, cb, &qb, &db);
     for (int t0 = 0; t0 < g; t0 += PBA_CHUNK) {
         const int nt = min(PBA_CHUNK, g - t0);
         if (active) {
@@ -397,11 +432,8 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
                         idx++;
                         qa = qb;
                         da = db;
-                        sa = idx < PBA_STACK ? L.scode[idx][lane] : sp[idx];
-                        if (idx + 1 < m) {
-                            const int sb = idx + 1 < PBA_STACK ? L.scode[idx + 1][lane] : sp[idx + 1];
-                            site_pos_off<AXIS, IDS>(sb, vox, ca, cb, &qb, &db);
-                        }
+                        sa = get(idx);
+                        if (idx + 1 < m) site_pos_off<AXIS, IDS>(get(idx + 1), vox, ca, cb, &qb, &db);
                     } else {
                         break;
                     }
@@ -410,19 +442,22 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
             }
         }
         __syncthreads();
-        for (int i = lane; i < PBA_CHUNK * PBA_COLS; i += 64) {
-            int cc, qq;
-            if (AXIS == 0) { cc = i / PBA_CHUNK; qq = i % PBA_CHUNK; } else { qq = i / PBA_COLS; cc = i % PBA_COLS; }
-            if (qq < nt && cc < ncols) {
-                int v = L.tile[qq][cc];
-                if (!IDS && idgrid)
-                    v = (v == G4D_MARKER) ? -1
-                                          : idgrid[((int64_t)G4D_DEC_Z(v) * g + G4D_DEC_Y(v)) * g + G4D_DEC_X(v)];
-                out[pba_gidx64<AXIS>(g, fixed, c0 + cc, t0 + qq)] = v;
-            }
+#pragma unroll
+        for (int k = 0; k < PBA_PER_LANE; k++) {
+            const int cc = pba_tile_c<AXIS>(k), qq = pba_tile_q<AXIS>(k);
+            int v = L.tile[qq][cc];
+            if (!IDS && idgrid && qq < nt && cc < ncols)
+                v = (v == G4D_MARKER) ? -1 : idgrid[((int64_t)G4D_DEC_Z(v) * g + G4D_DEC_Y(v)) * g + G4D_DEC_X(v)];
+            pre[k] = v;
+        }
+#pragma unroll
+        for (int k = 0; k < PBA_PER_LANE; k++) {
+            const int cc = pba_tile_c<AXIS>(k), qq = pba_tile_q<AXIS>(k);
+            if (qq < nt && cc < ncols) out[pba_gidx64<AXIS>(g, fixed, c0 + cc, t0 + qq)] = pre[k];
         }
         __syncthreads();
     }
+#undef get
 }
 
 int64_t pba_blocks(int g) { return (int64_t)g * ((g + PBA_COLS - 1) / PBA_COLS); }
@@ -524,8 +559,28 @@ __device__ __forceinline__ uint32_t cube_pairs(const int *v, uint64_t *pr)
 
 __host__ __device__ inline int grid_pair_chunk_dev(int g) { return g > 512 ? 32 : 16; }
 
-// One thread per (x, y) column of cubes and a chunk of zc cubes along z; pass 0 counts the
-// thread's pairs, pass 1 writes them at the scanned offset (deterministic order, no atomics).
+// Threads are laid out in tiles of GP_TILE x GP_TILE (x, y) columns of cubes (x, y from -1,
+// the outer Marker layer) and a chunk of zc cubes along z; a set's thread count is a whole
+// number of tiles, so a 256-thread work-group never spans two sets.
+static constexpr int GP_TILE = 16;
+static constexpr int GP_THREADS = GP_TILE * GP_TILE;
+
+__device__ __forceinline__ bool gp_coords(int64_t lt, int g, int zc, int *x, int *y, int *z0, int *z1)
+{
+    const int w = g + 1;
+    const int ntx = (w + GP_TILE - 1) / GP_TILE;
+    const int64_t tile = lt / GP_THREADS;
+    const int t = (int)(lt % GP_THREADS);
+    *x = (int)(tile % ntx) * GP_TILE + t % GP_TILE - 1;
+    *y = (int)((tile / ntx) % ntx) * GP_TILE + t / GP_TILE - 1;
+    const int zb = (int)(tile / ((int64_t)ntx * ntx));
+    *z0 = zb * zc - 1;
+    *z1 = min(*z0 + zc, g);
+    return *x < g && *y < g;
+}
+
+// Pass 0 counts each thread's raw pairs, pass 1 writes them at the scanned offset
+// (deterministic order, no atomics).
 template <bool WRITE>
 __global__ void __launch_bounds__(256) k_grid_pairs(const int *grid_all, GridSets S, int64_t nthr, int64_t *count,
                                                     uint64_t *out)
@@ -535,13 +590,11 @@ __global__ void __launch_bounds__(256) k_grid_pairs(const int *grid_all, GridSet
         const int64_t lt = tid - S.pair_thr[st];
         const int g = S.g[st];
         const int *grid = grid_all + S.vox_off[st];
-        const int zc = grid_pair_chunk_dev(g);
-        const int w = g + 1;
-        const int64_t ncol = (int64_t)w * w;
-        const int64_t col = lt % ncol;
-        const int zb = (int)(lt / ncol);
-        const int x = (int)(col % w) - 1, y = (int)(col / w) - 1;
-        const int z0 = zb * zc - 1, z1 = min(z0 + zc, g);
+        int x, y, z0, z1;
+        if (!gp_coords(lt, g, grid_pair_chunk_dev(g), &x, &y, &z0, &z1)) {
+            if (!WRITE) count[tid] = 0;
+            continue;
+        }
         int v[8];
         v[0] = grid_at(grid, g, x, y, z0);
         v[1] = grid_at(grid, g, x + 1, y, z0);
@@ -571,6 +624,122 @@ __global__ void __launch_bounds__(256) k_grid_pairs(const int *grid_all, GridSet
     }
 }
 
+// Pass 1 with the work-group's pairs deduplicated in an LDS hash set: one work-group per tile,
+// whose raw pairs have the region [off[first thread], off[first thread of the next tile]) of
+// `out`.  Distinct pairs go to the bottom of the region in table order; a pair that finds no
+// free slot within GP_PROBES probes is written as is from the top of the region down.  Either
+// way the region holds at most the tile's raw count.  cnt[tile] = (table count, overflow count).
+// Adjacent columns see the same Voronoi faces, so a 16 x 16 x zc tile keeps roughly one pair in
+// four; the global sort and unique that follow remove what remains duplicated across tiles.
+static constexpr int GP_HASH = 4096;
+static constexpr int GP_PROBES = 32;
+static constexpr uint64_t GP_EMPTY = ~0ull;
+
+__global__ void __launch_bounds__(GP_THREADS) k_grid_pairs_dedup(const int *grid_all, GridSets S, const int64_t *off,
+                                                                 uint64_t *out, uint2 *cnt)
+{
+    __shared__ unsigned long long tab[GP_HASH];
+    __shared__ unsigned int nover;
+    __shared__ unsigned int wsum[GP_THREADS / 64];
+    for (int i = threadIdx.x; i < GP_HASH; i += GP_THREADS) tab[i] = GP_EMPTY;
+    if (threadIdx.x == 0) nover = 0;
+    __syncthreads();
+    const int64_t tid = (int64_t)blockIdx.x * GP_THREADS + threadIdx.x;
+    const int st = S.nsets == 1 ? 0 : set_of_work(S.pair_thr, S.nsets, tid);
+    const int64_t lt = tid - S.pair_thr[st];
+    const int g = S.g[st];
+    const int *grid = grid_all + S.vox_off[st];
+    const int64_t rbeg = off[(int64_t)blockIdx.x * GP_THREADS];
+    const int64_t rend = off[(int64_t)(blockIdx.x + 1) * GP_THREADS];
+    int x, y, z0, z1;
+    if (gp_coords(lt, g, grid_pair_chunk_dev(g), &x, &y, &z0, &z1)) {
+        int v[8];
+        v[0] = grid_at(grid, g, x, y, z0);
+        v[1] = grid_at(grid, g, x + 1, y, z0);
+        v[2] = grid_at(grid, g, x, y + 1, z0);
+        v[3] = grid_at(grid, g, x + 1, y + 1, z0);
+        uint64_t pr[31];
+        for (int z = z0; z < z1; z++) {
+            v[4] = grid_at(grid, g, x, y, z + 1);
+            v[5] = grid_at(grid, g, x + 1, y, z + 1);
+            v[6] = grid_at(grid, g, x, y + 1, z + 1);
+            v[7] = grid_at(grid, g, x + 1, y + 1, z + 1);
+            const uint32_t mask = cube_pairs(v, pr);
+            if (!mask) goto next_cube;
+            // static slot indices keep pr[] in registers
+#pragma unroll
+            for (int j = 0; j < 31; j++) {
+                if (!(mask & (1u << j))) continue;
+                const uint64_t key = pr[j];
+                uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 52);
+                // a plain read settles repeats (most raw pairs) without an atomic
+                bool done = false;
+                for (int p = 0; p < GP_PROBES && !done; p++) {
+                    unsigned long long cur = ((volatile unsigned long long *)tab)[h];
+                    if (cur == GP_EMPTY) cur = atomicCAS(&tab[h], GP_EMPTY, (unsigned long long)key);
+                    done = cur == GP_EMPTY || cur == key;
+                    h = (h + 1) & (GP_HASH - 1);
+                }
+                if (!done) out[rend - 1 - atomicAdd(&nover, 1u)] = key;
+            }
+        next_cube:
+            v[0] = v[4];
+            v[1] = v[5];
+            v[2] = v[6];
+            v[3] = v[7];
+        }
+    }
+    __syncthreads();
+    // compact the table: GP_HASH / GP_THREADS slots per thread, block-wide exclusive scan
+    constexpr int PER = GP_HASH / GP_THREADS;
+    unsigned long long e[PER];
+    uint32_t c = 0;
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+        e[i] = tab[i * GP_THREADS + threadIdx.x];
+        c += e[i] != GP_EMPTY;
+    }
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    uint32_t inc = c;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += t;
+    }
+    if (lane == 63) wsum[wid] = inc;
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < GP_THREADS / 64; w++) {
+        if (w < wid) base += wsum[w];
+        total += wsum[w];
+    }
+    uint64_t *dst = out + rbeg + base + inc - c;
+#pragma unroll
+    for (int i = 0; i < PER; i++)
+        if (e[i] != GP_EMPTY) *dst++ = e[i];
+    if (threadIdx.x == 0) cnt[blockIdx.x] = make_uint2(total, nover);
+}
+
+// Packs every tile's table pairs and overflow pairs densely: tile b's pairs go to
+// dst + doff[b], doff being the exclusive scan of the tiles' total counts.
+__global__ void __launch_bounds__(256) k_gather_tiles(const uint64_t *src, const int64_t *off, const uint2 *cnt,
+                                                      const int64_t *doff, uint64_t *dst)
+{
+    const int64_t b = blockIdx.x;
+    const int64_t rbeg = off[b * GP_THREADS], rend = off[(b + 1) * GP_THREADS];
+    const uint2 c = cnt[b];
+    uint64_t *d = dst + doff[b];
+    for (uint32_t i = threadIdx.x; i < c.x; i += blockDim.x) d[i] = src[rbeg + i];
+    for (uint32_t i = threadIdx.x; i < c.y; i += blockDim.x) d[c.x + i] = src[rend - c.y + i];
+}
+
+__global__ void k_tile_totals(const uint2 *cnt, int64_t nb, int64_t *tot)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nb; i += (int64_t)gridDim.x * blockDim.x)
+        tot[i] = (int64_t)cnt[i].x + cnt[i].y;
+}
+
 // CSR offsets of sorted directed keys: off[s] = first index with star >= s (off has n+1).
 __global__ void k_csr_offsets(const uint64_t *keys, int64_t m, int n, int64_t *off)
 {
@@ -598,33 +767,36 @@ __device__ __forceinline__ bool list_has(const uint64_t *keys, int64_t b, int64_
 __global__ void k_pump(const uint64_t *keys, const int64_t *off, int n, GridSets S, int min_ws, uint64_t *out,
                        unsigned long long *counter, int64_t cap)
 {
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < n; s += gridDim.x * blockDim.x) {
-        const int64_t b = off[s], e = off[s + 1];
-        const int have = (int)(e - b);
-        if (have >= min_ws) continue;
-        const int need = min_ws - have;
+    // the loop bound is uniform per work-group, so whole waves reach the aggregated append
+    for (int s0 = blockIdx.x * blockDim.x; s0 < n; s0 += gridDim.x * blockDim.x) {
+        const int s = s0 + threadIdx.x;
         int got[64];
         int ng = 0;
-        for (int64_t k = b; k < e && ng < need; k++) {
-            const int nei = (int)(uint32_t)keys[k];
-            for (int64_t q = off[nei]; q < off[nei + 1] && ng < need; q++) {
-                const int cv = (int)(uint32_t)keys[q];
+        const int64_t b = s < n ? off[s] : 0, e = s < n ? off[s + 1] : 0;
+        const int have = (int)(e - b);
+        if (s < n && have < min_ws) {
+            const int need = min_ws - have;
+            for (int64_t k = b; k < e && ng < need; k++) {
+                const int nei = (int)(uint32_t)keys[k];
+                for (int64_t q = off[nei]; q < off[nei + 1] && ng < need; q++) {
+                    const int cv = (int)(uint32_t)keys[q];
+                    if (cv == s || list_has(keys, b, e, cv)) continue;
+                    bool dup = false;
+                    for (int r = 0; r < ng; r++) dup |= (got[r] == cv);
+                    if (!dup) got[ng++] = cv;
+                }
+            }
+            const int st = set_of(S, s);
+            for (int cv = (int)S.pt_off[st]; ng < need && cv < (int)S.pt_off[st + 1]; cv++) {
                 if (cv == s || list_has(keys, b, e, cv)) continue;
                 bool dup = false;
                 for (int r = 0; r < ng; r++) dup |= (got[r] == cv);
                 if (!dup) got[ng++] = cv;
             }
         }
-        const int st = set_of(S, s);
-        for (int cv = (int)S.pt_off[st]; ng < need && cv < (int)S.pt_off[st + 1]; cv++) {
-            if (cv == s || list_has(keys, b, e, cv)) continue;
-            bool dup = false;
-            for (int r = 0; r < ng; r++) dup |= (got[r] == cv);
-            if (!dup) got[ng++] = cv;
-        }
+        const unsigned long long slot = g4d_wave_append_n((unsigned)ng, counter);
         for (int r = 0; r < ng; r++) {
-            const unsigned long long slot = atomicAdd(counter, 1ull);
-            const int64_t k = 2 * (int64_t)slot;
+            const int64_t k = 2 * (int64_t)(slot + r);
             if (k + 1 < cap) {
                 out[k] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)got[r];
                 out[k + 1] = ((uint64_t)(uint32_t)got[r] << 32) | (uint32_t)s;
@@ -702,8 +874,33 @@ int grid_pair_chunk(int g) { return grid_pair_chunk_dev(g); }
 
 int64_t grid_pair_threads(int g, int zc)
 {
-    const int64_t w = g + 1;
-    return w * w * ((w + zc - 1) / zc);
+    const int64_t w = g + 1, ntx = (w + GP_TILE - 1) / GP_TILE;
+    return ntx * ntx * ((w + zc - 1) / zc) * GP_THREADS;
+}
+
+hipError_t launch_grid_pairs_dedup(const int *grid, const GridSets &S, int64_t nthr, const int64_t *off, uint64_t *out,
+                                   uint2 *cnt, hipStream_t s)
+{
+    if (nthr <= 0) return hipSuccess;
+    if (nthr % GP_THREADS || nthr / GP_THREADS > INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_grid_pairs_dedup, dim3((unsigned)(nthr / GP_THREADS)), dim3(GP_THREADS), 0, s, grid, S, off, out,
+                       cnt);
+    return hipGetLastError();
+}
+
+hipError_t launch_tile_totals(const uint2 *cnt, int64_t ntiles, int64_t *tot, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_tile_totals, dim3(g4d_blocks(ntiles, 256, 65536)), dim3(256), 0, s, cnt, ntiles, tot);
+    return hipGetLastError();
+}
+
+hipError_t launch_gather_tiles(const uint64_t *src, const int64_t *off, const uint2 *cnt, int64_t ntiles,
+                               const int64_t *doff, uint64_t *dst, hipStream_t s)
+{
+    if (ntiles <= 0) return hipSuccess;
+    if (ntiles > INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_gather_tiles, dim3((unsigned)ntiles), dim3(256), 0, s, src, off, cnt, doff, dst);
+    return hipGetLastError();
 }
 
 hipError_t launch_grid_pairs(const int *grid, const GridSets &S, int64_t nthr, int64_t *count, uint64_t *out,

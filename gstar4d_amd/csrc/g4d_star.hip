@@ -218,20 +218,31 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
     const int lane = threadIdx.x;
     unsigned long long *cnt = p.cnt;
 
-    // Work queue in chunks: one atomic per STAR_CHUNK stars for the filter-only kernels (many
-    // small stars), per star for the exact ones (uneven cost).
-    constexpr int CHUNK = EXACT ? 1 : STAR_CHUNK;
     if (p.nlist_dev) {
         const unsigned long long c = *p.nlist_dev;
         nitems = c < (unsigned long long)nitems ? (int)c : nitems;
     }
+    // A list no longer than the grid is spread one star per work-group, without the queue;
+    // work-groups beyond its length leave at once (thousands of idle work-groups contending for
+    // the queue counter cost ~0.3 ms).  Longer lists go through the work queue in chunks: one
+    // atomic per STAR_CHUNK stars for the filter-only kernels (many small stars), per star for
+    // the exact ones (uneven cost).
+    constexpr int CHUNK = EXACT ? 1 : STAR_CHUNK;
+    const bool one_each = nitems <= (int)gridDim.x;
+    if ((int)blockIdx.x >= nitems) return;
+    bool first = true;
     int nfilt = 0, ncof = 0; // this lane's cofactor-filter evaluations and cofactor computations
     for (;;) {
         int base = 0;
-        if (lane == 0) base = (int)atomicAdd(&cnt[G4D_CNT_WORK], (unsigned long long)CHUNK);
-        base = __shfl(base, 0, 64);
+        if (one_each) {
+            base = first ? (int)blockIdx.x : nitems;
+            first = false;
+        } else {
+            if (lane == 0) base = (int)atomicAdd(&cnt[G4D_CNT_WORK], (unsigned long long)CHUNK);
+            base = __shfl(base, 0, 64);
+        }
         if (base >= nitems) break;
-        const int bend = base + CHUNK < nitems ? base + CHUNK : nitems;
+        const int bend = one_each ? base + 1 : (base + CHUNK < nitems ? base + CHUNK : nitems);
 #pragma unroll 1
         for (int item = base; item < bend; item++) {
             const int s = p.star_list ? p.star_list[item] : item;
