@@ -203,9 +203,95 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t *keys,
     }
 }
 
+// Lists up to SMALL_SORT keys are sorted by one work-group in a single launch: per 8-bit
+// digit a histogram sweep and a scatter sweep over tiles of 1024 keys in order (ballot-matched
+// ranks within a wave, wave offsets per digit, running digit offsets), ping-ponging between
+// keys and tmp; the result ends in keys.
+static constexpr int64_t SMALL_SORT = 16384;
+
+__global__ void __launch_bounds__(1024) k_sort_small(uint64_t *keys, uint64_t *tmp, int n, int lo_bit, int hi_bit)
+{
+    __shared__ uint32_t goff[256];
+    __shared__ uint32_t wcnt[16][256];
+    __shared__ uint32_t part[4];
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint64_t *src = keys, *dst = tmp;
+    for (int shift = lo_bit; shift < hi_bit; shift += 8) {
+        if (threadIdx.x < 256) goff[threadIdx.x] = 0;
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += 1024) atomicAdd(&goff[(src[i] >> shift) & 255], 1u);
+        __syncthreads();
+        // exclusive scan of the 256 digit counts (4 waves)
+        uint32_t c = 0, inc = 0;
+        if (threadIdx.x < 256) {
+            c = goff[threadIdx.x];
+            inc = c;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const uint32_t t = __shfl_up(inc, o, 64);
+                if (lane >= o) inc += t;
+            }
+            if (lane == 63) part[wid] = inc;
+        }
+        __syncthreads();
+        if (threadIdx.x < 256) {
+            uint32_t b = 0;
+            for (int w = 0; w < wid; w++) b += part[w];
+            goff[threadIdx.x] = b + inc - c;
+        }
+        for (int t0 = 0; t0 < n; t0 += 1024) {
+            for (int i = threadIdx.x; i < 16 * 256; i += 1024) (&wcnt[0][0])[i] = 0;
+            __syncthreads();
+            const int k = t0 + threadIdx.x;
+            const bool valid = k < n;
+            const uint64_t key = valid ? src[k] : 0ull;
+            const int d = valid ? (int)((key >> shift) & 255) : 256;
+            uint64_t peers = __ballot(1);
+#pragma unroll
+            for (int b = 0; b < 9; b++) {
+                const bool bit = (d >> b) & 1;
+                const uint64_t bal = __ballot(bit);
+                peers &= bit ? bal : ~bal;
+            }
+            const uint32_t rank = (uint32_t)__popcll(peers & lt);
+            if (valid && (peers & lt) == 0) wcnt[wid][d] = (uint32_t)__popcll(peers);
+            __syncthreads();
+            if (threadIdx.x < 256) { // exclusive prefix over waves per digit; advance the digit offset
+                uint32_t r = 0;
+#pragma unroll
+                for (int w = 0; w < 16; w++) {
+                    const uint32_t t = wcnt[w][threadIdx.x];
+                    wcnt[w][threadIdx.x] = r;
+                    r += t;
+                }
+                c = r; // this tile's count of the digit
+            }
+            __syncthreads();
+            if (valid) dst[goff[d] + wcnt[wid][d] + rank] = key;
+            __syncthreads();
+            if (threadIdx.x < 256) goff[threadIdx.x] += c;
+        }
+        __syncthreads();
+        uint64_t *t = src;
+        src = dst;
+        dst = t;
+    }
+    if (src != keys) {
+        __syncthreads();
+        for (int i = threadIdx.x; i < n; i += 1024) keys[i] = src[i];
+    }
+}
+
 hipError_t radix_sort_u64(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int hi_bit, hipStream_t s)
 {
     if (n <= 1 || hi_bit <= lo_bit) return hipSuccess;
+    if (n <= SMALL_SORT) {
+        HIP_CHECK_RET(P.sort_tmp_keys.ensure(sizeof(uint64_t) * (size_t)n));
+        hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(1024), 0, s, keys, P.sort_tmp_keys.as<uint64_t>(), (int)n, lo_bit,
+                           hi_bit);
+        return hipGetLastError();
+    }
     const int64_t nb64 = (n + RS_TILE - 1) / RS_TILE;
     if (nb64 > (1 << 30)) return hipErrorInvalidValue;
     const int nb = (int)nb64;
@@ -258,6 +344,63 @@ __global__ void k_scatter_flagged(const uint64_t *in, const uint32_t *flags, con
         if (flags[i]) out[pos[i]] = in[i];
 }
 
+// Lists up to SMALL_COMPACT entries are compacted by one work-group in a single launch
+// (tiles of 1024 in order, ballot ranks, a running offset), which also works in place: an
+// entry kept from index j lands at a position <= j, and later tiles read beyond every
+// position written so far.  MODE 0: adjacent-unique, 1: keep[i] != 0, 2: in[i] != none.
+static constexpr int64_t SMALL_COMPACT = 32768;
+
+template <int MODE>
+__global__ void __launch_bounds__(1024) k_compact_small(const uint64_t *in, const uint8_t *keep, uint64_t none,
+                                                        uint64_t *out, int64_t n, uint32_t *count)
+{
+    __shared__ uint32_t wsum[16];
+    __shared__ uint64_t last;
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    uint32_t run = 0;
+    for (int64_t t0 = 0; t0 < n; t0 += 1024) {
+        const int64_t i = t0 + threadIdx.x;
+        uint64_t v = 0;
+        bool f = false;
+        if (i < n) {
+            v = in[i];
+            if (MODE == 0) f = i == 0 || v != (threadIdx.x == 0 ? last : in[i - 1]);
+            else if (MODE == 1) f = keep[i] != 0;
+            else f = v != none;
+        }
+        const uint64_t m = __ballot(f);
+        if (lane == 0) wsum[wid] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < 16; w++) {
+            const uint32_t c = wsum[w];
+            if (w < wid) before += c;
+            tot += c;
+        }
+        if (f) out[run + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
+        if (MODE == 0 && threadIdx.x == 1023) last = v;
+        run += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *count = run;
+}
+
+template <int MODE>
+static hipError_t compact_small(Prims &P, const uint64_t *in, const uint8_t *keep, uint64_t none, uint64_t *out,
+                                int64_t n, int64_t *n_out, hipStream_t s)
+{
+    HIP_CHECK_RET(P.pos.ensure(sizeof(uint32_t) * 4));
+    uint32_t *dcount = P.pos.as<uint32_t>();
+    hipLaunchKernelGGL(k_compact_small<MODE>, dim3(1), dim3(1024), 0, s, in, keep, none, out, n, dcount);
+    HIP_CHECK_RET(hipGetLastError());
+    uint32_t c = 0;
+    HIP_CHECK_RET(hipMemcpyAsync(&c, dcount, 4, hipMemcpyDeviceToHost, s));
+    HIP_CHECK_RET(hipStreamSynchronize(s));
+    *n_out = c;
+    return hipSuccess;
+}
+
 static hipError_t compact_with_flags(Prims &P, const uint64_t *in, uint64_t *out, int64_t n, int64_t *n_out,
                                      hipStream_t s)
 {
@@ -292,6 +435,7 @@ hipError_t unique_u64(Prims &P, uint64_t *keys, int64_t n, int64_t *n_out, hipSt
         *n_out = n;
         return hipSuccess;
     }
+    if (n <= SMALL_COMPACT) return compact_small<0>(P, keys, nullptr, 0, keys, n, n_out, s);
     HIP_CHECK_RET(P.flags.ensure(sizeof(uint32_t) * (size_t)n));
     HIP_CHECK_RET(P.pos.ensure(sizeof(uint32_t) * (size_t)n));
     hipLaunchKernelGGL(k_unique_flags, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, keys, n, P.flags.as<uint32_t>());
@@ -306,6 +450,7 @@ hipError_t compact_u64(Prims &P, const uint64_t *in, const uint8_t *keep, uint64
         *n_out = 0;
         return hipSuccess;
     }
+    if (n <= SMALL_COMPACT) return compact_small<1>(P, in, keep, 0, out, n, n_out, s);
     HIP_CHECK_RET(P.flags.ensure(sizeof(uint32_t) * (size_t)n));
     HIP_CHECK_RET(P.pos.ensure(sizeof(uint32_t) * (size_t)n));
     hipLaunchKernelGGL(k_keep_flags, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, keep, n, P.flags.as<uint32_t>());
@@ -320,6 +465,7 @@ hipError_t compact_valid_u64(Prims &P, const uint64_t *in, uint64_t *out, int64_
         *n_out = 0;
         return hipSuccess;
     }
+    if (n <= SMALL_COMPACT) return compact_small<2>(P, in, nullptr, none, out, n, n_out, s);
     HIP_CHECK_RET(P.flags.ensure(sizeof(uint32_t) * (size_t)n));
     HIP_CHECK_RET(P.pos.ensure(sizeof(uint32_t) * (size_t)n));
     hipLaunchKernelGGL(k_valid_flags, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, in, n, none,
