@@ -427,14 +427,15 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         HCK(hipMemcpyAsync(pr + ngd, pr + ucap + ngp, sizeof(uint64_t) * (size_t)nmiss, hipMemcpyDeviceToDevice, s));
     trace("n %d in %d set(s), grid %d: grid pairs %lld raw, %lld after tile dedup, missing %lld", n, S, gmax,
           (long long)ngp, (long long)ngd, (long long)nmiss);
-    int64_t u = ngd + nmiss;
-    HCK(sort_pairs(P, pr, u, nb, s));
-    HCK(g4d::unique_u64(P, pr, u, &u, s));
-    ctx->st.num_missing = nmiss;
-    ctx->st.num_grid_pairs = u - nmiss;
+    // Both directions of every pair, then one sort and one unique: each distinct pair leaves
+    // exactly its two directed keys (a missing point appears in no grid pair).
+    const int64_t u = ngd + nmiss;
     HCK(g4d::launch_reciprocate(pr, u, s));
     int64_t m = 2 * u;
     HCK(sort_pairs(P, pr, m, nb, s));
+    HCK(g4d::unique_u64(P, pr, m, &m, s));
+    ctx->st.num_missing = nmiss;
+    ctx->st.num_grid_pairs = m / 2 - nmiss;
     HCK(ctx->ws_off.ensure(sizeof(int64_t) * (size_t)(n + 1)));
     int64_t *ws_off = buf<int64_t>(ctx->ws_off);
     HCK(g4d::launch_csr_offsets(pr, m, n, ws_off, s));
@@ -443,11 +444,19 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     CK(read_counters(ctx, hc));
     const int64_t npump = (int64_t)hc[G4D_CNT_APPEND0];
     if (npump > 0) {
-        int64_t m2 = m + 2 * npump;
-        HCK(sort_pairs(P, pr, m2, nb, s));
-        HCK(g4d::unique_u64(P, pr, m2, &m2, s));
-        ctx->st.num_pumped = m2 - m;
-        m = m2;
+        // A pumped candidate is not in the star's list, and the lists are symmetric, so no
+        // pumped key is already present: sort and deduplicate the pumped keys alone (two stars
+        // may pump each other) and merge them into the sorted worksets.
+        int64_t np = 2 * npump;
+        uint64_t *pk = pr + m;
+        HCK(sort_pairs(P, pk, np, nb, s));
+        HCK(g4d::unique_u64(P, pk, np, &np, s));
+        HCK(P.sort_tmp_keys.ensure(sizeof(uint64_t) * (size_t)(m + np)));
+        uint64_t *merged = P.sort_tmp_keys.as<uint64_t>();
+        HCK(g4d::merge_u64(pr, m, pk, np, merged, s));
+        HCK(hipMemcpyAsync(pr, merged, sizeof(uint64_t) * (size_t)(m + np), hipMemcpyDeviceToDevice, s));
+        ctx->st.num_pumped = np;
+        m += np;
         HCK(g4d::launch_csr_offsets(pr, m, n, ws_off, s));
     }
     ctx->st.num_workset = m;

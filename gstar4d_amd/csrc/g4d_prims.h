@@ -64,6 +64,9 @@ hipError_t scan_excl_u32(Prims &P, const uint32_t *in, uint32_t *out, int64_t n,
 // Stable LSD radix sort of uint64 keys on bits [lo_bit, hi_bit).  Result in keys.
 hipError_t radix_sort_u64(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int hi_bit, hipStream_t s);
 
+// Merge of two sorted lists into out (no aliasing); equal keys keep a's before b's.
+hipError_t merge_u64(const uint64_t *a, int64_t na, const uint64_t *b, int64_t nb, uint64_t *out, hipStream_t s);
+
 // Order-preserving removal of adjacent duplicates; returns the new count via host sync.
 hipError_t unique_u64(Prims &P, uint64_t *keys, int64_t n, int64_t *n_out, hipStream_t s);
 

@@ -316,6 +316,78 @@ hipError_t radix_sort_u64(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int h
 }
 
 // ------------------------------------------------------------------------------------
+// Merge of two sorted lists, for a long list `a` and a short one `b`.  Work-group w writes
+// outputs [w*MG_TILE, (w+1)*MG_TILE): its first thread finds where the two tile boundaries cross
+// the inputs (merge path, ties take `a` first), the tile's part of `b` is staged in LDS, and
+// every element goes straight to its output: an `a` element after the staged `b` elements
+// smaller than it, a `b` element after the tile's `a` elements not greater than it.
+// ------------------------------------------------------------------------------------
+
+static constexpr int MG_TILE = 4096;
+
+__device__ __forceinline__ int64_t merge_split(const uint64_t *a, int64_t na, const uint64_t *b, int64_t nb, int64_t d)
+{
+    int64_t lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (a[mid] <= b[d - mid - 1]) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__global__ void __launch_bounds__(256) k_merge_u64(const uint64_t *a, int64_t na, const uint64_t *b, int64_t nb,
+                                                   uint64_t *out)
+{
+    __shared__ uint64_t bw[MG_TILE];
+    __shared__ int64_t sp[4];
+    const int64_t total = na + nb;
+    const int64_t o0 = (int64_t)blockIdx.x * MG_TILE;
+    const int64_t o1 = o0 + MG_TILE < total ? o0 + MG_TILE : total;
+    if (threadIdx.x == 0) {
+        const int64_t i0 = merge_split(a, na, b, nb, o0), i1 = merge_split(a, na, b, nb, o1);
+        sp[0] = i0;
+        sp[1] = o0 - i0;
+        sp[2] = i1;
+        sp[3] = o1 - i1;
+    }
+    __syncthreads();
+    const int64_t i0 = sp[0], j0 = sp[1], i1 = sp[2], j1 = sp[3];
+    const int nbw = (int)(j1 - j0);
+    for (int k = threadIdx.x; k < nbw; k += blockDim.x) bw[k] = b[j0 + k];
+    __syncthreads();
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += blockDim.x) {
+        const uint64_t v = a[i];
+        int lo = 0, hi = nbw; // staged b elements < v
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (bw[mid] < v) lo = mid + 1;
+            else hi = mid;
+        }
+        out[o0 + (i - i0) + lo] = v;
+    }
+    for (int k = threadIdx.x; k < nbw; k += blockDim.x) {
+        const uint64_t v = bw[k];
+        int64_t lo = i0, hi = i1; // tile's a elements <= v
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (a[mid] <= v) lo = mid + 1;
+            else hi = mid;
+        }
+        out[o0 + k + (lo - i0)] = v;
+    }
+}
+
+hipError_t merge_u64(const uint64_t *a, int64_t na, const uint64_t *b, int64_t nb, uint64_t *out, hipStream_t s)
+{
+    const int64_t nt = (na + nb + MG_TILE - 1) / MG_TILE;
+    if (nt <= 0) return hipSuccess;
+    if (nt > INT32_MAX) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_merge_u64, dim3((unsigned)nt), dim3(256), 0, s, a, na, b, nb, out);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------------------------
 // Compaction
 // ------------------------------------------------------------------------------------
 
