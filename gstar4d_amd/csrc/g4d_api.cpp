@@ -779,8 +779,8 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         const uint64_t tsz = next_pow2((uint64_t)std::max<int64_t>(8 * nt, 1024));
         HCK(ctx->table.ensure(sizeof(uint32_t) * (size_t)tsz));
         HCK(ctx->opp.ensure(sizeof(int) * 4 * (size_t)std::max<int64_t>(nt, 1)));
-        HCK(g4d::launch_adjacency(buf<int>(ctx->tets), nt, buf<int>(ctx->opp), buf<uint32_t>(ctx->table), (int64_t)tsz,
-                                  cnt, s));
+        HCK(g4d::launch_adjacency_tiled(buf<int>(ctx->tets), nt, buf<int>(ctx->opp), buf<uint32_t>(ctx->table),
+                                        (int64_t)tsz, cnt, s));
     }
     if (batched) {
         // per-set tetrahedron ranges (tetrahedra are grouped by owner star, stars by set) and

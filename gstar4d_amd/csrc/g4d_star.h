@@ -164,6 +164,10 @@ hipError_t launch_out_write(const OutParams &p, hipStream_t s);
 // Face adjacency via an open-addressing hash of face triples.
 hipError_t launch_adjacency(const int *tets, int64_t nt, int *opp, uint32_t *table, int64_t table_size,
                             unsigned long long *cnt, hipStream_t s);
+// Same result for tetrahedra grouped spatially (the pipeline's output order): faces are first
+// paired within tiles of 1,024 consecutive tetrahedra in LDS, the rest through `table`.
+hipError_t launch_adjacency_tiled(const int *tets, int64_t nt, int *opp, uint32_t *table, int64_t table_size,
+                                  unsigned long long *cnt, hipStream_t s);
 hipError_t launch_map_perm(int *tets, int64_t m4, const int *perm, hipStream_t s);
 // out[i] = a[idx[i]] for i < m.
 hipError_t launch_gather_i64(const int64_t *a, const int64_t *idx, int64_t m, int64_t *out, hipStream_t s);

@@ -1443,6 +1443,99 @@ __global__ void k_adjacency(const int *tets, int64_t nt, int *opp, uint32_t *tab
     }
 }
 
+__device__ __forceinline__ uint64_t face_hash(const int *f)
+{
+    uint64_t h = (uint64_t)(uint32_t)f[0] * 0x9E3779B97F4A7C15ull ^ (uint64_t)(uint32_t)f[1] * 0xC2B2AE3D27D4EB4Full ^
+                 (uint64_t)(uint32_t)f[2] * 0x165667B19E3779F9ull;
+    h ^= h >> 29;
+    h *= 0xBF58476D1CE4E5B9ull;
+    h ^= h >> 32;
+    return h;
+}
+
+// Tiled variant for the pipeline's output, whose tetrahedra come grouped by owner star in
+// Morton order: a work-group first pairs the faces of its ADJ_TETS consecutive tetrahedra in an
+// LDS hash table (most interior faces have both tetrahedra in the same tile), and only the
+// faces left unpaired go through the global table as in k_adjacency.
+static constexpr int ADJ_TETS = 1024;
+static constexpr int ADJ_HASH = 8192;
+
+__global__ void __launch_bounds__(256) k_adjacency_tiled(const int *tets, int64_t nt, int *opp, uint32_t *table,
+                                                         uint64_t mask, unsigned long long *cnt)
+{
+    __shared__ int4 lt[ADJ_TETS];
+    __shared__ uint32_t tab[ADJ_HASH];
+    __shared__ uint32_t done[ADJ_TETS * 4 / 32];
+    for (int64_t tile = blockIdx.x; tile * ADJ_TETS < nt; tile += gridDim.x) {
+        const int64_t t0 = tile * ADJ_TETS;
+        const int nloc = (int)(nt - t0 < ADJ_TETS ? nt - t0 : ADJ_TETS);
+        for (int k = threadIdx.x; k < nloc; k += blockDim.x) lt[k] = reinterpret_cast<const int4 *>(tets)[t0 + k];
+        for (int k = threadIdx.x; k < ADJ_HASH; k += blockDim.x) tab[k] = 0u;
+        for (int k = threadIdx.x; k < ADJ_TETS * 4 / 32; k += blockDim.x) done[k] = 0u;
+        __syncthreads();
+        for (int lf = threadIdx.x; lf < 4 * nloc; lf += blockDim.x) {
+            const int k = lf >> 2, i = lf & 3;
+            int f[3];
+            face_of(reinterpret_cast<const int *>(&lt[k]), i, f);
+            uint32_t h = (uint32_t)face_hash(f) & (ADJ_HASH - 1);
+            for (int probe = 0; probe < ADJ_HASH; probe++) { // fewer faces than slots: ends
+                const uint32_t cur = atomicCAS(&tab[h], 0u, (uint32_t)lf + 1u);
+                if (cur == 0u) break;
+                const int other = (int)cur - 1;
+                int g[3];
+                face_of(reinterpret_cast<const int *>(&lt[other >> 2]), other & 3, g);
+                if (g[0] == f[0] && g[1] == f[1] && g[2] == f[2]) {
+                    const uint32_t was = atomicOr(&done[other >> 5], 1u << (other & 31));
+                    if (was & (1u << (other & 31))) atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_FACE);
+                    atomicOr(&done[lf >> 5], 1u << (lf & 31));
+                    opp[4 * (t0 + (other >> 2)) + (other & 3)] = (int)(t0 + k);
+                    opp[4 * (t0 + k) + i] = (int)(t0 + (other >> 2));
+                    break;
+                }
+                h = (h + 1) & (ADJ_HASH - 1);
+            }
+        }
+        __syncthreads();
+        for (int lf = threadIdx.x; lf < 4 * nloc; lf += blockDim.x) {
+            if (done[lf >> 5] & (1u << (lf & 31))) continue;
+            const int k = lf >> 2, i = lf & 3;
+            const int64_t t = t0 + k;
+            int f[3];
+            face_of(reinterpret_cast<const int *>(&lt[k]), i, f);
+            uint64_t slot = face_hash(f) & mask;
+            const uint32_t mine = (uint32_t)((t << 2) | i) + 1u;
+            for (uint64_t probe = 0; probe <= mask; probe++) {
+                const uint32_t cur = atomicCAS(&table[slot], 0u, mine);
+                if (cur == 0u) break;
+                const int64_t t2 = (int64_t)((cur - 1u) >> 2);
+                const int i2 = (int)((cur - 1u) & 3u);
+                int g[3];
+                face_of(tets + 4 * t2, i2, g);
+                if (g[0] == f[0] && g[1] == f[1] && g[2] == f[2]) {
+                    if (atomicCAS(&opp[4 * t2 + i2], -1, (int)t) != -1)
+                        atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_FACE);
+                    opp[4 * t + i] = (int)t2;
+                    break;
+                }
+                slot = (slot + 1) & mask;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+hipError_t launch_adjacency_tiled(const int *tets, int64_t nt, int *opp, uint32_t *table, int64_t table_size,
+                                  unsigned long long *cnt, hipStream_t s)
+{
+    if (nt <= 0) return hipSuccess;
+    HIP_CHECK_RET(hipMemsetAsync(table, 0, sizeof(uint32_t) * (size_t)table_size, s));
+    HIP_CHECK_RET(hipMemsetAsync(opp, 0xff, sizeof(int) * 4 * (size_t)nt, s));
+    const int64_t tiles = (nt + ADJ_TETS - 1) / ADJ_TETS;
+    hipLaunchKernelGGL(k_adjacency_tiled, dim3((unsigned)(tiles < 65536 ? tiles : 65536)), dim3(256), 0, s, tets, nt,
+                       opp, table, (uint64_t)(table_size - 1), cnt);
+    return hipGetLastError();
+}
+
 hipError_t launch_adjacency(const int *tets, int64_t nt, int *opp, uint32_t *table, int64_t table_size,
                             unsigned long long *cnt, hipStream_t s)
 {
