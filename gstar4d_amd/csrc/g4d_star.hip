@@ -1309,32 +1309,37 @@ __device__ __forceinline__ uint64_t *out_keep_word(const OutParams &p, int64_t t
 }
 __device__ __forceinline__ uint64_t out_keep_bit(bool wide) { return wide ? (1ull << 63) : (1ull << 58); }
 
-// Decision pass: every owner triangle whose orient3d filter says "lower hull" gets its keep
-// bit and is counted for its star; inconclusive ones go to the list for the exact pass.
-__global__ void k_out_flag(OutParams p, uint64_t *list, int64_t cap)
+// Decision pass, one wave per star and one lane per link triangle (coalesced reads of the
+// star's triangle words): every owner triangle whose orient3d filter says "lower hull" gets its
+// keep bit and is counted for its star; inconclusive ones go to the list for the exact pass.
+__global__ void __launch_bounds__(256) k_out_flag(OutParams p, uint64_t *list, int64_t cap)
 {
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < p.n; s += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int w0 = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), nw = (int)((gridDim.x * blockDim.x) >> 6);
+    for (int s = w0; s < p.n; s += nw) {
         const int tn = p.tnum[s];
         const bool wide = p.vnum[s] >= STAR_WIDE_MIN;
         const int64_t vo = p.voff[s], to = p.toff[s];
         int64_t c = 0;
-        for (int t = 0; t < tn; t++) {
-            int lv[3];
-            star_tri_verts(p.tpool, to, t, wide, lv);
-            const int a = p.vpool[vo + lv[0]], b = p.vpool[vo + lv[1]], cc = p.vpool[vo + lv[2]];
-            if (!(s < a && s < b && s < cc)) continue;
-            const double *pp[4] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b, p.pts + 3 * (int64_t)cc,
-                                   p.pts + 3 * (int64_t)s};
-            const int o = filter_orient3d(pp);
-            if (o < 0) {
-                *out_keep_word(p, to, t, wide) |= out_keep_bit(wide);
-                c++;
-            } else if (o == 0) {
-                const unsigned long long k = atomicAdd(&p.cnt[G4D_CNT_APPEND5], 1ull);
-                if ((int64_t)k < cap) list[k] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)t;
+        for (int t0 = 0; t0 < tn; t0 += 64) {
+            const int t = t0 + lane;
+            int o = 1;
+            if (t < tn) {
+                int lv[3];
+                star_tri_verts(p.tpool, to, t, wide, lv);
+                const int a = p.vpool[vo + lv[0]], b = p.vpool[vo + lv[1]], cc = p.vpool[vo + lv[2]];
+                if (s < a && s < b && s < cc) {
+                    const double *pp[4] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b, p.pts + 3 * (int64_t)cc,
+                                           p.pts + 3 * (int64_t)s};
+                    o = filter_orient3d(pp);
+                    if (o < 0) *out_keep_word(p, to, t, wide) |= out_keep_bit(wide);
+                }
             }
+            c += __popcll(__ballot(o < 0));
+            const long long k = g4d_wave_append(o == 0, &p.cnt[G4D_CNT_APPEND5]);
+            if (o == 0 && k < cap) list[k] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)t;
         }
-        p.star_tet_count[s] = c;
+        if (lane == 0) p.star_tet_count[s] = c;
     }
 }
 
@@ -1357,30 +1362,40 @@ __global__ void k_out_exact(OutParams p, const uint64_t *list, int64_t m)
     }
 }
 
-// Write pass: the kept triangles of each star at its scanned offset.
-__global__ void k_out_write(OutParams p)
+// Write pass, one wave per star: the kept triangles of each star at its scanned offset, in
+// triangle order.
+__global__ void __launch_bounds__(256) k_out_write(OutParams p)
 {
-    for (int s = blockIdx.x * blockDim.x + threadIdx.x; s < p.n; s += gridDim.x * blockDim.x) {
+    const int lane = threadIdx.x & 63;
+    const int w0 = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), nw = (int)((gridDim.x * blockDim.x) >> 6);
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (int s = w0; s < p.n; s += nw) {
         const int tn = p.tnum[s];
         const bool wide = p.vnum[s] >= STAR_WIDE_MIN;
         const int64_t vo = p.voff[s], to = p.toff[s];
         int64_t k = p.star_tet_off[s];
-        for (int t = 0; t < tn; t++) {
-            if (!(*out_keep_word(p, to, t, wide) & out_keep_bit(wide))) continue;
-            int lv[3];
-            star_tri_verts(p.tpool, to, t, wide, lv);
-            int *out = p.tets + 4 * k++;
-            out[0] = p.vpool[vo + lv[0]];
-            out[1] = p.vpool[vo + lv[1]];
-            out[2] = p.vpool[vo + lv[2]];
-            out[3] = s;
+        for (int t0 = 0; t0 < tn; t0 += 64) {
+            const int t = t0 + lane;
+            const bool keep = t < tn && (*out_keep_word(p, to, t, wide) & out_keep_bit(wide));
+            const uint64_t m = __ballot(keep);
+            if (keep) {
+                int lv[3];
+                star_tri_verts(p.tpool, to, t, wide, lv);
+                int4 q;
+                q.x = p.vpool[vo + lv[0]];
+                q.y = p.vpool[vo + lv[1]];
+                q.z = p.vpool[vo + lv[2]];
+                q.w = s;
+                reinterpret_cast<int4 *>(p.tets)[k + __popcll(m & below)] = q;
+            }
+            k += __popcll(m);
         }
     }
 }
 
 hipError_t launch_out_flag(const OutParams &p, uint64_t *list, int64_t cap, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_out_flag, dim3(g4d_blocks(p.n, 64, 65536)), dim3(64), 0, s, p, list, cap);
+    hipLaunchKernelGGL(k_out_flag, dim3(g4d_blocks((int64_t)p.n * 64, 256, 16384)), dim3(256), 0, s, p, list, cap);
     return hipGetLastError();
 }
 
@@ -1393,7 +1408,7 @@ hipError_t launch_out_exact(const OutParams &p, const uint64_t *list, int64_t m,
 
 hipError_t launch_out_write(const OutParams &p, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_out_write, dim3(g4d_blocks(p.n, 64, 65536)), dim3(64), 0, s, p);
+    hipLaunchKernelGGL(k_out_write, dim3(g4d_blocks((int64_t)p.n * 64, 256, 16384)), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 
