@@ -3,7 +3,8 @@
 // The reference leans on thrust for sort_by_key / exclusive_scan / remove_if
 // (GD/GDelaunay/GDelCommon.cu:85-277).  Here: a three-phase tiled scan, a stable LSD radix
 // sort whose per-wave ranking uses ballot-matched digit peers (8 ballots per 8-bit digit),
-// and scan-based order-preserving compaction.
+// order-preserving compaction by tile counts and ballot ranks, a merge of a long and a short
+// sorted list, and single-work-group versions of the sort and the compaction for short lists.
 #include "g4d_prims.h"
 
 namespace g4d {
@@ -391,31 +392,6 @@ hipError_t merge_u64(const uint64_t *a, int64_t na, const uint64_t *b, int64_t n
 // Compaction
 // ------------------------------------------------------------------------------------
 
-__global__ void k_unique_flags(const uint64_t *keys, int64_t n, uint32_t *flags)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        flags[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
-}
-
-__global__ void k_keep_flags(const uint8_t *keep, int64_t n, uint32_t *flags)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        flags[i] = keep[i] ? 1u : 0u;
-}
-
-__global__ void k_valid_flags(const uint64_t *in, int64_t n, uint64_t none, uint32_t *flags)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        flags[i] = in[i] != none ? 1u : 0u;
-}
-
-__global__ void k_scatter_flagged(const uint64_t *in, const uint32_t *flags, const uint32_t *pos, uint64_t *out,
-                                  int64_t n)
-{
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        if (flags[i]) out[pos[i]] = in[i];
-}
-
 // Lists up to SMALL_COMPACT entries are compacted by one work-group in a single launch
 // (tiles of 1024 in order, ballot ranks, a running offset), which also works in place: an
 // entry kept from index j lands at a position <= j, and later tiles read beyond every
@@ -473,31 +449,95 @@ static hipError_t compact_small(Prims &P, const uint64_t *in, const uint8_t *kee
     return hipSuccess;
 }
 
-static hipError_t compact_with_flags(Prims &P, const uint64_t *in, uint64_t *out, int64_t n, int64_t *n_out,
-                                     hipStream_t s)
+// Longer lists: a counting pass per tile of CMP_TILE entries, a scan of the tile counts, and
+// a writing pass that ranks each round of 256 entries with ballots, so no flag or position
+// array goes through HBM.  Out of place only (tiles write concurrently).
+static constexpr int CMP_ROUNDS = 16;
+static constexpr int CMP_TILE = 256 * CMP_ROUNDS;
+
+template <int MODE>
+__device__ __forceinline__ bool cmp_keep(const uint64_t *in, const uint8_t *keep, uint64_t none, int64_t i,
+                                         uint64_t v)
 {
-    uint32_t *flags = P.flags.as<uint32_t>();
-    uint32_t *pos = P.pos.as<uint32_t>();
-    HIP_CHECK_RET(scan_excl_u32(P, flags, pos, n, s));
-    uint32_t last[2];
-    HIP_CHECK_RET(hipMemcpyAsync(&last[0], pos + n - 1, 4, hipMemcpyDeviceToHost, s));
-    HIP_CHECK_RET(hipMemcpyAsync(&last[1], flags + n - 1, 4, hipMemcpyDeviceToHost, s));
-    if (in == out) {
-        // scatter through the sort scratch then copy back
-        HIP_CHECK_RET(P.sort_tmp_keys.ensure(sizeof(uint64_t) * (size_t)n));
-        uint64_t *tmp = P.sort_tmp_keys.as<uint64_t>();
-        hipLaunchKernelGGL(k_scatter_flagged, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, in, flags, pos, tmp, n);
-        HIP_CHECK_RET(hipGetLastError());
-        HIP_CHECK_RET(hipStreamSynchronize(s));
-        const int64_t m = (int64_t)last[0] + last[1];
-        if (m > 0) HIP_CHECK_RET(hipMemcpyAsync(out, tmp, sizeof(uint64_t) * (size_t)m, hipMemcpyDeviceToDevice, s));
-        *n_out = m;
-        return hipSuccess;
+    if (MODE == 0) return i == 0 || v != in[i - 1];
+    if (MODE == 1) return keep[i] != 0;
+    return v != none;
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_cmp_count(const uint64_t *in, const uint8_t *keep, uint64_t none, int64_t n,
+                                                   int64_t *tile_cnt)
+{
+    __shared__ uint32_t wsum[4];
+    const int64_t t0 = (int64_t)blockIdx.x * CMP_TILE;
+    uint32_t c = 0;
+#pragma unroll 4
+    for (int r = 0; r < CMP_ROUNDS; r++) {
+        const int64_t i = t0 + r * 256 + threadIdx.x;
+        if (i < n) c += cmp_keep<MODE>(in, keep, none, i, in[i]) ? 1u : 0u;
     }
-    hipLaunchKernelGGL(k_scatter_flagged, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, in, flags, pos, out, n);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = c;
+    __syncthreads();
+    if (threadIdx.x == 0) tile_cnt[blockIdx.x] = (int64_t)wsum[0] + wsum[1] + wsum[2] + wsum[3];
+}
+
+template <int MODE>
+__global__ void __launch_bounds__(256) k_cmp_write(const uint64_t *in, const uint8_t *keep, uint64_t none, int64_t n,
+                                                   const int64_t *tile_off, uint64_t *out)
+{
+    __shared__ uint32_t wsum[4];
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    const int64_t t0 = (int64_t)blockIdx.x * CMP_TILE;
+    int64_t run = tile_off[blockIdx.x];
+    for (int r = 0; r < CMP_ROUNDS; r++) {
+        const int64_t i = t0 + r * 256 + threadIdx.x;
+        uint64_t v = 0;
+        bool f = false;
+        if (i < n) {
+            v = in[i];
+            f = cmp_keep<MODE>(in, keep, none, i, v);
+        }
+        const uint64_t m = __ballot(f);
+        if (lane == 0) wsum[wid] = (uint32_t)__popcll(m);
+        __syncthreads();
+        uint32_t before = 0, tot = 0;
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            const uint32_t x = wsum[w];
+            if (w < wid) before += x;
+            tot += x;
+        }
+        if (f) out[run + before + (uint32_t)__popcll(m & ((1ull << lane) - 1ull))] = v;
+        run += tot;
+        __syncthreads();
+    }
+}
+
+template <int MODE>
+static hipError_t compact_big(Prims &P, const uint64_t *in, const uint8_t *keep, uint64_t none, uint64_t *out,
+                              int64_t n, int64_t *n_out, hipStream_t s)
+{
+    const int64_t nt = (n + CMP_TILE - 1) / CMP_TILE;
+    if (nt > INT32_MAX) return hipErrorInvalidValue;
+    HIP_CHECK_RET(P.flags.ensure(sizeof(int64_t) * (size_t)(nt + 1)));
+    int64_t *tc = P.flags.as<int64_t>();
+    hipLaunchKernelGGL(k_cmp_count<MODE>, dim3((unsigned)nt), dim3(256), 0, s, in, keep, none, n, tc);
     HIP_CHECK_RET(hipGetLastError());
+    HIP_CHECK_RET(scan_excl_i64(P, tc, tc, nt, tc + nt, s));
+    uint64_t *dst = out;
+    if (in == out) {
+        HIP_CHECK_RET(P.sort_tmp_keys.ensure(sizeof(uint64_t) * (size_t)n));
+        dst = P.sort_tmp_keys.as<uint64_t>();
+    }
+    hipLaunchKernelGGL(k_cmp_write<MODE>, dim3((unsigned)nt), dim3(256), 0, s, in, keep, none, n, tc, dst);
+    HIP_CHECK_RET(hipGetLastError());
+    int64_t m = 0;
+    HIP_CHECK_RET(hipMemcpyAsync(&m, tc + nt, 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK_RET(hipStreamSynchronize(s));
-    *n_out = (int64_t)last[0] + last[1];
+    if (dst != out && m > 0) HIP_CHECK_RET(hipMemcpyAsync(out, dst, sizeof(uint64_t) * (size_t)m, hipMemcpyDeviceToDevice, s));
+    *n_out = m;
     return hipSuccess;
 }
 
@@ -508,11 +548,7 @@ hipError_t unique_u64(Prims &P, uint64_t *keys, int64_t n, int64_t *n_out, hipSt
         return hipSuccess;
     }
     if (n <= SMALL_COMPACT) return compact_small<0>(P, keys, nullptr, 0, keys, n, n_out, s);
-    HIP_CHECK_RET(P.flags.ensure(sizeof(uint32_t) * (size_t)n));
-    HIP_CHECK_RET(P.pos.ensure(sizeof(uint32_t) * (size_t)n));
-    hipLaunchKernelGGL(k_unique_flags, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, keys, n, P.flags.as<uint32_t>());
-    HIP_CHECK_RET(hipGetLastError());
-    return compact_with_flags(P, keys, keys, n, n_out, s);
+    return compact_big<0>(P, keys, nullptr, 0, keys, n, n_out, s);
 }
 
 hipError_t compact_u64(Prims &P, const uint64_t *in, const uint8_t *keep, uint64_t *out, int64_t n, int64_t *n_out,
@@ -523,11 +559,7 @@ hipError_t compact_u64(Prims &P, const uint64_t *in, const uint8_t *keep, uint64
         return hipSuccess;
     }
     if (n <= SMALL_COMPACT) return compact_small<1>(P, in, keep, 0, out, n, n_out, s);
-    HIP_CHECK_RET(P.flags.ensure(sizeof(uint32_t) * (size_t)n));
-    HIP_CHECK_RET(P.pos.ensure(sizeof(uint32_t) * (size_t)n));
-    hipLaunchKernelGGL(k_keep_flags, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, keep, n, P.flags.as<uint32_t>());
-    HIP_CHECK_RET(hipGetLastError());
-    return compact_with_flags(P, in, out, n, n_out, s);
+    return compact_big<1>(P, in, keep, 0, out, n, n_out, s);
 }
 
 hipError_t compact_valid_u64(Prims &P, const uint64_t *in, uint64_t *out, int64_t n, uint64_t none, int64_t *n_out,
@@ -538,12 +570,7 @@ hipError_t compact_valid_u64(Prims &P, const uint64_t *in, uint64_t *out, int64_
         return hipSuccess;
     }
     if (n <= SMALL_COMPACT) return compact_small<2>(P, in, nullptr, none, out, n, n_out, s);
-    HIP_CHECK_RET(P.flags.ensure(sizeof(uint32_t) * (size_t)n));
-    HIP_CHECK_RET(P.pos.ensure(sizeof(uint32_t) * (size_t)n));
-    hipLaunchKernelGGL(k_valid_flags, dim3(g4d_blocks(n, 256, 65536)), dim3(256), 0, s, in, n, none,
-                       P.flags.as<uint32_t>());
-    HIP_CHECK_RET(hipGetLastError());
-    return compact_with_flags(P, in, out, n, n_out, s);
+    return compact_big<2>(P, in, nullptr, none, out, n, n_out, s);
 }
 
 } // namespace g4d
