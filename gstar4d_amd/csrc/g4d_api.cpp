@@ -670,10 +670,9 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
             HCK(g4d::launch_drown_keys(buf<uint64_t>(ctx->drown), nd, dk, s));
             HCK(g4d::radix_sort_u64(P, dk, nd, 0, 1 + nb, s));
             HCK(g4d::radix_sort_u64(P, dk, nd, 33, 33 + nb, s));
-            HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8, s));
-            HCK(g4d::launch_nonmutual(dk, nd, buf<uint64_t>(ctx->tasks), cnt, s));
-            CK(read_counters(ctx, hc));
-            ntask = (int64_t)hc[G4D_CNT_APPEND0];
+            HCK(g4d::launch_nonmutual(dk, nd, buf<uint64_t>(ctx->tasks), s));
+            HCK(g4d::compact_valid_u64(P, buf<uint64_t>(ctx->tasks), buf<uint64_t>(ctx->tasks), nd,
+                                       g4d::STAR_TASK_NONE, &ntask, s));
             if (ntask > 0) {
                 g4d::ProofParams pp;
                 pp.pts = pts;

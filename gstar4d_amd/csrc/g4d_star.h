@@ -109,8 +109,10 @@ hipError_t launch_proofs(const ProofParams &p, bool exact, hipStream_t s);
 
 // Drowned pairs -> keys (min<<33 | max<<1 | dir) for the mutual test.
 hipError_t launch_drown_keys(const uint64_t *drown, int64_t m, uint64_t *keys, hipStream_t s);
-// Non-mutual drowned keys -> proof tasks (star<<32 | vertex); counter G4D_CNT_APPEND0.
-hipError_t launch_nonmutual(const uint64_t *keys, int64_t m, uint64_t *tasks, unsigned long long *cnt, hipStream_t s);
+// Sorted drowned keys -> tasks[i] = proof task (star<<32 | vertex), or STAR_TASK_NONE for a
+// mutual pair; compact the NONE entries away afterwards.
+static constexpr uint64_t STAR_TASK_NONE = ~0ull;
+hipError_t launch_nonmutual(const uint64_t *keys, int64_t m, uint64_t *tasks, hipStream_t s);
 // keep[i] = pair (a<<32|b) with a < b and b not in the history of star a.
 hipError_t launch_hist_filter(const uint64_t *pairs, int64_t m, const int *hpool, const int64_t *hoff, const int *hnum,
                               uint8_t *keep, hipStream_t s);

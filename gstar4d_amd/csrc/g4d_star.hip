@@ -1107,30 +1107,24 @@ hipError_t launch_drown_keys(const uint64_t *drown, int64_t m, uint64_t *keys, h
     return hipGetLastError();
 }
 
-__global__ void k_nonmutual(const uint64_t *keys, int64_t m, uint64_t *tasks, unsigned long long *cnt)
+// tasks[i] = (star << 32 | vertex) of the drowned insertion i unless it is mutual, else
+// STAR_TASK_NONE; the caller compacts (order-preserving, no atomics).
+__global__ void k_nonmutual(const uint64_t *keys, int64_t m, uint64_t *tasks)
 {
-    for (int64_t base = (int64_t)blockIdx.x * blockDim.x; base < m; base += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t i = base + threadIdx.x;
-        bool keep = false;
-        uint64_t task = 0;
-        if (i < m) {
-            const uint64_t k = keys[i] >> 1;
-            const bool mutual = (i > 0 && (keys[i - 1] >> 1) == k) || (i + 1 < m && (keys[i + 1] >> 1) == k);
-            keep = !mutual;
-            const uint32_t mn = (uint32_t)(keys[i] >> 33), mx = (uint32_t)((keys[i] >> 1) & 0xffffffffu);
-            const bool dir = keys[i] & 1;
-            const uint32_t s = dir ? mx : mn, v = dir ? mn : mx;
-            task = ((uint64_t)s << 32) | v;
-        }
-        const long long slot = g4d_wave_append(keep, &cnt[G4D_CNT_APPEND0]);
-        if (keep) tasks[slot] = task;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint64_t k = keys[i] >> 1;
+        const bool mutual = (i > 0 && (keys[i - 1] >> 1) == k) || (i + 1 < m && (keys[i + 1] >> 1) == k);
+        const uint32_t mn = (uint32_t)(keys[i] >> 33), mx = (uint32_t)((keys[i] >> 1) & 0xffffffffu);
+        const bool dir = keys[i] & 1;
+        const uint32_t st = dir ? mx : mn, v = dir ? mn : mx;
+        tasks[i] = mutual ? STAR_TASK_NONE : (((uint64_t)st << 32) | v);
     }
 }
 
-hipError_t launch_nonmutual(const uint64_t *keys, int64_t m, uint64_t *tasks, unsigned long long *cnt, hipStream_t s)
+hipError_t launch_nonmutual(const uint64_t *keys, int64_t m, uint64_t *tasks, hipStream_t s)
 {
     if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_nonmutual, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, keys, m, tasks, cnt);
+    hipLaunchKernelGGL(k_nonmutual, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, keys, m, tasks);
     return hipGetLastError();
 }
 
