@@ -287,18 +287,22 @@ __device__ __forceinline__ void site_pos_off(int s, const uint64_t *vox, int ca,
 // streamed through an LDS tile of PBA_CHUNK positions x 64 columns (coalesced loads and stores
 // for every axis).  The lower envelope of the candidate sites (Maurer stack, exact integer
 // dominance test) keeps its two top entries in registers, so a candidate costs one stack read
-// per pop; the stack itself lives in LDS up to PBA_STACK entries, deeper entries in a per-block
-// region of `spill` laid out [depth][lane] so that a wave's accesses share cache lines.  At
-// 32 voxels per site a 512-long column's envelope is about 115 entries deep (at most ~140).
+// per pop; the stack itself lives in LDS up to pba_stack(AXIS) entries, deeper entries in a
+// per-block region of `spill` laid out [depth][lane] so that a wave's accesses share cache
+// lines.  The LDS size sets how many waves share a CU, so it differs per pass: the z pass sees
+// only the sites themselves (a shallow envelope); at 32 voxels per site the y pass's envelope
+// of a 512-long column is about 115 entries deep (at most ~140), where a deep LDS stack pays;
+// in the x pass more resident waves pay more than the spill they avoid (measured at 512^3).
 // The colour walk writes the nearest site of every voxel back through the same tile.  Ties: a
 // later site along the axis wins (the walk advances on <=).
 static constexpr int PBA_COLS = 64;
 static constexpr int PBA_CHUNK = 32;
-static constexpr int PBA_STACK = 120;
+__host__ __device__ constexpr int pba_stack(int axis) { return axis == 2 ? 32 : axis == 1 ? 120 : 56; }
 
+template <int STACK>
 struct PbaLds {
     int tile[PBA_CHUNK][PBA_COLS + 1];
-    int scode[PBA_STACK][PBA_COLS];
+    int scode[STACK][PBA_COLS];
 };
 
 template <int AXIS>
@@ -341,16 +345,17 @@ __device__ __forceinline__ void pba_load_chunk(int (&pre)[PBA_PER_LANE], const i
 
 // Stack entry below the LDS part.  Kept out of line: inlined next to the LDS read, the
 // compiler (ROCm 7.2, gfx950) emits an illegal operand for the unrolled tile loops.
-__device__ __attribute__((noinline)) int pba_spill_get(const int *sp, int k)
+__device__ __attribute__((noinline)) int pba_spill_get(const int *sp, int k, int stack)
 {
-    return sp[(int64_t)(k - PBA_STACK) * PBA_COLS];
+    return sp[(int64_t)(k - stack) * PBA_COLS];
 }
 
 template <int AXIS, bool IDS>
 __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all, GridSets S, const int *idgrid_all,
                                                  int *spill_all, const uint64_t *vox)
 {
-    __shared__ PbaLds L;
+    constexpr int PBA_STACK = pba_stack(AXIS);
+    __shared__ PbaLds<PBA_STACK> L;
     const int lane = threadIdx.x;
     const int st = S.nsets == 1 ? 0 : set_of_work(S.pba_blk, S.nsets, blockIdx.x);
     const int64_t lb = blockIdx.x - S.pba_blk[st];
@@ -366,7 +371,7 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
     const int c = c0 + lane;
     const int ca = c, cb = fixed; // x (Z, Y passes) or y (X pass); y (Z), z (Y), z (X)
     int *sp = spill_all + S.spill[st] + lb * PBA_COLS * g + lane; // entry k >= PBA_STACK at (k - PBA_STACK) * 64
-#define get(k) ((k) < PBA_STACK ? L.scode[(k)][lane] : pba_spill_get(sp, (k)))
+#define get(k) ((k) < PBA_STACK ? L.scode[(k)][lane] : pba_spill_get(sp, (k), PBA_STACK))
 
     // Tile element k of this lane: (cc, qq) with every load of a chunk issued before the first
     // is consumed, and the next chunk's loads in flight while the current one is processed.
