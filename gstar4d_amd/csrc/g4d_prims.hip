@@ -146,15 +146,43 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_hist(const uint64_t *keys, in
 }
 
 // Each wave owns RS_ITEMS consecutive rounds of 64 keys (wave-contiguous sub-tile), so
-// (wave, round, lane) order equals input order: ranking in that order is stable.
+// (wave, round, lane) order equals input order: ranking in that order is stable.  SELF: the
+// histogram is raw (few tiles) and each work-group derives its own digit offsets from it,
+// which saves the scan's launches.
+static constexpr int RS_SELF_MAX_TILES = 128;
+
+template <bool SELF>
 __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t *keys, uint64_t *out, int64_t n, int shift,
                                                            const uint32_t *hist_scan, int nblocks)
 {
     __shared__ uint32_t cnt[RS_WAVES][256];
     __shared__ uint32_t goff[256];
+    __shared__ uint32_t part[RS_WAVES];
     const int lane = __lane_id(), wid = threadIdx.x >> 6;
     for (int i = threadIdx.x; i < RS_WAVES * 256; i += RS_THREADS) (&cnt[0][0])[i] = 0;
-    goff[threadIdx.x] = hist_scan[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+    if (SELF) {
+        // digit d: keys of digit d in earlier tiles, plus all keys of smaller digits
+        const uint32_t *h = hist_scan + (int64_t)threadIdx.x * nblocks;
+        uint32_t before = 0, total = 0;
+        for (int b = 0; b < nblocks; b++) {
+            const uint32_t c = h[b];
+            if (b < (int)blockIdx.x) before += c;
+            total += c;
+        }
+        uint32_t inc = total;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += t;
+        }
+        if (lane == 63) part[wid] = inc;
+        __syncthreads();
+        uint32_t wb = 0;
+        for (int w = 0; w < wid; w++) wb += part[w];
+        goff[threadIdx.x] = wb + inc - total + before;
+    } else {
+        goff[threadIdx.x] = hist_scan[(int64_t)threadIdx.x * nblocks + blockIdx.x];
+    }
     __syncthreads();
     const int64_t wbase = (int64_t)blockIdx.x * RS_TILE + (int64_t)wid * (RS_ITEMS * 64);
     uint64_t kv[RS_ITEMS];
@@ -304,8 +332,14 @@ hipError_t radix_sort_u64(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int h
     for (int shift = lo_bit; shift < hi_bit; shift += 8) {
         hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, s, src, n, shift, hist, nb);
         HIP_CHECK_RET(hipGetLastError());
-        HIP_CHECK_RET(scan_excl_u32(P, hist, hist, (int64_t)256 * nb, s));
-        hipLaunchKernelGGL(k_rs_scatter, dim3(nb), dim3(RS_THREADS), 0, s, src, dst, n, shift, hist, nb);
+        if (nb <= RS_SELF_MAX_TILES) {
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rs_scatter<true>), dim3(nb), dim3(RS_THREADS), 0, s, src, dst, n, shift,
+                               hist, nb);
+        } else {
+            HIP_CHECK_RET(scan_excl_u32(P, hist, hist, (int64_t)256 * nb, s));
+            hipLaunchKernelGGL(HIP_KERNEL_NAME(k_rs_scatter<false>), dim3(nb), dim3(RS_THREADS), 0, s, src, dst, n, shift,
+                               hist, nb);
+        }
         HIP_CHECK_RET(hipGetLastError());
         uint64_t *t = src;
         src = dst;
