@@ -148,9 +148,7 @@ static T *buf(DevBuf &b)
 // Sort directed keys (a<<32 | b) with a, b < 2^nb.
 static hipError_t sort_pairs(g4d::Prims &P, uint64_t *k, int64_t m, int nb, hipStream_t s)
 {
-    hipError_t e = g4d::radix_sort_u64(P, k, m, 0, nb, s);
-    if (e != hipSuccess) return e;
-    return g4d::radix_sort_u64(P, k, m, 32, 32 + nb, s);
+    return g4d::radix_sort_u64_2(P, k, m, 0, nb, 32, 32 + nb, s);
 }
 
 static int read_counters(g4d_ctx *ctx, unsigned long long *h)
@@ -668,8 +666,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
             HCK(ctx->tasks.ensure(sizeof(uint64_t) * (size_t)nd));
             uint64_t *dk = buf<uint64_t>(ctx->dkeys);
             HCK(g4d::launch_drown_keys(buf<uint64_t>(ctx->drown), nd, dk, s));
-            HCK(g4d::radix_sort_u64(P, dk, nd, 0, 1 + nb, s));
-            HCK(g4d::radix_sort_u64(P, dk, nd, 33, 33 + nb, s));
+            HCK(g4d::radix_sort_u64_2(P, dk, nd, 0, 1 + nb, 33, 33 + nb, s));
             HCK(g4d::launch_nonmutual(dk, nd, buf<uint64_t>(ctx->tasks), s));
             HCK(g4d::compact_valid_u64(P, buf<uint64_t>(ctx->tasks), buf<uint64_t>(ctx->tasks), nd,
                                        g4d::STAR_TASK_NONE, &ntask, s));

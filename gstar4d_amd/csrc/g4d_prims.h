@@ -63,6 +63,10 @@ hipError_t scan_excl_u32(Prims &P, const uint32_t *in, uint32_t *out, int64_t n,
 
 // Stable LSD radix sort of uint64 keys on bits [lo_bit, hi_bit).  Result in keys.
 hipError_t radix_sort_u64(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int hi_bit, hipStream_t s);
+// The same over the digits of [lo_bit, hi_bit) and then of [lo2, hi2) in one ping-pong
+// sequence (an even total number of passes ends in keys without a copy).
+hipError_t radix_sort_u64_2(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int hi_bit, int lo2, int hi2,
+                            hipStream_t s);
 
 // Merge of two sorted lists into out (no aliasing); equal keys keep a's before b's.
 hipError_t merge_u64(const uint64_t *a, int64_t na, const uint64_t *b, int64_t nb, uint64_t *out, hipStream_t s);

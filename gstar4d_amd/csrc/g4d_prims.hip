@@ -238,7 +238,8 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t *keys,
 // keys and tmp; the result ends in keys.
 static constexpr int64_t SMALL_SORT = 16384;
 
-__global__ void __launch_bounds__(1024) k_sort_small(uint64_t *keys, uint64_t *tmp, int n, int lo_bit, int hi_bit)
+__global__ void __launch_bounds__(1024) k_sort_small(uint64_t *keys, uint64_t *tmp, int n, int lo_bit, int hi_bit,
+                                                     int lo2, int hi2)
 {
     __shared__ uint32_t goff[256];
     __shared__ uint32_t wcnt[16][256];
@@ -246,7 +247,10 @@ __global__ void __launch_bounds__(1024) k_sort_small(uint64_t *keys, uint64_t *t
     const int lane = __lane_id(), wid = threadIdx.x >> 6;
     const uint64_t lt = (1ull << lane) - 1ull;
     uint64_t *src = keys, *dst = tmp;
-    for (int shift = lo_bit; shift < hi_bit; shift += 8) {
+    // digits of [lo_bit, hi_bit), then of [lo2, hi2)
+    const int np1 = hi_bit > lo_bit ? (hi_bit - lo_bit + 7) / 8 : 0, np2 = hi2 > lo2 ? (hi2 - lo2 + 7) / 8 : 0;
+    for (int pass = 0; pass < np1 + np2; pass++) {
+        const int shift = pass < np1 ? lo_bit + 8 * pass : lo2 + 8 * (pass - np1);
         if (threadIdx.x < 256) goff[threadIdx.x] = 0;
         __syncthreads();
         for (int i = threadIdx.x; i < n; i += 1024) atomicAdd(&goff[(src[i] >> shift) & 255], 1u);
@@ -314,13 +318,27 @@ __global__ void __launch_bounds__(1024) k_sort_small(uint64_t *keys, uint64_t *t
 
 hipError_t radix_sort_u64(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int hi_bit, hipStream_t s)
 {
-    if (n <= 1 || hi_bit <= lo_bit) return hipSuccess;
+    return radix_sort_u64_2(P, keys, n, lo_bit, hi_bit, 0, 0, s);
+}
+
+hipError_t radix_sort_u64_2(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int hi_bit, int lo2, int hi2,
+                            hipStream_t s)
+{
+    if (n <= 1 || (hi_bit <= lo_bit && hi2 <= lo2)) return hipSuccess;
+    if (hi_bit <= lo_bit) {
+        lo_bit = lo2;
+        hi_bit = hi2;
+        lo2 = hi2 = 0;
+    }
     if (n <= SMALL_SORT) {
         HIP_CHECK_RET(P.sort_tmp_keys.ensure(sizeof(uint64_t) * (size_t)n));
         hipLaunchKernelGGL(k_sort_small, dim3(1), dim3(1024), 0, s, keys, P.sort_tmp_keys.as<uint64_t>(), (int)n, lo_bit,
-                           hi_bit);
+                           hi_bit, lo2, hi2);
         return hipGetLastError();
     }
+    std::vector<int> shifts;
+    for (int sh = lo_bit; sh < hi_bit; sh += 8) shifts.push_back(sh);
+    for (int sh = lo2; sh < hi2; sh += 8) shifts.push_back(sh);
     const int64_t nb64 = (n + RS_TILE - 1) / RS_TILE;
     if (nb64 > (1 << 30)) return hipErrorInvalidValue;
     const int nb = (int)nb64;
@@ -329,7 +347,7 @@ hipError_t radix_sort_u64(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int h
     uint64_t *src = keys, *dst = P.sort_tmp_keys.as<uint64_t>();
     uint32_t *hist = P.sort_hist.as<uint32_t>();
     int passes = 0;
-    for (int shift = lo_bit; shift < hi_bit; shift += 8) {
+    for (const int shift : shifts) {
         hipLaunchKernelGGL(k_rs_hist, dim3(nb), dim3(RS_THREADS), 0, s, src, n, shift, hist, nb);
         HIP_CHECK_RET(hipGetLastError());
         if (nb <= RS_SELF_MAX_TILES) {
