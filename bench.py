@@ -176,6 +176,11 @@ def main():
         stages.append(st)
     total_s = time.perf_counter() - t_begin
     value = n * args.steps / total_s
+    # Outside the timed region: one call through the host-buffer entry point (points copied in,
+    # tetrahedra and adjacency copied out), the PCIe-inclusive rate.
+    h0 = time.perf_counter()
+    t.triangulate(x)
+    host_ms = (time.perf_counter() - h0) * 1e3
     G3 = grid ** 3
     pba_bytes = 24 * G3
     pba_avg_ms = statistics.mean(pba_ms)
@@ -226,6 +231,8 @@ def main():
         "exact_calls": last["exact_calls"],
         "sos_calls": last["sos_calls"],
         "step_ms": [round(v, 3) for v in step_ms],
+        "host_buffers": {"ms": round(host_ms, 3), "points_per_s": round(n / (host_ms * 1e-3), 1),
+                         "note": "g4d_triangulate: host points in, host tetrahedra + adjacency out (PCIe-inclusive)"},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.seed)
