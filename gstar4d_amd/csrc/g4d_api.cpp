@@ -470,7 +470,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     HCK(ctx->huge.ensure(sizeof(int) * (size_t)n));
     HCK(ctx->xstars.ensure(sizeof(int) * (size_t)n));
     int64_t *vcap = buf<int64_t>(ctx->vcap), *tcap = buf<int64_t>(ctx->tcap), *hcap = buf<int64_t>(ctx->hcap);
-    const int nblk_star = 8192; // 256 CUs x 16 resident waves of the 32-vertex tier, twice over
+    const int nblk_star = 5120; // 256 CUs x 20 resident waves of the 32-vertex tier (5 per SIMD)
     const int lds_limit = std::min(g4d::STAR_LDS_LIMIT, ctx->opt.lds_star_limit > 0 ? ctx->opt.lds_star_limit
                                                                                   : g4d::STAR_LDS_LIMIT);
 
