@@ -167,7 +167,7 @@ hipError_t launch_out_write(const OutParams &p, hipStream_t s);
 hipError_t launch_adjacency(const int *tets, int64_t nt, int *opp, uint32_t *table, int64_t table_size,
                             unsigned long long *cnt, hipStream_t s);
 // Same result for tetrahedra grouped spatially (the pipeline's output order): faces are first
-// paired within tiles of 1,024 consecutive tetrahedra in LDS, the rest through `table`.
+// paired within tiles of 512 consecutive tetrahedra in LDS, the rest through `table`.
 hipError_t launch_adjacency_tiled(const int *tets, int64_t nt, int *opp, uint32_t *table, int64_t table_size,
                                   unsigned long long *cnt, hipStream_t s);
 hipError_t launch_map_perm(int *tets, int64_t m4, const int *perm, hipStream_t s);
