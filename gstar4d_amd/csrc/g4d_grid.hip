@@ -296,7 +296,7 @@ __device__ __forceinline__ void site_pos_off(int s, const uint64_t *vox, int ca,
 // The colour walk writes the nearest site of every voxel back through the same tile.  Ties: a
 // later site along the axis wins (the walk advances on <=).
 static constexpr int PBA_COLS = 64;
-static constexpr int PBA_CHUNK = 32;
+static constexpr int PBA_CHUNK = 16;
 __host__ __device__ constexpr int pba_stack(int axis) { return axis == 2 ? 32 : axis == 1 ? 120 : 56; }
 
 template <int STACK>
