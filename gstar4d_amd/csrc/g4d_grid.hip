@@ -638,12 +638,37 @@ __global__ void __launch_bounds__(256) k_grid_pairs(const int *grid_all, GridSet
 // four; the global sort and unique that follow remove what remains duplicated across tiles.
 static constexpr int GP_HASH = 4096;
 static constexpr int GP_PROBES = 32;
+static constexpr int GP_STAGE = 256;
 static constexpr uint64_t GP_EMPTY = ~0ull;
+
+__device__ __forceinline__ void gp_wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Inserts a pair into the tile's hash set (a plain read settles a repeat without an atomic);
+// a pair that finds no free slot within GP_PROBES probes goes straight to the top of the region.
+__device__ __forceinline__ void gp_insert(unsigned long long *tab, uint64_t key, uint64_t *out, int64_t rend,
+                                          unsigned int *nover)
+{
+    uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 52);
+    bool done = false;
+    for (int p = 0; p < GP_PROBES && !done; p++) {
+        unsigned long long cur = ((volatile unsigned long long *)tab)[h];
+        if (cur == GP_EMPTY) cur = atomicCAS(&tab[h], GP_EMPTY, (unsigned long long)key);
+        done = cur == GP_EMPTY || cur == key;
+        h = (h + 1) & (GP_HASH - 1);
+    }
+    if (!done) out[rend - 1 - atomicAdd(nover, 1u)] = key;
+}
 
 __global__ void __launch_bounds__(GP_THREADS) k_grid_pairs_dedup(const int *grid_all, GridSets S, const int64_t *off,
                                                                  uint64_t *out, uint2 *cnt)
 {
     __shared__ unsigned long long tab[GP_HASH];
+    __shared__ unsigned long long stage[GP_THREADS / 64][GP_STAGE];
     __shared__ unsigned int nover;
     __shared__ unsigned int wsum[GP_THREADS / 64];
     for (int i = threadIdx.x; i < GP_HASH; i += GP_THREADS) tab[i] = GP_EMPTY;
@@ -656,43 +681,52 @@ __global__ void __launch_bounds__(GP_THREADS) k_grid_pairs_dedup(const int *grid
     const int *grid = grid_all + S.vox_off[st];
     const int64_t rbeg = off[(int64_t)blockIdx.x * GP_THREADS];
     const int64_t rend = off[(int64_t)(blockIdx.x + 1) * GP_THREADS];
+    // Every lane walks its column (a lane outside the grid sees only Marker cubes, which emit
+    // nothing), so the wave steps through its cubes in lockstep: a step's pairs are first
+    // packed into the wave's LDS stage (exclusive scan of the lanes' counts) and then inserted
+    // by all 64 lanes together, instead of one pass per pair slot that any lane uses.  A step
+    // with more pairs than the stage holds inserts slot by slot.
     int x, y, z0, z1;
-    if (gp_coords(lt, g, grid_pair_chunk_dev(g), &x, &y, &z0, &z1)) {
-        int v[8];
-        v[0] = grid_at(grid, g, x, y, z0);
-        v[1] = grid_at(grid, g, x + 1, y, z0);
-        v[2] = grid_at(grid, g, x, y + 1, z0);
-        v[3] = grid_at(grid, g, x + 1, y + 1, z0);
-        uint64_t pr[31];
-        for (int z = z0; z < z1; z++) {
-            v[4] = grid_at(grid, g, x, y, z + 1);
-            v[5] = grid_at(grid, g, x + 1, y, z + 1);
-            v[6] = grid_at(grid, g, x, y + 1, z + 1);
-            v[7] = grid_at(grid, g, x + 1, y + 1, z + 1);
-            const uint32_t mask = cube_pairs(v, pr);
-            if (!mask) goto next_cube;
-            // static slot indices keep pr[] in registers
+    (void)gp_coords(lt, g, grid_pair_chunk_dev(g), &x, &y, &z0, &z1);
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    unsigned long long *stg = stage[wid];
+    int v[8];
+    v[0] = grid_at(grid, g, x, y, z0);
+    v[1] = grid_at(grid, g, x + 1, y, z0);
+    v[2] = grid_at(grid, g, x, y + 1, z0);
+    v[3] = grid_at(grid, g, x + 1, y + 1, z0);
+    uint64_t pr[31];
+    for (int z = z0; z < z1; z++) {
+        v[4] = grid_at(grid, g, x, y, z + 1);
+        v[5] = grid_at(grid, g, x + 1, y, z + 1);
+        v[6] = grid_at(grid, g, x, y + 1, z + 1);
+        v[7] = grid_at(grid, g, x + 1, y + 1, z + 1);
+        const uint32_t mask = cube_pairs(v, pr);
+        const uint32_t c = (uint32_t)__popc(mask);
+        uint32_t inc = c;
 #pragma unroll
-            for (int j = 0; j < 31; j++) {
-                if (!(mask & (1u << j))) continue;
-                const uint64_t key = pr[j];
-                uint32_t h = (uint32_t)((key * 0x9E3779B97F4A7C15ull) >> 52);
-                // a plain read settles repeats (most raw pairs) without an atomic
-                bool done = false;
-                for (int p = 0; p < GP_PROBES && !done; p++) {
-                    unsigned long long cur = ((volatile unsigned long long *)tab)[h];
-                    if (cur == GP_EMPTY) cur = atomicCAS(&tab[h], GP_EMPTY, (unsigned long long)key);
-                    done = cur == GP_EMPTY || cur == key;
-                    h = (h + 1) & (GP_HASH - 1);
-                }
-                if (!done) out[rend - 1 - atomicAdd(&nover, 1u)] = key;
-            }
-        next_cube:
-            v[0] = v[4];
-            v[1] = v[5];
-            v[2] = v[6];
-            v[3] = v[7];
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += t;
         }
+        const uint32_t tot = __shfl(inc, 63, 64);
+        if (tot <= GP_STAGE) {
+            uint32_t pos = inc - c;
+#pragma unroll
+            for (int j = 0; j < 31; j++)
+                if (mask & (1u << j)) stg[pos++] = pr[j];
+            gp_wave_sync();
+            for (uint32_t k = lane; k < tot; k += 64) gp_insert(tab, stg[k], out, rend, &nover);
+            gp_wave_sync();
+        } else {
+#pragma unroll
+            for (int j = 0; j < 31; j++)
+                if (mask & (1u << j)) gp_insert(tab, pr[j], out, rend, &nover);
+        }
+        v[0] = v[4];
+        v[1] = v[5];
+        v[2] = v[6];
+        v[3] = v[7];
     }
     __syncthreads();
     // compact the table: GP_HASH / GP_THREADS slots per thread, block-wide exclusive scan
@@ -704,7 +738,6 @@ __global__ void __launch_bounds__(GP_THREADS) k_grid_pairs_dedup(const int *grid
         e[i] = tab[i * GP_THREADS + threadIdx.x];
         c += e[i] != GP_EMPTY;
     }
-    const int lane = __lane_id(), wid = threadIdx.x >> 6;
     uint32_t inc = c;
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
