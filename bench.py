@@ -148,8 +148,7 @@ def main():
     ap.add_argument("--no-per-set", action="store_true", help="batched workload: skip the per-set baseline")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one MI355X; multi-GPU runs are not part of this build "
-              "(sharding is described in DESIGN.md).", file=sys.stderr)
+        print("bench.py measures one MI355X; runs on more than one GPU are not supported.", file=sys.stderr)
         sys.exit(2)
     if args.workload.startswith("c5_batch"):
         return run_batch(args)
