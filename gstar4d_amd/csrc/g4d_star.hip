@@ -1305,20 +1305,25 @@ __device__ __forceinline__ uint64_t *out_keep_word(const OutParams &p, int64_t t
 }
 __device__ __forceinline__ uint64_t out_keep_bit(bool wide) { return wide ? (1ull << 63) : (1ull << 58); }
 
-// Decision pass, one wave per star and one lane per link triangle (coalesced reads of the
-// star's triangle words): every owner triangle whose orient3d filter says "lower hull" gets its
-// keep bit and is counted for its star; inconclusive ones go to the list for the exact pass.
+// Decision pass, one half-wave per star and one lane per link triangle (a star has ~27 link
+// triangles, so two stars fill a wave; coalesced reads of each star's triangle words): every
+// owner triangle whose orient3d filter says "lower hull" gets its keep bit and is counted for
+// its star; inconclusive ones go to the list for the exact pass.
 __global__ void __launch_bounds__(256) k_out_flag(OutParams p, uint64_t *list, int64_t cap)
 {
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, half = lane >> 5, hl = lane & 31;
+    const uint64_t hmask = half ? 0xffffffff00000000ull : 0x00000000ffffffffull;
     const int w0 = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), nw = (int)((gridDim.x * blockDim.x) >> 6);
-    for (int s = w0; s < p.n; s += nw) {
-        const int tn = p.tnum[s];
-        const bool wide = p.vnum[s] >= STAR_WIDE_MIN;
-        const int64_t vo = p.voff[s], to = p.toff[s];
+    for (int s0 = 2 * w0; s0 < p.n; s0 += 2 * nw) {
+        const int s = s0 + half;
+        const bool has = s < p.n;
+        const int tn = has ? p.tnum[s] : 0;
+        const bool wide = has && p.vnum[s] >= STAR_WIDE_MIN;
+        const int64_t vo = has ? p.voff[s] : 0, to = has ? p.toff[s] : 0;
+        const int tmax = max(tn, __shfl_xor(tn, 32, 64));
         int64_t c = 0;
-        for (int t0 = 0; t0 < tn; t0 += 64) {
-            const int t = t0 + lane;
+        for (int t0 = 0; t0 < tmax; t0 += 32) {
+            const int t = t0 + hl;
             int o = 1;
             if (t < tn) {
                 int lv[3];
@@ -1331,11 +1336,11 @@ __global__ void __launch_bounds__(256) k_out_flag(OutParams p, uint64_t *list, i
                     if (o < 0) *out_keep_word(p, to, t, wide) |= out_keep_bit(wide);
                 }
             }
-            c += __popcll(__ballot(o < 0));
+            c += __popcll(__ballot(o < 0) & hmask);
             const long long k = g4d_wave_append(o == 0, &p.cnt[G4D_CNT_APPEND5]);
             if (o == 0 && k < cap) list[k] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)t;
         }
-        if (lane == 0) p.star_tet_count[s] = c;
+        if (has && hl == 0) p.star_tet_count[s] = c;
     }
 }
 
@@ -1358,20 +1363,24 @@ __global__ void k_out_exact(OutParams p, const uint64_t *list, int64_t m)
     }
 }
 
-// Write pass, one wave per star: the kept triangles of each star at its scanned offset, in
-// triangle order.
+// Write pass, one half-wave per star: the kept triangles of each star at its scanned offset,
+// in triangle order.
 __global__ void __launch_bounds__(256) k_out_write(OutParams p)
 {
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, half = lane >> 5, hl = lane & 31;
+    const uint64_t hmask = half ? 0xffffffff00000000ull : 0x00000000ffffffffull;
     const int w0 = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), nw = (int)((gridDim.x * blockDim.x) >> 6);
-    const uint64_t below = (1ull << lane) - 1ull;
-    for (int s = w0; s < p.n; s += nw) {
-        const int tn = p.tnum[s];
-        const bool wide = p.vnum[s] >= STAR_WIDE_MIN;
-        const int64_t vo = p.voff[s], to = p.toff[s];
-        int64_t k = p.star_tet_off[s];
-        for (int t0 = 0; t0 < tn; t0 += 64) {
-            const int t = t0 + lane;
+    const uint64_t below = ((1ull << lane) - 1ull) & hmask;
+    for (int s0 = 2 * w0; s0 < p.n; s0 += 2 * nw) {
+        const int s = s0 + half;
+        const bool has = s < p.n;
+        const int tn = has ? p.tnum[s] : 0;
+        const bool wide = has && p.vnum[s] >= STAR_WIDE_MIN;
+        const int64_t vo = has ? p.voff[s] : 0, to = has ? p.toff[s] : 0;
+        int64_t k = has ? p.star_tet_off[s] : 0;
+        const int tmax = max(tn, __shfl_xor(tn, 32, 64));
+        for (int t0 = 0; t0 < tmax; t0 += 32) {
+            const int t = t0 + hl;
             const bool keep = t < tn && (*out_keep_word(p, to, t, wide) & out_keep_bit(wide));
             const uint64_t m = __ballot(keep);
             if (keep) {
@@ -1384,7 +1393,7 @@ __global__ void __launch_bounds__(256) k_out_write(OutParams p)
                 q.w = s;
                 reinterpret_cast<int4 *>(p.tets)[k + __popcll(m & below)] = q;
             }
-            k += __popcll(m);
+            k += __popcll(m & hmask);
         }
     }
 }
