@@ -571,10 +571,15 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         sp.exact_list = buf<int>(ctx->xstars);
         sp.exact_cap = n;
         sp.exact_counter = G4D_CNT_APPEND4;
-        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND2, 0, 8, s));
-        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND3, 0, 8 * 2, s));
-        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND6, 0, 8, s));
-        HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
+        // list counters, filter counts and the six work-queue heads of this pass, in one launch
+        {
+            uint32_t zmask = (1u << G4D_CNT_APPEND2) | (1u << G4D_CNT_APPEND3) | (1u << G4D_CNT_APPEND4) |
+                             (1u << G4D_CNT_APPEND6) | (1u << G4D_CNT_ORIENT4) | (1u << G4D_CNT_COFACTOR);
+            for (int q = 0; q < 6; q++) zmask |= 1u << (G4D_CNT_QUEUE0 + q);
+            HCK(g4d::launch_zero_counters(cnt, zmask, s));
+        }
+        int queue = G4D_CNT_QUEUE0;
+        sp.work_counter = queue++;
         sp.star_list = list;
         sp.nlist = nitems;
         sp.nlist_dev = nullptr;
@@ -583,8 +588,6 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         // to the exact list.  Then the exact-capable kernels of the same tiers over the exact
         // list.  Every later launch reads its list length on the device, so the chain runs
         // without host round trips; lists and counters are reused in stream order.
-        HCK(hipMemsetAsync(cnt + G4D_CNT_ORIENT4, 0, 8, s));
-        HCK(hipMemsetAsync(cnt + G4D_CNT_COFACTOR, 0, 8, s));
         if (ctx->ev_ok) (void)hipEventRecord(ctx->ev_t0[0], s);
         HCK(g4d::launch_star_kernel(sp, g4d::STAR_VMAX_T0, false, nblk_star, s));
         if (ctx->ev_ok) (void)hipEventRecord(ctx->ev_t0[1], s);
@@ -594,7 +597,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
             sp.nlist_dev = cnt + counter;
             sp.defer_list = defer;
             sp.defer_counter = defer_counter;
-            HCK(hipMemsetAsync(cnt + G4D_CNT_WORK, 0, 8, s));
+            sp.work_counter = queue++;
             HCK(g4d::launch_star_kernel(sp, vmax, exact, vmax == g4d::STAR_VMAX_T1 && !exact ? 4096 : 1024, s));
             return G4D_OK;
         };

@@ -51,7 +51,7 @@ enum g4d_counter : int {
     G4D_CNT_SOS = 6,      // SoS resolutions
     G4D_CNT_ORIENT3 = 7,
     G4D_CNT_MISC = 8,
-    G4D_CNT_WORK = 9,     // work-queue head
+    G4D_CNT_WORK = 9,     // spare
     G4D_CNT_APPEND3 = 10,
     G4D_CNT_APPEND4 = 11,
     G4D_CNT_APPEND5 = 12,
@@ -59,6 +59,7 @@ enum g4d_counter : int {
     G4D_CNT_COFACTOR = 14, // cached-cofactor computations in the LDS star kernels
     G4D_CNT_SNAP0 = 15,    // copies of list counters kept for the statistics
     G4D_CNT_SNAP1 = 16,
+    G4D_CNT_QUEUE0 = 17,   // work-queue heads of the six star-kernel launches of one pass (17..22)
     G4D_CNT_NUM = 24
 };
 

@@ -66,6 +66,7 @@ struct StarParams {
     int *exact_list;      // stars whose filters were inconclusive (filter-only kernels),
     int64_t exact_cap;    // counter `exact_counter`
     int exact_counter;
+    int work_counter; // counter word of this launch's work queue (zero at launch)
     int lds_limit;        // largest vn + m handled in LDS (<= STAR_LDS_LIMIT)
     // work list (nullptr = all stars); with nlist_dev the list length is read on the device
     // (the counter a previous kernel appended with), nlist being its capacity
@@ -79,6 +80,8 @@ struct StarParams {
 // insertions; larger ones go to p.defer_list.  exact = false: filters only (no multiprecision
 // code, no private scratch), a star with an inconclusive orientation goes to p.exact_list
 // untouched.  exact = true: exact predicates and SoS, for the stars of star_list.
+// Zeroes the counter words whose bits are set in mask (one launch instead of one fill per word).
+hipError_t launch_zero_counters(unsigned long long *cnt, uint32_t mask, hipStream_t s);
 hipError_t launch_star_kernel(const StarParams &p, int vmax, bool exact, int nblocks, hipStream_t s);
 // Global-memory star kernel for the stars in `list` (any size up to vcap link vertices);
 // each block owns a workspace of star_global_bytes(vcap) bytes at ws + blockIdx * stride.

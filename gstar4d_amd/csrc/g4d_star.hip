@@ -238,7 +238,7 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
             base = first ? (int)blockIdx.x : nitems;
             first = false;
         } else {
-            if (lane == 0) base = (int)atomicAdd(&cnt[G4D_CNT_WORK], (unsigned long long)CHUNK);
+            if (lane == 0) base = (int)atomicAdd(&cnt[p.work_counter], (unsigned long long)CHUNK);
             base = __shfl(base, 0, 64);
         }
         if (base >= nitems) break;
@@ -573,6 +573,18 @@ template <int VMAX, bool EXACT>
 __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
 {
     star_body<VMAX, EXACT>(p, nitems);
+}
+
+__global__ void k_zero_counters(unsigned long long *cnt, uint32_t mask)
+{
+    const int i = threadIdx.x;
+    if (i < 32 && ((mask >> i) & 1u)) cnt[i] = 0ull;
+}
+
+hipError_t launch_zero_counters(unsigned long long *cnt, uint32_t mask, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, cnt, mask);
+    return hipGetLastError();
 }
 
 hipError_t launch_star_kernel(const StarParams &p, int vmax, bool exact, int nblocks, hipStream_t s)
