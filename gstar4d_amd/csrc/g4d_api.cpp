@@ -574,7 +574,8 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         // list counters, filter counts and the six work-queue heads of this pass, in one launch
         {
             uint32_t zmask = (1u << G4D_CNT_APPEND2) | (1u << G4D_CNT_APPEND3) | (1u << G4D_CNT_APPEND4) |
-                             (1u << G4D_CNT_APPEND6) | (1u << G4D_CNT_ORIENT4) | (1u << G4D_CNT_COFACTOR);
+                             (1u << G4D_CNT_APPEND6) | (1u << G4D_CNT_XMID) | (1u << G4D_CNT_XBIG) |
+                             (1u << G4D_CNT_ORIENT4) | (1u << G4D_CNT_COFACTOR);
             for (int q = 0; q < 6; q++) zmask |= 1u << (G4D_CNT_QUEUE0 + q);
             HCK(g4d::launch_zero_counters(cnt, zmask, s));
         }
@@ -604,14 +605,11 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         // filter tiers: mid list (A6) -> 64, big list (A2) -> 256, beyond -> huge (A3)
         CK(chain(buf<int>(ctx->mid), G4D_CNT_APPEND6, buf<int>(ctx->big), G4D_CNT_APPEND2, g4d::STAR_VMAX_T1, false));
         CK(chain(buf<int>(ctx->big), G4D_CNT_APPEND2, buf<int>(ctx->huge), G4D_CNT_APPEND3, g4d::STAR_VMAX_T2, false));
-        // keep the tier counts for the statistics, then reuse mid/big for the exact chain
-        HCK(hipMemcpyAsync(cnt + G4D_CNT_SNAP0, cnt + G4D_CNT_APPEND6, 8, hipMemcpyDeviceToDevice, s));
-        HCK(hipMemcpyAsync(cnt + G4D_CNT_SNAP1, cnt + G4D_CNT_APPEND2, 8, hipMemcpyDeviceToDevice, s));
-        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND6, 0, 8, s));
-        HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND2, 0, 8, s));
-        CK(chain(buf<int>(ctx->xstars), G4D_CNT_APPEND4, buf<int>(ctx->mid), G4D_CNT_APPEND6, g4d::STAR_VMAX_T0, true));
-        CK(chain(buf<int>(ctx->mid), G4D_CNT_APPEND6, buf<int>(ctx->big), G4D_CNT_APPEND2, g4d::STAR_VMAX_T1, true));
-        CK(chain(buf<int>(ctx->big), G4D_CNT_APPEND2, buf<int>(ctx->huge), G4D_CNT_APPEND3, g4d::STAR_VMAX_T2, true));
+        // the exact chain reuses the mid/big lists (the filter chain is done with them in stream
+        // order) under counters of its own, so the filter tiers' counts stay for the statistics
+        CK(chain(buf<int>(ctx->xstars), G4D_CNT_APPEND4, buf<int>(ctx->mid), G4D_CNT_XMID, g4d::STAR_VMAX_T0, true));
+        CK(chain(buf<int>(ctx->mid), G4D_CNT_XMID, buf<int>(ctx->big), G4D_CNT_XBIG, g4d::STAR_VMAX_T1, true));
+        CK(chain(buf<int>(ctx->big), G4D_CNT_XBIG, buf<int>(ctx->huge), G4D_CNT_APPEND3, g4d::STAR_VMAX_T2, true));
         sp.nlist_dev = nullptr;
         CK(read_counters(ctx, hc));
         {
@@ -622,7 +620,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
             ctx->st.star_t0_cofactors += (int64_t)hc[G4D_CNT_COFACTOR];
             ctx->st.star_t0_launches++;
         }
-        const int64_t nmid = (int64_t)hc[G4D_CNT_SNAP0], nbig = (int64_t)hc[G4D_CNT_SNAP1];
+        const int64_t nmid = (int64_t)hc[G4D_CNT_APPEND6], nbig = (int64_t)hc[G4D_CNT_APPEND2];
         const int64_t nexact = (int64_t)hc[G4D_CNT_APPEND4];
         ctx->st.big_stars += (int32_t)nbig;
         ctx->st.exact_stars += nexact;

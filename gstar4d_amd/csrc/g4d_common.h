@@ -57,8 +57,8 @@ enum g4d_counter : int {
     G4D_CNT_APPEND5 = 12,
     G4D_CNT_APPEND6 = 13,
     G4D_CNT_COFACTOR = 14, // cached-cofactor computations in the LDS star kernels
-    G4D_CNT_SNAP0 = 15,    // copies of list counters kept for the statistics
-    G4D_CNT_SNAP1 = 16,
+    G4D_CNT_XMID = 15,     // list lengths of the exact chain's 64- and 256-vertex tiers
+    G4D_CNT_XBIG = 16,
     G4D_CNT_QUEUE0 = 17,   // work-queue heads of the six star-kernel launches of one pass (17..22)
     G4D_CNT_NUM = 24
 };
