@@ -12,7 +12,7 @@ import numpy as np
 
 __all__ = ["Triangulator", "Options", "Stats", "CheckReport", "G4DError", "load_library", "gdel_init",
            "gdel_compute", "get_tetra", "gdel_deinit", "canonical", "LIB_PATH", "PointFile", "read_asc",
-           "write_asc", "write_ply", "BatchResult", "batch_grid_size"]
+           "write_asc", "write_ply", "BatchResult", "batch_grid_size", "check_sizes"]
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgstar4d.so")
@@ -107,7 +107,7 @@ EXPORTS = ["g4d_default_options", "g4d_create", "g4d_destroy", "g4d_last_error",
            "g4d_diag_morton", "g4d_diag_pba", "g4d_diag_grid_pairs", "g4d_diag_copy",
            "g4d_check_result", "g4d_check_mesh", "g4d_read_asc", "g4d_free_point_file", "g4d_write_asc",
            "g4d_write_ply", "g4d_triangulate_batch", "g4d_triangulate_batch_device", "g4d_batch_grid_size",
-           "g4d_struct_sizes"]
+           "g4d_struct_sizes", "g4d_check_sizes"]
 
 
 def load_library(path=None):
@@ -144,6 +144,7 @@ def load_library(path=None):
     L.g4d_triangulate_batch_device.argtypes = [_vp, _vp, _lp, ctypes.c_int32, _ip, _lp]
     L.g4d_batch_grid_size.argtypes = [ctypes.c_int64]
     L.g4d_struct_sizes.argtypes = [_lp, ctypes.c_int32]
+    L.g4d_check_sizes.argtypes = [ctypes.c_int64, ctypes.c_int64]
     L.g4d_read_asc.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(PointFile)]
     L.g4d_free_point_file.argtypes = [ctypes.POINTER(PointFile)]
     L.g4d_free_point_file.restype = None
@@ -171,6 +172,21 @@ def read_asc(path, grid_size):
     finally:
         L.g4d_free_point_file(ctypes.byref(f))
     return pts, sc, info
+
+
+def check_sizes(sort_keys, num_tets):
+    """Status of g4d_check_sizes: 0 within the library's list and result limits, 7 (CAPACITY)
+    beyond them."""
+    return int(load_library().g4d_check_sizes(int(sort_keys), int(num_tets)))
+
+
+def _grids_arg(grids, nsets):
+    if grids is None:
+        return None
+    gr = np.ascontiguousarray(grids, dtype=np.int32)
+    if gr.ndim != 1 or gr.shape[0] != nsets:
+        raise ValueError("grids must hold one grid width per set (%d), got shape %s" % (nsets, gr.shape))
+    return gr
 
 
 def batch_grid_size(n):
@@ -272,7 +288,7 @@ class Triangulator:
         xyz = np.ascontiguousarray(xyz, dtype=np.float64).reshape(-1, 3)
         off = np.ascontiguousarray(set_offsets, dtype=np.int64)
         S = off.shape[0] - 1
-        gr = None if grids is None else np.ascontiguousarray(grids, dtype=np.int32)
+        gr = _grids_arg(grids, S)
         res = BatchResult()
         self._check(self._L.g4d_triangulate_batch(self._ctx, _ptr(xyz, _dp), _ptr(off, _lp), S,
                                                   None if gr is None else _ptr(gr, _ip), ctypes.byref(res)))
@@ -285,7 +301,7 @@ class Triangulator:
 
     def triangulate_batch_device(self, d_xyz, set_offsets, grids=None):
         off = np.ascontiguousarray(set_offsets, dtype=np.int64)
-        gr = None if grids is None else np.ascontiguousarray(grids, dtype=np.int32)
+        gr = _grids_arg(grids, off.shape[0] - 1)
         nt = ctypes.c_int64(0)
         self._check(self._L.g4d_triangulate_batch_device(self._ctx, _vp(d_xyz), _ptr(off, _lp), off.shape[0] - 1,
                                                          None if gr is None else _ptr(gr, _ip), ctypes.byref(nt)))

@@ -151,6 +151,23 @@ static hipError_t sort_pairs(g4d::Prims &P, uint64_t *k, int64_t m, int nb, hipS
     return g4d::radix_sort_u64_2(P, k, m, 0, nb, 32, 32 + nb, s);
 }
 
+// Size limits of the device primitives: the radix sort and the compactions count with 32-bit
+// histograms and offsets, and the face matching encodes (tetrahedron << 2 | face) + 1 in 32
+// bits.  Lists beyond them are refused instead of wrapping.
+static int guard_keys(g4d_ctx *ctx, int64_t keys, const char *what)
+{
+    if (g4d_check_sizes(keys, 0) == G4D_OK) return G4D_OK;
+    return set_err(ctx, G4D_ERR_CAPACITY, "%s: %lld keys exceed the sort limit of %lld", what, (long long)keys,
+                   (long long)G4D_MAX_SORT_KEYS);
+}
+
+static int guard_tets(g4d_ctx *ctx, int64_t nt)
+{
+    if (g4d_check_sizes(0, nt) == G4D_OK) return G4D_OK;
+    return set_err(ctx, G4D_ERR_CAPACITY, "%lld tetrahedra exceed the limit of %lld", (long long)nt,
+                   (long long)G4D_MAX_TETS);
+}
+
 static int read_counters(g4d_ctx *ctx, unsigned long long *h)
 {
     HCK(hipMemcpyAsync(h, ctx->cnt.ptr, sizeof(unsigned long long) * G4D_CNT_NUM, hipMemcpyDeviceToHost, ctx->stream));
@@ -407,6 +424,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     int64_t ngp = 0;
     HCK(hipMemcpyAsync(&ngp, pcnt + nthr, 8, hipMemcpyDeviceToHost, s));
     HCK(hipStreamSynchronize(s));
+    CK(guard_keys(ctx, 2 * (ngp + (int64_t)n), "grid pairs"));
     const int64_t ucap = ngp + n;
     HCK(ctx->pairs.ensure(sizeof(uint64_t) * (size_t)(2 * ucap + 2 * (int64_t)n * min_ws + 64)));
     uint64_t *pr = buf<uint64_t>(ctx->pairs);
@@ -446,6 +464,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         // pumped key is already present: sort and deduplicate the pumped keys alone (two stars
         // may pump each other) and merge them into the sorted worksets.
         int64_t np = 2 * npump;
+        CK(guard_keys(ctx, m + np, "pumped worksets"));
         uint64_t *pk = pr + m;
         HCK(sort_pairs(P, pk, np, nb, s));
         HCK(g4d::unique_u64(P, pk, np, &np, s));
@@ -536,6 +555,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         for (int k = 0; k < 3; k++) store.top[k] += need[k];
         const int64_t dcap = std::max<int64_t>(nins, 1);
         const int64_t npb = 2 * need[1];
+        CK(guard_keys(ctx, std::max(npb, nins), "star pass outputs"));
         const int64_t tpcap = npb + 4 * dcap + 64;
         HCK(ctx->drown.ensure(sizeof(uint64_t) * (size_t)dcap));
         HCK(ctx->dflag.ensure((size_t)dcap));
@@ -663,6 +683,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         // F1-F3: drowned insertions, mutual removal, containment proofs
         int64_t ntask = 0;
         if (nd > 0) {
+            CK(guard_keys(ctx, nd, "drowned insertions"));
             HCK(ctx->dkeys.ensure(sizeof(uint64_t) * (size_t)nd));
             HCK(ctx->tasks.ensure(sizeof(uint64_t) * (size_t)nd));
             uint64_t *dk = buf<uint64_t>(ctx->dkeys);
@@ -699,6 +720,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         const int64_t ncand = ntp + 4 * ntask;
         int64_t nins = 0;
         if (ncand > 0) {
+            CK(guard_keys(ctx, 2 * ncand, "splaying candidates"));
             HCK(ctx->keep.ensure((size_t)ncand));
             HCK(ctx->ins.ensure(sizeof(uint64_t) * (size_t)(2 * ncand + 2)));
             uint64_t *ins = buf<uint64_t>(ctx->ins);
@@ -768,6 +790,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     int64_t nt = 0;
     HCK(hipMemcpyAsync(&nt, toffs + n, 8, hipMemcpyDeviceToHost, s));
     HCK(hipStreamSynchronize(s));
+    CK(guard_tets(ctx, nt));
     HCK(ctx->tets.ensure(sizeof(int) * 4 * (size_t)std::max<int64_t>(nt, 1)));
     op.tets = buf<int>(ctx->tets);
     HCK(g4d::launch_out_write(op, s));
@@ -928,6 +951,13 @@ int g4d_triangulate_device(g4d_ctx *ctx, const double *d_xyz, int32_t n, int64_t
     const int r = run_device(ctx, d_xyz, n, single_set(n, ctx->opt.grid_size));
     if (r != G4D_OK) return r;
     if (num_tets) *num_tets = ctx->num_tets;
+    return G4D_OK;
+}
+
+int g4d_check_sizes(int64_t sort_keys, int64_t num_tets)
+{
+    if (sort_keys < 0 || num_tets < 0) return G4D_ERR_INVALID_INPUT;
+    if (sort_keys > G4D_MAX_SORT_KEYS || num_tets > G4D_MAX_TETS) return G4D_ERR_CAPACITY;
     return G4D_OK;
 }
 
@@ -1186,6 +1216,8 @@ static int check_device(g4d_ctx *ctx, const double *pts, int n, const int *tets,
     rep->num_tets = nt;
     rep->empty_sphere_failures = -1;
     if (nt <= 0) return G4D_OK;
+    CK(guard_tets(ctx, nt));
+    CK(guard_keys(ctx, 6 * nt, "validator edges"));
     HCK(ctx->cnt.ensure(sizeof(unsigned long long) * G4D_CNT_NUM));
     unsigned long long *cnt = buf<unsigned long long>(ctx->cnt);
     HCK(hipMemsetAsync(cnt, 0, sizeof(unsigned long long) * G4D_CNT_NUM, s));

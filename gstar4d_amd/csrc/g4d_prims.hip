@@ -325,6 +325,7 @@ hipError_t radix_sort_u64_2(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int
                             hipStream_t s)
 {
     if (n <= 1 || (hi_bit <= lo_bit && hi2 <= lo2)) return hipSuccess;
+    if (n > (int64_t)0xFFFFFFFF) return hipErrorInvalidValue; // 32-bit histograms and digit offsets
     if (hi_bit <= lo_bit) {
         lo_bit = lo2;
         hi_bit = hi2;

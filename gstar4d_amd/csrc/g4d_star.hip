@@ -575,14 +575,18 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
     star_body<VMAX, EXACT>(p, nitems);
 }
 
+static_assert(G4D_CNT_QUEUE0 + 6 <= G4D_CNT_NUM, "six queue heads must fit the counter block");
+static_assert(G4D_CNT_NUM <= 32, "counter masks are 32 bits wide");
+
 __global__ void k_zero_counters(unsigned long long *cnt, uint32_t mask)
 {
     const int i = threadIdx.x;
-    if (i < 32 && ((mask >> i) & 1u)) cnt[i] = 0ull;
+    if (i < G4D_CNT_NUM && ((mask >> i) & 1u)) cnt[i] = 0ull;
 }
 
 hipError_t launch_zero_counters(unsigned long long *cnt, uint32_t mask, hipStream_t s)
 {
+    if (G4D_CNT_NUM < 32 && (mask >> G4D_CNT_NUM) != 0u) return hipErrorInvalidValue; // beyond the block
     hipLaunchKernelGGL(k_zero_counters, dim3(1), dim3(64), 0, s, cnt, mask);
     return hipGetLastError();
 }

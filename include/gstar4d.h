@@ -115,6 +115,18 @@ typedef struct {
     int64_t empty_sphere_tests;    /* exact in-sphere tests of that check */
 } g4d_check_report;
 
+/* Size limits.  Intermediate key lists (sorted and compacted with 32-bit counts and offsets)
+   hold at most G4D_MAX_SORT_KEYS entries, and a result at most G4D_MAX_TETS tetrahedra
+   (face keys (t << 2 | i) + 1 are 32-bit).  The largest list is the reciprocated worksets,
+   about 20 keys per point for uniform inputs, and a result has about 6.8 tetrahedra per
+   point, so uniform inputs reach these limits near 150 M points; a call that would pass them
+   returns G4D_ERR_CAPACITY. */
+#define G4D_MAX_SORT_KEYS 4294967295LL
+#define G4D_MAX_TETS 1073741823LL
+/* G4D_OK when lists of sort_keys keys and num_tets tetrahedra are within the limits,
+   G4D_ERR_CAPACITY otherwise (host only; no device work). */
+int g4d_check_sizes(int64_t sort_keys, int64_t num_tets);
+
 int g4d_default_options(g4d_options *opt);
 int g4d_create(g4d_ctx **ctx, const g4d_options *opt);
 void g4d_destroy(g4d_ctx *ctx);
@@ -149,7 +161,8 @@ typedef struct {
 /* Triangulates num_sets independent point sets: set k is xyz[set_offsets[k] .. set_offsets[k+1])
    (set_offsets[0] = 0, every set at least 5 points, at most 2^24 with several sets).  Each set's
    tetrahedra are those g4d_triangulate returns for that set alone with grid_size = grids[k];
-   grids NULL picks them with g4d_batch_grid_size (several sets: at most 256).  All sets go
+   grids (NULL, or num_sets entries) NULL picks them with g4d_batch_grid_size (several sets:
+   at most 256).  All sets go
    through every stage together (one set table, one launch per stage).  Arrays belong to ctx
    until its next call. */
 int g4d_triangulate_batch(g4d_ctx *ctx, const double *xyz, const int64_t *set_offsets, int32_t num_sets,

@@ -93,6 +93,12 @@ int orc_pba_codes(const int32_t *vox_sorted, int32_t n, int32_t grid, int32_t *o
    unique, each as (min<<32 | max).  Returns count, or -needed if cap too small. */
 int64_t orc_grid_pairs(const int32_t *grid, int32_t g, uint64_t *out, int64_t cap);
 
+/* Threads of the parallel stages (PBA columns, grid-pair slabs, star creation, drowned
+   checks, proofs, insertions); < 1 selects every processor.  Returns the count in use.  The
+   result does not depend on it: every parallel loop owns whole stars or columns and its
+   outputs are placed in input order. */
+int orc_set_threads(int nthreads);
+
 /* Full triangulation.  xyz: input points (N x 3 FP64).  Output tetrahedra in MORTON index
    space, orient3d < 0, vertex order (owner-star triangle v0 v1 v2, owner star).
    tets_out (4*cap) may be NULL to query the count.  Returns tetra count or <0 on error. */

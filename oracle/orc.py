@@ -69,6 +69,7 @@ def lib():
         L.orc_canonicalize.argtypes = [_ip, ctypes.c_int64, _ip]
         L.orc_adjacency.argtypes = [_ip, ctypes.c_int64, ctypes.c_int32, _ip]
         L.orc_validate.argtypes = [_dp, ctypes.c_int32, _ip, ctypes.c_int64, ctypes.c_int, _lp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
     return _LIB
 
 
@@ -157,9 +158,17 @@ def grid_pairs(id_grid):
     return out[:m]
 
 
-def triangulate(xyz, grid, min_workset=8, max_rounds=1000, canonical=True):
+def set_threads(nthreads):
+    """Threads of the oracle's parallel stages (< 1: every processor); returns the count."""
+    return lib().orc_set_threads(int(nthreads))
+
+
+def triangulate(xyz, grid, min_workset=8, max_rounds=1000, canonical=True, threads=None):
     """Returns (tets, perm, stats).  tets are canonical (input index space, sorted) when
-    canonical=True, else Morton-space with the reference's vertex order."""
+    canonical=True, else Morton-space with the reference's vertex order.  threads: OpenMP
+    threads for this call (None keeps the current setting); the output does not depend on it."""
+    if threads is not None:
+        set_threads(threads)
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     n = xyz.shape[0]
     opt = OrcOptions(grid, min_workset, max_rounds, 0)

@@ -16,6 +16,8 @@
 #include <string.h>
 #include <time.h>
 
+#include <omp.h>
+
 #include "orc.h"
 
 #define MARKER (-1)
@@ -199,49 +201,58 @@ static inline int64_t site_coord(int32_t s, int c, const int32_t *vox_sorted, in
 static void pba_sweeps(int32_t g, int32_t *codes, const int32_t *vox_sorted, int by_id)
 {
     const int64_t G = g;
-    int32_t *cs = xmalloc(sizeof(int32_t) * g), *out = xmalloc(sizeof(int32_t) * g);
-    int64_t *cd = xmalloc(sizeof(int64_t) * g);
-    int32_t *sq = xmalloc(sizeof(int32_t) * g), *ss = xmalloc(sizeof(int32_t) * g);
-    int64_t *sd = xmalloc(sizeof(int64_t) * g);
-    /* Z pass: column (x,y), candidates are the sites in the column (offset 0) */
-    for (int64_t y = 0; y < G; y++)
-        for (int64_t x = 0; x < G; x++) {
-            for (int64_t z = 0; z < G; z++) {
-                cs[z] = codes[(z * G + y) * G + x];
-                cd[z] = 0;
-            }
-            envelope_1d(g, cs, cd, out, sq, sd, ss);
-            for (int64_t z = 0; z < G; z++) codes[(z * G + y) * G + x] = out[z];
-        }
-    /* Y pass: column (x,z); candidate at row y is the Z-pass site of (x,y,z) */
-    for (int64_t z = 0; z < G; z++)
-        for (int64_t x = 0; x < G; x++) {
-            for (int64_t y = 0; y < G; y++) {
-                const int32_t s = codes[(z * G + y) * G + x];
-                cs[y] = s;
-                if (s != MARKER) {
-                    const int64_t dz = z - site_coord(s, 2, vox_sorted, by_id);
-                    cd[y] = dz * dz;
-                }
-            }
-            envelope_1d(g, cs, cd, out, sq, sd, ss);
-            for (int64_t y = 0; y < G; y++) codes[(z * G + y) * G + x] = out[y];
-        }
-    /* X pass: row (y,z); candidate at x is the Y-pass site of (x,y,z) */
-    for (int64_t z = 0; z < G; z++)
-        for (int64_t y = 0; y < G; y++) {
+    /* the columns of one pass are independent: each thread sweeps whole columns with
+       scratch of its own */
+#pragma omp parallel
+    {
+        int32_t *cs = xmalloc(sizeof(int32_t) * g), *out = xmalloc(sizeof(int32_t) * g);
+        int64_t *cd = xmalloc(sizeof(int64_t) * g);
+        int32_t *sq = xmalloc(sizeof(int32_t) * g), *ss = xmalloc(sizeof(int32_t) * g);
+        int64_t *sd = xmalloc(sizeof(int64_t) * g);
+        /* Z pass: column (x,y), candidates are the sites in the column (offset 0) */
+#pragma omp for schedule(static)
+        for (int64_t y = 0; y < G; y++)
             for (int64_t x = 0; x < G; x++) {
-                const int32_t s = codes[(z * G + y) * G + x];
-                cs[x] = s;
-                if (s != MARKER) {
-                    const int64_t dy = y - site_coord(s, 1, vox_sorted, by_id), dz = z - site_coord(s, 2, vox_sorted, by_id);
-                    cd[x] = dy * dy + dz * dz;
+                for (int64_t z = 0; z < G; z++) {
+                    cs[z] = codes[(z * G + y) * G + x];
+                    cd[z] = 0;
                 }
+                envelope_1d(g, cs, cd, out, sq, sd, ss);
+                for (int64_t z = 0; z < G; z++) codes[(z * G + y) * G + x] = out[z];
             }
-            envelope_1d(g, cs, cd, out, sq, sd, ss);
-            for (int64_t x = 0; x < G; x++) codes[(z * G + y) * G + x] = out[x];
-        }
-    free(cs); free(out); free(cd); free(sq); free(ss); free(sd);
+        /* Y pass: column (x,z); candidate at row y is the Z-pass site of (x,y,z) */
+#pragma omp for schedule(static)
+        for (int64_t z = 0; z < G; z++)
+            for (int64_t x = 0; x < G; x++) {
+                for (int64_t y = 0; y < G; y++) {
+                    const int32_t s = codes[(z * G + y) * G + x];
+                    cs[y] = s;
+                    if (s != MARKER) {
+                        const int64_t dz = z - site_coord(s, 2, vox_sorted, by_id);
+                        cd[y] = dz * dz;
+                    }
+                }
+                envelope_1d(g, cs, cd, out, sq, sd, ss);
+                for (int64_t y = 0; y < G; y++) codes[(z * G + y) * G + x] = out[y];
+            }
+        /* X pass: row (y,z); candidate at x is the Y-pass site of (x,y,z) */
+#pragma omp for schedule(static)
+        for (int64_t z = 0; z < G; z++)
+            for (int64_t y = 0; y < G; y++) {
+                for (int64_t x = 0; x < G; x++) {
+                    const int32_t s = codes[(z * G + y) * G + x];
+                    cs[x] = s;
+                    if (s != MARKER) {
+                        const int64_t dy = y - site_coord(s, 1, vox_sorted, by_id),
+                                      dz = z - site_coord(s, 2, vox_sorted, by_id);
+                        cd[x] = dy * dy + dz * dz;
+                    }
+                }
+                envelope_1d(g, cs, cd, out, sq, sd, ss);
+                for (int64_t x = 0; x < G; x++) codes[(z * G + y) * G + x] = out[x];
+            }
+        free(cs); free(out); free(cd); free(sq); free(ss); free(sd);
+    }
 }
 
 int orc_pba_codes(const int32_t *vox_sorted, int32_t n, int32_t g, int32_t *codes)
@@ -310,34 +321,49 @@ static void grab(u64vec *out, int a, int b)
     u64_push(out, pair_key(a, b));
 }
 
-static void read_grid_pairs(const int32_t *grid, int g, u64vec *out)
+static void read_grid_pairs_rows(const int32_t *grid, int g, int y, u64vec *out)
 {
     static const int link[7] = {6, 2, 3, 1, 5, 4, 6};
-    for (int y = -1; y < g; y++)
-        for (int x = -1; x < g; x++)
-            for (int z = -1; z < g; z++) {
-                int v[8];
-                for (int c = 0; c < 8; c++)
-                    v[c] = grid_at(grid, g, x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1));
-                const int root = v[0], opp = v[7];
-                if (root == opp) continue;
-                const int has_marker = (root == MARKER) || (opp == MARKER);
-                int has_quad = 0;
-                int a = v[link[0]];
-                for (int i = 0; i < 6; i++) {
-                    const int b = v[link[i + 1]];
-                    if (a != b && a != root && a != opp && b != root && b != opp) {
-                        grab(out, root, a);
-                        grab(out, opp, a);
-                        grab(out, root, b);
-                        grab(out, opp, b);
-                        grab(out, a, b);
-                        has_quad = 1;
-                    }
-                    a = b;
+    for (int x = -1; x < g; x++)
+        for (int z = -1; z < g; z++) {
+            int v[8];
+            for (int c = 0; c < 8; c++)
+                v[c] = grid_at(grid, g, x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1));
+            const int root = v[0], opp = v[7];
+            if (root == opp) continue;
+            const int has_marker = (root == MARKER) || (opp == MARKER);
+            int has_quad = 0;
+            int a = v[link[0]];
+            for (int i = 0; i < 6; i++) {
+                const int b = v[link[i + 1]];
+                if (a != b && a != root && a != opp && b != root && b != opp) {
+                    grab(out, root, a);
+                    grab(out, opp, a);
+                    grab(out, root, b);
+                    grab(out, opp, b);
+                    grab(out, a, b);
+                    has_quad = 1;
                 }
-                if (has_quad && !has_marker) grab(out, root, opp);
+                a = b;
             }
+            if (has_quad && !has_marker) grab(out, root, opp);
+        }
+}
+
+/* Every 2x2x2 cube, including the outer Marker layer; the pairs of each y slab of cubes are
+   collected per thread and concatenated (the caller sorts them). */
+static void read_grid_pairs(const int32_t *grid, int g, u64vec *out)
+{
+    const int nt = omp_get_max_threads();
+    u64vec *part = calloc((size_t)nt, sizeof(u64vec));
+    if (!part) abort();
+#pragma omp parallel for schedule(dynamic, 4)
+    for (int y = -1; y < g; y++) read_grid_pairs_rows(grid, g, y, &part[omp_get_thread_num()]);
+    for (int t = 0; t < nt; t++) {
+        for (int64_t k = 0; k < part[t].n; k++) u64_push(out, part[t].a[k]);
+        free(part[t].a);
+    }
+    free(part);
 }
 
 int64_t orc_grid_pairs(const int32_t *grid, int32_t g, uint64_t *out, int64_t cap)
@@ -378,10 +404,33 @@ typedef struct {
     const double *pts; /* Morton-sorted FP64 points */
     int n;
     ostar *st;
-    /* scratch */
+} octx;
+
+/* Per-thread scratch of the star loops (stars are independent, so each thread owns the
+   stars it works on and only its own scratch). */
+typedef struct {
     unsigned char *vis;
     int vis_cap;
-} octx;
+    int *lbuf, lcap; /* sorted link of a star */
+    int *wbuf, wcap; /* a star's insertion list */
+} oscratch;
+
+static void scratch_free(oscratch *w)
+{
+    free(w->vis);
+    free(w->lbuf);
+    free(w->wbuf);
+    memset(w, 0, sizeof(*w));
+}
+
+static int *scratch_w(oscratch *w, int m)
+{
+    if (w->wcap < m) {
+        w->wcap = m;
+        w->wbuf = xrealloc(w->wbuf, sizeof(int) * m);
+    }
+    return w->wbuf;
+}
 
 static int orient4(const octx *c, int a, int b, int d, int e, int f)
 {
@@ -439,19 +488,20 @@ static void star_make_cone(octx *c, int s, const int *w4)
 /* Beneath/beyond insertion of v into star s (GD/GDelaunay/GDelPredKernels.cu:233-420 and
    GD/GDelaunay/GDelKernels.cu:518-783): free every triangle (a,b,c) with
    orient4(a,b,c,s,v) < 0 and fan the hole boundary to v.  Returns 0 if v is drowned. */
-static int star_insert(octx *c, int s, int v)
+static int star_insert(octx *c, oscratch *w, int s, int v)
 {
     ostar *S = &c->st[s];
-    if (c->vis_cap < S->nt) {
-        c->vis_cap = 2 * S->nt + 64;
-        c->vis = xrealloc(c->vis, c->vis_cap);
+    if (w->vis_cap < S->nt) {
+        w->vis_cap = 2 * S->nt + 64;
+        w->vis = xrealloc(w->vis, w->vis_cap);
     }
+    unsigned char *vis = w->vis;
     int nvis = 0;
     for (int t = 0; t < S->nt; t++) {
-        c->vis[t] = 0;
+        vis[t] = 0;
         if (!(S->fl[t] & TRI_ALIVE)) continue;
         if (orient4(c, S->t[t].v[0], S->t[t].v[1], S->t[t].v[2], s, v) < 0) {
-            c->vis[t] = 1;
+            vis[t] = 1;
             nvis++;
         }
     }
@@ -464,11 +514,11 @@ static int star_insert(octx *c, int s, int v)
     int nfree = 0;
     int *freel = xmalloc(sizeof(int) * nvis);
     for (int t = 0; t < S->nt; t++) {
-        if (!c->vis[t]) continue;
+        if (!vis[t]) continue;
         freel[nfree++] = t;
         for (int i = 0; i < 3; i++) {
             const int u = S->t[t].opp[i];
-            if (c->vis[u]) continue;
+            if (vis[u]) continue;
             int j = 0;
             while (j < 3 && S->t[u].opp[j] != t) j++;
             if (j == 3) {
@@ -483,7 +533,7 @@ static int star_insert(octx *c, int s, int v)
         }
     }
     for (int t = 0; t < S->nt; t++)
-        if (c->vis[t]) S->fl[t] = 0;
+        if (vis[t]) S->fl[t] = 0;
     star_reserve(S, S->nt + nb);
     int nt = S->nt;
     for (int k = 0; k < nb; k++) slots[k] = (k < nfree) ? freel[k] : nt++;
@@ -605,6 +655,14 @@ static void pairs_to_csr(const uint64_t *p, int64_t m, int n, int64_t *off)
     for (int i = 0; i <= n; i++) off[i] = 0;
     for (int64_t k = 0; k < m; k++) off[(p[k] >> 32) + 1]++;
     for (int i = 0; i < n; i++) off[i + 1] += off[i];
+}
+
+int orc_set_threads(int nthreads)
+{
+    if (nthreads < 1) nthreads = omp_get_num_procs();
+    if (nthreads > 256) nthreads = 256;
+    omp_set_num_threads(nthreads);
+    return nthreads;
 }
 
 int64_t orc_triangulate(const double *xyz, int32_t n, const orc_options *opt, int32_t *perm_out,
@@ -736,29 +794,30 @@ int64_t orc_triangulate(const double *xyz, int32_t n, const orc_options *opt, in
     for (int s = 0; s < n; s++)
         if (off[s + 1] - off[s] > st.max_workset) st.max_workset = (int)(off[s + 1] - off[s]);
 
-    /* ---- E: stars ---- */
+    /* ---- E: stars (each star is built from its own workset alone: one per thread) ---- */
     octx c;
     memset(&c, 0, sizeof(c));
     c.pts = pts;
     c.n = n;
     c.st = calloc(n, sizeof(ostar));
     if (!c.st) abort();
-    int *wbuf = NULL;
-    int wcap = 0;
-    for (int s = 0; s < n; s++) {
-        const int m = (int)(off[s + 1] - off[s]);
-        if (wcap < m) {
-            wcap = m;
-            wbuf = xrealloc(wbuf, sizeof(int) * wcap);
+#pragma omp parallel
+    {
+        oscratch w = {0};
+#pragma omp for schedule(dynamic, 256)
+        for (int s = 0; s < n; s++) {
+            const int m = (int)(off[s + 1] - off[s]);
+            int *wb = scratch_w(&w, m);
+            for (int k = 0; k < m; k++) wb[k] = (int)(uint32_t)ws.a[off[s] + k];
+            star_make_cone(&c, s, wb);
+            for (int k = 4; k < m; k++) star_insert(&c, &w, s, wb[k]);
+            ostar *S = &c.st[s];
+            S->hist = xmalloc(sizeof(int) * (m + 8));
+            S->caphist = m + 8;
+            memcpy(S->hist, wb, sizeof(int) * m);
+            S->nhist = m;
         }
-        for (int k = 0; k < m; k++) wbuf[k] = (int)(uint32_t)ws.a[off[s] + k];
-        star_make_cone(&c, s, wbuf);
-        for (int k = 4; k < m; k++) star_insert(&c, s, wbuf[k]);
-        ostar *S = &c.st[s];
-        S->hist = xmalloc(sizeof(int) * (m + 8));
-        S->caphist = m + 8;
-        memcpy(S->hist, wbuf, sizeof(int) * m);
-        S->nhist = m;
+        scratch_free(&w);
     }
     double t3 = now_s();
     st.t_star = t3 - t2;
@@ -768,29 +827,38 @@ int64_t orc_triangulate(const double *xyz, int32_t n, const orc_options *opt, in
     uint64_t *batch = ws.a; /* sorted (star, vert) pairs */
     int64_t nbatch = ws.n;
     ws.a = NULL;
-    int *lbuf = NULL;
-    int lcap = 0;
     int rounds = 0;
     for (;;) {
         if (rounds >= opt->max_rounds) {
             fprintf(stderr, "gstar4d oracle: splaying did not converge in %d rounds\n", rounds);
             break;
         }
-        /* F1: drowned insertions (GD/GDelaunay/GDelKernels.cu:1492-1527) */
+        /* F1: drowned insertions (GD/GDelaunay/GDelKernels.cu:1492-1527): a flag per entry of
+           the last batch, star groups in parallel, then an in-order compaction */
         u64vec drown = {0};
         {
-            int64_t k = 0;
-            while (k < nbatch) {
-                const int s = (int)(batch[k] >> 32);
-                int64_t e = k;
-                while (e < nbatch && (int)(batch[e] >> 32) == s) e++;
-                const int nl = star_link(&c.st[s], &lbuf, &lcap);
-                for (int64_t q = k; q < e; q++) {
-                    const int v = (int)(uint32_t)batch[q];
-                    if (!in_sorted(lbuf, nl, v)) u64_push(&drown, batch[q]);
+            unsigned char *dfl = xmalloc((size_t)nbatch + 1);
+            int64_t *gs = xmalloc(sizeof(int64_t) * ((size_t)nbatch + 1));
+            int64_t ng = 0;
+            for (int64_t k = 0; k < nbatch; k++)
+                if (k == 0 || (batch[k] >> 32) != (batch[k - 1] >> 32)) gs[ng++] = k;
+            gs[ng] = nbatch;
+#pragma omp parallel
+            {
+                oscratch w = {0};
+#pragma omp for schedule(dynamic, 256)
+                for (int64_t q = 0; q < ng; q++) {
+                    const int s = (int)(batch[gs[q]] >> 32);
+                    const int nl = star_link(&c.st[s], &w.lbuf, &w.lcap);
+                    for (int64_t e = gs[q]; e < gs[q + 1]; e++)
+                        dfl[e] = !in_sorted(w.lbuf, nl, (int)(uint32_t)batch[e]);
                 }
-                k = e;
+                scratch_free(&w);
             }
+            for (int64_t k = 0; k < nbatch; k++)
+                if (dfl[k]) u64_push(&drown, batch[k]);
+            free(dfl);
+            free(gs);
         }
         /* F2: remove mutually drowned pairs (GD/GDelaunay/GDelHost.cu:832-875) */
         u64vec ins = {0};
@@ -801,27 +869,45 @@ int64_t orc_triangulate(const double *xyz, int32_t n, const orc_options *opt, in
                 key[k] = pair_key((int)s, (int)v);
             }
             radix_u64(key, drown.n, 64);
-            /* count occurrences of each unordered pair */
-            uint64_t *proof = xmalloc(sizeof(uint64_t) * 4);
-            for (int64_t k = 0; k < drown.n; k++) {
-                const uint32_t s = (uint32_t)(drown.a[k] >> 32), v = (uint32_t)drown.a[k];
-                const uint64_t pk = pair_key((int)s, (int)v);
-                /* binary search count of pk in key[] */
-                int64_t lo = 0, hi = drown.n;
-                while (lo < hi) {
-                    int64_t mid = (lo + hi) / 2;
-                    if (key[mid] < pk) lo = mid + 1; else hi = mid;
+            /* F3: a containment proof for every drowned insertion whose pair is drowned one
+               way only; proofs land in per-entry slots, then in order into `ins` */
+            uint64_t *proof = xmalloc(sizeof(uint64_t) * 4 * (drown.n + 1));
+            unsigned char *has = xmalloc((size_t)drown.n + 1);
+            int failed = 0;
+#pragma omp parallel
+            {
+                oscratch w = {0};
+#pragma omp for schedule(dynamic, 64)
+                for (int64_t k = 0; k < drown.n; k++) {
+                    const uint32_t s = (uint32_t)(drown.a[k] >> 32), v = (uint32_t)drown.a[k];
+                    const uint64_t pk = pair_key((int)s, (int)v);
+                    /* binary search count of pk in key[] */
+                    int64_t lo = 0, hi = drown.n;
+                    while (lo < hi) {
+                        int64_t mid = (lo + hi) / 2;
+                        if (key[mid] < pk) lo = mid + 1; else hi = mid;
+                    }
+                    const int mutual = (lo + 1 < drown.n && key[lo + 1] == pk);
+                    has[k] = 0;
+                    if (mutual) continue;
+                    const int nl = star_link(&c.st[s], &w.lbuf, &w.lcap);
+                    if (nl == 0 || star_proof(&c, (int)s, (int)v, w.lbuf[0], proof + 4 * k) != 0) {
+#pragma omp critical
+                        {
+                            fprintf(stderr, "gstar4d oracle: no containment proof (star %u vert %u)\n", s, v);
+                            failed = 1;
+                        }
+                        continue;
+                    }
+                    has[k] = 1;
                 }
-                const int mutual = (lo + 1 < drown.n && key[lo + 1] == pk);
-                if (mutual) continue;
-                /* F3: containment proof */
-                const int nl = star_link(&c.st[s], &lbuf, &lcap);
-                if (nl == 0 || star_proof(&c, (int)s, (int)v, lbuf[0], proof) != 0) {
-                    fprintf(stderr, "gstar4d oracle: no containment proof (star %u vert %u)\n", s, v);
-                    abort();
-                }
-                for (int q = 0; q < 4; q++) u64_push(&ins, proof[q]);
+                scratch_free(&w);
             }
+            if (failed) abort();
+            for (int64_t k = 0; k < drown.n; k++)
+                if (has[k])
+                    for (int q = 0; q < 4; q++) u64_push(&ins, proof[4 * k + q]);
+            free(has);
             free(proof);
             free(key);
         }
@@ -861,25 +947,31 @@ int64_t orc_triangulate(const double *xyz, int32_t n, const orc_options *opt, in
         }
         radix_u64(ins.a, ins.n, 32 + nb);
         st.splay_insertions += ins.n;
-        /* F6-F9: insert per star in ascending vertex order; extend history */
+        /* F6-F9: insert per star in ascending vertex order; extend history (star groups in
+           parallel: a group touches only its own star) */
         {
-            int64_t k = 0;
-            while (k < ins.n) {
-                const int s = (int)(ins.a[k] >> 32);
-                int64_t e = k;
-                while (e < ins.n && (int)(ins.a[e] >> 32) == s) e++;
-                const int cnt = (int)(e - k);
-                if (wcap < cnt) {
-                    wcap = cnt;
-                    wbuf = xrealloc(wbuf, sizeof(int) * wcap);
+            int64_t *gs = xmalloc(sizeof(int64_t) * ((size_t)ins.n + 1));
+            int64_t ng = 0;
+            for (int64_t k = 0; k < ins.n; k++)
+                if (k == 0 || (ins.a[k] >> 32) != (ins.a[k - 1] >> 32)) gs[ng++] = k;
+            gs[ng] = ins.n;
+#pragma omp parallel
+            {
+                oscratch w = {0};
+#pragma omp for schedule(dynamic, 64)
+                for (int64_t q = 0; q < ng; q++) {
+                    const int s = (int)(ins.a[gs[q]] >> 32);
+                    const int cnt = (int)(gs[q + 1] - gs[q]);
+                    int *wb = scratch_w(&w, cnt);
+                    for (int r = 0; r < cnt; r++) {
+                        wb[r] = (int)(uint32_t)ins.a[gs[q] + r];
+                        star_insert(&c, &w, s, wb[r]);
+                    }
+                    hist_add(&c.st[s], wb, cnt);
                 }
-                for (int q = 0; q < cnt; q++) {
-                    wbuf[q] = (int)(uint32_t)ins.a[k + q];
-                    star_insert(&c, s, wbuf[q]);
-                }
-                hist_add(&c.st[s], wbuf, cnt);
-                k = e;
+                scratch_free(&w);
             }
+            free(gs);
         }
         batch = ins.a;
         nbatch = ins.n;
@@ -923,9 +1015,6 @@ int64_t orc_triangulate(const double *xyz, int32_t n, const orc_options *opt, in
         free(c.st[s].hist);
     }
     free(c.st);
-    free(c.vis);
-    free(wbuf);
-    free(lbuf);
     free(off);
     free(pts);
     free(vs);

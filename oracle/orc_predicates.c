@@ -26,17 +26,28 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <omp.h>
+
 #include "orc.h"
 
 #define ORC_EPS 1.1102230246251565e-16 /* 2^-53, unit roundoff of FP64 */
 
-static int64_t g_orient4_calls, g_exact_calls, g_sos_calls;
+/* Call counters, one padded row per OpenMP thread (summed when read), so parallel star
+   loops count without contention. */
+#define ORC_MAX_THREADS 256
+static struct {
+    int64_t orient4, exact, sos, pad[5];
+} g_cnt[ORC_MAX_THREADS];
+#define CNT (g_cnt[omp_get_thread_num() & (ORC_MAX_THREADS - 1)])
 
 int64_t orc_pred_counter(int which)
 {
-    return which == 0 ? g_orient4_calls : which == 1 ? g_exact_calls : g_sos_calls;
+    int64_t s = 0;
+    for (int t = 0; t < ORC_MAX_THREADS; t++)
+        s += which == 0 ? g_cnt[t].orient4 : which == 1 ? g_cnt[t].exact : g_cnt[t].sos;
+    return s;
 }
-void orc_pred_counter_reset(void) { g_orient4_calls = g_exact_calls = g_sos_calls = 0; }
+void orc_pred_counter_reset(void) { memset(g_cnt, 0, sizeof(g_cnt)); }
 
 /* ------------------------------------------------------------------------------------ */
 /* Expansion arithmetic                                                                  */
@@ -202,7 +213,7 @@ int orc_orient3d(const double *pa, const double *pb, const double *pc, const dou
         if (det > bound) return 1;
         if (-det > bound) return -1;
     }
-    g_exact_calls++;
+    CNT.exact++;
     return orc_orient3d_exact(pa, pb, pc, pd);
 }
 
@@ -244,7 +255,7 @@ int orc_insphere(const double *pa, const double *pb, const double *pc, const dou
         if (det > bound) return 1;
         if (-det > bound) return -1;
     }
-    g_exact_calls++;
+    CNT.exact++;
     return orc_insphere_exact(pa, pb, pc, pd, pe);
 }
 
@@ -318,13 +329,13 @@ static int col_code(char c) { return c == 'x' ? 0 : c == 'y' ? 1 : c == 'z' ? 2 
 int orc_orient4_sos_xyz(const double *p0, const double *p1, const double *p2, const double *p3,
                         const double *p4, int i0, int i1, int i2, int i3, int i4, int *depth_out)
 {
-    g_orient4_calls++;
+    CNT.orient4++;
     const int s = orc_insphere(p0, p1, p2, p3, p4);
     if (s != 0) {
         if (depth_out) *depth_out = 0;
         return s;
     }
-    g_sos_calls++;
+    CNT.sos++;
     /* sort the indices ascending, tracking the permutation parity */
     int idx[5] = {i0, i1, i2, i3, i4};
     const double *pt[5] = {p0, p1, p2, p3, p4};

@@ -35,7 +35,7 @@ def main(names):
         cfg = datasets.CONFIGS[name]
         x = datasets.make(name, seed=0)
         t0 = time.time()
-        tets, perm, st = orc.triangulate(x, cfg["grid"])
+        tets, perm, st = orc.triangulate(x, cfg["grid"], threads=0)
         dt = time.time() - t0
         data[name] = {"n": int(x.shape[0]), "grid": cfg["grid"], "seed": 0,
                       "num_tets": int(tets.shape[0]), "sha256": canonical_hash(tets),

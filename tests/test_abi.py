@@ -65,3 +65,34 @@ def test_python_structs_match_c_sizes():
     assert L.g4d_struct_sizes(out, 6) == 6
     mine = [ctypes.sizeof(c) for c in (g.Options, g.Result, g.Stats, g.CheckReport, g.PointFile, g.BatchResult)]
     assert list(out) == mine
+
+
+def test_size_guard_limits():
+    """g4d_check_sizes refuses lists the 32-bit sort offsets and face keys cannot hold."""
+    import gstar4d_amd as g
+    ok, cap = 0, 7
+    assert g.check_sizes(0, 0) == ok
+    assert g.check_sizes(2 ** 32 - 1, 0) == ok
+    assert g.check_sizes(2 ** 32, 0) == cap
+    assert g.check_sizes(0, 2 ** 30 - 1) == ok       # face key 4 t + 3 + 1 <= 2^32 - 1
+    assert g.check_sizes(0, 2 ** 30) == cap
+    assert g.check_sizes(-1, 0) == 1
+    # uniform inputs: ~20 workset keys and ~6.8 tetrahedra per point
+    n = 150_000_000
+    assert g.check_sizes(20 * n, int(6.8 * n)) == ok
+    n = 220_000_000
+    assert g.check_sizes(20 * n, int(6.8 * n)) == cap
+
+
+def test_batch_grids_length_is_checked():
+    """A grids array shorter than the set count is refused before any device call."""
+    import numpy as np
+    import gstar4d_amd as g
+    t = g.Triangulator.__new__(g.Triangulator)  # no device context: the argument check comes first
+    t._L = g.load_library()
+    t._ctx = None
+    x = np.zeros((30, 3))
+    with pytest.raises(ValueError):
+        t.triangulate_batch(x, np.array([0, 10, 20, 30]), grids=np.array([16, 16]))
+    with pytest.raises(ValueError):
+        t.triangulate_batch_device(0, np.array([0, 10, 20, 30]), grids=np.array([16, 16, 16, 16]))
