@@ -89,6 +89,10 @@ int orc_pba(const int32_t *vox_sorted, int32_t n, int32_t grid, int32_t *out_gri
 /* Site-code grid before id conversion (for kernel parity tests); codes x<<20|y<<10|z. */
 int orc_pba_codes(const int32_t *vox_sorted, int32_t n, int32_t grid, int32_t *out_codes);
 
+/* Pairs (min<<32 | max, Marker = -1) of one 2x2x2 cube of labels v[8], corner c at offset
+   (c & 1, c >> 1 & 1, c >> 2) (GD/GDelaunay/GDelKernels.cu:196-270); returns the count (<= 31),
+   duplicates included, in the reference's emission order. */
+int orc_cube_pairs(const int32_t *v, uint64_t *out);
 /* Voronoi-neighbour pairs from the id grid (GD/GDelaunay/GDelKernels.cu:137-299), sorted and
    unique, each as (min<<32 | max).  Returns count, or -needed if cap too small. */
 int64_t orc_grid_pairs(const int32_t *grid, int32_t g, uint64_t *out, int64_t cap);

@@ -70,6 +70,7 @@ def lib():
         L.orc_adjacency.argtypes = [_ip, ctypes.c_int64, ctypes.c_int32, _ip]
         L.orc_validate.argtypes = [_dp, ctypes.c_int32, _ip, ctypes.c_int64, ctypes.c_int, _lp]
         L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_cube_pairs.argtypes = [_ip, _up]
     return _LIB
 
 
@@ -147,6 +148,16 @@ def pba(vox_sorted, grid, codes=False):
     if f(_p(v, _ip), v.shape[0], grid, _p(out, _ip)) != 0:
         raise ValueError("orc_pba failed")
     return out.reshape(grid, grid, grid)  # [z, y, x]
+
+
+def cube_pairs(v8):
+    """Pairs of one 2x2x2 cube of labels (corner c at offset (c&1, c>>1&1, c>>2)), in emission
+    order, each as (min, max) with Marker = -1."""
+    v = np.ascontiguousarray(v8, dtype=np.int32)
+    out = np.zeros(32, np.uint64)
+    m = lib().orc_cube_pairs(_p(v, _ip), _p(out, _up))
+    return [(int(np.int32(np.uint32(k >> np.uint64(32)))), int(np.int32(np.uint32(k & np.uint64(0xFFFFFFFF)))))
+            for k in out[:m]]
 
 
 def grid_pairs(id_grid):

@@ -315,38 +315,51 @@ static inline int32_t grid_at(const int32_t *grid, int g, int x, int y, int z)
 }
 
 /* grabPair (GD/GDelaunay/GDelKernels.cu:155-176): skipped when the first value is Marker */
-static void grab(u64vec *out, int a, int b)
+static int grab(uint64_t *out, int m, int a, int b)
 {
-    if (a == MARKER) return;
-    u64_push(out, pair_key(a, b));
+    if (a == MARKER) return m;
+    out[m] = pair_key(a, b);
+    return m + 1;
+}
+
+/* The pairs of one 2x2x2 voxel cube (GD/GDelaunay/GDelKernels.cu:196-270): corner c holds the
+   label of voxel (x + (c & 1), y + (c >> 1 & 1), z + (c >> 2)); root = corner 0, opp = corner
+   7, link cycle 6, 2, 3, 1, 5, 4.  At most 6 * 5 + 1 pairs. */
+int orc_cube_pairs(const int32_t *v, uint64_t *out)
+{
+    static const int link[7] = {6, 2, 3, 1, 5, 4, 6};
+    const int root = v[0], opp = v[7];
+    int m = 0;
+    if (root == opp) return 0;
+    const int has_marker = (root == MARKER) || (opp == MARKER);
+    int has_quad = 0;
+    int a = v[link[0]];
+    for (int i = 0; i < 6; i++) {
+        const int b = v[link[i + 1]];
+        if (a != b && a != root && a != opp && b != root && b != opp) {
+            m = grab(out, m, root, a);
+            m = grab(out, m, opp, a);
+            m = grab(out, m, root, b);
+            m = grab(out, m, opp, b);
+            m = grab(out, m, a, b);
+            has_quad = 1;
+        }
+        a = b;
+    }
+    if (has_quad && !has_marker) m = grab(out, m, root, opp);
+    return m;
 }
 
 static void read_grid_pairs_rows(const int32_t *grid, int g, int y, u64vec *out)
 {
-    static const int link[7] = {6, 2, 3, 1, 5, 4, 6};
+    uint64_t pk[32];
     for (int x = -1; x < g; x++)
         for (int z = -1; z < g; z++) {
-            int v[8];
+            int32_t v[8];
             for (int c = 0; c < 8; c++)
                 v[c] = grid_at(grid, g, x + (c & 1), y + ((c >> 1) & 1), z + ((c >> 2) & 1));
-            const int root = v[0], opp = v[7];
-            if (root == opp) continue;
-            const int has_marker = (root == MARKER) || (opp == MARKER);
-            int has_quad = 0;
-            int a = v[link[0]];
-            for (int i = 0; i < 6; i++) {
-                const int b = v[link[i + 1]];
-                if (a != b && a != root && a != opp && b != root && b != opp) {
-                    grab(out, root, a);
-                    grab(out, opp, a);
-                    grab(out, root, b);
-                    grab(out, opp, b);
-                    grab(out, a, b);
-                    has_quad = 1;
-                }
-                a = b;
-            }
-            if (has_quad && !has_marker) grab(out, root, opp);
+            const int m = orc_cube_pairs(v, pk);
+            for (int k = 0; k < m; k++) u64_push(out, pk[k]);
         }
 }
 

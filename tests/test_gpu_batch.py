@@ -83,3 +83,51 @@ def test_batch_rejects_bad_sets():
             t.triangulate_batch(x, np.array([0, 20, 50]), np.array([64, 512]))  # batched grid above 256
         with pytest.raises(g.G4DError):
             t.triangulate_batch(x, np.array([0, 30, 30, 50]), None)  # empty set
+
+
+def test_c5_batch_4096_sets(orc):
+    """The 4096-set batched workload (sizes log-uniform in [1000, 200000], uniform or one
+    Gaussian each, about 153.5 M points): batched calls over consecutive sets of at most 24 M
+    points; every set's tetrahedra and adjacency equal its own single-set call, in order;
+    every set passes the GPU validator with the full empty-sphere test; the 64 sets with
+    oracle hashes (sizes spanning the range) match them."""
+    import hashlib
+    import json
+    import os
+    from gstar4d_amd import datasets
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "batch_golden.json")))
+    want = {e["set"]: e for e in gold["sets"]}
+    sets = datasets.batch_sets(gold["nsets"], seed=gold["seed"])
+    sizes = np.array([x.shape[0] for x in sets], np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    print("c5: %d sets, %d points" % (len(sets), int(off[-1])), flush=True)
+    chunks, a = [], 0
+    while a < len(sets):
+        b = a + 1
+        while b < len(sets) and off[b + 1] - off[a] <= 24_000_000:
+            b += 1
+        chunks.append((a, b))
+        a = b
+    hashed = checked = 0
+    with g.Triangulator(grid_size=64) as t:
+        for a, b in chunks:
+            xyz = np.concatenate(sets[a:b])
+            tets, opp, toff, gs = t.triangulate_batch(xyz, off[a:b + 1] - off[a])
+            for k in range(a, b):
+                mine = tets[toff[k - a]:toff[k - a + 1]]
+                mo = opp[toff[k - a]:toff[k - a + 1]]
+                st, so, stoff, _ = t.triangulate_batch(sets[k], np.array([0, sizes[k]]), gs[k - a:k - a + 1])
+                assert np.array_equal(mine, st) and np.array_equal(mo, so), k
+                rep = t.check_mesh(sets[k], mine, mo, full_empty_sphere=True)
+                assert rep["euler"] == 1 and rep["orientation_failures"] == 0 and rep["flat_tets"] == 0, k
+                assert rep["insphere_failures"] == 0 and rep["adjacency_failures"] == 0, k
+                assert rep["empty_sphere_failures"] == 0 and rep["empty_sphere_tests"] > 0, k
+                checked += 1
+                if k in want:
+                    c = g.canonical(mine)
+                    assert c.shape[0] == want[k]["num_tets"], k
+                    assert hashlib.sha256(np.ascontiguousarray(c, "<i4").tobytes()).hexdigest() == want[k]["sha256"], k
+                    hashed += 1
+            print("c5: sets %d..%d (%d points, grids %d..%d) equal to single-set calls, validated" %
+                  (a, b - 1, int(off[b] - off[a]), int(gs.min()), int(gs.max())), flush=True)
+    assert checked == len(sets) and hashed == len(want) == 64

@@ -61,6 +61,44 @@ def test_orient4_sos_matches_oracle(tri, orc):
     assert (depth > 0).sum() > 100
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_sos_every_depth_matches_oracle(tri, orc, mode):
+    """Tuples built to stop SoS at each depth 1..13 (all reachable depths): the GPU's exact +
+    SoS path (behind the insphere filter, and exact only) returns the oracle's sign and depth;
+    the star kernels' cofactor filter never claims a sign for them."""
+    import sos_families as F
+    rng = np.random.default_rng(41)
+    pts, idx, want_d = [], [], []
+    for depth in range(1, 14):
+        for _ in range(40):
+            p, i = F.family(rng, depth)
+            pts.append(p)
+            idx.append(i)
+            want_d.append(depth)
+    pts = np.array(pts)
+    idx = np.array(idx, np.int32)
+    sign, depth = tri.diag_orient4(pts.reshape(-1, 15), idx, mode=mode)
+    ref = [orc.orient4_sos(pts[j], [int(v) for v in idx[j]]) for j in range(pts.shape[0])]
+    assert (np.array([r[1] for r in ref]) == np.array(want_d)).all()
+    assert (sign == np.array([r[0] for r in ref])).all()
+    assert (depth == np.array(want_d)).all()
+    filt, _ = tri.diag_orient4(pts.reshape(-1, 15), idx, mode=2)
+    assert (filt == 0).all()
+
+
+def test_grid_pairs_equal_freudenthal_restatement(tri):
+    """The GPU pair kernels on random label grids against the rule restated as the Freudenthal
+    triangulation of each 2x2x2 voxel cube (no oracle involved)."""
+    from test_pair_rule import freudenthal_pairs
+    rng = np.random.default_rng(13)
+    for g_ in (4, 5, 7, 9):
+        for nlab in (3, 5, 9, 40):
+            grid = rng.integers(0, nlab, size=(g_, g_, g_)).astype(np.int32)
+            got = tri.diag_grid_pairs(grid)
+            got = sorted((int(k >> np.uint64(32)), int(k & np.uint64(0xFFFFFFFF))) for k in got)
+            assert got == freudenthal_pairs(grid), (g_, nlab)
+
+
 def test_cofactor_filter_never_wrong(tri, orc):
     rng = np.random.default_rng(12)
     t = _tuples(rng, 3000)
@@ -347,21 +385,35 @@ def test_errors_are_reported(orc):
         assert tets.shape[0] > 0
 
 
-def test_full_size_goldens(orc):
-    """C2 (4M uniform, G=512) and C3 (2M hard mix, G=512) against oracle hashes."""
+def _full_size(name, gold, check=True):
+    x = datasets.make(name)
+    print("%s: %d points generated" % (name, x.shape[0]), flush=True)
+    with g.Triangulator(grid_size=gold["grid"], adjacency=check) as t:
+        tets, _, _ = t.triangulate(x)
+        st = t.stats()
+        print("%s: %d tetrahedra, %.1f ms on the GPU, %d rounds" % (name, tets.shape[0], st["t_total_ms"],
+                                                                   st["splay_rounds"]), flush=True)
+        rep = t.check(full_empty_sphere=True) if check else None
+    got = g.canonical(tets)
+    assert got.shape[0] == gold["num_tets"], name
+    assert _h(got) == gold["sha256"], name
+    if rep is not None:
+        print("%s: validator %s" % (name, rep), flush=True)
+        assert rep["num_tets"] == gold["num_tets"] and rep["euler"] == 1
+        assert rep["orientation_failures"] == rep["flat_tets"] == rep["insphere_failures"] == 0
+        assert rep["adjacency_failures"] == 0
+        assert rep["empty_sphere_failures"] == 0 and rep["empty_sphere_tests"] > 0
+    return st
+
+
+@pytest.mark.parametrize("name", ["c2_uniform_4m_g512", "c3_hard_2m_g512", "c4_clustered_8m_g512"])
+def test_full_size_goldens(orc, name):
+    """C2 (4M uniform), C3 (2M hard mix) and C4 (8M clustered mix), all at G = 512, against the
+    oracle's hashes, and the full validator (Euler, orientation, adjacent in-sphere, adjacency,
+    every point near each tetrahedron outside its circumsphere) on the GPU result."""
     gold = json.load(open(GOLD_PIPE))
-    ran = 0
-    for name in ("c2_uniform_4m_g512", "c3_hard_2m_g512"):
-        if name not in gold:
-            continue
-        x = datasets.make(name)
-        print("%s: %d points generated" % (name, x.shape[0]), flush=True)
-        with g.Triangulator(grid_size=gold[name]["grid"], adjacency=False) as t:
-            tets, _, _ = t.triangulate(x)
-            print("%s: %d tetrahedra, %.1f ms on the GPU" % (name, tets.shape[0], t.stats()["t_total_ms"]),
-                  flush=True)
-        got = g.canonical(tets)
-        assert got.shape[0] == gold[name]["num_tets"], name
-        assert _h(got) == gold[name]["sha256"], name
-        ran += 1
-    assert ran > 0
+    if name not in gold:
+        pytest.skip("no oracle hash for %s" % name)
+    st = _full_size(name, gold[name])
+    if name == "c3_hard_2m_g512":
+        assert st["sos_calls"] > 0

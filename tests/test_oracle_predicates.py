@@ -128,3 +128,16 @@ def test_sos_equals_exact_when_nonzero(orc):
         p = rng.random((5, 3))
         s, d = orc.orient4_sos(p, [0, 1, 2, 3, 4])
         assert d == 0 and s == X.insphere(*p)
+
+
+def test_sos_reaches_every_depth_1_to_13(orc):
+    """Families built to stop SoS at each depth 1..13 (every depth reachable with distinct
+    points): the oracle stops exactly there and its sign equals the brute-force expansion."""
+    import sos_families as F
+    rng = np.random.default_rng(31)
+    for depth in range(1, 14):
+        for _ in range(6):
+            p, idx = F.family(rng, depth)
+            s, d = orc.orient4_sos(p, [int(i) for i in idx])
+            assert d == depth, (depth, p.tolist(), idx.tolist())
+            assert s == X.sos_orient4(p, [int(i) for i in idx]), (depth, p.tolist(), idx.tolist())
