@@ -1,23 +1,30 @@
 """Benchmark: points/s to the exact 3D Delaunay tetrahedralisation on one MI355X.
 
-Workload (default): 4,194,304 FP64 points i.i.d. uniform in [0,1)^3 (SplitMix64 stream, seed 0),
-PBA grid 512^3, tetrahedra with face adjacency.  One step = one full triangulation of the
-HBM-resident points (Morton sort, PBA, pairs and worksets, star creation, splaying rounds,
+Headline workload (`value`): 4,194,304 FP64 points i.i.d. uniform in [0,1)^3 (SplitMix64 stream,
+seed 0), PBA grid 512^3, tetrahedra with face adjacency.  One step = one full triangulation of
+the HBM-resident points (Morton sort, PBA, pairs and worksets, star creation, splaying rounds,
 lower-hull output, adjacency), result left in HBM.
 
-Prints one JSON line.  Roofline: the dominant kernel, the 32-vertex LDS star kernel (k_star32,
-FP64 vector work): algorithmic flops = 111 FP64 flops per filtered orient4 evaluation (the
-reference's insphere filter) times the evaluations the kernel counts on the device, over its
-launch time measured with HIP events on the library's stream; `executed_flops` is what the
-cached-cofactor filter actually spends (15 per evaluation plus 64 per cofactor computation).
-`roofline_pba`: the three PBA axis passes (HBM bound), 24 G^3 algorithmic bytes per
-triangulation.  Traffic figures come from rocprofv3 FETCH_SIZE / WRITE_SIZE runs summarised
-under profiles/ when present.  CPU baseline: the oracle's C restatement of the same pipeline,
-single thread, on a 524,288-point sample at the same density (G = 256), timed on this host.
+Sub-records on the same line (each timed the same way, inputs resident in HBM):
+  c4: 8,000,000 clustered points (70 % from 256 Gaussian clusters, sigma log-uniform in
+      [0.002, 0.05], + 30 % uniform), G = 512: the plain pipeline and, when a trained proposal
+      network is present, the learned-proposal pipeline; their ratio.
+  c5: 4096 independent sets (sizes log-uniform in [1000, 200000], uniform or one Gaussian each,
+      153.5 M points): batched calls over consecutive sets against one call per set.
+
+Roofline: the dominant kernel, the 32-vertex LDS star kernel (k_star32, FP64 vector ALU work,
+no MFMA): algorithmic flops = 111 FP64 flops per filtered orient4 evaluation (the reference's
+insphere filter) times the evaluations the kernel counts on the device, over its launch time
+measured with HIP events on the library's stream.  `roofline_pba`: the three PBA axis passes
+(HBM bound), 24 G^3 algorithmic bytes per triangulation.  Traffic figures come from the
+rocprofv3 FETCH_SIZE / WRITE_SIZE summaries under profiles/ when present.
+CPU baseline: the oracle's C restatement of the same pipeline on the 100,000-point uniform
+workload at G = 128, single-threaded and with OpenMP over this host's cores, median of 3.
 """
 import argparse
 import json
 import os
+import platform
 import statistics
 import sys
 import time
@@ -33,18 +40,52 @@ FLOPS_FILTER_EXECUTED = 15  # cached-cofactor filter: 4-term dot product and err
 FLOPS_COFACTORS = 64        # cofactors and permanents of one link triangle
 
 
-def cpu_baseline(seed=0):
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def host_threads():
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit() and int(env) > 0:
+        n = min(n, int(env))
+    return max(1, n)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(seed=0, reps=3):
+    """The oracle (C restatement of the reference's stages with its own host predicates) on the
+    100,000-point uniform workload at G = 128, 1 thread and all host threads, median of reps."""
     from gstar4d_amd import datasets
     from oracle import orc
-    n, grid = 524288, 256
-    x = datasets.uniform_cube(n, seed=seed)
-    t0 = time.perf_counter()
-    tets, _, st = orc.triangulate(x, grid, canonical=False)
-    dt = time.perf_counter() - t0
-    return {"value": round(x.shape[0] / dt, 1), "unit": "points/s", "cores": 1, "kind": "port",
-            "sample": "524288 uniform points, G=256 (same points-per-voxel as 4M at 512^3), "
-                      "oracle C restatement single-threaded, %.1f s" % dt,
-            "seconds": round(dt, 2), "num_tets": int(tets.shape[0])}
+    x = datasets.make("c1_uniform_100k_g128", seed=seed)
+    n = x.shape[0]
+    res = {}
+    for th in (1, host_threads()):
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            tets, _, _ = orc.triangulate(x, 128, canonical=False, threads=th)
+            ts.append(time.perf_counter() - t0)
+        res[th] = (statistics.median(ts), int(tets.shape[0]))
+        log("cpu baseline: %d thread(s) %.2f s" % (th, res[th][0]))
+    nall = host_threads()
+    return {"value": round(n / res[nall][0], 1), "unit": "points/s", "cores": nall, "kind": "port",
+            "sample": "100000 uniform points, G=128 (the reference's CPU-runnable case); oracle C restatement, "
+                      "OpenMP over %d threads, median of %d" % (nall, reps),
+            "seconds": round(res[nall][0], 3),
+            "single_thread": {"value": round(n / res[1][0], 1), "seconds": round(res[1][0], 3), "cores": 1},
+            "cpu_model": cpu_model(), "num_tets": res[nall][1]}
 
 
 def _profile_value(name, workload, grid, key):
@@ -59,7 +100,67 @@ def _profile_value(name, workload, grid, key):
     return rec.get(key)
 
 
-def run_batch(args):
+def time_plain(t, d_xyz, n, steps, warmup, label):
+    """Times `steps` triangulations of HBM-resident points; returns (seconds, step_ms, stats)."""
+    for _ in range(warmup):
+        t.triangulate_device(d_xyz, n)
+    step_ms, stages = [], []
+    nt = 0
+    t_begin = time.perf_counter()
+    for _ in range(steps):
+        s0 = time.perf_counter()
+        nt = t.triangulate_device(d_xyz, n)  # returns after the device work completed
+        step_ms.append((time.perf_counter() - s0) * 1e3)
+        stages.append(t.stats())
+        log("%s step %.1f ms" % (label, step_ms[-1]))
+    return time.perf_counter() - t_begin, step_ms, stages, nt
+
+
+def run_c4(args):
+    """Clustered 8M points at G = 512: plain pipeline, and the learned proposal when present."""
+    import gstar4d_amd as g
+    from gstar4d_amd import datasets
+    x = datasets.make("c4_clustered_8m_g512", seed=args.seed)
+    n = x.shape[0]
+    out = {"workload": "c4_clustered_8m_g512", "points": n, "grid": 512}
+    with g.Triangulator(grid_size=512, adjacency=True) as t:
+        d_xyz, _ = t.upload(x)
+        secs, step_ms, stages, nt = time_plain(t, d_xyz, n, args.c4_steps, 1, "c4 plain")
+        last = stages[-1]
+        out["plain"] = {"value": round(n * len(step_ms) / secs, 1), "ms_per_step": round(secs * 1e3 / len(step_ms), 3),
+                        "splay_rounds": last["splay_rounds"], "store_repacks": last["store_repacks"],
+                        "splay_insertions": last["splay_insertions"],
+                        "num_tets": int(nt), "stages_ms": _stage_means(stages)}
+        from gstar4d_amd import proposal
+        if proposal.available():
+            params, thr, rec = proposal.load_weights()
+            out["proposal"] = {"weights": os.path.relpath(proposal.WEIGHTS_PATH, ROOT), "threshold": thr,
+                               "trained_on": "%d sets of %d points, G=%d" % (rec["sets"], rec["points"], rec["grid"]),
+                               "holdout_metrics": rec.get("holdout_metrics")}
+            t.set_proposal(params, thr)
+            secs2, step2, stages2, nt2 = time_plain(t, d_xyz, n, args.c4_steps, 1, "c4 learned")
+            l2 = stages2[-1]
+            out["learned"] = {"value": round(n * len(step2) / secs2, 1),
+                              "ms_per_step": round(secs2 * 1e3 / len(step2), 3),
+                              "splay_rounds": l2["splay_rounds"], "store_repacks": l2["store_repacks"],
+                              "num_tets": int(nt2), "same_tet_count": int(nt2) == int(nt),
+                              "prop_pairs": l2["prop_pairs"], "t_prop_ms": round(l2["t_prop_ms"], 3),
+                              "splay_insertions": l2["splay_insertions"], "stages_ms": _stage_means(stages2)}
+            out["learned_vs_plain"] = round(out["learned"]["value"] / out["plain"]["value"], 3)
+            t.set_proposal(enable=False)
+        else:
+            out["learned"] = None
+            out["learned_vs_plain"] = None
+    return out
+
+
+def _stage_means(stages):
+    return {k: round(statistics.mean(s[k] for s in stages), 3) for k in
+            ("t_morton_ms", "t_pba_ms", "t_pairs_ms", "t_star_ms", "t_splay_ms", "t_out_ms", "t_adj_ms",
+             "t_total_ms")}
+
+
+def run_c5(args, steps, warmup, per_set=True):
     """Batched workload: independent sets (sizes log-uniform in [1000, 200000], each uniform or
     one Gaussian), every set's grid from g4d_batch_grid_size.  One step = all sets triangulated
     by batched calls over consecutive sets of at most --chunk-points points; the per-set baseline
@@ -82,9 +183,8 @@ def run_batch(args):
             b += 1
         chunks.append((a, b))
         a = b
-    print("batch: %d sets, %d points, %d chunks, generated in %.1f s" % (len(sizes), int(off[-1]), len(chunks),
-                                                                         time.perf_counter() - t_gen),
-          file=sys.stderr, flush=True)
+    log("batch: %d sets, %d points, %d chunks, generated in %.1f s" % (len(sizes), int(off[-1]), len(chunks),
+                                                                     time.perf_counter() - t_gen))
     t = g.Triangulator(grid_size=64, adjacency=True)
     d_xyz, n = t.upload(xyz)
 
@@ -95,64 +195,39 @@ def run_batch(args):
             nt += t.triangulate_batch_device(d_xyz + 24 * int(off[a]), sub, grids[a:b])
         return nt
 
-    for _ in range(args.warmup):
+    for _ in range(warmup):
         batch_pass()
     step_ms = []
     t_begin = time.perf_counter()
-    for _ in range(args.steps):
+    nt = 0
+    for _ in range(steps):
         s0 = time.perf_counter()
         nt = batch_pass()
         step_ms.append((time.perf_counter() - s0) * 1e3)
-        print("batch step %.1f ms" % step_ms[-1], file=sys.stderr, flush=True)
+        log("batch step %.1f ms" % step_ms[-1])
     total_s = time.perf_counter() - t_begin
-    value = n * args.steps / total_s
-    per_set = None
-    if not args.no_per_set:
+    value = n * steps / total_s
+    base = None
+    if per_set:
         s0 = time.perf_counter()
         nt1 = 0
         for k in range(len(sizes)):
             nt1 += t.triangulate_batch_device(d_xyz + 24 * int(off[k]), np.array([0, sizes[k]]), grids[k:k + 1])
-            if k % 512 == 511:
-                print("per-set %d / %d" % (k + 1, len(sizes)), file=sys.stderr, flush=True)
+            if k % 1024 == 1023:
+                log("per-set %d / %d" % (k + 1, len(sizes)))
         per_s = time.perf_counter() - s0
-        per_set = {"seconds": round(per_s, 3), "points_per_s": round(n / per_s, 1), "num_tets": int(nt1),
-                   "same_tet_count": int(nt1) == int(nt)}
+        base = {"seconds": round(per_s, 3), "value": round(n / per_s, 1), "num_tets": int(nt1),
+                "same_tet_count": int(nt1) == int(nt)}
     t.close()
-    out = {
-        "metric": METRIC, "value": round(value, 1), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(total_s * 1e3 / args.steps, 3), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": args.workload, "sets": int(len(sizes)), "points": int(n),
-                   "set_sizes": [int(sizes.min()), int(sizes.max())], "grids": [int(grids.min()), int(grids.max())],
-                   "calls_per_step": len(chunks), "chunk_points": args.chunk_points,
-                   "distribution": "per set: uniform or one Gaussian (sigma 0.1)",
-                   "output": "tetrahedra + face adjacency per set (HBM)"},
-        "num_tets": int(nt), "step_ms": [round(v, 3) for v in step_ms],
-        "per_set_baseline": per_set,
-        "batch_speedup_vs_per_set": round(per_set["seconds"] / (total_s / args.steps), 3) if per_set else None,
-    }
-    print(json.dumps(out), flush=True)
+    return {"workload": "c5_batch_4096", "value": round(value, 1), "ms_per_step": round(total_s * 1e3 / steps, 3),
+            "steps": steps, "sets": int(len(sizes)), "points": int(n),
+            "set_sizes": [int(sizes.min()), int(sizes.max())], "grids": [int(grids.min()), int(grids.max())],
+            "calls_per_step": len(chunks), "chunk_points": args.chunk_points, "num_tets": int(nt),
+            "step_ms": [round(v, 3) for v in step_ms], "per_set": base,
+            "batched_vs_per_set": round(value / base["value"], 3) if base else None}
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="c2_uniform_4m_g512")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--sets", type=int, default=4096, help="batched workload: number of point sets")
-    ap.add_argument("--chunk-points", type=int, default=24_000_000,
-                    help="batched workload: points per batched call (consecutive sets)")
-    ap.add_argument("--no-per-set", action="store_true", help="batched workload: skip the per-set baseline")
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print("bench.py measures one MI355X; runs on more than one GPU are not supported.", file=sys.stderr)
-        sys.exit(2)
-    if args.workload.startswith("c5_batch"):
-        return run_batch(args)
-
+def run_headline(args):
     import gstar4d_amd as g
     from gstar4d_amd import datasets
 
@@ -161,28 +236,17 @@ def main():
     n, grid = x.shape[0], cfg["grid"]
     t = g.Triangulator(grid_size=grid, adjacency=True)
     d_xyz, _ = t.upload(x)
-    for _ in range(args.warmup):
-        t.triangulate_device(d_xyz, n)
-    step_ms, pba_ms, stages = [], [], []
-    nt = 0
-    t_begin = time.perf_counter()
-    for _ in range(args.steps):
-        s0 = time.perf_counter()
-        nt = t.triangulate_device(d_xyz, n)  # returns after the device work completed
-        step_ms.append((time.perf_counter() - s0) * 1e3)
-        st = t.stats()
-        pba_ms.append(sum(st["t_pba_pass_ms"]))
-        stages.append(st)
-    total_s = time.perf_counter() - t_begin
+    total_s, step_ms, stages, nt = time_plain(t, d_xyz, n, args.steps, args.warmup, args.workload)
     value = n * args.steps / total_s
     # Outside the timed region: one call through the host-buffer entry point (points copied in,
     # tetrahedra and adjacency copied out), the PCIe-inclusive rate.
     h0 = time.perf_counter()
     t.triangulate(x)
     host_ms = (time.perf_counter() - h0) * 1e3
+    t.close()
     G3 = grid ** 3
     pba_bytes = 24 * G3
-    pba_avg_ms = statistics.mean(pba_ms)
+    pba_avg_ms = statistics.mean(sum(s["t_pba_pass_ms"]) for s in stages)
     achieved = pba_bytes / (pba_avg_ms * 1e-3) / 1e9
     traffic = _profile_value("pba_pass_pmc.json", args.workload, grid, "hbm_bytes_per_triangulation")
     last = stages[-1]
@@ -195,7 +259,7 @@ def main():
     exe_flops = FLOPS_FILTER_EXECUTED * evals + FLOPS_COFACTORS * cofs
     k_tflops = alg_flops / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
     star_traffic = _profile_value("star_t0_pmc.json", args.workload, grid, "hbm_bytes_per_launch")
-    out = {
+    return {
         "metric": METRIC,
         "value": round(value, 1),
         "unit": "points/s",
@@ -210,7 +274,7 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "points": n, "grid": grid, "seed": args.seed,
                    "distribution": cfg["gen"], "output": "tetrahedra + face adjacency (HBM)"},
-        "roofline": {"bound": "mfma", "compute": "FP64 vector ALU (no MFMA); latency-bound", "kernel": "k_star32",
+        "roofline": {"bound": "valu", "compute": "FP64 vector ALU (no MFMA); latency-bound", "kernel": "k_star32",
                      "achieved": round(k_tflops, 4), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(k_tflops / FP64_PEAK_TFLOPS, 5), "traffic": star_traffic,
                      "algorithmic_flops_per_launch": round(alg_flops / max(k_launch, 1)),
@@ -221,9 +285,7 @@ def main():
                          "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                          "algorithmic_bytes": pba_bytes, "avg_ms": round(pba_avg_ms / 3, 4)},
-        "stages_ms": {k: round(statistics.mean(s[k] for s in stages), 3) for k in
-                      ("t_morton_ms", "t_pba_ms", "t_pairs_ms", "t_star_ms", "t_splay_ms", "t_out_ms",
-                       "t_adj_ms", "t_total_ms")},
+        "stages_ms": _stage_means(stages),
         "pba_pass_ms": [round(v, 4) for v in last["t_pba_pass_ms"]],
         "num_tets": int(nt),
         "splay_rounds": last["splay_rounds"],
@@ -233,9 +295,43 @@ def main():
         "host_buffers": {"ms": round(host_ms, 3), "points_per_s": round(n / (host_ms * 1e-3), 1),
                          "note": "g4d_triangulate: host points in, host tetrahedra + adjacency out (PCIe-inclusive)"},
     }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c2_uniform_4m_g512")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c4", action="store_true", help="skip the clustered 8M sub-record")
+    ap.add_argument("--no-c5", action="store_true", help="skip the 4096-set batched sub-record")
+    ap.add_argument("--c4-steps", type=int, default=3)
+    ap.add_argument("--c5-steps", type=int, default=2)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--sets", type=int, default=4096, help="batched workload: number of point sets")
+    ap.add_argument("--chunk-points", type=int, default=24_000_000,
+                    help="batched workload: points per batched call (consecutive sets)")
+    ap.add_argument("--no-per-set", action="store_true", help="batched workload: skip the per-set baseline")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one MI355X; runs on more than one GPU are not supported.", file=sys.stderr)
+        sys.exit(2)
+    if args.workload.startswith("c5_batch"):
+        rec = run_c5(args, args.steps, args.warmup, per_set=not args.no_per_set)
+        out = {"metric": METRIC, "value": rec["value"], "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+               "warmup": args.warmup, "ms_per_step": rec["ms_per_step"], "higher_is_better": True,
+               "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+               "config": {"workload": args.workload, "sets": rec["sets"], "points": rec["points"]}, "c5": rec}
+        print(json.dumps(out), flush=True)
+        return
+    out = run_headline(args)
+    if not args.no_c4 and args.workload != "c4_clustered_8m_g512":
+        out["c4"] = run_c4(args)
+    if not args.no_c5:
+        out["c5"] = run_c5(args, args.c5_steps, 1, per_set=not args.no_per_set)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(args.seed)
-    t.close()
     print(json.dumps(out), flush=True)
 
 

@@ -58,7 +58,8 @@ class Stats(ctypes.Structure):
                 ("star_store_bytes", ctypes.c_int64), ("store_repacks", ctypes.c_int32),
                 ("star_passes", ctypes.c_int32), ("star_t0_launches", ctypes.c_int32), ("pad1", ctypes.c_int32),
                 ("t_star_t0_ms", ctypes.c_double), ("star_t0_filter_evals", ctypes.c_int64),
-                ("star_t0_cofactors", ctypes.c_int64)]
+                ("star_t0_cofactors", ctypes.c_int64), ("prop_enabled", ctypes.c_int32), ("pad2", ctypes.c_int32),
+                ("prop_pairs", ctypes.c_int64), ("t_prop_ms", ctypes.c_double)]
 
     def as_dict(self):
         d = {}
@@ -107,7 +108,13 @@ EXPORTS = ["g4d_default_options", "g4d_create", "g4d_destroy", "g4d_last_error",
            "g4d_diag_morton", "g4d_diag_pba", "g4d_diag_grid_pairs", "g4d_diag_copy",
            "g4d_check_result", "g4d_check_mesh", "g4d_read_asc", "g4d_free_point_file", "g4d_write_asc",
            "g4d_write_ply", "g4d_triangulate_batch", "g4d_triangulate_batch_device", "g4d_batch_grid_size",
-           "g4d_struct_sizes", "g4d_check_sizes"]
+           "g4d_struct_sizes", "g4d_check_sizes", "g4d_set_proposal", "g4d_set_sample_export", "g4d_get_samples",
+           "g4d_mlp_forward", "g4d_mlp_grad", "g4d_mlp_train"]
+
+MLP_PARAMS = 7312
+PROP_FEATURES = 12
+PROP_SLOTS = 8
+PROP_BINS = 8
 
 
 def load_library(path=None):
@@ -145,6 +152,15 @@ def load_library(path=None):
     L.g4d_batch_grid_size.argtypes = [ctypes.c_int64]
     L.g4d_struct_sizes.argtypes = [_lp, ctypes.c_int32]
     L.g4d_check_sizes.argtypes = [ctypes.c_int64, ctypes.c_int64]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    _u32p = ctypes.POINTER(ctypes.c_uint32)
+    L.g4d_set_proposal.argtypes = [_vp, _fp, ctypes.c_int32, ctypes.c_float, ctypes.c_int32]
+    L.g4d_set_sample_export.argtypes = [_vp, ctypes.c_int32]
+    L.g4d_get_samples.argtypes = [_vp, ctypes.c_int64, ctypes.c_int64, _fp, _u32p, _ip, ctypes.c_int64, _lp]
+    L.g4d_mlp_forward.argtypes = [_vp, _fp, _fp, ctypes.c_int64, _fp]
+    L.g4d_mlp_grad.argtypes = [_vp, _fp, _fp, _u32p, ctypes.c_int64, _fp, ctypes.POINTER(ctypes.c_double)]
+    L.g4d_mlp_train.argtypes = [_vp, _fp, _fp, _u32p, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_float,
+                                ctypes.c_uint64, ctypes.POINTER(ctypes.c_double)]
     L.g4d_read_asc.argtypes = [ctypes.c_char_p, ctypes.c_int32, ctypes.POINTER(PointFile)]
     L.g4d_free_point_file.argtypes = [ctypes.POINTER(PointFile)]
     L.g4d_free_point_file.restype = None
@@ -342,6 +358,65 @@ class Triangulator:
                                            None if o is None else _ptr(o, _ip), t.shape[0],
                                            int(full_empty_sphere), ctypes.byref(rep)))
         return rep.as_dict()
+
+    # ---- learned proposal ----
+    def set_proposal(self, params=None, threshold=0.0, enable=True):
+        """Turns the learned proposal on with `params` (G4D_MLP_PARAMS floats) or off."""
+        if not enable:
+            self._check(self._L.g4d_set_proposal(self._ctx, None, 0, 0.0, 0))
+            return
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        self._check(self._L.g4d_set_proposal(self._ctx, _ptr(p, ctypes.POINTER(ctypes.c_float)), p.shape[0],
+                                              float(threshold), 1))
+
+    def set_sample_export(self, enable=True):
+        self._check(self._L.g4d_set_sample_export(self._ctx, int(bool(enable))))
+
+    def get_samples(self, stride=1, offset=0, candidates=False):
+        """(features (k,12) float32, labels (k,) uint32[, candidates (k,8) int32]) of the last call."""
+        n = self.stats()["num_points"]
+        cap = max(0, (n - offset + stride - 1) // stride)
+        f = np.empty((max(cap, 1), PROP_FEATURES), np.float32)
+        y = np.empty(max(cap, 1), np.uint32)
+        c = np.empty((max(cap, 1), PROP_SLOTS), np.int32) if candidates else None
+        cnt = ctypes.c_int64(0)
+        self._check(self._L.g4d_get_samples(self._ctx, int(offset), int(stride),
+                                             _ptr(f, ctypes.POINTER(ctypes.c_float)),
+                                             _ptr(y, ctypes.POINTER(ctypes.c_uint32)),
+                                             _ptr(c, _ip) if candidates else None, cap, ctypes.byref(cnt)))
+        k = cnt.value
+        return (f[:k], y[:k], c[:k]) if candidates else (f[:k], y[:k])
+
+    def mlp_forward(self, params, features):
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        x = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, PROP_FEATURES)
+        out = np.empty((x.shape[0], 16), np.float32)
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._check(self._L.g4d_mlp_forward(self._ctx, _ptr(p, fp), _ptr(x, fp), x.shape[0], _ptr(out, fp)))
+        return out
+
+    def mlp_grad(self, params, features, labels):
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        x = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, PROP_FEATURES)
+        y = np.ascontiguousarray(labels, dtype=np.uint32).reshape(-1)
+        g = np.empty(MLP_PARAMS, np.float32)
+        loss = ctypes.c_double(0.0)
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._check(self._L.g4d_mlp_grad(self._ctx, _ptr(p, fp), _ptr(x, fp), _ptr(y, ctypes.POINTER(ctypes.c_uint32)),
+                                         x.shape[0], _ptr(g, fp), ctypes.byref(loss)))
+        return g, loss.value
+
+    def mlp_train(self, params, features, labels, epochs=4, batch=4096, lr=2e-3, seed=0):
+        """Trains a copy of params on the GPU; returns (params, mean loss per epoch)."""
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1).copy()
+        x = np.ascontiguousarray(features, dtype=np.float32).reshape(-1, PROP_FEATURES)
+        y = np.ascontiguousarray(labels, dtype=np.uint32).reshape(-1)
+        el = np.zeros(epochs, np.float64)
+        fp = ctypes.POINTER(ctypes.c_float)
+        self._check(self._L.g4d_mlp_train(self._ctx, _ptr(p, fp), _ptr(x, fp), _ptr(y, ctypes.POINTER(ctypes.c_uint32)),
+                                          x.shape[0], int(epochs), int(batch), float(lr), int(seed),
+                                          _ptr(el, ctypes.POINTER(ctypes.c_double))))
+        return p, el
 
     def stats(self):
         st = Stats()

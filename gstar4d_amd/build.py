@@ -10,8 +10,8 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgstar4d.so")
 
-SOURCES = ["g4d_prims.hip", "g4d_grid.hip", "g4d_star.hip", "g4d_check.hip", "g4d_diag.hip", "g4d_api.cpp",
-           "g4d_io.cpp"]
+SOURCES = ["g4d_prims.hip", "g4d_grid.hip", "g4d_star.hip", "g4d_proposal.hip", "g4d_check.hip", "g4d_diag.hip",
+           "g4d_api.cpp", "g4d_io.cpp"]
 
 # -ffp-contract=off: the grid scaling must round exactly like the host, and the filters'
 # error bounds are derived for separately rounded operations (explicit fma() is still used

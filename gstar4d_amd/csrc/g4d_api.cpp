@@ -22,6 +22,7 @@
 #include "g4d_diag.h"
 #include "g4d_grid.h"
 #include "g4d_prims.h"
+#include "g4d_proposal.h"
 #include "g4d_star.h"
 
 using g4d::DevBuf;
@@ -51,7 +52,8 @@ struct Store {
 };
 constexpr size_t Store::esz[3];
 
-enum { EV_START, EV_MORTON, EV_PBA0, EV_PBA1, EV_PBA2, EV_PBA3, EV_PAIRS, EV_STAR, EV_SPLAY, EV_OUT, EV_ADJ, EV_NUM };
+enum { EV_START, EV_MORTON, EV_PBA0, EV_PBA1, EV_PBA2, EV_PBA3, EV_PROP0, EV_PROP1, EV_PAIRS, EV_STAR, EV_SPLAY, EV_OUT,
+       EV_ADJ, EV_NUM };
 
 } // namespace
 
@@ -64,6 +66,11 @@ struct g4d_ctx {
     Store store;
     DevBuf settet, drown, dflag, pairbuf, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, aflags, apos, active, mid, big, huge, xstars,
         xtasks, xtris, gwork, tetcnt, tetoff, tets, opp, table;
+    // learned proposal: parameters, candidates, features, proposed pairs, capacity bins, labels
+    DevBuf mlp_params, fkeys, ford, frank, pcand, pd2, pfeat, ppairs, pcap, plab, pvcap;
+    bool prop_on = false, prop_export = false, have_samples = false;
+    float prop_threshold = 0.f;
+    int samples_n = 0;
     std::vector<int32_t> h_tets, h_opp, h_perm, h_set_grid;
     std::vector<int64_t> h_set_tet_off;
     const double *last_xyz = nullptr; // device points of the last triangulation (input order)
@@ -316,6 +323,72 @@ static int setup_sets(g4d_ctx *ctx, const SetSpec &spec, int n, g4d::GridSets &G
     return G4D_OK;
 }
 
+// Learned proposal stage.  Candidates: the 8 nearest points within windows of two fine
+// Morton orders; features of the digital Voronoi diagram; with the proposal on, the MLP's
+// chosen candidates are added to the worksets (both directions, merged into the sorted keys
+// pr[0, *m) without duplicates) and *pred_vertices receives the predicted total of final link
+// vertices.  With export on, candidates and features stay for the training samples.
+static int proposal_stage(g4d_ctx *ctx, int n, const g4d::GridSets &GS, const int *idg, uint64_t *pr, int64_t *m,
+                          const int64_t *ws_off, int nb, int64_t *pred_vertices)
+{
+    hipStream_t s = ctx->stream;
+    g4d::Prims &P = ctx->P;
+    const int S = GS.nsets;
+    const int sb = S > 1 ? bits_for(S) : 0;
+    const int cb = std::min(21, (64 - nb - sb) / 3);
+    if (cb < 6) return set_err(ctx, G4D_ERR_INVALID_INPUT, "proposal: too many points for fine Morton keys");
+    const int hi_bit = sb + 3 * cb + nb;
+    HCK(ctx->fkeys.ensure(sizeof(uint64_t) * (size_t)n));
+    HCK(ctx->ford.ensure(2 * sizeof(int) * (size_t)n));
+    HCK(ctx->frank.ensure(2 * sizeof(int) * (size_t)n));
+    int *ord = buf<int>(ctx->ford), *rank = buf<int>(ctx->frank);
+    uint64_t *fk = buf<uint64_t>(ctx->fkeys);
+    for (int o = 0; o < 2; o++) {
+        HCK(g4d::launch_fine_keys(buf<double>(ctx->pts), n, GS, cb, nb, o, fk, s));
+        HCK(g4d::radix_sort_u64(P, fk, n, nb, hi_bit, s));
+        HCK(g4d::launch_fine_ranks(fk, n, nb, ord + (size_t)o * n, rank + (size_t)o * n, s));
+    }
+    HCK(ctx->pcand.ensure(sizeof(int) * g4d::PROP_SLOTS * (size_t)n));
+    HCK(ctx->pd2.ensure(sizeof(float) * g4d::PROP_SLOTS * (size_t)n));
+    HCK(ctx->pfeat.ensure(sizeof(float) * g4d::PROP_FEATS * (size_t)n));
+    HCK(g4d::launch_candidates(buf<double>(ctx->pts), n, GS, ord, rank, ord + n, rank + n, buf<int>(ctx->pcand),
+                               buf<float>(ctx->pd2), s));
+    HCK(g4d::launch_features(buf<double>(ctx->pts), buf<uint64_t>(ctx->vox), n, GS, idg, pr, ws_off,
+                             buf<int>(ctx->pcand), buf<float>(ctx->pd2), buf<float>(ctx->pfeat), s));
+    if (!ctx->prop_on) return G4D_OK;
+    // proposals: per point up to 8 pairs (i << 32 | candidate), compacted, then both directions
+    HCK(ctx->ppairs.ensure(sizeof(uint64_t) * 2 * g4d::PROP_SLOTS * (size_t)n));
+    HCK(ctx->pcap.ensure((size_t)n));
+    uint64_t *pp = buf<uint64_t>(ctx->ppairs);
+    HCK(g4d::launch_mlp_propose(buf<float>(ctx->mlp_params), buf<float>(ctx->pfeat), buf<int>(ctx->pcand), n,
+                                ctx->prop_threshold, pp, buf<uint8_t>(ctx->pcap), s));
+    int64_t np = 0;
+    HCK(g4d::compact_valid_u64(P, pp, pp, g4d::PROP_SLOTS * (int64_t)n, g4d::STAR_PAIR_NONE, &np, s));
+    ctx->st.prop_pairs = np;
+    // predicted final link sizes (at least the current workset) for the arena sizing
+    HCK(ctx->pvcap.ensure(sizeof(int64_t) * (size_t)(n + 1)));
+    int64_t *pv = buf<int64_t>(ctx->pvcap);
+    HCK(g4d::launch_capacity_vertices(buf<uint8_t>(ctx->pcap), ws_off, n, pv, s));
+    HCK(g4d::scan_excl_i64(P, pv, pv, n, pv + n, s));
+    HCK(hipMemcpyAsync(pred_vertices, pv + n, 8, hipMemcpyDeviceToHost, s));
+    if (np > 0) {
+        CK(guard_keys(ctx, *m + 2 * np, "proposed worksets"));
+        HCK(g4d::launch_reciprocate(pp, np, s));
+        np *= 2;
+        HCK(sort_pairs(P, pp, np, nb, s));
+        HCK(g4d::unique_u64(P, pp, np, &np, s));
+        HCK(P.sort_tmp_keys.ensure(sizeof(uint64_t) * (size_t)(*m + np)));
+        uint64_t *merged = P.sort_tmp_keys.as<uint64_t>();
+        HCK(g4d::merge_u64(pr, *m, pp, np, merged, s));
+        int64_t mm = *m + np;
+        HCK(g4d::unique_u64(P, merged, mm, &mm, s)); // proposals already in a workset
+        HCK(hipMemcpyAsync(pr, merged, sizeof(uint64_t) * (size_t)mm, hipMemcpyDeviceToDevice, s));
+        *m = mm;
+    }
+    HCK(hipStreamSynchronize(s));
+    return G4D_OK;
+}
+
 static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &spec)
 {
     ctx->last_xyz = d_xyz;
@@ -347,6 +420,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     ctx->st.num_points = n;
     ctx->st.grid_size = gmax;
     ctx->have_result = false;
+    ctx->have_samples = false;
     unsigned long long hc[G4D_CNT_NUM];
 
     HCK(ctx->cnt.ensure(sizeof(unsigned long long) * G4D_CNT_NUM));
@@ -426,7 +500,8 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     HCK(hipStreamSynchronize(s));
     CK(guard_keys(ctx, 2 * (ngp + (int64_t)n), "grid pairs"));
     const int64_t ucap = ngp + n;
-    HCK(ctx->pairs.ensure(sizeof(uint64_t) * (size_t)(2 * ucap + 2 * (int64_t)n * min_ws + 64)));
+    const int64_t prop_room = ctx->prop_on ? 2 * (int64_t)g4d::PROP_SLOTS * n : 0;
+    HCK(ctx->pairs.ensure(sizeof(uint64_t) * (size_t)(2 * ucap + prop_room + 2 * (int64_t)n * min_ws + 64)));
     uint64_t *pr = buf<uint64_t>(ctx->pairs);
     HCK(g4d::launch_grid_pairs_dedup(idg, GS, nthr, pcnt, pr + ucap, tcnt, s));
     HCK(g4d::launch_tile_totals(tcnt, ntiles, ttot, s));
@@ -455,6 +530,15 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     HCK(ctx->ws_off.ensure(sizeof(int64_t) * (size_t)(n + 1)));
     int64_t *ws_off = buf<int64_t>(ctx->ws_off);
     HCK(g4d::launch_csr_offsets(pr, m, n, ws_off, s));
+    // learned proposal (plug-in point of _pumpWorksets, GD/GDelaunay/GDelHost.cu:352-529):
+    // candidates and features from the digital Voronoi diagram, then extra workset pairs
+    ev_record(ctx, EV_PROP0);
+    int64_t pred_vertices = 0;
+    if (ctx->prop_on || ctx->prop_export) {
+        CK(proposal_stage(ctx, n, GS, idg, pr, &m, ws_off, nb, &pred_vertices));
+        HCK(g4d::launch_csr_offsets(pr, m, n, ws_off, s));
+    }
+    ev_record(ctx, EV_PROP1);
     HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8, s));
     HCK(g4d::launch_pump(pr, ws_off, n, GS, min_ws, pr + m, cnt + G4D_CNT_APPEND0, 2 * (int64_t)n * min_ws + 64, s));
     CK(read_counters(ctx, hc));
@@ -541,9 +625,16 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         }
         HCK(hipStreamSynchronize(s));
         if (mode == 0) {
-            // room for the splaying rounds: their regions come to about twice the creation's
+            // room for the splaying rounds: their regions come to about twice the creation's;
+            // with the proposal's predicted link sizes, room for that many rebuilt stars too
+            int64_t want[3] = {3 * need[0], 3 * need[1], 3 * need[2]};
+            if (pred_vertices > 0) {
+                want[0] = std::max(want[0], 3 * pred_vertices);
+                want[1] = std::max(want[1], 3 * (2 * pred_vertices - 4 * (int64_t)n));
+                want[2] = std::max(want[2], 3 * pred_vertices);
+            }
             for (int k = 0; k < 3; k++)
-                HCK(store.arena[k].ensure(Store::esz[k] * (size_t)std::max<int64_t>(3 * need[k], 1024)));
+                HCK(store.arena[k].ensure(Store::esz[k] * (size_t)std::max<int64_t>(want[k], 1024)));
         } else if (store.top[0] + need[0] > store.capacity(0) || store.top[1] + need[1] > store.capacity(1) ||
                    store.top[2] + need[2] > store.capacity(2)) {
             CK(repack_store(ctx, n, need));
@@ -818,6 +909,14 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         HCK(g4d::launch_map_perm(buf<int>(ctx->tets), 4 * nt, perm, s));
     }
     ev_record(ctx, EV_ADJ);
+    if (ctx->prop_export) {
+        // training labels: which candidates ended in each star's final link, and its size bin
+        HCK(ctx->plab.ensure(sizeof(uint32_t) * (size_t)n));
+        HCK(g4d::launch_labels(buf<int>(ctx->pcand), n, buf<int>(store.vnum), buf<int64_t>(store.voff),
+                               buf<int>(store.arena[0]), buf<uint32_t>(ctx->plab), s));
+        ctx->have_samples = true;
+        ctx->samples_n = n;
+    }
     CK(read_counters(ctx, hc));
     CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], "output"));
     ctx->num_tets = nt;
@@ -836,6 +935,8 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     ctx->st.t_pba_pass_ms[1] = ev_ms(ctx, EV_PBA1, EV_PBA2);
     ctx->st.t_pba_pass_ms[2] = ev_ms(ctx, EV_PBA2, EV_PBA3);
     ctx->st.t_pairs_ms = ev_ms(ctx, EV_PBA3, EV_PAIRS);
+    ctx->st.t_prop_ms = ev_ms(ctx, EV_PROP0, EV_PROP1);
+    ctx->st.prop_enabled = ctx->prop_on ? 1 : 0;
     ctx->st.t_star_ms = ev_ms(ctx, EV_PAIRS, EV_STAR);
     ctx->st.t_splay_ms = ev_ms(ctx, EV_STAR, EV_SPLAY);
     ctx->st.t_out_ms = ev_ms(ctx, EV_SPLAY, EV_OUT);
@@ -908,7 +1009,9 @@ void g4d_destroy(g4d_ctx *ctx)
     DevBuf *bufs[] = {&ctx->xyz_up, &ctx->keys, &ctx->pts, &ctx->perm, &ctx->vox, &ctx->bounds, &ctx->settab, &ctx->ptset, &ctx->grid,
                       &ctx->gtmp, &ctx->idgrid, &ctx->pbaspill, &ctx->pcount, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->settet, &ctx->drown, &ctx->dflag, &ctx->pairbuf, &ctx->dkeys,
                       &ctx->tasks, &ctx->cand, &ctx->keep, &ctx->ins, &ctx->ins_off, &ctx->vcap, &ctx->tcap,
-                      &ctx->hcap, &ctx->aflags, &ctx->apos, &ctx->active, &ctx->mid, &ctx->big, &ctx->huge, &ctx->xstars, &ctx->xtasks, &ctx->xtris, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table};
+                      &ctx->hcap, &ctx->aflags, &ctx->apos, &ctx->active, &ctx->mid, &ctx->big, &ctx->huge, &ctx->xstars, &ctx->xtasks, &ctx->xtris, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table,
+                      &ctx->mlp_params, &ctx->fkeys, &ctx->ford, &ctx->frank, &ctx->pcand, &ctx->pd2, &ctx->pfeat,
+                      &ctx->ppairs, &ctx->pcap, &ctx->plab, &ctx->pvcap};
     for (DevBuf *b : bufs) b->release();
     ctx->store.release();
     ctx->P.release();
@@ -1075,6 +1178,168 @@ int g4d_triangulate(g4d_ctx *ctx, const double *xyz, int32_t n, g4d_result *out)
     out->opp = ctx->opt.compute_adjacency ? ctx->h_opp.data() : nullptr;
     out->perm = ctx->h_perm.data();
     out->num_points = n;
+    return G4D_OK;
+}
+
+int g4d_set_proposal(g4d_ctx *ctx, const float *params, int32_t nparams, float threshold, int32_t enable)
+{
+    if (!ctx) return G4D_ERR_INVALID_INPUT;
+    if (!enable) {
+        ctx->prop_on = false;
+        return G4D_OK;
+    }
+    if (!params || nparams != g4d::MLP_NPARAM)
+        return set_err(ctx, G4D_ERR_INVALID_INPUT, "proposal needs %d parameters", g4d::MLP_NPARAM);
+    if (!std::isfinite(threshold)) return set_err(ctx, G4D_ERR_INVALID_INPUT, "threshold must be finite");
+    HCK(hipSetDevice(ctx->opt.device));
+    HCK(ctx->mlp_params.ensure(sizeof(float) * g4d::MLP_NPARAM));
+    HCK(hipMemcpyAsync(ctx->mlp_params.ptr, params, sizeof(float) * g4d::MLP_NPARAM, hipMemcpyHostToDevice,
+                       ctx->stream));
+    HCK(hipStreamSynchronize(ctx->stream));
+    ctx->prop_threshold = threshold;
+    ctx->prop_on = true;
+    return G4D_OK;
+}
+
+int g4d_set_sample_export(g4d_ctx *ctx, int32_t enable)
+{
+    if (!ctx) return G4D_ERR_INVALID_INPUT;
+    ctx->prop_export = enable != 0;
+    return G4D_OK;
+}
+
+int g4d_get_samples(g4d_ctx *ctx, int64_t offset, int64_t stride, float *features, uint32_t *labels,
+                    int32_t *candidates, int64_t cap, int64_t *count)
+{
+    if (!ctx || !features || !labels || !count || offset < 0 || stride < 1 || cap < 0) return G4D_ERR_INVALID_INPUT;
+    if (!ctx->have_samples) return set_err(ctx, G4D_ERR_INVALID_INPUT, "no samples: enable export before the call");
+    HCK(hipSetDevice(ctx->opt.device));
+    const int n = ctx->samples_n;
+    std::vector<float> f((size_t)n * g4d::PROP_FEATS);
+    std::vector<uint32_t> y((size_t)n);
+    std::vector<int32_t> c(candidates ? (size_t)n * g4d::PROP_SLOTS : 0);
+    hipStream_t s = ctx->stream;
+    HCK(hipMemcpyAsync(f.data(), ctx->pfeat.ptr, sizeof(float) * f.size(), hipMemcpyDeviceToHost, s));
+    HCK(hipMemcpyAsync(y.data(), ctx->plab.ptr, sizeof(uint32_t) * y.size(), hipMemcpyDeviceToHost, s));
+    if (candidates) HCK(hipMemcpyAsync(c.data(), ctx->pcand.ptr, sizeof(int32_t) * c.size(), hipMemcpyDeviceToHost, s));
+    HCK(hipStreamSynchronize(s));
+    int64_t k = 0;
+    for (int64_t i = offset; i < n && k < cap; i += stride, k++) {
+        memcpy(features + k * g4d::PROP_FEATS, &f[(size_t)i * g4d::PROP_FEATS], sizeof(float) * g4d::PROP_FEATS);
+        labels[k] = y[(size_t)i];
+        if (candidates) memcpy(candidates + k * g4d::PROP_SLOTS, &c[(size_t)i * g4d::PROP_SLOTS], sizeof(int32_t) * g4d::PROP_SLOTS);
+    }
+    *count = k;
+    return G4D_OK;
+}
+
+int g4d_mlp_forward(g4d_ctx *ctx, const float *params, const float *features, int64_t n, float *logits)
+{
+    if (!ctx || !params || !features || !logits || n < 0) return G4D_ERR_INVALID_INPUT;
+    if (n == 0) return G4D_OK;
+    HCK(hipSetDevice(ctx->opt.device));
+    hipStream_t s = ctx->stream;
+    DevBuf dp, df, dl;
+    HCK(dp.ensure(sizeof(float) * g4d::MLP_NPARAM));
+    HCK(df.ensure(sizeof(float) * g4d::PROP_FEATS * (size_t)n));
+    HCK(dl.ensure(sizeof(float) * g4d::MLP_OUT * (size_t)n));
+    HCK(hipMemcpyAsync(dp.ptr, params, sizeof(float) * g4d::MLP_NPARAM, hipMemcpyHostToDevice, s));
+    HCK(hipMemcpyAsync(df.ptr, features, sizeof(float) * g4d::PROP_FEATS * (size_t)n, hipMemcpyHostToDevice, s));
+    HCK(g4d::launch_mlp_forward(buf<float>(dp), buf<float>(df), n, buf<float>(dl), s));
+    HCK(hipMemcpyAsync(logits, dl.ptr, sizeof(float) * g4d::MLP_OUT * (size_t)n, hipMemcpyDeviceToHost, s));
+    HCK(hipStreamSynchronize(s));
+    dp.release();
+    df.release();
+    dl.release();
+    return G4D_OK;
+}
+
+int g4d_mlp_grad(g4d_ctx *ctx, const float *params, const float *features, const uint32_t *labels, int64_t n,
+                 float *grad, double *loss)
+{
+    if (!ctx || !params || !features || !labels || !grad || !loss || n < 0) return G4D_ERR_INVALID_INPUT;
+    HCK(hipSetDevice(ctx->opt.device));
+    hipStream_t s = ctx->stream;
+    DevBuf dp, df, dy, dg, dloss;
+    HCK(dp.ensure(sizeof(float) * g4d::MLP_NPARAM));
+    HCK(dg.ensure(sizeof(float) * g4d::MLP_NPARAM));
+    HCK(dloss.ensure(sizeof(double)));
+    HCK(df.ensure(sizeof(float) * g4d::PROP_FEATS * (size_t)std::max<int64_t>(n, 1)));
+    HCK(dy.ensure(sizeof(uint32_t) * (size_t)std::max<int64_t>(n, 1)));
+    HCK(hipMemcpyAsync(dp.ptr, params, sizeof(float) * g4d::MLP_NPARAM, hipMemcpyHostToDevice, s));
+    if (n > 0) {
+        HCK(hipMemcpyAsync(df.ptr, features, sizeof(float) * g4d::PROP_FEATS * (size_t)n, hipMemcpyHostToDevice, s));
+        HCK(hipMemcpyAsync(dy.ptr, labels, sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice, s));
+    }
+    HCK(g4d::launch_fill_f32(buf<float>(dg), g4d::MLP_NPARAM, 0.f, s));
+    HCK(hipMemsetAsync(dloss.ptr, 0, sizeof(double), s));
+    HCK(g4d::launch_mlp_grad(buf<float>(dp), buf<float>(df), buf<uint32_t>(dy), nullptr, n, buf<float>(dg),
+                             buf<double>(dloss), s));
+    HCK(hipMemcpyAsync(grad, dg.ptr, sizeof(float) * g4d::MLP_NPARAM, hipMemcpyDeviceToHost, s));
+    HCK(hipMemcpyAsync(loss, dloss.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
+    HCK(hipStreamSynchronize(s));
+    dp.release();
+    df.release();
+    dy.release();
+    dg.release();
+    dloss.release();
+    return G4D_OK;
+}
+
+int g4d_mlp_train(g4d_ctx *ctx, float *params, const float *features, const uint32_t *labels, int64_t n,
+                  int32_t epochs, int32_t batch, float lr, uint64_t seed, double *epoch_loss)
+{
+    if (!ctx || !params || !features || !labels || n <= 0 || epochs < 1 || batch < 32 || !(lr > 0.f))
+        return G4D_ERR_INVALID_INPUT;
+    if (n > INT32_MAX) return set_err(ctx, G4D_ERR_INVALID_INPUT, "at most 2^31 - 1 samples");
+    HCK(hipSetDevice(ctx->opt.device));
+    hipStream_t s = ctx->stream;
+    DevBuf dp, dm, dv, dg, dloss, df, dy, didx;
+    const size_t np = g4d::MLP_NPARAM;
+    HCK(dp.ensure(sizeof(float) * np));
+    HCK(dm.ensure(sizeof(float) * np));
+    HCK(dv.ensure(sizeof(float) * np));
+    HCK(dg.ensure(sizeof(float) * np));
+    HCK(dloss.ensure(sizeof(double)));
+    HCK(df.ensure(sizeof(float) * g4d::PROP_FEATS * (size_t)n));
+    HCK(dy.ensure(sizeof(uint32_t) * (size_t)n));
+    HCK(didx.ensure(sizeof(int) * (size_t)n));
+    HCK(hipMemcpyAsync(dp.ptr, params, sizeof(float) * np, hipMemcpyHostToDevice, s));
+    HCK(hipMemcpyAsync(df.ptr, features, sizeof(float) * g4d::PROP_FEATS * (size_t)n, hipMemcpyHostToDevice, s));
+    HCK(hipMemcpyAsync(dy.ptr, labels, sizeof(uint32_t) * (size_t)n, hipMemcpyHostToDevice, s));
+    HCK(g4d::launch_fill_f32(buf<float>(dm), (int64_t)np, 0.f, s));
+    HCK(g4d::launch_fill_f32(buf<float>(dv), (int64_t)np, 0.f, s));
+    HCK(g4d::launch_fill_f32(buf<float>(dg), (int64_t)np, 0.f, s));
+    std::vector<int> order((size_t)n);
+    for (int64_t i = 0; i < n; i++) order[(size_t)i] = (int)i;
+    uint64_t rs = seed * 0x9E3779B97F4A7C15ull + 0x2545F4914F6CDD1Dull;
+    int step = 0;
+    for (int e = 0; e < epochs; e++) {
+        for (int64_t i = n - 1; i > 0; i--) { // Fisher-Yates with xorshift64*
+            rs ^= rs >> 12;
+            rs ^= rs << 25;
+            rs ^= rs >> 27;
+            const uint64_t r = (rs * 0x2545F4914F6CDD1Dull) % (uint64_t)(i + 1);
+            std::swap(order[(size_t)i], order[(size_t)r]);
+        }
+        HCK(hipMemcpyAsync(didx.ptr, order.data(), sizeof(int) * (size_t)n, hipMemcpyHostToDevice, s));
+        HCK(hipMemsetAsync(dloss.ptr, 0, sizeof(double), s));
+        for (int64_t b0 = 0; b0 < n; b0 += batch) {
+            const int64_t bs = std::min<int64_t>(batch, n - b0);
+            HCK(g4d::launch_mlp_grad(buf<float>(dp), buf<float>(df), buf<uint32_t>(dy), buf<int>(didx) + b0, bs,
+                                     buf<float>(dg), buf<double>(dloss), s));
+            HCK(g4d::launch_adam(buf<float>(dp), buf<float>(dm), buf<float>(dv), buf<float>(dg), 1.f / (float)bs, lr,
+                                 ++step, s));
+        }
+        double tot = 0.0;
+        HCK(hipMemcpyAsync(&tot, dloss.ptr, sizeof(double), hipMemcpyDeviceToHost, s));
+        HCK(hipStreamSynchronize(s));
+        if (epoch_loss) epoch_loss[e] = tot / (double)n;
+    }
+    HCK(hipMemcpyAsync(params, dp.ptr, sizeof(float) * np, hipMemcpyDeviceToHost, s));
+    HCK(hipStreamSynchronize(s));
+    DevBuf *all[] = {&dp, &dm, &dv, &dg, &dloss, &df, &dy, &didx};
+    for (DevBuf *b : all) b->release();
     return G4D_OK;
 }
 

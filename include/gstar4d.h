@@ -98,6 +98,13 @@ typedef struct {
     double t_star_t0_ms;
     int64_t star_t0_filter_evals;
     int64_t star_t0_cofactors;
+    /* Learned proposal: enabled for the call, candidate pairs it added (one direction, before
+       removing those already in a workset), time of the proposal stage (candidates, features,
+       network, merge). */
+    int32_t prop_enabled;
+    int32_t pad2;
+    int64_t prop_pairs;
+    double t_prop_ms;
 } g4d_stats;
 
 /* Mesh validation report (GDelaunay/Common/Geometry.cu:200-393 plus adjacency and a full
@@ -172,6 +179,45 @@ int g4d_triangulate_batch_device(g4d_ctx *ctx, const double *d_xyz, const int64_
                                  const int32_t *grids, int64_t *num_tets);
 /* The grid width chosen for a set of n points: about 24 voxels per point, in steps of 16, 32..256. */
 int g4d_batch_grid_size(int64_t n);
+
+/* ---- Learned proposal (no reference counterpart; it plugs in where the reference builds
+   worksets, GD/GDelaunay/GDelHost.cu:352-529, and sizes star storage, :1139-1186).  For every
+   point the 8 nearest points among its neighbours in two fine Morton orders are candidates;
+   12 features of the digital Voronoi diagram feed an MLP 12 -> 64 -> 64 -> 16 (ReLU; BF16
+   weights, FP32 accumulation on the MFMA) whose first 8 outputs score the candidates and
+   last 8 the final link-size bin.  Candidates scoring above the threshold join the worksets
+   (both directions); predicted link sizes pre-size the star arenas.  Splaying still runs to
+   full consistency, so the tetrahedra are those of the plain pipeline. ---- */
+#define G4D_PROP_FEATURES 12
+#define G4D_PROP_SLOTS 8
+#define G4D_PROP_BINS 8
+/* Parameters (FP32): W1[32][64] (rows 12..31 zero), b1[64], W2[64][64], b2[64], W3[64][16],
+   b3[16]; W[i][o] maps input i to output o. */
+#define G4D_MLP_PARAMS 7312
+/* enable != 0 with params (G4D_MLP_PARAMS floats, copied) turns the proposal on for the
+   following calls on ctx; a candidate is proposed when its logit exceeds `threshold`. */
+int g4d_set_proposal(g4d_ctx *ctx, const float *params, int32_t nparams, float threshold, int32_t enable);
+/* enable != 0 keeps each call's features, candidates and labels (which candidates are in the
+   final link, and the link-size bin) for g4d_get_samples. */
+int g4d_set_sample_export(g4d_ctx *ctx, int32_t enable);
+/* Copies samples of the last call: points offset, offset + stride, ... (at most cap):
+   features (12 floats), label words (bit j: candidate j in the final link; bit 8 + j:
+   candidate j exists; bits 16..18: link-size bin) and candidates (8 ids, -1 when absent; may
+   be NULL).  *count receives the number copied. */
+int g4d_get_samples(g4d_ctx *ctx, int64_t offset, int64_t stride, float *features, uint32_t *labels,
+                    int32_t *candidates, int64_t cap, int64_t *count);
+/* Network outputs (16 logits per sample) of n samples (host arrays). */
+int g4d_mlp_forward(g4d_ctx *ctx, const float *params, const float *features, int64_t n, float *logits);
+/* Gradient of the summed loss over n samples (binary cross-entropy with logits over the
+   existing candidates plus the link-size-bin cross-entropy) with respect to the parameters,
+   and the summed loss. */
+int g4d_mlp_grad(g4d_ctx *ctx, const float *params, const float *features, const uint32_t *labels, int64_t n,
+                 float *grad, double *loss);
+/* Trains params in place on the GPU: Adam (betas 0.9, 0.999), minibatches of `batch` samples
+   in a seeded shuffled order, `epochs` passes; epoch_loss (may be NULL) receives the mean loss
+   per sample of each epoch. */
+int g4d_mlp_train(g4d_ctx *ctx, float *params, const float *features, const uint32_t *labels, int64_t n,
+                  int32_t epochs, int32_t batch, float lr, uint64_t seed, double *epoch_loss);
 
 /* Validate the last result of ctx on the GPU (TetraMesh::check, GD/Common/Geometry.cu:200-393);
    full_empty_sphere != 0 adds the empty-sphere test of every point near each tetrahedron. */

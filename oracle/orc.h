@@ -109,6 +109,12 @@ int orc_set_threads(int nthreads);
 int64_t orc_triangulate(const double *xyz, int32_t n, const orc_options *opt,
                         int32_t *perm_out, int32_t *tets_out, int64_t cap, orc_stats *stats);
 
+/* The same with extra candidate pairs (a << 32 | b, Morton index space) added to the
+   worksets in both directions before pumping (the learned proposal's plug-in point,
+   GD/GDelaunay/GDelHost.cu:352-529); pairs with a == b or ids out of range are ignored. */
+int64_t orc_triangulate_ex(const double *xyz, int32_t n, const orc_options *opt, const uint64_t *extra,
+                           int64_t n_extra, int32_t *perm_out, int32_t *tets_out, int64_t cap, orc_stats *stats);
+
 /* Canonical form: map tets through perm into input index space, sort each quadruple
    ascending, sort the list lexicographically.  In place. */
 void orc_canonicalize(int32_t *tets, int64_t nt, const int32_t *perm);

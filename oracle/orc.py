@@ -66,6 +66,9 @@ def lib():
         L.orc_triangulate.argtypes = [_dp, ctypes.c_int32, ctypes.POINTER(OrcOptions), _ip, _ip,
                                       ctypes.c_int64, ctypes.POINTER(OrcStats)]
         L.orc_triangulate.restype = ctypes.c_int64
+        L.orc_triangulate_ex.argtypes = [_dp, ctypes.c_int32, ctypes.POINTER(OrcOptions), _up, ctypes.c_int64,
+                                         _ip, _ip, ctypes.c_int64, ctypes.POINTER(OrcStats)]
+        L.orc_triangulate_ex.restype = ctypes.c_int64
         L.orc_canonicalize.argtypes = [_ip, ctypes.c_int64, _ip]
         L.orc_adjacency.argtypes = [_ip, ctypes.c_int64, ctypes.c_int32, _ip]
         L.orc_validate.argtypes = [_dp, ctypes.c_int32, _ip, ctypes.c_int64, ctypes.c_int, _lp]
@@ -174,10 +177,12 @@ def set_threads(nthreads):
     return lib().orc_set_threads(int(nthreads))
 
 
-def triangulate(xyz, grid, min_workset=8, max_rounds=1000, canonical=True, threads=None):
+def triangulate(xyz, grid, min_workset=8, max_rounds=1000, canonical=True, threads=None, extra_pairs=None):
     """Returns (tets, perm, stats).  tets are canonical (input index space, sorted) when
     canonical=True, else Morton-space with the reference's vertex order.  threads: OpenMP
-    threads for this call (None keeps the current setting); the output does not depend on it."""
+    threads for this call (None keeps the current setting); the output does not depend on it.
+    extra_pairs: uint64 (a << 32 | b) candidate pairs in Morton index space added to the
+    worksets (the proposal's plug-in point)."""
     if threads is not None:
         set_threads(threads)
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
@@ -187,14 +192,18 @@ def triangulate(xyz, grid, min_workset=8, max_rounds=1000, canonical=True, threa
     st = OrcStats()
     cap = 8 * n + 64
     tets = np.empty((cap, 4), np.int32)
-    cnt = lib().orc_triangulate(_p(xyz), n, ctypes.byref(opt), _p(perm, _ip), _p(tets, _ip),
-                                cap, ctypes.byref(st))
+    ex = np.ascontiguousarray(extra_pairs if extra_pairs is not None else np.zeros(1, np.uint64), dtype=np.uint64)
+    nex = 0 if extra_pairs is None else ex.shape[0]
+
+    def call(t, c):
+        return lib().orc_triangulate_ex(_p(xyz), n, ctypes.byref(opt), _p(ex, _up), nex, _p(perm, _ip),
+                                        _p(t, _ip), c, ctypes.byref(st))
+    cnt = call(tets, cap)
     if cnt < 0:
         raise ValueError("orc_triangulate failed with %d" % cnt)
     if cnt > cap:
         tets = np.empty((cnt, 4), np.int32)
-        cnt = lib().orc_triangulate(_p(xyz), n, ctypes.byref(opt), _p(perm, _ip),
-                                    _p(tets, _ip), cnt, ctypes.byref(st))
+        cnt = call(tets, cnt)
     tets = tets[:cnt]
     if canonical:
         lib().orc_canonicalize(_p(tets, _ip), cnt, _p(perm, _ip))

@@ -681,6 +681,12 @@ int orc_set_threads(int nthreads)
 int64_t orc_triangulate(const double *xyz, int32_t n, const orc_options *opt, int32_t *perm_out,
                         int32_t *tets_out, int64_t cap, orc_stats *stats)
 {
+    return orc_triangulate_ex(xyz, n, opt, NULL, 0, perm_out, tets_out, cap, stats);
+}
+
+int64_t orc_triangulate_ex(const double *xyz, int32_t n, const orc_options *opt, const uint64_t *extra,
+                           int64_t n_extra, int32_t *perm_out, int32_t *tets_out, int64_t cap, orc_stats *stats)
+{
     orc_stats st;
     memset(&st, 0, sizeof(st));
     st.num_points = n;
@@ -749,6 +755,14 @@ int64_t orc_triangulate(const double *xyz, int32_t n, const orc_options *opt, in
         free(gp.a);
     }
     free(grid);
+    /* extra candidate pairs (a learned proposal, Morton index space), both directions; they
+       change which vertices the stars start from, never the result */
+    for (int64_t k = 0; k < n_extra; k++) {
+        const uint32_t a = (uint32_t)(extra[k] >> 32), b = (uint32_t)extra[k];
+        if (a == b || a >= (uint32_t)n || b >= (uint32_t)n) continue;
+        u64_push(&ws, ((uint64_t)a << 32) | b);
+        u64_push(&ws, ((uint64_t)b << 32) | a);
+    }
     const int nb = bits_for(n);
     radix_u64(ws.a, ws.n, 32 + nb);
     ws.n = unique_u64(ws.a, ws.n);
