@@ -88,16 +88,18 @@ def cpu_baseline(seed=0, reps=3):
             "cpu_model": cpu_model(), "num_tets": res[nall][1]}
 
 
-def _profile_value(name, workload, grid, key):
-    """A figure from a rocprofv3 summary under profiles/ for this workload, or None."""
-    path = os.path.join(ROOT, "profiles", name)
+def _traffic(workload, kernel):
+    """HBM bytes per dispatch of the kernels whose names contain `kernel`, summed, from the
+    rocprofv3 traffic summary of this workload under profiles/ (None when absent)."""
+    path = os.path.join(ROOT, "profiles", "%s_traffic.json" % workload)
     if not os.path.exists(path):
         return None
     with open(path) as f:
         rec = json.load(f)
-    if rec.get("grid") != grid or rec.get("workload", workload) != workload:
+    if rec.get("workload") != workload:
         return None
-    return rec.get(key)
+    vals = [v["hbm_bytes"] for k, v in rec["per_dispatch"].items() if kernel in k]
+    return round(sum(vals)) if vals else None
 
 
 def time_plain(t, d_xyz, n, steps, warmup, label):
@@ -248,7 +250,7 @@ def run_headline(args):
     pba_bytes = 24 * G3
     pba_avg_ms = statistics.mean(sum(s["t_pba_pass_ms"]) for s in stages)
     achieved = pba_bytes / (pba_avg_ms * 1e-3) / 1e9
-    traffic = _profile_value("pba_pass_pmc.json", args.workload, grid, "hbm_bytes_per_triangulation")
+    traffic = _traffic(args.workload, "k_pba_pass")
     last = stages[-1]
     # dominant kernel: k_star32
     k_ms = statistics.mean(s["t_star_t0_ms"] for s in stages)
@@ -258,7 +260,7 @@ def run_headline(args):
     alg_flops = FLOPS_PER_ORIENT4 * evals
     exe_flops = FLOPS_FILTER_EXECUTED * evals + FLOPS_COFACTORS * cofs
     k_tflops = alg_flops / (k_ms * 1e-3) / 1e12 if k_ms > 0 else 0.0
-    star_traffic = _profile_value("star_t0_pmc.json", args.workload, grid, "hbm_bytes_per_launch")
+    star_traffic = _traffic(args.workload, "k_star32")
     return {
         "metric": METRIC,
         "value": round(value, 1),
@@ -297,6 +299,121 @@ def run_headline(args):
     }
 
 
+# ---------------------------------------------------------------------------------------
+# Profiling support: the workload driver that rocprofv3 runs, and the summarisers that turn
+# its CSV output into the files under profiles/.
+#
+#   rocprofv3 --kernel-trace --stats --output-format csv -d OUT/trace -o run -- \
+#       python3 bench.py --profile-run --workload c2_uniform_4m_g512 --steps 1
+#   rocprofv3 --pmc FETCH_SIZE --output-format csv -d OUT/fetch -o run -- python3 bench.py --profile-run ...
+#   rocprofv3 --pmc WRITE_SIZE --output-format csv -d OUT/write -o run -- python3 bench.py --profile-run ...
+#   python bench.py --summarize-trace OUT/trace --out profiles/<kernel>_<workload>_kernel_stats.txt --cmd "..."
+#   python bench.py --summarize-traffic OUT/fetch OUT/write --kernels "k_star32|k_pba_pass" --out ... --cmd "..."
+#   python bench.py --summarize-counters OUT/c1 OUT/c2 ... --kernels "k_star32" --out ... --cmd "..."
+# ---------------------------------------------------------------------------------------
+
+def profile_run(args):
+    """One plain copy kernel of known size (it calibrates the HBM traffic counters), then
+    `steps` triangulations of the workload on HBM-resident points (learned proposal on when
+    --learned and weights are present)."""
+    import gstar4d_amd as g
+    from gstar4d_amd import datasets
+    cfg = datasets.CONFIGS[args.workload]
+    x = datasets.make(args.workload, seed=args.seed)
+    with g.Triangulator(grid_size=cfg["grid"], adjacency=True) as t:
+        copy_bytes = args.copy_mib << 20
+        copy_ms = t.diag_copy(copy_bytes)
+        if args.learned:
+            from gstar4d_amd import proposal
+            params, thr, _ = proposal.load_weights()
+            t.set_proposal(params, thr)
+        d, n = t.upload(x)
+        nt = 0
+        for k in range(args.steps):
+            nt = t.triangulate_device(d, n)
+            log("profile step %d done" % k)
+        st = t.stats()
+    print(json.dumps({"workload": args.workload, "points": n, "num_tets": nt, "copy_bytes": copy_bytes,
+                      "copy_ms": round(copy_ms, 4), "copy_GBps": round(2 * copy_bytes / copy_ms / 1e6, 1),
+                      "t_pba_pass_ms": st["t_pba_pass_ms"], "t_total_ms": st["t_total_ms"]}), flush=True)
+
+
+def _csv_rows(path):
+    import csv
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _find(root, suffix):
+    for dp, _, files in sorted(os.walk(root)):
+        for fn in sorted(files):
+            if fn.endswith(suffix):
+                return os.path.join(dp, fn)
+    raise FileNotFoundError("no *%s under %s" % (suffix, root))
+
+
+def summarize_trace(prof_dir, command, top=30):
+    """Text table of a rocprofv3 --kernel-trace --stats run (its kernel_stats.csv)."""
+    path = _find(prof_dir, "kernel_stats.csv")
+    rows = _csv_rows(path)
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    total = sum(float(r["TotalDurationNs"]) for r in rows)
+    out = ["command: " + command, "source: " + os.path.basename(path), "",
+           "%-58s %7s %12s %12s %6s" % ("kernel", "calls", "avg_us", "total_ms", "pct")]
+    for r in rows[:top]:
+        name = r["Name"].split("(")[0][:58]
+        out.append("%-58s %7d %12.2f %12.3f %6.2f" % (name, int(r["Calls"]), float(r["AverageNs"]) / 1e3,
+                                                      float(r["TotalDurationNs"]) / 1e6,
+                                                      100.0 * float(r["TotalDurationNs"]) / total))
+    out.append("%-58s %7s %12s %12.3f" % ("all kernels", "", "", total / 1e6))
+    return "\n".join(out) + "\n"
+
+
+def pmc_per_kernel(prof_dir):
+    """{counter: {kernel: mean value per dispatch}} from a --pmc run's counter_collection.csv."""
+    acc = {}
+    for r in _csv_rows(_find(prof_dir, "counter_collection.csv")):
+        name = r["Kernel_Name"].split("(")[0]
+        acc.setdefault(r["Counter_Name"], {}).setdefault(name, []).append(float(r["Counter_Value"]))
+    return {c: {k: sum(v) / len(v) for k, v in ks.items()} for c, ks in acc.items()}
+
+
+def summarize_traffic(fetch_dir, write_dir, kernels, copy_bytes, workload, command):
+    """HBM bytes per dispatch of the kernels matching `kernels` (a regex) from FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch), each scaled by what the same counter reads for the plain
+    copy kernel, which reads and writes copy_bytes once."""
+    import re
+    f = pmc_per_kernel(fetch_dir)["FETCH_SIZE"]
+    w = pmc_per_kernel(write_dir)["WRITE_SIZE"]
+    ck = [k for k in f if "k_diag_copy" in k][0]
+    sf = copy_bytes / (f[ck] * 1024.0)
+    sw = copy_bytes / (w[ck] * 1024.0)
+    per = {}
+    for k in f:
+        if k != ck and re.search(kernels, k):
+            per[k] = {"fetch_bytes": f[k] * 1024.0 * sf, "write_bytes": w.get(k, 0.0) * 1024.0 * sw}
+            per[k]["hbm_bytes"] = per[k]["fetch_bytes"] + per[k]["write_bytes"]
+    return {"workload": workload, "command": command, "copy_bytes": copy_bytes, "fetch_scale": sf,
+            "write_scale": sw, "copy_fetch_kib": f[ck], "copy_write_kib": w[ck], "per_dispatch": per}
+
+
+def summarize_counters(dirs, kernels, command):
+    """Mean value per dispatch of every counter collected in the --pmc runs `dirs`, for the
+    kernels matching `kernels`, with the derived lane utilisation of the vector ALU
+    (SQ_THREAD_CYCLES_VALU / (64 SQ_ACTIVE_INST_VALU)) when both are present."""
+    import re
+    out = {}
+    for d in dirs:
+        for c, ks in pmc_per_kernel(d).items():
+            for k, v in ks.items():
+                if re.search(kernels, k):
+                    out.setdefault(k, {})[c] = v
+    for k, cs in out.items():
+        if cs.get("SQ_ACTIVE_INST_VALU") and "SQ_THREAD_CYCLES_VALU" in cs:
+            cs["valu_lane_utilisation"] = cs["SQ_THREAD_CYCLES_VALU"] / (64.0 * cs["SQ_ACTIVE_INST_VALU"])
+    return {"command": command, "per_dispatch": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -313,7 +430,35 @@ def main():
     ap.add_argument("--chunk-points", type=int, default=24_000_000,
                     help="batched workload: points per batched call (consecutive sets)")
     ap.add_argument("--no-per-set", action="store_true", help="batched workload: skip the per-set baseline")
+    ap.add_argument("--profile-run", action="store_true",
+                    help="profiling driver: one copy kernel, then --steps triangulations of --workload")
+    ap.add_argument("--learned", action="store_true", help="profiling driver: learned proposal on")
+    ap.add_argument("--copy-mib", type=int, default=1024, help="profiling driver: size of the copy kernel")
+    ap.add_argument("--summarize-trace", metavar="DIR", help="write a kernel table of a --kernel-trace --stats run")
+    ap.add_argument("--summarize-traffic", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="HBM bytes per dispatch from FETCH_SIZE and WRITE_SIZE runs")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="DIR", help="counters per dispatch of --pmc runs")
+    ap.add_argument("--kernels", default="k_star32|k_pba_pass", help="summaries: kernel name regex")
+    ap.add_argument("--cmd", default="", help="summaries: the command lines that produced the runs")
+    ap.add_argument("--out", help="summaries: output file")
     args = ap.parse_args()
+    if args.profile_run:
+        profile_run(args)
+        return
+    if args.summarize_trace or args.summarize_traffic or args.summarize_counters:
+        if args.summarize_trace:
+            text = summarize_trace(args.summarize_trace, args.cmd)
+        elif args.summarize_traffic:
+            text = json.dumps(summarize_traffic(args.summarize_traffic[0], args.summarize_traffic[1], args.kernels,
+                                                args.copy_mib << 20, args.workload, args.cmd), indent=1) + "\n"
+        else:
+            text = json.dumps(summarize_counters(args.summarize_counters, args.kernels, args.cmd), indent=1) + "\n"
+        if args.out:
+            with open(args.out, "w") as f:
+                f.write(text)
+        else:
+            sys.stdout.write(text)
+        return
     if args.gpus != 1:
         print("bench.py measures one MI355X; runs on more than one GPU are not supported.", file=sys.stderr)
         sys.exit(2)

@@ -109,7 +109,7 @@ EXPORTS = ["g4d_default_options", "g4d_create", "g4d_destroy", "g4d_last_error",
            "g4d_check_result", "g4d_check_mesh", "g4d_read_asc", "g4d_free_point_file", "g4d_write_asc",
            "g4d_write_ply", "g4d_triangulate_batch", "g4d_triangulate_batch_device", "g4d_batch_grid_size",
            "g4d_struct_sizes", "g4d_check_sizes", "g4d_set_proposal", "g4d_set_sample_export", "g4d_get_samples",
-           "g4d_mlp_forward", "g4d_mlp_grad", "g4d_mlp_train"]
+           "g4d_mlp_forward", "g4d_mlp_grad", "g4d_mlp_train", "g4d_check_stars", "g4d_diag_set_link_vertex"]
 
 MLP_PARAMS = 7312
 PROP_FEATURES = 12
@@ -145,6 +145,8 @@ def load_library(path=None):
     L.g4d_diag_grid_pairs.argtypes = [_vp, _ip, ctypes.c_int32, _up, ctypes.c_int64, _lp]
     L.g4d_diag_copy.argtypes = [_vp, ctypes.c_int64, _dp]
     L.g4d_check_result.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(CheckReport)]
+    L.g4d_check_stars.argtypes = [_vp, _lp, _lp]
+    L.g4d_diag_set_link_vertex.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
     L.g4d_check_mesh.argtypes = [_vp, _dp, ctypes.c_int32, _ip, _ip, ctypes.c_int64, ctypes.c_int32,
                                  ctypes.POINTER(CheckReport)]
     L.g4d_triangulate_batch.argtypes = [_vp, _dp, _lp, ctypes.c_int32, _ip, ctypes.POINTER(BatchResult)]
@@ -347,6 +349,18 @@ class Triangulator:
         rep = CheckReport()
         self._check(self._L.g4d_check_result(self._ctx, int(full_empty_sphere), ctypes.byref(rep)))
         return rep.as_dict()
+
+    def check_stars(self):
+        """Star consistency of the last result's stars on the GPU: (inconsistent link
+        triangles, link triangles checked)."""
+        bad = ctypes.c_int64(0)
+        tot = ctypes.c_int64(0)
+        self._check(self._L.g4d_check_stars(self._ctx, ctypes.byref(bad), ctypes.byref(tot)))
+        return int(bad.value), int(tot.value)
+
+    def diag_set_link_vertex(self, star, local, v):
+        """Overwrites one link vertex of a star of the last result (planted defects)."""
+        self._check(self._L.g4d_diag_set_link_vertex(self._ctx, int(star), int(local), int(v)))
 
     def check_mesh(self, xyz, tets, opp=None, full_empty_sphere=True):
         """Validate any tetrahedral mesh of the points xyz on the GPU."""

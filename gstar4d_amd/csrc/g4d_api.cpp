@@ -1611,6 +1611,51 @@ int g4d_check_mesh(g4d_ctx *ctx, const double *xyz, int32_t n, const int32_t *te
     return r;
 }
 
+static g4d::OutParams star_params(g4d_ctx *ctx)
+{
+    Store &store = ctx->store;
+    g4d::OutParams op;
+    memset(&op, 0, sizeof(op));
+    op.n = ctx->last_n;
+    op.vnum = buf<int>(store.vnum);
+    op.voff = buf<int64_t>(store.voff);
+    op.vpool = buf<int>(store.arena[0]);
+    op.tnum = buf<int>(store.tnum);
+    op.toff = buf<int64_t>(store.toff);
+    op.tpool = buf<uint64_t>(store.arena[1]);
+    op.cnt = buf<unsigned long long>(ctx->cnt);
+    return op;
+}
+
+int g4d_check_stars(g4d_ctx *ctx, int64_t *inconsistent, int64_t *triangles)
+{
+    if (!ctx || !inconsistent || !ctx->have_result) return G4D_ERR_INVALID_INPUT;
+    HCK(hipSetDevice(ctx->opt.device));
+    hipStream_t s = ctx->stream;
+    const g4d::OutParams op = star_params(ctx);
+    HCK(g4d::launch_zero_counters(op.cnt, (1u << G4D_CNT_APPEND4) | (1u << G4D_CNT_APPEND5), s));
+    HCK(g4d::launch_star_consistency(op, s));
+    unsigned long long hc[G4D_CNT_NUM];
+    CK(read_counters(ctx, hc));
+    *inconsistent = (int64_t)hc[G4D_CNT_APPEND5];
+    if (triangles) *triangles = (int64_t)hc[G4D_CNT_APPEND4];
+    return G4D_OK;
+}
+
+int g4d_diag_set_link_vertex(g4d_ctx *ctx, int32_t star, int32_t local, int32_t v)
+{
+    if (!ctx || !ctx->have_result || star < 0 || star >= ctx->last_n) return G4D_ERR_INVALID_INPUT;
+    HCK(hipSetDevice(ctx->opt.device));
+    Store &store = ctx->store;
+    int vn = 0;
+    int64_t vo = 0;
+    HCK(hipMemcpy(&vn, buf<int>(store.vnum) + star, sizeof(int), hipMemcpyDeviceToHost));
+    HCK(hipMemcpy(&vo, buf<int64_t>(store.voff) + star, sizeof(int64_t), hipMemcpyDeviceToHost));
+    if (local < 0 || local >= vn) return set_err(ctx, G4D_ERR_INVALID_INPUT, "star %d has %d link vertices", star, vn);
+    HCK(hipMemcpy(buf<int>(store.arena[0]) + vo + local, &v, sizeof(int), hipMemcpyHostToDevice));
+    return G4D_OK;
+}
+
 int g4d_diag_copy(g4d_ctx *ctx, int64_t bytes, double *ms)
 {
     if (!ctx || bytes < 16) return G4D_ERR_INVALID_INPUT;

@@ -166,6 +166,11 @@ struct OutParams {
 hipError_t launch_out_flag(const OutParams &p, uint64_t *list, int64_t cap, hipStream_t s);
 hipError_t launch_out_exact(const OutParams &p, const uint64_t *list, int64_t m, hipStream_t s);
 hipError_t launch_out_write(const OutParams &p, hipStream_t s);
+// Star consistency of the stars in p (pts, star_tet_* and tets unused): every link triangle
+// (a, b, c) of star s must appear as {s, b, c} in star a, {s, c, a} in star b and {s, a, b} in
+// star c.  Adds the failing link triangles to cnt[G4D_CNT_APPEND5], the checked ones to
+// cnt[G4D_CNT_APPEND4].
+hipError_t launch_star_consistency(const OutParams &p, hipStream_t s);
 // Face adjacency via an open-addressing hash of face triples.
 hipError_t launch_adjacency(const int *tets, int64_t nt, int *opp, uint32_t *table, int64_t table_size,
                             unsigned long long *cnt, hipStream_t s);

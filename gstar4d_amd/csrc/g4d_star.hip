@@ -1433,6 +1433,79 @@ hipError_t launch_out_write(const OutParams &p, hipStream_t s)
     return hipGetLastError();
 }
 
+// ------------------------------------------------------------------------------------
+// Star consistency (kerCheckStarConsistency / checkIfStarHasTriangle, GD/GDelaunay/
+// GDelKernels.cu:1193-1271): link triangle (a, b, c) of star s stands for tetrahedron
+// {s, a, b, c}, so star a must hold a link triangle with the vertex set {s, b, c}, and likewise
+// stars b and c.  Vertex sets are compared without orientation, as the reference does.
+// ------------------------------------------------------------------------------------
+
+// Does star u hold a link triangle with vertex set {x, y, z} (global ids)?
+__device__ bool star_has_triangle(const OutParams &p, int u, int x, int y, int z)
+{
+    if (u < 0 || u >= p.n) return false;
+    const int vn = p.vnum[u], tn = p.tnum[u];
+    const int64_t vo = p.voff[u], to = p.toff[u];
+    int lx = -1, ly = -1, lz = -1;
+    for (int v = 0; v < vn; v++) {
+        const int g = p.vpool[vo + v];
+        lx = g == x ? v : lx;
+        ly = g == y ? v : ly;
+        lz = g == z ? v : lz;
+    }
+    if (lx < 0 || ly < 0 || lz < 0) return false;
+    const bool wide = vn >= STAR_WIDE_MIN;
+    for (int t = 0; t < tn; t++) {
+        int lv[3];
+        star_tri_verts(p.tpool, to, t, wide, lv);
+        const bool hx = lv[0] == lx || lv[1] == lx || lv[2] == lx;
+        const bool hy = lv[0] == ly || lv[1] == ly || lv[2] == ly;
+        const bool hz = lv[0] == lz || lv[1] == lz || lv[2] == lz;
+        if (hx && hy && hz) return true;
+    }
+    return false;
+}
+
+// One half-wave per star, one lane per link triangle; cnt[G4D_CNT_APPEND5] counts the link
+// triangles that fail, cnt[G4D_CNT_APPEND4] those checked.
+__global__ void __launch_bounds__(256) k_star_consistency(OutParams p)
+{
+    const int lane = threadIdx.x & 63, half = lane >> 5, hl = lane & 31;
+    const int w0 = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), nw = (int)((gridDim.x * blockDim.x) >> 6);
+    int bad = 0, seen = 0;
+    for (int s0 = 2 * w0; s0 < p.n; s0 += 2 * nw) {
+        const int s = s0 + half;
+        const bool has = s < p.n;
+        const int tn = has ? p.tnum[s] : 0;
+        const bool wide = has && p.vnum[s] >= STAR_WIDE_MIN;
+        const int64_t vo = has ? p.voff[s] : 0, to = has ? p.toff[s] : 0;
+        const int tmax = max(tn, __shfl_xor(tn, 32, 64));
+        for (int t0 = 0; t0 < tmax; t0 += 32) {
+            const int t = t0 + hl;
+            if (t >= tn) continue;
+            int lv[3];
+            star_tri_verts(p.tpool, to, t, wide, lv);
+            const int g[3] = {p.vpool[vo + lv[0]], p.vpool[vo + lv[1]], p.vpool[vo + lv[2]]};
+            bool ok = true;
+            for (int i = 0; i < 3 && ok; i++) ok = star_has_triangle(p, g[i], s, g[(i + 1) % 3], g[(i + 2) % 3]);
+            seen++;
+            bad += ok ? 0 : 1;
+        }
+    }
+    const int wb = g4d_wave_sum(bad), ws = g4d_wave_sum(seen);
+    if (lane == 0) {
+        if (wb) atomicAdd(&p.cnt[G4D_CNT_APPEND5], (unsigned long long)wb);
+        atomicAdd(&p.cnt[G4D_CNT_APPEND4], (unsigned long long)ws);
+    }
+}
+
+hipError_t launch_star_consistency(const OutParams &p, hipStream_t s)
+{
+    if (p.n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_star_consistency, dim3(g4d_blocks((int64_t)p.n * 32, 256, 16384)), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
 __device__ __forceinline__ void face_of(const int *tt, int i, int *f)
 {
     int q = 0;

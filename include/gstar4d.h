@@ -226,6 +226,14 @@ int g4d_check_result(g4d_ctx *ctx, int32_t full_empty_sphere, g4d_check_report *
 int g4d_check_mesh(g4d_ctx *ctx, const double *xyz, int32_t n, const int32_t *tets, const int32_t *opp,
                    int64_t num_tets, int32_t full_empty_sphere, g4d_check_report *rep);
 
+/* Star consistency of the last result's stars (areStarsConsistent -> kerCheckStarConsistency,
+   GD/GDelaunay/GDelHost.cu:1362-1387, GD/GDelaunay/GDelKernels.cu:1193-1271): link triangle
+   (a, b, c) of star s stands for tetrahedron {s, a, b, c}, so stars a, b and c must each hold the
+   link triangle made of s and the other two (vertex sets compared without orientation).
+   *inconsistent receives the number of link triangles that fail (0 after splaying ends),
+   *triangles (may be NULL) the number checked. */
+int g4d_check_stars(g4d_ctx *ctx, int64_t *inconsistent, int64_t *triangles);
+
 /* ---- Point and mesh files (host only; no context, no device work) ---- */
 
 /* Points of an .asc file (GD/Main/Application.cpp:475-566): whitespace-separated tokens read as
@@ -271,6 +279,9 @@ int g4d_diag_pba(g4d_ctx *ctx, const int32_t *vox_sorted, int32_t n, int32_t gri
 /* Plain device-to-device copy of `bytes` (rounded down to 16) between two buffers the context
    allocates; *ms receives the kernel time.  Calibrates HBM traffic counters and bandwidth. */
 int g4d_diag_copy(g4d_ctx *ctx, int64_t bytes, double *ms);
+/* Overwrites link vertex `local` of star `star` (Morton index space) of the last result with
+   point id v: plants a defect for the star-consistency tests. */
+int g4d_diag_set_link_vertex(g4d_ctx *ctx, int32_t star, int32_t local, int32_t v);
 /* Sorted unique Voronoi-neighbour pairs (min<<32|max) from an id grid. */
 int g4d_diag_grid_pairs(g4d_ctx *ctx, const int32_t *id_grid, int32_t grid, uint64_t *out, int64_t cap,
                         int64_t *count);

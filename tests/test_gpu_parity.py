@@ -224,9 +224,12 @@ def test_c1_full_size_against_golden(orc):
     with g.Triangulator(grid_size=128) as t:
         tets, opp, perm = t.triangulate(x)
         st = t.stats()
+        bad, tot = t.check_stars()
     got = g.canonical(tets)
     assert got.shape[0] == gold["num_tets"] and _h(got) == gold["sha256"]
     assert st["splay_rounds"] == gold["splay_rounds"]
+    # every link triangle of every star is in its three other stars (areStarsConsistent)
+    assert bad == 0 and tot >= 4 * gold["num_tets"]
 
 
 @pytest.mark.parametrize("n,grid", [(20000, 128), (100000, 256)])
@@ -327,6 +330,21 @@ def test_validator_finds_planted_defects(orc):
         assert rep["empty_sphere_failures"] > 0 and ref["empty_sphere"] > 0
 
 
+def test_star_consistency_finds_planted_defect():
+    """The star-consistency check passes after splaying and fails once one link vertex of
+    one star is replaced by a point that is not in its link."""
+    x = datasets.uniform_cube(5000, seed=21)
+    with g.Triangulator(grid_size=32, index_space="morton") as t:
+        tets, _, _ = t.triangulate(x)
+        bad, tot = t.check_stars()
+        assert bad == 0 and tot >= 4 * tets.shape[0]
+        far = x.shape[0] - 1  # Morton-last point: not in the Morton-first star's link
+        t.diag_set_link_vertex(0, 0, far)
+        bad2, tot2 = t.check_stars()
+        assert tot2 == tot and bad2 > 0
+        print("planted defect: %d of %d link triangles inconsistent" % (bad2, tot2))
+
+
 def _flip23(orc, x, tets, opp):
     """Replace two tetrahedra sharing face abc by (a,b,d,e), (b,c,d,e), (c,a,d,e) where the
     segment de crosses abc (exact orientations), each oriented with orient3d < 0."""
@@ -394,6 +412,9 @@ def _full_size(name, gold, check=True):
         print("%s: %d tetrahedra, %.1f ms on the GPU, %d rounds" % (name, tets.shape[0], st["t_total_ms"],
                                                                    st["splay_rounds"]), flush=True)
         rep = t.check(full_empty_sphere=True) if check else None
+        bad, tot = t.check_stars()
+        print("%s: star consistency %d of %d link triangles fail" % (name, bad, tot), flush=True)
+    assert bad == 0 and tot >= 4 * gold["num_tets"], name
     got = g.canonical(tets)
     assert got.shape[0] == gold["num_tets"], name
     assert _h(got) == gold["sha256"], name
