@@ -381,8 +381,9 @@ static int proposal_stage(g4d_ctx *ctx, int n, const g4d::GridSets &GS, const in
         uint64_t *merged = P.sort_tmp_keys.as<uint64_t>();
         HCK(g4d::merge_u64(pr, *m, pp, np, merged, s));
         int64_t mm = *m + np;
-        HCK(g4d::unique_u64(P, merged, mm, &mm, s)); // proposals already in a workset
-        HCK(hipMemcpyAsync(pr, merged, sizeof(uint64_t) * (size_t)mm, hipMemcpyDeviceToDevice, s));
+        // back into pr without the proposals already in a workset (out of place: unique_u64
+        // would compact merged through the very buffer that holds it)
+        HCK(g4d::unique_copy_u64(P, merged, pr, mm, &mm, s));
         *m = mm;
     }
     HCK(hipStreamSynchronize(s));

@@ -72,7 +72,10 @@ hipError_t radix_sort_u64_2(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int
 hipError_t merge_u64(const uint64_t *a, int64_t na, const uint64_t *b, int64_t nb, uint64_t *out, hipStream_t s);
 
 // Order-preserving removal of adjacent duplicates; returns the new count via host sync.
+// keys must not be P.sort_tmp_keys (longer lists go through that buffer when compacted in place).
 hipError_t unique_u64(Prims &P, uint64_t *keys, int64_t n, int64_t *n_out, hipStream_t s);
+// The same from in to out (distinct buffers; in may be P.sort_tmp_keys).
+hipError_t unique_copy_u64(Prims &P, const uint64_t *in, uint64_t *out, int64_t n, int64_t *n_out, hipStream_t s);
 
 // Order-preserving compaction keep[i] != 0.  Count returned via host sync.
 hipError_t compact_u64(Prims &P, const uint64_t *in, const uint8_t *keep, uint64_t *out, int64_t n,

@@ -581,6 +581,7 @@ static hipError_t compact_big(Prims &P, const uint64_t *in, const uint8_t *keep,
     HIP_CHECK_RET(scan_excl_i64(P, tc, tc, nt, tc + nt, s));
     uint64_t *dst = out;
     if (in == out) {
+        if (in == P.sort_tmp_keys.ptr) return hipErrorInvalidValue; // would read and write one buffer
         HIP_CHECK_RET(P.sort_tmp_keys.ensure(sizeof(uint64_t) * (size_t)n));
         dst = P.sort_tmp_keys.as<uint64_t>();
     }
@@ -602,6 +603,17 @@ hipError_t unique_u64(Prims &P, uint64_t *keys, int64_t n, int64_t *n_out, hipSt
     }
     if (n <= SMALL_COMPACT) return compact_small<0>(P, keys, nullptr, 0, keys, n, n_out, s);
     return compact_big<0>(P, keys, nullptr, 0, keys, n, n_out, s);
+}
+
+hipError_t unique_copy_u64(Prims &P, const uint64_t *in, uint64_t *out, int64_t n, int64_t *n_out, hipStream_t s)
+{
+    if (n <= 0) {
+        *n_out = 0;
+        return hipSuccess;
+    }
+    if (in == out) return hipErrorInvalidValue;
+    if (n <= SMALL_COMPACT) return compact_small<0>(P, in, nullptr, 0, out, n, n_out, s);
+    return compact_big<0>(P, in, nullptr, 0, out, n, n_out, s);
 }
 
 hipError_t compact_u64(Prims &P, const uint64_t *in, const uint8_t *keep, uint64_t *out, int64_t n, int64_t *n_out,

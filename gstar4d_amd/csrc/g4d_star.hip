@@ -24,6 +24,9 @@
 // two per-vertex maps (boundary edge starting / ending at each vertex).  The whole insertion
 // list of a star runs inside one kernel, so there is no host loop per insertion index.
 #include "g4d_predicates.h"
+#include <cstdlib>
+#include <type_traits>
+
 #include "g4d_star.h"
 
 namespace g4d {
@@ -76,7 +79,9 @@ __device__ __forceinline__ void star_tri_verts(const uint64_t *tpool, int64_t to
 template <int VMAX>
 struct StarLds {
     static constexpr int TMAX = 2 * VMAX;
-    double cof[TMAX][8]; // per triangle slot: cofactors C[4], permanents P[4]
+    // per triangle slot: cofactors C[0..3] and permanents P[0..3], stored component-major so
+    // that the lanes of a wave, each reading its own slot, touch consecutive 8-byte words
+    double cof[8][TMAX];
     int vid[VMAX];
     double vc[VMAX][4];
     uint32_t tv[TMAX];
@@ -93,6 +98,20 @@ struct StarLds {
     uint16_t vremap[VMAX];
     int sc[4];
 };
+
+// Cofactors and permanents of the triangle (r0, r1, r2) into the component-major slot table.
+template <int TMAX>
+__device__ __forceinline__ void lds_cofactors(const double *r0, const double *r1, const double *r2,
+                                              double (*cof)[TMAX], int slot)
+{
+    double C[4], P[4];
+    tri_cofactors(r0, r1, r2, C, P);
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        cof[j][slot] = C[j];
+        cof[4 + j][slot] = P[j];
+    }
+}
 
 // Exclusive scan across the wave of per-lane counts below 2^BITS; returns the prefix, the total
 // in *tot.  One ballot and a lane-masked bit count per bit: no cross-lane data movement.
@@ -333,7 +352,7 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                 const int slot = lane + 64 * k;
                 if (slot < T) {
                     const uint32_t t = L.tv[slot];
-                    tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], L.cof[slot], L.cof[slot] + 4);
+                    lds_cofactors<TMAX>(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], L.cof, slot);
                     ncof++;
                 }
             }
@@ -351,7 +370,10 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                         const uint32_t t = L.tv[slot];
                         if (t & TF_ALIVE) {
                             nfilt++;
-                            int sg = cofactor_filter_sign(r, L.cof[slot], L.cof[slot] + 4);
+                            double c[8];
+#pragma unroll
+                            for (int j = 0; j < 8; j++) c[j] = L.cof[j][slot];
+                            int sg = cofactor_filter_sign(r, c, c + 4);
                             if (sg == 0) {
                                 if (EXACT) {
                                     const int a = L.vid[tvert(t, 0)], b = L.vid[tvert(t, 1)], c = L.vid[tvert(t, 2)];
@@ -452,8 +474,8 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                     if (slot < T) {
                         const uint32_t t = L.tv[slot];
                         if (t & TF_FRESH) {
-                            tri_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], L.cof[slot],
-                                          L.cof[slot] + 4);
+                            lds_cofactors<TMAX>(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], L.cof,
+                                                slot);
                             ncof++;
                             L.tv[slot] = t & ~TF_FRESH;
                         }
@@ -575,6 +597,507 @@ __global__ void __launch_bounds__(64) k_star(StarParams p, int nitems)
     star_body<VMAX, EXACT>(p, nitems);
 }
 
+// ------------------------------------------------------------------------------------
+// Grouped 32-vertex star kernel: one star per group of GL lanes, 64 / GL stars per wave.
+//
+// The one-star-per-wave kernel leaves most lanes idle on a star of this tier (at most 60
+// triangle slots, about 27 live) and pays its chain of ballots and wave-level barriers per
+// insertion of one star; here the chain is shared by 64 / GL stars.  Each lane owns KPL =
+// 64 / GL triangle slots (gl, gl + GL, ...) and keeps their cofactors and permanents in
+// registers (only the owning lane ever reads or writes them), so a star's LDS holds only its
+// link vertices, triangles and hole-boundary lists (about 3 KB).  Every barrier and ballot is
+// reached by the whole wave in uniform control flow: the insertion loop runs to the largest
+// insertion count of the wave's stars, and a group with nothing to do in a step has its work
+// masked off.  Filters only (no multiprecision code): a star with an inconclusive orientation
+// goes to p.exact_list untouched, a star beyond the tier to p.defer_list, as in k_star32.
+// ------------------------------------------------------------------------------------
+
+struct StarGrpLds {
+    static constexpr int VMAX = STAR_VMAX_T0;
+    static constexpr int TMAX = 2 * VMAX;
+    double vc[VMAX][4];
+    int vid[VMAX];
+    int ins[VMAX];
+    uint32_t tv[TMAX];
+    uint16_t to[TMAX][3];
+    int16_t vstart[VMAX];
+    int16_t vend[VMAX];
+    uint16_t freel[TMAX];
+    uint16_t be_a[TMAX], be_b[TMAX], be_u[TMAX], be_slot[TMAX];
+    uint8_t be_j[TMAX];
+    int16_t ins_local[VMAX];
+    uint8_t present[VMAX];
+    uint16_t remap[TMAX];
+    uint16_t vremap[VMAX];
+};
+
+// Ballot of this lane's group of GL lanes, shifted down to bit 0.
+template <int GL>
+__device__ __forceinline__ uint64_t grp_ballot(bool pred, int g)
+{
+    const uint64_t m = __ballot(pred);
+    return GL == 64 ? m : (m >> (g * GL)) & ((1ull << GL) - 1ull);
+}
+
+// Exclusive scan of per-lane counts below 2^BITS within the group; total in *tot.
+template <int GL, int BITS>
+__device__ __forceinline__ int grp_excl(int v, int g, int gl, int *tot)
+{
+    const uint64_t below = gl == 0 ? 0ull : (~0ull >> (64 - gl));
+    int pre = 0, all = 0;
+#pragma unroll
+    for (int b = 0; b < BITS; b++) {
+        const uint64_t m = grp_ballot<GL>((v >> b) & 1, g);
+        pre += __popcll(m & below) << b;
+        all += __popcll(m) << b;
+    }
+    *tot = all;
+    return pre;
+}
+
+// Cofactors of a triangle slot, with either the four permanents in FP64 or only their largest,
+// rounded up to FP32.  The filter's error bound 32 u sum_j |v'_j| P_j is then replaced by
+// 32 u (sum_j |v'_j|) max_j P_j, which is no smaller, so a decided sign stays certain; an
+// overflow to +inf or a bound that vanishes only makes the filter inconclusive.
+__device__ __forceinline__ void grp_cofactors(const double *r0, const double *r1, const double *r2, double *C,
+                                              double (&P)[4])
+{
+    tri_cofactors(r0, r1, r2, C, P);
+}
+__device__ __forceinline__ void grp_cofactors(const double *r0, const double *r1, const double *r2, double *C,
+                                              float &P)
+{
+    double Pd[4];
+    tri_cofactors(r0, r1, r2, C, Pd);
+    P = __double2float_ru(fmax(fmax(Pd[0], Pd[1]), fmax(Pd[2], Pd[3])));
+}
+// rs = |v'_x| + |v'_y| + |v'_z| + v'_l of the vertex being inserted (used with the FP32 form).
+__device__ __forceinline__ int grp_filter_sign(const double *r, double, const double *C, const double (&P)[4])
+{
+    return cofactor_filter_sign(r, C, P);
+}
+__device__ __forceinline__ int grp_filter_sign(const double *r, double rs, const double *C, const float &P)
+{
+    const double val = r[0] * C[0] + r[1] * C[1] + r[2] * C[2] + r[3] * C[3];
+    const double per = rs * (double)P;
+    const double bnd = 32.0 * 1.1102230246251565e-16 * per;
+    if (!(per > 1e-250)) return 0;
+    return val > bnd ? 1 : (-val > bnd ? -1 : 0);
+}
+
+__device__ __forceinline__ int wave_max_i(int v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        const int o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+template <int GL, bool PF>
+__device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
+{
+    constexpr int NG = 64 / GL;
+    constexpr int VMAX = StarGrpLds::VMAX, TMAX = StarGrpLds::TMAX;
+    constexpr int KPL = TMAX / GL;
+    __shared__ StarGrpLds Ls[NG];
+    const int lane = threadIdx.x, g = lane / GL, gl = lane % GL;
+    StarGrpLds &L = Ls[g];
+    unsigned long long *cnt = p.cnt;
+
+    if (p.nlist_dev) {
+        const unsigned long long c = *p.nlist_dev;
+        nitems = c < (unsigned long long)nitems ? (int)c : nitems;
+    }
+    // Work unit: a batch of NG consecutive list entries, one per group.  As many batches as
+    // work-groups or fewer: one batch each, no queue; otherwise STAR_CHUNK batches per atomic.
+    const int nbatch = (nitems + NG - 1) / NG;
+    const bool one_each = nbatch <= (int)gridDim.x;
+    if ((int)blockIdx.x >= nbatch) return;
+    bool first = true;
+    int nfilt = 0, ncof = 0;
+    double C[KPL][4];
+    typename std::conditional<PF, float, double[4]>::type P[KPL];
+    for (;;) {
+        int base = 0;
+        if (one_each) {
+            base = first ? (int)blockIdx.x : nbatch;
+            first = false;
+        } else {
+            if (lane == 0) base = (int)atomicAdd(&cnt[p.work_counter], (unsigned long long)STAR_CHUNK);
+            base = __shfl(base, 0, 64);
+        }
+        if (base >= nbatch) break;
+        const int bend = one_each ? base + 1 : (base + STAR_CHUNK < nbatch ? base + STAR_CHUNK : nbatch);
+#pragma unroll 1
+        for (int batch = base; batch < bend; batch++) {
+            const int item = batch * NG + g;
+            // ---- this group's star: checks, then its link and insertions into LDS ----
+            bool act = item < nitems;
+            const int s = act ? (p.star_list ? p.star_list[item] : item) : 0;
+            const int64_t ib = act ? p.ins_off[s] : 0;
+            const int m = act ? (int)(p.ins_off[s + 1] - ib) : 0;
+            const int vn = act && p.mode ? p.vnum[s] : 0;
+            const int need = vn + m;
+            const int lim = VMAX < p.lds_limit ? VMAX : p.lds_limit;
+            if (act && need > lim) {
+                if (gl == 0) {
+                    const unsigned long long k = atomicAdd(&cnt[p.defer_counter], 1ull);
+                    if ((int64_t)k < p.defer_cap) p.defer_list[k] = s;
+                    else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
+                }
+                act = false;
+            }
+            if (act && p.mode == 0 && m < 4) {
+                if (gl == 0) {
+                    atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_CONE);
+                    p.vnum[s] = 0;
+                    p.tnum[s] = 0;
+                }
+                act = false;
+            }
+            if (act && p.mode == 1 && m == 0) act = false;
+            double sx = 0.0, sy = 0.0, sz = 0.0;
+            if (act) {
+                sx = p.pts[3 * (int64_t)s];
+                sy = p.pts[3 * (int64_t)s + 1];
+                sz = p.pts[3 * (int64_t)s + 2];
+            }
+            const int q0 = p.mode ? 0 : 4;
+            if (act) {
+                for (int q = gl; q < m; q += GL) {
+                    const int gv = (int)(uint32_t)p.ins_keys[ib + q];
+                    L.ins[q] = gv;
+                    L.ins_local[q] = -1;
+                    if (q >= q0) {
+                        L.vid[vn + q] = gv;
+                        rel_coords(p.pts, gv, sx, sy, sz, L.vc[vn + q]);
+                    }
+                }
+            }
+            int T = 0, nfree = 0;
+            bool unsure = false, failed = false;
+            if (act && p.mode == 1) {
+                const int tn = p.tnum[s];
+                const int64_t vi = p.voff[s], ti = p.toff[s];
+                for (int v = gl; v < vn; v += GL) {
+                    const int gv = p.vpool[vi + v];
+                    L.vid[v] = gv;
+                    rel_coords(p.pts, gv, sx, sy, sz, L.vc[v]);
+                }
+                for (int t = gl; t < tn; t += GL) {
+                    const uint64_t h = p.tpool[ti + t];
+                    L.tv[t] = tpack(hvert(h, 0), hvert(h, 1), hvert(h, 2), TF_ALIVE | (hchecked(h) ? TF_CHECKED : 0u));
+                    L.to[t][0] = (uint16_t)hopp(h, 0);
+                    L.to[t][1] = (uint16_t)hopp(h, 1);
+                    L.to[t][2] = (uint16_t)hopp(h, 2);
+                }
+                T = tn;
+            } else if (act) {
+                // initial 4-simplex (GD/GDelaunay/GDelPredKernels.cu:45-158)
+                int l[4];
+                for (int i = 0; i < 4; i++) l[i] = (int)(uint32_t)p.ins_keys[ib + i];
+                const double *pp[5] = {p.pts + 3 * (int64_t)l[0], p.pts + 3 * (int64_t)l[1], p.pts + 3 * (int64_t)l[2],
+                                       p.pts + 3 * (int64_t)l[3], p.pts + 3 * (int64_t)s};
+                const int o = filter_insphere(pp);
+                if (o == 0) {
+                    unsure = true; // pushed to the exact list below, untouched
+                } else {
+                    const bool sw = o < 0;
+                    if (sw) {
+                        const int t = l[0];
+                        l[0] = l[1];
+                        l[1] = t;
+                    }
+                    if (gl < 4) {
+                        L.vid[gl] = l[gl];
+                        rel_coords(p.pts, l[gl], sx, sy, sz, L.vc[gl]);
+                        const int lt[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
+                        L.tv[gl] = tpack(lt[gl][0], lt[gl][1], lt[gl][2], TF_ALIVE);
+                        L.to[gl][0] = (uint16_t)lt[gl][0];
+                        L.to[gl][1] = (uint16_t)lt[gl][1];
+                        L.to[gl][2] = (uint16_t)lt[gl][2];
+                        const int local = (gl < 2 && sw) ? 1 - gl : gl;
+                        L.ins_local[gl] = (int16_t)local;
+                    }
+                    T = 4;
+                }
+            }
+            wave_sync();
+            bool live = act && !unsure; // the group is building its star
+#pragma unroll
+            for (int k = 0; k < KPL; k++) {
+                const int slot = gl + GL * k;
+                if (live && slot < T) {
+                    const uint32_t t = L.tv[slot];
+                    grp_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], C[k], P[k]);
+                    ncof++;
+                }
+            }
+            // ---- insertions: every group steps through its list in lock-step ----
+            const int qend = wave_max_i(live ? m : 0);
+            for (int q = q0; q < qend; q++) {
+                bool step = live && q < m;
+                const int Vn = vn + q;
+                double r[4] = {0.0, 0.0, 0.0, 0.0};
+                if (step) {
+                    r[0] = L.vc[Vn][0];
+                    r[1] = L.vc[Vn][1];
+                    r[2] = L.vc[Vn][2];
+                    r[3] = L.vc[Vn][3];
+                }
+                const double rs = fabs(r[0]) + fabs(r[1]) + fabs(r[2]) + r[3];
+                int nv = 0;
+                bool unc = false;
+#pragma unroll
+                for (int k = 0; k < KPL; k++) {
+                    const int slot = gl + GL * k;
+                    if (step && slot < T) {
+                        const uint32_t t = L.tv[slot];
+                        if (t & TF_ALIVE) {
+                            nfilt++;
+                            const int sg = grp_filter_sign(r, rs, C[k], P[k]);
+                            if (sg == 0) unc = true;
+                            if (sg < 0) {
+                                L.tv[slot] = t | TF_VIS;
+                                nv++;
+                            }
+                        }
+                    }
+                }
+                if (grp_ballot<GL>(unc, g) != 0ull) {
+                    unsure = true;
+                    live = false;
+                    step = false;
+                }
+                int nvis;
+                const int vbase = grp_excl<GL, count_bits(KPL)>(nv, g, gl, &nvis);
+                if (nvis == 0) step = false; // drowned at insertion
+                if (step && gl == 0) L.ins_local[q] = (int16_t)Vn;
+                wave_sync();
+                // hole boundary: edges of freed triangles whose neighbour survives
+                int nb = 0;
+#pragma unroll
+                for (int k = 0; k < KPL; k++) {
+                    const int slot = gl + GL * k;
+                    if (step && slot < T && (L.tv[slot] & TF_VIS))
+                        for (int i = 0; i < 3; i++) nb += (L.tv[L.to[slot][i]] & TF_VIS) ? 0 : 1;
+                }
+                int nbtot;
+                int e = grp_excl<GL, count_bits(3 * KPL)>(nb, g, gl, &nbtot);
+                if (step && (nbtot > TMAX || nbtot < 3)) {
+                    failed = true;
+                    live = false;
+                    step = false;
+                }
+                int fpos = nfree + vbase;
+#pragma unroll
+                for (int k = 0; k < KPL; k++) {
+                    const int slot = gl + GL * k;
+                    if (step && slot < T && (L.tv[slot] & TF_VIS)) {
+                        const uint32_t t = L.tv[slot];
+                        for (int i = 0; i < 3; i++) {
+                            const int u = L.to[slot][i];
+                            if (L.tv[u] & TF_VIS) continue;
+                            int j = 0;
+                            while (j < 2 && L.to[u][j] != slot) j++;
+                            L.be_a[e] = (uint16_t)tvert(t, (i + 1) % 3);
+                            L.be_b[e] = (uint16_t)tvert(t, (i + 2) % 3);
+                            L.be_u[e] = (uint16_t)u;
+                            L.be_j[e] = (uint8_t)j;
+                            e++;
+                        }
+                        L.freel[fpos++] = (uint16_t)slot;
+                    }
+                }
+                wave_sync();
+                const int nfree_tot = nfree + nvis;
+#pragma unroll
+                for (int k = 0; k < KPL; k++) {
+                    const int slot = gl + GL * k;
+                    if (step && slot < T && (L.tv[slot] & TF_VIS)) L.tv[slot] = 0u;
+                }
+                if (step)
+                    for (int k = gl; k < nbtot; k += GL)
+                        L.be_slot[k] = (uint16_t)(k < nfree_tot ? L.freel[nfree_tot - 1 - k] : T + (k - nfree_tot));
+                const int newT = T + (nbtot > nfree_tot ? nbtot - nfree_tot : 0);
+                if (step && (newT > TMAX || Vn + 1 > VMAX)) {
+                    failed = true;
+                    live = false;
+                    step = false;
+                }
+                wave_sync();
+                if (step)
+                    for (int k = gl; k < nbtot; k += GL) {
+                        const int slot = L.be_slot[k], a = L.be_a[k], b = L.be_b[k], u = L.be_u[k];
+                        L.tv[slot] = tpack(a, b, Vn, TF_ALIVE | TF_FRESH);
+                        L.to[slot][2] = (uint16_t)u;
+                        L.to[u][L.be_j[k]] = (uint16_t)slot;
+                        L.vstart[a] = (int16_t)slot;
+                        L.vend[b] = (int16_t)slot;
+                    }
+                wave_sync();
+                if (step) {
+                    for (int k = gl; k < nbtot; k += GL) {
+                        const int slot = L.be_slot[k];
+                        L.to[slot][0] = (uint16_t)L.vstart[L.be_b[k]];
+                        L.to[slot][1] = (uint16_t)L.vend[L.be_a[k]];
+                    }
+                    nfree = nfree_tot - (nbtot < nfree_tot ? nbtot : nfree_tot);
+                    T = newT;
+                }
+                wave_sync();
+#pragma unroll
+                for (int k = 0; k < KPL; k++) {
+                    const int slot = gl + GL * k;
+                    if (step && slot < T) {
+                        const uint32_t t = L.tv[slot];
+                        if (t & TF_FRESH) {
+                            grp_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], C[k], P[k]);
+                            ncof++;
+                            L.tv[slot] = t & ~TF_FRESH;
+                        }
+                    }
+                }
+                wave_sync();
+            }
+            // ---- outcome per group ----
+            if (act && unsure && gl == 0) {
+                const unsigned long long k = atomicAdd(&cnt[p.exact_counter], 1ull);
+                if ((int64_t)k < p.exact_cap) p.exact_list[k] = s;
+                else atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
+            }
+            if (act && failed && gl == 0) {
+                atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
+                p.vnum[s] = 0;
+                p.tnum[s] = 0;
+            }
+            bool fin = act && !unsure && !failed;
+            // ---- finalize: link set, drowned insertions, compaction, pairs, history ----
+            const int V = need; // drowned insertions' slots are never referenced and drop out
+            if (fin)
+                for (int v = gl; v < V; v += GL) L.present[v] = 0;
+            wave_sync();
+#pragma unroll
+            for (int k = 0; k < KPL; k++) {
+                const int slot = gl + GL * k;
+                if (fin && slot < T) {
+                    const uint32_t t = L.tv[slot];
+                    if (t & TF_ALIVE) {
+                        L.present[tvert(t, 0)] = 1;
+                        L.present[tvert(t, 1)] = 1;
+                        L.present[tvert(t, 2)] = 1;
+                    }
+                }
+            }
+            wave_sync();
+            if (fin)
+                for (int q = gl; q < m; q += GL) {
+                    const int lv = L.ins_local[q];
+                    p.dflag[ib + q] = (lv < 0 || !L.present[lv]) ? 1 : 0;
+                }
+            const int64_t vo = fin ? p.nvoff[s] : 0, to = fin ? p.ntoff[s] : 0;
+            int vcount = 0;
+#pragma unroll
+            for (int b0 = 0; b0 < VMAX; b0 += GL) {
+                const int v = b0 + gl;
+                const bool pr = fin && v < V && L.present[v];
+                int tot;
+                const int rk = grp_excl<GL, 1>(pr ? 1 : 0, g, gl, &tot);
+                if (pr) {
+                    L.vremap[v] = (uint16_t)(vcount + rk);
+                    p.vpool[vo + vcount + rk] = L.vid[v];
+                }
+                vcount += tot;
+            }
+            int tcount = 0;
+#pragma unroll
+            for (int b0 = 0; b0 < TMAX; b0 += GL) {
+                const int t = b0 + gl;
+                const bool al = fin && t < T && (L.tv[t] & TF_ALIVE);
+                int tot;
+                const int rk = grp_excl<GL, 1>(al ? 1 : 0, g, gl, &tot);
+                if (al) L.remap[t] = (uint16_t)(tcount + rk);
+                tcount += tot;
+            }
+            if (fin && (tcount > 2 * need - 4 || tcount != 2 * vcount - 4)) {
+                // not a triangulated 2-sphere: report instead of writing past the star's region
+                if (gl == 0) {
+                    atomicOr(&cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_HOLE);
+                    p.vnum[s] = 0;
+                    p.tnum[s] = 0;
+                }
+                fin = false;
+            }
+            wave_sync();
+            const int64_t pb = 2 * (to - p.tbase);
+#pragma unroll
+            for (int b0 = 0; b0 < TMAX; b0 += GL) {
+                const int t = b0 + gl;
+                const uint32_t tt = (fin && t < T) ? L.tv[t] : 0u;
+                if (tt & TF_ALIVE) {
+                    const int a = tvert(tt, 0), b = tvert(tt, 1), c = tvert(tt, 2);
+                    const int d = L.remap[t];
+                    p.tpool[to + d] = hpack(L.vremap[a], L.vremap[b], L.vremap[c], L.remap[L.to[t][0]],
+                                            L.remap[L.to[t][1]], L.remap[L.to[t][2]], true);
+                    const TriPairs pr = star_tri_pairs(L.vid[a], L.vid[b], L.vid[c], !(tt & TF_CHECKED));
+                    p.pairbuf[pb + 2 * d] = pr.x;
+                    p.pairbuf[pb + 2 * d + 1] = pr.y;
+                }
+            }
+            int hn = 0;
+            if (fin) {
+                for (int64_t j = 2 * (int64_t)tcount + gl; j < 2 * star_tcap_words(need); j += GL)
+                    p.pairbuf[pb + j] = STAR_PAIR_NONE;
+                hn = star_history_merge(p, s, L.ins, m, gl, GL);
+            }
+            wave_sync();
+            if (fin && gl == 0) star_commit(p, s, vcount, vo, tcount, to, hn + m);
+            wave_sync();
+        }
+    }
+    const int wf = g4d_wave_sum(nfilt), wc = g4d_wave_sum(ncof);
+    if (lane == 0) {
+        atomicAdd(&cnt[G4D_CNT_ORIENT4], (unsigned long long)wf);
+        atomicAdd(&cnt[G4D_CNT_COFACTOR], (unsigned long long)wc);
+    }
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) k_star32g16(StarParams p, int nitems)
+{
+    star_body_grp<16, false>(p, nitems);
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(3, 8))) k_star32g16f(StarParams p, int nitems)
+{
+    star_body_grp<16, true>(p, nitems);
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) k_star32g16f4(StarParams p, int nitems)
+{
+    star_body_grp<16, true>(p, nitems);
+}
+
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8))) k_star32g32f(StarParams p, int nitems)
+{
+    star_body_grp<32, true>(p, nitems);
+}
+
+// The 32-vertex filter-only kernel: 0 one star per wave (k_star32), 1 k_star32g16 (3 waves per
+// SIMD), 2 k_star32g16f (FP32 permanents, 3 waves), 3 k_star32g16f4 (4 waves), 4 k_star32g32f
+// (two stars per wave, 4 waves).  G4D_STAR_T0 selects one (measurement).
+static int star_t0_variant()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("G4D_STAR_T0");
+        v = e ? atoi(e) : 2;
+        if (v < 0 || v > 4) v = 2;
+    }
+    return v;
+}
+
 static_assert(G4D_CNT_QUEUE0 + 6 <= G4D_CNT_NUM, "six queue heads must fit the counter block");
 static_assert(G4D_CNT_NUM <= 32, "counter masks are 32 bits wide");
 
@@ -608,7 +1131,18 @@ hipError_t launch_star_kernel(const StarParams &p, int vmax, bool exact, int nbl
         else
             return hipErrorInvalidValue;
     } else if (vmax == STAR_VMAX_T0) {
-        hipLaunchKernelGGL(k_star32, dim3(nblocks), dim3(64), 0, s, p, nitems);
+        const int v = star_t0_variant();
+        if (v == 0) {
+            hipLaunchKernelGGL(k_star32, dim3(nblocks), dim3(64), 0, s, p, nitems);
+        } else {
+            // one work-group per resident wave: 256 CUs x 4 SIMDs x the variant's occupancy
+            const int per_simd = v <= 2 ? 3 : 4;
+            const int nb = 1024 * per_simd;
+            if (v == 1) hipLaunchKernelGGL(k_star32g16, dim3(nb), dim3(64), 0, s, p, nitems);
+            else if (v == 2) hipLaunchKernelGGL(k_star32g16f, dim3(nb), dim3(64), 0, s, p, nitems);
+            else if (v == 3) hipLaunchKernelGGL(k_star32g16f4, dim3(nb), dim3(64), 0, s, p, nitems);
+            else hipLaunchKernelGGL(k_star32g32f, dim3(nb), dim3(64), 0, s, p, nitems);
+        }
     } else if (vmax == STAR_VMAX_T1) {
         hipLaunchKernelGGL(HIP_KERNEL_NAME(k_star<STAR_VMAX_T1, false>), dim3(nblocks), dim3(64), 0, s, p, nitems);
     } else if (vmax == STAR_VMAX_T2) {
