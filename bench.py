@@ -149,6 +149,16 @@ def run_c4(args):
                               "prop_pairs": l2["prop_pairs"], "t_prop_ms": round(l2["t_prop_ms"], 3),
                               "splay_insertions": l2["splay_insertions"], "stages_ms": _stage_means(stages2)}
             out["learned_vs_plain"] = round(out["learned"]["value"] / out["plain"]["value"], 3)
+            # ablation: the same candidates, every one proposed (the network's decisions removed)
+            t.set_proposal(params, -1e30)
+            secs3, step3, stages3, nt3 = time_plain(t, d_xyz, n, args.c4_steps, 1, "c4 all candidates")
+            l3 = stages3[-1]
+            out["all_candidates"] = {"value": round(n * len(step3) / secs3, 1),
+                                     "ms_per_step": round(secs3 * 1e3 / len(step3), 3),
+                                     "splay_rounds": l3["splay_rounds"], "prop_pairs": l3["prop_pairs"],
+                                     "splay_insertions": l3["splay_insertions"], "num_tets": int(nt3),
+                                     "stages_ms": _stage_means(stages3)}
+            out["learned_vs_all_candidates"] = round(out["learned"]["value"] / out["all_candidates"]["value"], 3)
             t.set_proposal(enable=False)
         else:
             out["learned"] = None

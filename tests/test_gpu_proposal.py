@@ -191,3 +191,33 @@ def test_proposal_in_batched_mode(orc):
         assert (g.canonical(ta) == g.canonical(tb)).all(), k
     ref, _, _ = orc.triangulate(sets[5], int(gs[5]))
     assert (g.canonical(b[boff[5]:boff[6]]) == ref).all()
+
+
+def test_c4_learned_proposal_matches_golden():
+    """C4 (8M clustered mix, G = 512) with the committed trained weights: the tetrahedra equal the
+    oracle's hash (the proposal changes only the splaying work), every star is consistent, and the
+    trained network needs fewer splaying insertions and rounds than the plain pipeline."""
+    import hashlib
+    import json
+    import os
+    gold = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "pipeline_golden.json")))
+    want = gold["c4_clustered_8m_g512"]
+    params, thr, rec = P.load_weights()
+    x = datasets.make("c4_clustered_8m_g512")
+    with g.Triangulator(grid_size=want["grid"], adjacency=False) as t:
+        d, n = t.upload(x)
+        t.triangulate_device(d, n)
+        plain = t.stats()
+        t.set_proposal(params, thr)
+        tets, _, _ = t.triangulate(x)
+        st = t.stats()
+        bad, tot = t.check_stars()
+    print("c4 learned: %d tetrahedra, rounds %d -> %d, insertions %d -> %d, %.1f -> %.1f ms" % (
+        tets.shape[0], plain["splay_rounds"], st["splay_rounds"], plain["splay_insertions"], st["splay_insertions"],
+        plain["t_total_ms"], st["t_total_ms"]), flush=True)
+    c = g.canonical(tets)
+    assert c.shape[0] == want["num_tets"]
+    assert hashlib.sha256(np.ascontiguousarray(c, "<i4").tobytes()).hexdigest() == want["sha256"]
+    assert bad == 0 and tot >= 4 * want["num_tets"]
+    assert st["prop_enabled"] == 1 and st["prop_pairs"] > 0
+    assert st["splay_insertions"] < plain["splay_insertions"] and st["splay_rounds"] <= plain["splay_rounds"]
