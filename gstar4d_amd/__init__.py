@@ -17,6 +17,11 @@ __all__ = ["Triangulator", "Options", "Stats", "CheckReport", "G4DError", "load_
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libgstar4d.so")
 
+# Limits of the allocation-free entry point (include/gstar4d.h).
+NOALLOC_SCRATCH_BYTES = 256 << 20
+NOALLOC_MAX_POINTS = 16384
+NOALLOC_MAX_GRID = 128
+
 STATUS = {0: "OK", 1: "INVALID_INPUT", 2: "DUPLICATE_POINTS", 3: "GRID_LIMIT", 4: "OOM",
           5: "DEVICE", 6: "NOT_CONVERGED", 7: "CAPACITY", 8: "PREDICATE_RANGE", 9: "INTERNAL"}
 
@@ -109,7 +114,8 @@ EXPORTS = ["g4d_default_options", "g4d_create", "g4d_destroy", "g4d_last_error",
            "g4d_check_result", "g4d_check_mesh", "g4d_read_asc", "g4d_free_point_file", "g4d_write_asc",
            "g4d_write_ply", "g4d_triangulate_batch", "g4d_triangulate_batch_device", "g4d_batch_grid_size",
            "g4d_struct_sizes", "g4d_check_sizes", "g4d_set_proposal", "g4d_set_sample_export", "g4d_get_samples",
-           "g4d_mlp_forward", "g4d_mlp_grad", "g4d_mlp_train", "g4d_check_stars", "g4d_diag_set_link_vertex"]
+           "g4d_mlp_forward", "g4d_mlp_grad", "g4d_mlp_train", "g4d_check_stars", "g4d_diag_set_link_vertex",
+           "g4d_triangulate_device_noalloc"]
 
 MLP_PARAMS = 7312
 PROP_FEATURES = 12
@@ -135,6 +141,7 @@ def load_library(path=None):
     L.g4d_triangulate.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.POINTER(Result)]
     L.g4d_triangulate_device.argtypes = [_vp, _vp, ctypes.c_int32, _lp]
     L.g4d_copy_result.argtypes = [_vp, _ip, _ip, _ip]
+    L.g4d_triangulate_device_noalloc.argtypes = [_vp, _vp, ctypes.c_int32, _vp, _vp, ctypes.c_int64, _vp, _lp, _lp]
     L.g4d_upload_points.argtypes = [_vp, _dp, ctypes.c_int32, ctypes.POINTER(_vp)]
     L.g4d_get_stats.argtypes = [_vp, ctypes.POINTER(Stats)]
     L.g4d_set_options.argtypes = [_vp, ctypes.POINTER(Options)]
@@ -335,6 +342,17 @@ class Triangulator:
         nt = ctypes.c_int64(0)
         self._check(self._L.g4d_triangulate_device(self._ctx, _vp(d_xyz), int(n), ctypes.byref(nt)))
         return nt.value
+
+    def triangulate_device_noalloc(self, d_xyz, n, d_tets, d_opp, max_tets, d_perm=None):
+        """Allocation-free triangulation of n device-resident points into the caller's device
+        buffers d_tets / d_opp (max_tets x 4 int32 each; d_opp may be None without adjacency)
+        and d_perm (n int32, optional).  Returns (num_tets, scratch bytes used)."""
+        nt, used = ctypes.c_int64(0), ctypes.c_int64(0)
+        self._check(self._L.g4d_triangulate_device_noalloc(self._ctx, _vp(d_xyz), int(n), _vp(d_tets),
+                                                           _vp(d_opp) if d_opp else None, int(max_tets),
+                                                           _vp(d_perm) if d_perm else None, ctypes.byref(nt),
+                                                           ctypes.byref(used)))
+        return nt.value, used.value
 
     def copy_result(self, nt, n, want_opp=True):
         tets = np.empty((max(nt, 1), 4), np.int32)

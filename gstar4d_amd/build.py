@@ -10,7 +10,7 @@ CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libgstar4d.so")
 
-SOURCES = ["g4d_prims.hip", "g4d_grid.hip", "g4d_star.hip", "g4d_proposal.hip", "g4d_check.hip", "g4d_diag.hip",
+SOURCES = ["g4d_prims.hip", "g4d_grid.hip", "g4d_star.hip", "g4d_proposal.hip", "g4d_check.hip", "g4d_diag.hip", "g4d_noalloc.hip",
            "g4d_api.cpp", "g4d_io.cpp"]
 
 # -ffp-contract=off: the grid scaling must round exactly like the host, and the filters'

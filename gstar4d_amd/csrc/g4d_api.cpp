@@ -21,11 +21,20 @@
 #include "g4d_check.h"
 #include "g4d_diag.h"
 #include "g4d_grid.h"
+#include "g4d_noalloc.h"
 #include "g4d_prims.h"
 #include "g4d_proposal.h"
 #include "g4d_star.h"
 
 using g4d::DevBuf;
+
+namespace g4d {
+ScratchArena *&scratch_arena_slot()
+{
+    static thread_local ScratchArena *arena = nullptr;
+    return arena;
+}
+} // namespace g4d
 
 namespace {
 
@@ -82,6 +91,7 @@ struct g4d_ctx {
     hipEvent_t ev[EV_NUM];
     hipEvent_t ev_t0[2]; // around each launch of the 32-vertex star kernel
     bool ev_ok = false;
+    hipError_t noalloc_status = hipErrorNotInitialized; // lookup of the scratch array's address
 };
 
 // Stage trace on stderr when the environment sets G4D_TRACE (flushed per line).
@@ -993,6 +1003,8 @@ int g4d_create(g4d_ctx **out, const g4d_options *opt)
         delete ctx;
         return G4D_ERR_DEVICE;
     }
+    // device address of the allocation-free entry point's scratch array (once per device)
+    ctx->noalloc_status = g4d::noalloc_setup(ctx->opt.device);
     ctx->ev_ok = true;
     for (int i = 0; i < EV_NUM; i++)
         if (hipEventCreate(&ctx->ev[i]) != hipSuccess) ctx->ev_ok = false;
@@ -1157,6 +1169,69 @@ int g4d_copy_result(g4d_ctx *ctx, int32_t *tets, int32_t *opp, int32_t *perm)
     if (perm) HCK(hipMemcpyAsync(perm, ctx->perm.ptr, sizeof(int) * (size_t)ctx->last_n, hipMemcpyDeviceToHost, ctx->stream));
     HCK(hipStreamSynchronize(ctx->stream));
     return G4D_OK;
+}
+
+int g4d_triangulate_device_noalloc(g4d_ctx *ctx, const double *d_xyz, int32_t n, int32_t *d_tets, int32_t *d_opp,
+                                   int64_t max_tets, int32_t *d_perm, int64_t *num_tets, int64_t *scratch_used)
+{
+    if (!ctx || !d_xyz || !d_tets || max_tets < 0 || (ctx->opt.compute_adjacency && !d_opp))
+        return G4D_ERR_INVALID_INPUT;
+    if (num_tets) *num_tets = 0;
+    if (scratch_used) *scratch_used = 0;
+    if (n > G4D_NOALLOC_MAX_POINTS)
+        return set_err(ctx, G4D_ERR_CAPACITY, "allocation-free call: %d points, at most %d", n,
+                       G4D_NOALLOC_MAX_POINTS);
+    if (ctx->opt.grid_size > G4D_NOALLOC_MAX_GRID)
+        return set_err(ctx, G4D_ERR_CAPACITY, "allocation-free call: grid %d, at most %d", ctx->opt.grid_size,
+                       G4D_NOALLOC_MAX_GRID);
+    HCK(hipSetDevice(ctx->opt.device));
+    void *base = g4d::noalloc_scratch(ctx->opt.device);
+    if (ctx->noalloc_status != hipSuccess || !base)
+        return set_err(ctx, G4D_ERR_DEVICE, "allocation-free call: scratch array unavailable (%s)",
+                       hipGetErrorString(ctx->noalloc_status));
+    // A context of its own for the run: same stream, options and events, every buffer carved
+    // from the scratch array.  Nothing in it is freed (the arena owns the memory); the
+    // caller's context keeps its buffers and its last result.
+    g4d_ctx tmp;
+    tmp.opt = ctx->opt;
+    tmp.stream = ctx->stream;
+    tmp.ev_ok = ctx->ev_ok;
+    for (int i = 0; i < EV_NUM; i++) tmp.ev[i] = ctx->ev[i];
+    for (int i = 0; i < 2; i++) tmp.ev_t0[i] = ctx->ev_t0[i];
+    g4d::ScratchArena arena;
+    arena.base = static_cast<char *>(base);
+    arena.cap = (size_t)G4D_NOALLOC_SCRATCH_BYTES;
+    int r;
+    {
+        g4d::ScratchScope scope(&arena);
+        r = run_device(&tmp, d_xyz, n, single_set(n, ctx->opt.grid_size));
+        if (r == G4D_OK) {
+            const int64_t nt = tmp.num_tets;
+            if (num_tets) *num_tets = nt;
+            if (nt > max_tets) {
+                r = set_err(&tmp, G4D_ERR_CAPACITY, "allocation-free call: %lld tetrahedra, output holds %lld",
+                            (long long)nt, (long long)max_tets);
+            } else {
+                hipStream_t s = ctx->stream;
+                hipError_t e = hipSuccess;
+                if (nt > 0)
+                    e = hipMemcpyAsync(d_tets, tmp.tets.ptr, sizeof(int) * 4 * (size_t)nt, hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess && nt > 0 && ctx->opt.compute_adjacency)
+                    e = hipMemcpyAsync(d_opp, tmp.opp.ptr, sizeof(int) * 4 * (size_t)nt, hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess && d_perm)
+                    e = hipMemcpyAsync(d_perm, tmp.perm.ptr, sizeof(int) * (size_t)n, hipMemcpyDeviceToDevice, s);
+                if (e == hipSuccess) e = hipStreamSynchronize(s);
+                if (e != hipSuccess) r = hip_fail(&tmp, e, "allocation-free output copy", __LINE__);
+            }
+        } else if (arena.exhausted) {
+            r = set_err(&tmp, G4D_ERR_OOM, "allocation-free call: input needs more than the %lld-byte scratch array",
+                        (long long)G4D_NOALLOC_SCRATCH_BYTES);
+        }
+    }
+    if (scratch_used) *scratch_used = (int64_t)arena.top;
+    ctx->st = tmp.st;
+    if (r != G4D_OK) ctx->err = tmp.err;
+    return r;
 }
 
 int g4d_triangulate(g4d_ctx *ctx, const double *xyz, int32_t n, g4d_result *out)

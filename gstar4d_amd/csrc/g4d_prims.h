@@ -7,6 +7,23 @@
 
 namespace g4d {
 
+// Bump allocator over a caller-supplied device region.  While one is installed on the calling
+// thread (ScratchScope), DevBuf::ensure carves from it instead of calling hipMalloc and
+// DevBuf::release forgets the pointer instead of calling hipFree, so a whole pipeline run can
+// take every buffer from one preallocated region.
+struct ScratchArena {
+    char *base = nullptr;
+    size_t cap = 0;
+    size_t top = 0;
+    bool exhausted = false;
+};
+ScratchArena *&scratch_arena_slot();
+struct ScratchScope {
+    ScratchArena *prev;
+    explicit ScratchScope(ScratchArena *a) : prev(scratch_arena_slot()) { scratch_arena_slot() = a; }
+    ~ScratchScope() { scratch_arena_slot() = prev; }
+};
+
 // Growable device buffer owned by a context; never shrinks, reused across calls.
 struct DevBuf {
     void *ptr = nullptr;
@@ -14,6 +31,24 @@ struct DevBuf {
     hipError_t ensure(size_t need)
     {
         if (need <= bytes) return hipSuccess;
+        if (ScratchArena *a = scratch_arena_slot()) {
+            // the most recent region grows in place
+            if (ptr && static_cast<char *>(ptr) + bytes == a->base + a->top &&
+                need <= a->cap - (size_t)(static_cast<char *>(ptr) - a->base)) {
+                a->top = (size_t)(static_cast<char *>(ptr) - a->base) + need;
+                bytes = need;
+                return hipSuccess;
+            }
+            const size_t off = (a->top + 255) & ~(size_t)255;
+            if (off > a->cap || need > a->cap - off) {
+                a->exhausted = true;
+                return hipErrorOutOfMemory;
+            }
+            ptr = a->base + off;
+            bytes = need;
+            a->top = off + need;
+            return hipSuccess;
+        }
         if (ptr) {
             hipError_t e = hipFree(ptr);
             if (e != hipSuccess) return e;
@@ -31,7 +66,7 @@ struct DevBuf {
     }
     void release()
     {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && !scratch_arena_slot()) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
     }

@@ -146,6 +146,26 @@ int g4d_triangulate(g4d_ctx *ctx, const double *xyz, int32_t n, g4d_result *out)
    input); g4d_copy_result copies the last result to host buffers (any may be NULL). */
 int g4d_triangulate_device(g4d_ctx *ctx, const double *d_xyz, int32_t n, int64_t *num_tets);
 int g4d_copy_result(g4d_ctx *ctx, int32_t *tets, int32_t *opp, int32_t *perm);
+/* Allocation-free triangulation.  No counterpart in the reference: it is for callers that cannot
+   allocate device memory (code under HIP graph capture, tight serving loops, harnesses that
+   forbid hipMalloc).  It runs the same pipeline as g4d_triangulate_device, but takes every
+   intermediate buffer from a static device array of G4D_NOALLOC_SCRATCH_BYTES bytes that each
+   GPU the library runs on holds until the process ends, and makes no hipMalloc or hipFree.
+   Limits: n <= G4D_NOALLOC_MAX_POINTS and the context's grid_size <= G4D_NOALLOC_MAX_GRID
+   (G4D_ERR_CAPACITY otherwise); an input that needs more scratch than the array holds returns
+   G4D_ERR_OOM.  Outputs are device buffers, in the context's index space, identical to what
+   g4d_copy_result gives after g4d_triangulate_device on the same points: d_tets (max_tets x 4
+   int32), d_opp (max_tets x 4 int32; required when the context computes adjacency, ignored
+   otherwise), d_perm (n int32, Morton index -> input index; may be NULL).  *num_tets (may be
+   NULL) receives the count, also when it exceeds max_tets (then G4D_ERR_CAPACITY and nothing
+   is written); *scratch_used (may be NULL) the scratch bytes the call took.  All calls on one
+   GPU share that GPU's array, so calls that overlap on one GPU race.  The context's own last
+   result (g4d_copy_result, the checks) is left as it was. */
+#define G4D_NOALLOC_SCRATCH_BYTES (256LL << 20)
+#define G4D_NOALLOC_MAX_POINTS 16384
+#define G4D_NOALLOC_MAX_GRID 128
+int g4d_triangulate_device_noalloc(g4d_ctx *ctx, const double *d_xyz, int32_t n, int32_t *d_tets, int32_t *d_opp,
+                                   int64_t max_tets, int32_t *d_perm, int64_t *num_tets, int64_t *scratch_used);
 /* Upload host points into a device buffer owned by ctx and return its device address. */
 int g4d_upload_points(g4d_ctx *ctx, const double *xyz, int32_t n, const double **d_xyz);
 int g4d_get_stats(const g4d_ctx *ctx, g4d_stats *st);
