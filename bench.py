@@ -187,7 +187,10 @@ def run_c5(args, steps, warmup, per_set=True):
     off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
     xyz = np.concatenate(sets)
     del sets
-    grids = np.array([g.batch_grid_size(int(n)) for n in sizes], np.int32)
+    from gstar4d_amd import autotune
+    t = g.Triangulator(grid_size=64, adjacency=True)
+    tuned = autotune.apply(t)  # per-size grid table and star kernel from the autotuner, when present
+    grids = (autotune.grids(t, sizes) if tuned else np.array([g.batch_grid_size(int(n)) for n in sizes], np.int32))
     chunks, a = [], 0
     while a < len(sizes):
         b = a + 1
@@ -197,7 +200,6 @@ def run_c5(args, steps, warmup, per_set=True):
         a = b
     log("batch: %d sets, %d points, %d chunks, generated in %.1f s" % (len(sizes), int(off[-1]), len(chunks),
                                                                      time.perf_counter() - t_gen))
-    t = g.Triangulator(grid_size=64, adjacency=True)
     d_xyz, n = t.upload(xyz)
 
     def batch_pass():
@@ -235,6 +237,8 @@ def run_c5(args, steps, warmup, per_set=True):
             "steps": steps, "sets": int(len(sizes)), "points": int(n),
             "set_sizes": [int(sizes.min()), int(sizes.max())], "grids": [int(grids.min()), int(grids.max())],
             "calls_per_step": len(chunks), "chunk_points": args.chunk_points, "num_tets": int(nt),
+            "autotuned": ({"grid_table": [[b["max_points"], b["voxels_per_point"]] for b in tuned["buckets"]],
+                           "star_kernel": tuned.get("star_kernel")} if tuned else None),
             "step_ms": [round(v, 3) for v in step_ms], "per_set": base,
             "batched_vs_per_set": round(value / base["value"], 3) if base else None}
 

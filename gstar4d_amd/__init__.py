@@ -63,7 +63,7 @@ class Stats(ctypes.Structure):
                 ("star_store_bytes", ctypes.c_int64), ("store_repacks", ctypes.c_int32),
                 ("star_passes", ctypes.c_int32), ("star_t0_launches", ctypes.c_int32), ("pad1", ctypes.c_int32),
                 ("t_star_t0_ms", ctypes.c_double), ("star_t0_filter_evals", ctypes.c_int64),
-                ("star_t0_cofactors", ctypes.c_int64), ("prop_enabled", ctypes.c_int32), ("pad2", ctypes.c_int32),
+                ("star_t0_cofactors", ctypes.c_int64), ("prop_enabled", ctypes.c_int32), ("pba_bands", ctypes.c_int32),
                 ("prop_pairs", ctypes.c_int64), ("t_prop_ms", ctypes.c_double)]
 
     def as_dict(self):
@@ -115,7 +115,8 @@ EXPORTS = ["g4d_default_options", "g4d_create", "g4d_destroy", "g4d_last_error",
            "g4d_write_ply", "g4d_triangulate_batch", "g4d_triangulate_batch_device", "g4d_batch_grid_size",
            "g4d_struct_sizes", "g4d_check_sizes", "g4d_set_proposal", "g4d_set_sample_export", "g4d_get_samples",
            "g4d_mlp_forward", "g4d_mlp_grad", "g4d_mlp_train", "g4d_check_stars", "g4d_diag_set_link_vertex",
-           "g4d_triangulate_device_noalloc"]
+           "g4d_triangulate_device_noalloc", "g4d_set_star_kernel", "g4d_set_grid_table", "g4d_grid_for_set",
+           "g4d_set_pba_bands"]
 
 MLP_PARAMS = 7312
 PROP_FEATURES = 12
@@ -159,6 +160,10 @@ def load_library(path=None):
     L.g4d_triangulate_batch.argtypes = [_vp, _dp, _lp, ctypes.c_int32, _ip, ctypes.POINTER(BatchResult)]
     L.g4d_triangulate_batch_device.argtypes = [_vp, _vp, _lp, ctypes.c_int32, _ip, _lp]
     L.g4d_batch_grid_size.argtypes = [ctypes.c_int64]
+    L.g4d_set_star_kernel.argtypes = [_vp, ctypes.c_int32]
+    L.g4d_set_grid_table.argtypes = [_vp, ctypes.c_int32, _lp, ctypes.POINTER(ctypes.c_float)]
+    L.g4d_grid_for_set.argtypes = [_vp, ctypes.c_int64]
+    L.g4d_set_pba_bands.argtypes = [_vp, ctypes.c_int32]
     L.g4d_struct_sizes.argtypes = [_lp, ctypes.c_int32]
     L.g4d_check_sizes.argtypes = [ctypes.c_int64, ctypes.c_int64]
     _fp = ctypes.POINTER(ctypes.c_float)
@@ -342,6 +347,27 @@ class Triangulator:
         nt = ctypes.c_int64(0)
         self._check(self._L.g4d_triangulate_device(self._ctx, _vp(d_xyz), int(n), ctypes.byref(nt)))
         return nt.value
+
+    def set_star_kernel(self, variant=-1):
+        """Variant of the 32-vertex star kernel (-1: built-in default; see g4d_set_star_kernel)."""
+        self._check(self._L.g4d_set_star_kernel(self._ctx, int(variant)))
+
+    def set_grid_table(self, max_points=(), voxels_per_point=()):
+        """Grids of batched sets without explicit grids from a per-size table (empty: the
+        built-in formula)."""
+        mp = np.ascontiguousarray(max_points, dtype=np.int64)
+        vp = np.ascontiguousarray(voxels_per_point, dtype=np.float32)
+        if mp.shape != vp.shape:
+            raise ValueError("max_points and voxels_per_point differ in length")
+        self._check(self._L.g4d_set_grid_table(self._ctx, int(mp.shape[0]), _ptr(mp, _lp) if mp.size else None,
+                                               _ptr(vp, ctypes.POINTER(ctypes.c_float)) if vp.size else None))
+
+    def set_pba_bands(self, bands=0):
+        """Bands of the y and x PBA passes (0: by grid width; 1: whole columns)."""
+        self._check(self._L.g4d_set_pba_bands(self._ctx, int(bands)))
+
+    def grid_for_set(self, n):
+        return int(self._L.g4d_grid_for_set(self._ctx, int(n)))
 
     def triangulate_device_noalloc(self, d_xyz, n, d_tets, d_opp, max_tets, d_perm=None):
         """Allocation-free triangulation of n device-resident points into the caller's device

@@ -71,7 +71,8 @@ struct g4d_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     g4d::Prims P;
-    DevBuf xyz_up, keys, pts, perm, vox, bounds, settab, ptset, grid, gtmp, idgrid, pbaspill, pcount, pairs, ws_off, cnt;
+    DevBuf xyz_up, keys, pts, perm, vox, bounds, settab, ptset, grid, gtmp, idgrid, pbaspill, pbaenv, pbacnt, pcount, pairs,
+        ws_off, cnt;
     Store store;
     DevBuf settet, drown, dflag, pairbuf, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, aflags, apos, active, mid, big, huge, xstars,
         xtasks, xtris, gwork, tetcnt, tetoff, tets, opp, table;
@@ -92,6 +93,11 @@ struct g4d_ctx {
     hipEvent_t ev_t0[2]; // around each launch of the 32-vertex star kernel
     bool ev_ok = false;
     hipError_t noalloc_status = hipErrorNotInitialized; // lookup of the scratch array's address
+    // tuning (g4d_set_star_kernel, g4d_set_grid_table)
+    int star_t0_variant = -1;
+    int pba_bands = 0; // bands of the y and x PBA passes (0: by grid size)
+    std::vector<int64_t> grid_tab_n;   // ascending upper bounds of the set-size buckets
+    std::vector<float> grid_tab_vpp;   // voxels per point of each bucket
 };
 
 // Stage trace on stderr when the environment sets G4D_TRACE (flushed per line).
@@ -400,6 +406,55 @@ static int proposal_stage(g4d_ctx *ctx, int n, const g4d::GridSets &GS, const in
     return G4D_OK;
 }
 
+// Bands of the y and x PBA passes: the context's choice, else G4D_PBA_BANDS (measurement), else
+// by grid width (bands pay once a column is long enough to give each band's wave real work).
+static int pba_bands_for(const g4d_ctx *ctx, int gmax)
+{
+    int b = ctx->pba_bands;
+    if (b <= 0) {
+        static int env = -2;
+        if (env == -2) {
+            const char *e = getenv("G4D_PBA_BANDS");
+            env = e ? atoi(e) : 0;
+        }
+        b = env > 0 ? env : (gmax >= 384 ? 4 : gmax >= 192 ? 2 : 1);
+    }
+    return std::max(1, std::min(b, g4d::PBA_MAX_BANDS));
+}
+
+// The three PBA axis passes: codes in grid -> (z) gtmp -> (y) grid -> (x) gtmp, the last with
+// point ids (idgrid, or the ids the grids hold when vox is given).  The y and x passes run in
+// bands when pba_bands_for says so.
+static int run_pba(g4d_ctx *ctx, const g4d::GridSets &GS, const SetTotals &tot, int gmax, int *grid, int *gtmp,
+                   const int *idgrid_last, const uint64_t *pvox, bool events)
+{
+    hipStream_t s = ctx->stream;
+    HCK(ctx->pbaspill.ensure(sizeof(int) * (size_t)tot.spill));
+    int *spill = buf<int>(ctx->pbaspill);
+    const int bands = pba_bands_for(ctx, gmax);
+    int *benv = nullptr, *bcnt = nullptr, *fcnt = nullptr;
+    if (bands > 1) {
+        HCK(ctx->pbaenv.ensure(sizeof(int) * (size_t)tot.spill));
+        HCK(ctx->pbacnt.ensure(sizeof(int) * 64 * (size_t)tot.pba_blocks * (size_t)(bands + 1)));
+        benv = buf<int>(ctx->pbaenv);
+        bcnt = buf<int>(ctx->pbacnt);
+        fcnt = bcnt + 64 * tot.pba_blocks * bands;
+    }
+    if (events) ev_record(ctx, EV_PBA0);
+    HCK(g4d::pba_pass(2, GS, tot.pba_blocks, grid, gtmp, nullptr, spill, pvox, s));
+    if (events) ev_record(ctx, EV_PBA1);
+    if (bands > 1) HCK(g4d::pba_pass_banded(1, GS, tot.pba_blocks, bands, gtmp, grid, nullptr, spill, benv, bcnt, fcnt,
+                                             pvox, s));
+    else HCK(g4d::pba_pass(1, GS, tot.pba_blocks, gtmp, grid, nullptr, spill, pvox, s));
+    if (events) ev_record(ctx, EV_PBA2);
+    if (bands > 1) HCK(g4d::pba_pass_banded(0, GS, tot.pba_blocks, bands, grid, gtmp, idgrid_last, spill, benv, bcnt,
+                                             fcnt, pvox, s));
+    else HCK(g4d::pba_pass(0, GS, tot.pba_blocks, grid, gtmp, idgrid_last, spill, pvox, s));
+    if (events) ev_record(ctx, EV_PBA3);
+    ctx->st.pba_bands = bands;
+    return G4D_OK;
+}
+
 static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &spec)
 {
     ctx->last_xyz = d_xyz;
@@ -483,15 +538,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     const uint64_t *pvox = id_mode ? vox : nullptr;
     HCK(g4d::launch_fill_i32(grid, G3, G4D_MARKER, s));
     HCK(g4d::launch_scatter_sites(vox, n, GS, grid, idgrid, s));
-    ev_record(ctx, EV_PBA0);
-    HCK(ctx->pbaspill.ensure(sizeof(int) * (size_t)tot.spill));
-    int *spill = buf<int>(ctx->pbaspill);
-    HCK(g4d::pba_pass(2, GS, tot.pba_blocks, grid, gtmp, nullptr, spill, pvox, s));
-    ev_record(ctx, EV_PBA1);
-    HCK(g4d::pba_pass(1, GS, tot.pba_blocks, gtmp, grid, nullptr, spill, pvox, s));
-    ev_record(ctx, EV_PBA2);
-    HCK(g4d::pba_pass(0, GS, tot.pba_blocks, grid, gtmp, idgrid, spill, pvox, s)); // point ids in gtmp
-    ev_record(ctx, EV_PBA3);
+    CK(run_pba(ctx, GS, tot, gmax, grid, gtmp, idgrid, pvox, true)); // point ids in gtmp
     const int *idg = gtmp;
 
     // ---- D: missing points, grid pairs, worksets, pumping (GD/GDelaunay/GDelHost.cu:259-529) ----
@@ -693,6 +740,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         sp.exact_list = buf<int>(ctx->xstars);
         sp.exact_cap = n;
         sp.exact_counter = G4D_CNT_APPEND4;
+        sp.t0_variant = ctx->star_t0_variant;
         // list counters, filter counts and the six work-queue heads of this pass, in one launch
         {
             uint32_t zmask = (1u << G4D_CNT_APPEND2) | (1u << G4D_CNT_APPEND3) | (1u << G4D_CNT_APPEND4) |
@@ -1020,7 +1068,7 @@ void g4d_destroy(g4d_ctx *ctx)
     (void)hipSetDevice(ctx->opt.device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->xyz_up, &ctx->keys, &ctx->pts, &ctx->perm, &ctx->vox, &ctx->bounds, &ctx->settab, &ctx->ptset, &ctx->grid,
-                      &ctx->gtmp, &ctx->idgrid, &ctx->pbaspill, &ctx->pcount, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->settet, &ctx->drown, &ctx->dflag, &ctx->pairbuf, &ctx->dkeys,
+                      &ctx->gtmp, &ctx->idgrid, &ctx->pbaspill, &ctx->pbaenv, &ctx->pbacnt, &ctx->pcount, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->settet, &ctx->drown, &ctx->dflag, &ctx->pairbuf, &ctx->dkeys,
                       &ctx->tasks, &ctx->cand, &ctx->keep, &ctx->ins, &ctx->ins_off, &ctx->vcap, &ctx->tcap,
                       &ctx->hcap, &ctx->aflags, &ctx->apos, &ctx->active, &ctx->mid, &ctx->big, &ctx->huge, &ctx->xstars, &ctx->xtasks, &ctx->xtris, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table,
                       &ctx->mlp_params, &ctx->fkeys, &ctx->ford, &ctx->frank, &ctx->pcand, &ctx->pd2, &ctx->pfeat,
@@ -1077,12 +1125,56 @@ int g4d_check_sizes(int64_t sort_keys, int64_t num_tets)
     return G4D_OK;
 }
 
+// Smallest grid of the batched sizes (multiples of 8 within [32, 256]) with at least vpp
+// voxels per point.
+static int grid_for_density(int64_t n, double vpp)
+{
+    int g = 32;
+    while (g < g4d::BATCH_MAX_GRID && (double)g * g * g < vpp * (double)n) g += 8;
+    return g;
+}
+
 int g4d_batch_grid_size(int64_t n)
 {
     // about 24 voxels per point (C1 has 21, C2 32), in steps of 16, within [32, 256]
     int g = 32;
     while (g < g4d::BATCH_MAX_GRID && (int64_t)g * g * g < 24 * n) g += 16;
     return g;
+}
+
+int g4d_set_star_kernel(g4d_ctx *ctx, int32_t variant)
+{
+    if (!ctx || variant < -1 || variant >= g4d::STAR_T0_NUM) return G4D_ERR_INVALID_INPUT;
+    ctx->star_t0_variant = variant;
+    return G4D_OK;
+}
+
+int g4d_set_grid_table(g4d_ctx *ctx, int32_t nbuckets, const int64_t *max_points, const float *voxels_per_point)
+{
+    if (!ctx || nbuckets < 0 || nbuckets > 64 || (nbuckets > 0 && (!max_points || !voxels_per_point)))
+        return G4D_ERR_INVALID_INPUT;
+    for (int k = 0; k < nbuckets; k++) {
+        if (!(voxels_per_point[k] > 0.f) || max_points[k] <= 0 || (k > 0 && max_points[k] <= max_points[k - 1]))
+            return set_err(ctx, G4D_ERR_INVALID_INPUT, "grid table entry %d invalid", k);
+    }
+    ctx->grid_tab_n.assign(max_points, max_points + nbuckets);
+    ctx->grid_tab_vpp.assign(voxels_per_point, voxels_per_point + nbuckets);
+    return G4D_OK;
+}
+
+int g4d_set_pba_bands(g4d_ctx *ctx, int32_t bands)
+{
+    if (!ctx || bands < 0 || bands > g4d::PBA_MAX_BANDS) return G4D_ERR_INVALID_INPUT;
+    ctx->pba_bands = bands;
+    return G4D_OK;
+}
+
+int g4d_grid_for_set(const g4d_ctx *ctx, int64_t n)
+{
+    if (!ctx || ctx->grid_tab_n.empty()) return g4d_batch_grid_size(n);
+    size_t k = 0;
+    while (k + 1 < ctx->grid_tab_n.size() && n > ctx->grid_tab_n[k]) k++;
+    return grid_for_density(n, (double)ctx->grid_tab_vpp[k]);
 }
 
 static int batch_spec(g4d_ctx *ctx, const int64_t *set_offsets, int32_t nsets, const int32_t *grids, int64_t *n_out,
@@ -1096,7 +1188,7 @@ static int batch_spec(g4d_ctx *ctx, const int64_t *set_offsets, int32_t nsets, c
     spec.g.resize((size_t)nsets);
     for (int k = 0; k < nsets; k++) {
         if (set_offsets[k + 1] <= set_offsets[k]) return set_err(ctx, G4D_ERR_INVALID_INPUT, "set %d is empty", k);
-        spec.g[(size_t)k] = grids ? grids[k] : g4d_batch_grid_size(set_offsets[k + 1] - set_offsets[k]);
+        spec.g[(size_t)k] = grids ? grids[k] : g4d_grid_for_set(ctx, set_offsets[k + 1] - set_offsets[k]);
     }
     if (set_offsets[nsets] > (1 << 29)) return set_err(ctx, G4D_ERR_INVALID_INPUT, "too many points in the batch");
     *n_out = set_offsets[nsets];
@@ -1195,6 +1287,8 @@ int g4d_triangulate_device_noalloc(g4d_ctx *ctx, const double *d_xyz, int32_t n,
     g4d_ctx tmp;
     tmp.opt = ctx->opt;
     tmp.stream = ctx->stream;
+    tmp.star_t0_variant = ctx->star_t0_variant;
+    tmp.pba_bands = ctx->pba_bands;
     tmp.ev_ok = ctx->ev_ok;
     for (int i = 0; i < EV_NUM; i++) tmp.ev[i] = ctx->ev[i];
     for (int i = 0; i < 2; i++) tmp.ev_t0[i] = ctx->ev_t0[i];
@@ -1534,12 +1628,8 @@ int g4d_diag_pba(g4d_ctx *ctx, const int32_t *vox_sorted, int32_t n, int32_t gri
     int *idgrid = id_mode ? nullptr : buf<int>(ctx->idgrid);
     HCK(g4d::launch_fill_i32(buf<int>(ctx->grid), G3, G4D_MARKER, s));
     HCK(g4d::launch_scatter_sites(vox, n, GS, buf<int>(ctx->grid), idgrid, s));
-    HCK(ctx->pbaspill.ensure(sizeof(int) * (size_t)tot.spill));
-    int *spill = buf<int>(ctx->pbaspill);
-    HCK(g4d::pba_pass(2, GS, tot.pba_blocks, buf<int>(ctx->grid), buf<int>(ctx->gtmp), nullptr, spill, pvox, s));
-    HCK(g4d::pba_pass(1, GS, tot.pba_blocks, buf<int>(ctx->gtmp), buf<int>(ctx->grid), nullptr, spill, pvox, s));
-    HCK(g4d::pba_pass(0, GS, tot.pba_blocks, buf<int>(ctx->grid), buf<int>(ctx->gtmp), codes == 1 ? nullptr : idgrid,
-                      spill, pvox, s));
+    CK(run_pba(ctx, GS, tot, grid, buf<int>(ctx->grid), buf<int>(ctx->gtmp), codes == 1 ? nullptr : idgrid, pvox,
+               false));
     HCK(hipMemcpyAsync(out, ctx->gtmp.ptr, sizeof(int) * (size_t)G3, hipMemcpyDeviceToHost, s));
     HCK(hipStreamSynchronize(s));
     return G4D_OK;

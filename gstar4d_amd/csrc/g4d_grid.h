@@ -53,6 +53,13 @@ int64_t pba_spill_ints(int g);
 // point ids throughout and the sites' coordinates are read from vox.
 hipError_t pba_pass(int axis, const GridSets &S, int64_t nblocks, const int *in, int *out, const int *idgrid,
                     int *spill, const uint64_t *vox, hipStream_t s);
+// The y (1) or x (0) pass in `bands` bands (2..PBA_MAX_BANDS): band envelopes, their merge, and
+// the colouring of each band (same output as pba_pass).  benv has the spill area's size; bcnt
+// holds 64 * bands ints per work-group of pba_pass, fcnt 64.
+static constexpr int PBA_MAX_BANDS = 16;
+hipError_t pba_pass_banded(int axis, const GridSets &S, int64_t nblocks, int bands, const int *in, int *out,
+                           const int *idgrid, int *spill, int *benv, int *bcnt, int *fcnt, const uint64_t *vox,
+                           hipStream_t s);
 // Ordered pairs (min<<32 | max) of each non-leader point with its voxel's leader.
 hipError_t launch_missing(const uint64_t *vox, int n, const GridSets &S, const int *grid, uint64_t *out,
                           unsigned long long *counter, int64_t cap, hipStream_t s);

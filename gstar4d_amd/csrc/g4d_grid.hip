@@ -414,20 +414,32 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
         __syncthreads();
     }
     // ---- colour walk ----
-    int idx = 0, qa = 0, qb = 0, sa = G4D_MARKER;
+    // With an id grid (the last pass), each envelope entry's point id is looked up once, when
+    // it becomes the next entry of the walk (the load is in flight while the walk covers the
+    // current entry's run of voxels), instead of once per voxel.
+    const bool ids_out = !IDS && idgrid;
+    auto id_of = [&](int code) {
+        return idgrid[((int64_t)G4D_DEC_Z(code) * g + G4D_DEC_Y(code)) * g + G4D_DEC_X(code)];
+    };
+    int idx = 0, qa = 0, qb = 0, sa = G4D_MARKER, sb = G4D_MARKER;
     int da = 0, db = 0;
     if (active && m > 0) {
         sa = L.scode[0][lane];
         site_pos_off<AXIS, IDS>(sa, vox, ca, cb, &qa, &da);
+        if (ids_out) sa = id_of(sa);
     }
-    if (active && m > 1) site_pos_off<AXIS, IDS>(get(1), vox, ca, cb, &qb, &db);
+    if (active && m > 1) {
+        sb = get(1);
+        site_pos_off<AXIS, IDS>(sb, vox, ca, cb, &qb, &db);
+        if (ids_out) sb = id_of(sb);
+    }
     for (int t0 = 0; t0 < g; t0 += PBA_CHUNK) {
         const int nt = min(PBA_CHUNK, g - t0);
         if (active) {
             for (int tt = 0; tt < nt; tt++) {
                 const int t = t0 + tt;
                 if (m == 0) {
-                    L.tile[tt][lane] = G4D_MARKER;
+                    L.tile[tt][lane] = ids_out ? -1 : G4D_MARKER;
                     continue;
                 }
                 while (idx + 1 < m) {
@@ -437,8 +449,12 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
                         idx++;
                         qa = qb;
                         da = db;
-                        sa = get(idx);
-                        if (idx + 1 < m) site_pos_off<AXIS, IDS>(get(idx + 1), vox, ca, cb, &qb, &db);
+                        sa = sb;
+                        if (idx + 1 < m) {
+                            sb = get(idx + 1);
+                            site_pos_off<AXIS, IDS>(sb, vox, ca, cb, &qb, &db);
+                            if (ids_out) sb = id_of(sb);
+                        }
                     } else {
                         break;
                     }
@@ -450,10 +466,7 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
 #pragma unroll
         for (int k = 0; k < PBA_PER_LANE; k++) {
             const int cc = pba_tile_c<AXIS>(k), qq = pba_tile_q<AXIS>(k);
-            int v = L.tile[qq][cc];
-            if (!IDS && idgrid && qq < nt && cc < ncols)
-                v = (v == G4D_MARKER) ? -1 : idgrid[((int64_t)G4D_DEC_Z(v) * g + G4D_DEC_Y(v)) * g + G4D_DEC_X(v)];
-            pre[k] = v;
+            pre[k] = L.tile[qq][cc];
         }
 #pragma unroll
         for (int k = 0; k < PBA_PER_LANE; k++) {
@@ -463,6 +476,280 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
         __syncthreads();
     }
 #undef get
+}
+
+// ------------------------------------------------------------------------------------
+// Banded axis pass (the reference's bands: kernelMaurerAxis over m2 bands, kernelMergeBands,
+// kernelColorAxis over m3 bands, GD/PBA/pba3DKernel.h:215-487).  A column of g positions is cut
+// into `bands` bands of pba_band_len(g, bands) positions, and the pass runs in three launches:
+//   1. k_pba_band_env: one wave per (column tile, band) builds the lower envelope of the band's
+//      candidate sites (a band's envelope is short, so the LDS stack is small and many waves
+//      share a CU) and writes it to `benv`;
+//   2. k_pba_band_merge: one wave per column tile pushes the bands' envelopes, in order, through
+//      one Maurer stack: a site hidden inside its band is hidden on the whole line, so the
+//      column's envelope is the envelope of the bands' envelopes, and this walk sees only their
+//      entries instead of every position; the result goes back to `benv`;
+//   3. k_pba_band_color: one wave per (column tile, band) colours the band's positions, starting
+//      at the envelope entry that owns the band's first position (binary search: entry i owns
+//      position t up to the first i whose successor is strictly farther at t).
+// `benv` and `spill` have the layout of the spill area: per column tile 64 g ints, band b's
+// region at 64 b L, entry k of lane l at 64 k + l.  bcnt holds 64 counts per (tile, band),
+// fcnt 64 per tile.
+// ------------------------------------------------------------------------------------
+
+__host__ __device__ inline int pba_band_len(int g, int bands)
+{
+    const int L = (g + bands - 1) / bands;
+    return (L + PBA_CHUNK - 1) / PBA_CHUNK * PBA_CHUNK;
+}
+static constexpr int PBA_BAND_STACK = 48;
+
+template <int AXIS, bool IDS>
+__global__ void __launch_bounds__(64) k_pba_band_env(const int *in_all, GridSets S, int *spill_all, int *benv_all,
+                                                     int *bcnt, const uint64_t *vox, int bands)
+{
+    __shared__ PbaLds<PBA_BAND_STACK> L;
+    const int lane = threadIdx.x;
+    const int64_t bb = blockIdx.x / bands;
+    const int band = (int)(blockIdx.x % bands);
+    const int st = S.nsets == 1 ? 0 : set_of_work(S.pba_blk, S.nsets, bb);
+    const int64_t lb = bb - S.pba_blk[st];
+    const int g = S.g[st];
+    const int *in = in_all + S.vox_off[st];
+    const int ntiles = (g + PBA_COLS - 1) / PBA_COLS;
+    const int fixed = (int)(lb / ntiles);
+    const int c0 = (int)(lb % ntiles) * PBA_COLS;
+    const int ncols = min(PBA_COLS, g - c0);
+    const bool active = lane < ncols;
+    const int ca = c0 + lane, cb = fixed;
+    const int Lb = pba_band_len(g, bands);
+    const int rb = band * Lb, re = min(g, rb + Lb);
+    const int64_t region = S.spill[st] + lb * PBA_COLS * g + (int64_t)band * PBA_COLS * Lb + lane;
+    int *sp = spill_all + region;
+    int *ev = benv_all + region;
+#define get(k) ((k) < PBA_BAND_STACK ? L.scode[(k)][lane] : pba_spill_get(sp, (k), PBA_BAND_STACK))
+    int pre[PBA_PER_LANE];
+    int m = 0;
+    int q0 = 0, d0 = 0, q1 = 0, d1 = 0;
+    if (rb < re) pba_load_chunk<AXIS>(pre, in, g, fixed, c0, ncols, rb);
+    for (int r0 = rb; r0 < re; r0 += PBA_CHUNK) {
+        const int nq = min(PBA_CHUNK, re - r0);
+#pragma unroll
+        for (int k = 0; k < PBA_PER_LANE; k++) {
+            const int cc = pba_tile_c<AXIS>(k), qq = pba_tile_q<AXIS>(k);
+            L.tile[qq][cc] = pre[k];
+        }
+        __syncthreads();
+        if (r0 + PBA_CHUNK < re) pba_load_chunk<AXIS>(pre, in, g, fixed, c0, ncols, r0 + PBA_CHUNK);
+        if (active) {
+            for (int qq = 0; qq < nq; qq++) {
+                const int sv = L.tile[qq][lane];
+                if (sv == G4D_MARKER) continue;
+                int qs, ds;
+                site_pos_off<AXIS, IDS>(sv, vox, ca, cb, &qs, &ds);
+                while (m >= 2 && env_dominated(q0, d0, q1, d1, qs, ds)) {
+                    m--;
+                    q1 = q0;
+                    d1 = d0;
+                    if (m >= 2) site_pos_off<AXIS, IDS>(get(m - 2), vox, ca, cb, &q0, &d0);
+                }
+                if (m < PBA_BAND_STACK) L.scode[m][lane] = sv;
+                else sp[(int64_t)(m - PBA_BAND_STACK) * PBA_COLS] = sv;
+                q0 = q1;
+                d0 = d1;
+                q1 = qs;
+                d1 = ds;
+                m++;
+            }
+        }
+        __syncthreads();
+    }
+    if (active) {
+        for (int k = 0; k < m; k++) ev[(int64_t)k * PBA_COLS] = get(k);
+        bcnt[(bb * bands + band) * PBA_COLS + lane] = m;
+    }
+#undef get
+}
+
+template <int AXIS, bool IDS>
+__global__ void __launch_bounds__(64) k_pba_band_merge(GridSets S, int *spill_all, int *benv_all, const int *bcnt,
+                                                       int *fcnt, const uint64_t *vox, int bands)
+{
+    constexpr int STACK = pba_stack(AXIS);
+    __shared__ int scode[STACK][PBA_COLS];
+    const int lane = threadIdx.x;
+    const int64_t bb = blockIdx.x;
+    const int st = S.nsets == 1 ? 0 : set_of_work(S.pba_blk, S.nsets, bb);
+    const int64_t lb = bb - S.pba_blk[st];
+    const int g = S.g[st];
+    const int ntiles = (g + PBA_COLS - 1) / PBA_COLS;
+    const int fixed = (int)(lb / ntiles);
+    const int c0 = (int)(lb % ntiles) * PBA_COLS;
+    if (lane >= min(PBA_COLS, g - c0)) return; // no barriers below: lanes are independent
+    const int ca = c0 + lane, cb = fixed;
+    const int Lb = pba_band_len(g, bands);
+    const int64_t base = S.spill[st] + lb * PBA_COLS * g + lane;
+    int *sp = spill_all + base; // entry k >= STACK of the merged stack at (k - STACK) * 64
+    int *ev = benv_all + base;
+#define get(k) ((k) < STACK ? scode[(k)][lane] : pba_spill_get(sp, (k), STACK))
+    int m = 0;
+    int q0 = 0, d0 = 0, q1 = 0, d1 = 0;
+    for (int band = 0; band < bands; band++) {
+        const int nb = band * Lb < g ? bcnt[(bb * bands + band) * PBA_COLS + lane] : 0;
+        const int *bev = ev + (int64_t)band * PBA_COLS * Lb;
+        for (int k = 0; k < nb; k++) {
+            const int sv = bev[(int64_t)k * PBA_COLS];
+            int qs, ds;
+            site_pos_off<AXIS, IDS>(sv, vox, ca, cb, &qs, &ds);
+            while (m >= 2 && env_dominated(q0, d0, q1, d1, qs, ds)) {
+                m--;
+                q1 = q0;
+                d1 = d0;
+                if (m >= 2) site_pos_off<AXIS, IDS>(get(m - 2), vox, ca, cb, &q0, &d0);
+            }
+            if (m < STACK) scode[m][lane] = sv;
+            else sp[(int64_t)(m - STACK) * PBA_COLS] = sv;
+            q0 = q1;
+            d0 = d1;
+            q1 = qs;
+            d1 = ds;
+            m++;
+        }
+    }
+    // every band entry has been read: the merged envelope goes to the front of the region
+    for (int k = 0; k < m; k++) ev[(int64_t)k * PBA_COLS] = get(k);
+    fcnt[bb * PBA_COLS + lane] = m;
+#undef get
+}
+
+template <int AXIS, bool IDS>
+__global__ void __launch_bounds__(64) k_pba_band_color(GridSets S, const int *benv_all, const int *fcnt, int *out_all,
+                                                       const int *idgrid_all, const uint64_t *vox, int bands)
+{
+    __shared__ int tile[PBA_CHUNK][PBA_COLS + 1];
+    const int lane = threadIdx.x;
+    const int64_t bb = blockIdx.x / bands;
+    const int band = (int)(blockIdx.x % bands);
+    const int st = S.nsets == 1 ? 0 : set_of_work(S.pba_blk, S.nsets, bb);
+    const int64_t lb = bb - S.pba_blk[st];
+    const int g = S.g[st];
+    int *out = out_all + S.vox_off[st];
+    const int *idgrid = idgrid_all ? idgrid_all + S.vox_off[st] : nullptr;
+    const int ntiles = (g + PBA_COLS - 1) / PBA_COLS;
+    const int fixed = (int)(lb / ntiles);
+    const int c0 = (int)(lb % ntiles) * PBA_COLS;
+    const int ncols = min(PBA_COLS, g - c0);
+    const bool active = lane < ncols;
+    const int ca = c0 + lane, cb = fixed;
+    const int Lb = pba_band_len(g, bands);
+    const int rb = band * Lb, re = min(g, rb + Lb);
+    if (rb >= re) return; // an empty band: the whole work-group leaves before any barrier
+    const int *ev = benv_all + S.spill[st] + lb * PBA_COLS * g + lane;
+    const int m = active ? fcnt[bb * PBA_COLS + lane] : 0;
+    const bool ids_out = !IDS && idgrid;
+    auto id_of = [&](int code) {
+        return idgrid[((int64_t)G4D_DEC_Z(code) * g + G4D_DEC_Y(code)) * g + G4D_DEC_X(code)];
+    };
+    // owner of position rb: the first entry whose successor is strictly farther there
+    int idx = 0;
+    if (active && m > 1) {
+        int lo = 0, hi = m - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            int qa, da, qb, db;
+            site_pos_off<AXIS, IDS>(ev[(int64_t)mid * PBA_COLS], vox, ca, cb, &qa, &da);
+            site_pos_off<AXIS, IDS>(ev[(int64_t)(mid + 1) * PBA_COLS], vox, ca, cb, &qb, &db);
+            const int fa = (rb - qa) * (rb - qa) + da, fb = (rb - qb) * (rb - qb) + db;
+            if (fb > fa) hi = mid;
+            else lo = mid + 1;
+        }
+        idx = lo;
+    }
+    int qa = 0, qb = 0, sa = G4D_MARKER, sb = G4D_MARKER, da = 0, db = 0;
+    if (active && m > 0) {
+        sa = ev[(int64_t)idx * PBA_COLS];
+        site_pos_off<AXIS, IDS>(sa, vox, ca, cb, &qa, &da);
+        if (ids_out) sa = id_of(sa);
+    }
+    if (active && idx + 1 < m) {
+        sb = ev[(int64_t)(idx + 1) * PBA_COLS];
+        site_pos_off<AXIS, IDS>(sb, vox, ca, cb, &qb, &db);
+        if (ids_out) sb = id_of(sb);
+    }
+    int pre[PBA_PER_LANE];
+    for (int t0 = rb; t0 < re; t0 += PBA_CHUNK) {
+        const int nt = min(PBA_CHUNK, re - t0);
+        if (active) {
+            for (int tt = 0; tt < nt; tt++) {
+                const int t = t0 + tt;
+                if (m == 0) {
+                    tile[tt][lane] = ids_out ? -1 : G4D_MARKER;
+                    continue;
+                }
+                while (idx + 1 < m) {
+                    const int fa = (t - qa) * (t - qa) + da;
+                    const int fb = (t - qb) * (t - qb) + db;
+                    if (fb <= fa) {
+                        idx++;
+                        qa = qb;
+                        da = db;
+                        sa = sb;
+                        if (idx + 1 < m) {
+                            sb = ev[(int64_t)(idx + 1) * PBA_COLS];
+                            site_pos_off<AXIS, IDS>(sb, vox, ca, cb, &qb, &db);
+                            if (ids_out) sb = id_of(sb);
+                        }
+                    } else {
+                        break;
+                    }
+                }
+                tile[tt][lane] = sa;
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < PBA_PER_LANE; k++) {
+            const int cc = pba_tile_c<AXIS>(k), qq = pba_tile_q<AXIS>(k);
+            pre[k] = tile[qq][cc];
+        }
+#pragma unroll
+        for (int k = 0; k < PBA_PER_LANE; k++) {
+            const int cc = pba_tile_c<AXIS>(k), qq = pba_tile_q<AXIS>(k);
+            if (qq < nt && cc < ncols) out[pba_gidx64<AXIS>(g, fixed, c0 + cc, t0 + qq)] = pre[k];
+        }
+        __syncthreads();
+    }
+}
+
+template <int AXIS, bool IDS>
+static void launch_band_pass(const GridSets &S, int64_t nblocks, int bands, const int *in, int *out,
+                             const int *idgrid, int *spill, int *benv, int *bcnt, int *fcnt, const uint64_t *vox,
+                             hipStream_t s)
+{
+    const dim3 bl(64);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pba_band_env<AXIS, IDS>), dim3((unsigned)(nblocks * bands)), bl, 0, s, in, S,
+                       spill, benv, bcnt, vox, bands);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pba_band_merge<AXIS, IDS>), dim3((unsigned)nblocks), bl, 0, s, S, spill, benv,
+                       bcnt, fcnt, vox, bands);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_pba_band_color<AXIS, IDS>), dim3((unsigned)(nblocks * bands)), bl, 0, s, S,
+                       benv, fcnt, out, idgrid, vox, bands);
+}
+
+hipError_t pba_pass_banded(int axis, const GridSets &S, int64_t nblocks, int bands, const int *in, int *out,
+                           const int *idgrid, int *spill, int *benv, int *bcnt, int *fcnt, const uint64_t *vox,
+                           hipStream_t s)
+{
+    if (nblocks <= 0 || bands < 2 || bands > PBA_MAX_BANDS || nblocks * bands > INT32_MAX) return hipErrorInvalidValue;
+    if (axis != 0 && axis != 1) return hipErrorInvalidValue;
+    if (vox && S.nsets != 1) return hipErrorInvalidValue;
+    if (vox) {
+        if (axis == 1) launch_band_pass<1, true>(S, nblocks, bands, in, out, idgrid, spill, benv, bcnt, fcnt, vox, s);
+        else launch_band_pass<0, true>(S, nblocks, bands, in, out, idgrid, spill, benv, bcnt, fcnt, vox, s);
+    } else {
+        if (axis == 1) launch_band_pass<1, false>(S, nblocks, bands, in, out, idgrid, spill, benv, bcnt, fcnt, vox, s);
+        else launch_band_pass<0, false>(S, nblocks, bands, in, out, idgrid, spill, benv, bcnt, fcnt, vox, s);
+    }
+    return hipGetLastError();
 }
 
 int64_t pba_blocks(int g) { return (int64_t)g * ((g + PBA_COLS - 1) / PBA_COLS); }

@@ -741,25 +741,26 @@ __device__ __forceinline__ int orient3d_full(const double *const *p, unsigned lo
 __device__ __forceinline__ void tri_cofactors(const double *r0, const double *r1, const double *r2, double *C,
                                               double *P)
 {
-    // 2x2 minors of rows (r1, r2) over column pairs (p, q): r1p r2q - r1q r2p
+    // 2x2 minors of rows (r1, r2) over column pairs (p, q): r1p r2q - r1q r2p, and their
+    // permanents |r1p r2q| + |r1q r2p|, each with one rounded product and one fused step
     double m[4][4], pm[4][4];
 #pragma unroll
     for (int p = 0; p < 4; p++)
 #pragma unroll
         for (int q = p + 1; q < 4; q++) {
-            const double u = r1[p] * r2[q], w = r1[q] * r2[p];
-            m[p][q] = u - w;
-            pm[p][q] = fabs(u) + fabs(w);
+            const double w = r1[q] * r2[p];
+            m[p][q] = fma(r1[p], r2[q], -w);
+            pm[p][q] = fma(fabs(r1[p]), fabs(r2[q]), fabs(w));
         }
     // M_x: cols (y,z,l); M_y: (x,z,l); M_z: (x,y,l); M_l: (x,y,z)
-    const double Mx = r0[1] * m[2][3] - r0[2] * m[1][3] + r0[3] * m[1][2];
-    const double My = r0[0] * m[2][3] - r0[2] * m[0][3] + r0[3] * m[0][2];
-    const double Mz = r0[0] * m[1][3] - r0[1] * m[0][3] + r0[3] * m[0][1];
-    const double Ml = r0[0] * m[1][2] - r0[1] * m[0][2] + r0[2] * m[0][1];
-    P[0] = fabs(r0[1]) * pm[2][3] + fabs(r0[2]) * pm[1][3] + fabs(r0[3]) * pm[1][2];
-    P[1] = fabs(r0[0]) * pm[2][3] + fabs(r0[2]) * pm[0][3] + fabs(r0[3]) * pm[0][2];
-    P[2] = fabs(r0[0]) * pm[1][3] + fabs(r0[1]) * pm[0][3] + fabs(r0[3]) * pm[0][1];
-    P[3] = fabs(r0[0]) * pm[1][2] + fabs(r0[1]) * pm[0][2] + fabs(r0[2]) * pm[0][1];
+    const double Mx = fma(r0[3], m[1][2], fma(-r0[2], m[1][3], r0[1] * m[2][3]));
+    const double My = fma(r0[3], m[0][2], fma(-r0[2], m[0][3], r0[0] * m[2][3]));
+    const double Mz = fma(r0[3], m[0][1], fma(-r0[1], m[0][3], r0[0] * m[1][3]));
+    const double Ml = fma(r0[2], m[0][1], fma(-r0[1], m[0][2], r0[0] * m[1][2]));
+    P[0] = fma(fabs(r0[3]), pm[1][2], fma(fabs(r0[2]), pm[1][3], fabs(r0[1]) * pm[2][3]));
+    P[1] = fma(fabs(r0[3]), pm[0][2], fma(fabs(r0[2]), pm[0][3], fabs(r0[0]) * pm[2][3]));
+    P[2] = fma(fabs(r0[3]), pm[0][1], fma(fabs(r0[1]), pm[0][3], fabs(r0[0]) * pm[1][3]));
+    P[3] = fma(fabs(r0[2]), pm[0][1], fma(fabs(r0[1]), pm[0][2], fabs(r0[0]) * pm[1][2]));
     C[0] = Mx;
     C[1] = -My;
     C[2] = Mz;
@@ -782,8 +783,8 @@ __device__ __forceinline__ void rel_coords(const double *pts, int v, double sx, 
 // permanents; 32 u is used.  Fused multiply-adds only remove roundings from this count.
 __device__ __forceinline__ int cofactor_filter_sign(const double *r, const double *C, const double *P)
 {
-    const double val = r[0] * C[0] + r[1] * C[1] + r[2] * C[2] + r[3] * C[3];
-    const double per = fabs(r[0]) * P[0] + fabs(r[1]) * P[1] + fabs(r[2]) * P[2] + r[3] * P[3];
+    const double val = fma(r[3], C[3], fma(r[2], C[2], fma(r[1], C[1], r[0] * C[0])));
+    const double per = fma(r[3], P[3], fma(fabs(r[2]), P[2], fma(fabs(r[1]), P[1], fabs(r[0]) * P[0])));
     const double bnd = 32.0 * 1.1102230246251565e-16 * per;
     if (!(per > 1e-250)) return 0;
     return val > bnd ? 1 : (-val > bnd ? -1 : 0);

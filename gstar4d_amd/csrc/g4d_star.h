@@ -74,6 +74,18 @@ struct StarParams {
     int nlist;
     const unsigned long long *nlist_dev;
     unsigned long long *cnt;
+    int t0_variant; // 32-vertex filter-only kernel: STAR_T0_* (STAR_T0_DEFAULT: the tuned default)
+};
+
+// Variants of the 32-vertex filter-only star kernel.
+enum {
+    STAR_T0_DEFAULT = -1,
+    STAR_T0_WAVE = 0,     // one star per wave (k_star32)
+    STAR_T0_G16 = 1,      // four stars per wave, FP64 permanents, 3 waves per SIMD (k_star32g16)
+    STAR_T0_G16F = 2,     // four stars per wave, FP32 largest permanent, 3 waves per SIMD (k_star32g16f)
+    STAR_T0_G16F4 = 3,    // the same at 4 waves per SIMD (k_star32g16f4)
+    STAR_T0_G32F = 4,     // two stars per wave, FP32 largest permanent, 4 waves per SIMD (k_star32g32f)
+    STAR_T0_NUM = 5
 };
 
 // LDS star kernel for stars of up to vmax (STAR_VMAX_T0, T1 or T2) link vertices plus

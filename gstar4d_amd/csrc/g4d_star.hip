@@ -678,7 +678,7 @@ __device__ __forceinline__ int grp_filter_sign(const double *r, double, const do
 }
 __device__ __forceinline__ int grp_filter_sign(const double *r, double rs, const double *C, const float &P)
 {
-    const double val = r[0] * C[0] + r[1] * C[1] + r[2] * C[2] + r[3] * C[3];
+    const double val = fma(r[3], C[3], fma(r[2], C[2], fma(r[1], C[1], r[0] * C[0])));
     const double per = rs * (double)P;
     const double bnd = 32.0 * 1.1102230246251565e-16 * per;
     if (!(per > 1e-250)) return 0;
@@ -1084,18 +1084,20 @@ __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
     star_body_grp<32, true>(p, nitems);
 }
 
-// The 32-vertex filter-only kernel: 0 one star per wave (k_star32), 1 k_star32g16 (3 waves per
-// SIMD), 2 k_star32g16f (FP32 permanents, 3 waves), 3 k_star32g16f4 (4 waves), 4 k_star32g32f
-// (two stars per wave, 4 waves).  G4D_STAR_T0 selects one (measurement).
-static int star_t0_variant()
+// The 32-vertex filter-only kernel variant of a launch: the caller's choice, else the
+// G4D_STAR_T0 environment variable (measurement), else the measured best (four stars per wave
+// with the FP32 largest permanent: 73.4 ms per triangulation at 4,194,304 uniform points, G =
+// 512, against 89.1 for one star per wave).
+static int star_t0_variant(int requested)
 {
-    static int v = -1;
-    if (v < 0) {
+    if (requested >= 0 && requested < STAR_T0_NUM) return requested;
+    static int env = -2;
+    if (env == -2) {
         const char *e = getenv("G4D_STAR_T0");
-        v = e ? atoi(e) : 2;
-        if (v < 0 || v > 4) v = 2;
+        env = e ? atoi(e) : -1;
+        if (env < 0 || env >= STAR_T0_NUM) env = -1;
     }
-    return v;
+    return env >= 0 ? env : STAR_T0_G16F;
 }
 
 static_assert(G4D_CNT_QUEUE0 + 6 <= G4D_CNT_NUM, "six queue heads must fit the counter block");
@@ -1131,16 +1133,16 @@ hipError_t launch_star_kernel(const StarParams &p, int vmax, bool exact, int nbl
         else
             return hipErrorInvalidValue;
     } else if (vmax == STAR_VMAX_T0) {
-        const int v = star_t0_variant();
-        if (v == 0) {
+        const int v = star_t0_variant(p.t0_variant);
+        if (v == STAR_T0_WAVE) {
             hipLaunchKernelGGL(k_star32, dim3(nblocks), dim3(64), 0, s, p, nitems);
         } else {
             // one work-group per resident wave: 256 CUs x 4 SIMDs x the variant's occupancy
-            const int per_simd = v <= 2 ? 3 : 4;
+            const int per_simd = v <= STAR_T0_G16F ? 3 : 4;
             const int nb = 1024 * per_simd;
-            if (v == 1) hipLaunchKernelGGL(k_star32g16, dim3(nb), dim3(64), 0, s, p, nitems);
-            else if (v == 2) hipLaunchKernelGGL(k_star32g16f, dim3(nb), dim3(64), 0, s, p, nitems);
-            else if (v == 3) hipLaunchKernelGGL(k_star32g16f4, dim3(nb), dim3(64), 0, s, p, nitems);
+            if (v == STAR_T0_G16) hipLaunchKernelGGL(k_star32g16, dim3(nb), dim3(64), 0, s, p, nitems);
+            else if (v == STAR_T0_G16F) hipLaunchKernelGGL(k_star32g16f, dim3(nb), dim3(64), 0, s, p, nitems);
+            else if (v == STAR_T0_G16F4) hipLaunchKernelGGL(k_star32g16f4, dim3(nb), dim3(64), 0, s, p, nitems);
             else hipLaunchKernelGGL(k_star32g32f, dim3(nb), dim3(64), 0, s, p, nitems);
         }
     } else if (vmax == STAR_VMAX_T1) {

@@ -102,7 +102,7 @@ typedef struct {
        removing those already in a workset), time of the proposal stage (candidates, features,
        network, merge). */
     int32_t prop_enabled;
-    int32_t pad2;
+    int32_t pba_bands;         /* bands of the y and x PBA passes (1: whole columns) */
     int64_t prop_pairs;
     double t_prop_ms;
 } g4d_stats;
@@ -166,6 +166,21 @@ int g4d_copy_result(g4d_ctx *ctx, int32_t *tets, int32_t *opp, int32_t *perm);
 #define G4D_NOALLOC_MAX_GRID 128
 int g4d_triangulate_device_noalloc(g4d_ctx *ctx, const double *d_xyz, int32_t n, int32_t *d_tets, int32_t *d_opp,
                                    int64_t max_tets, int32_t *d_perm, int64_t *num_tets, int64_t *scratch_used);
+/* Tuning (no reference counterpart; gstar4d_amd/autotune.py measures the choices).
+   g4d_set_star_kernel: variant of the 32-vertex star kernel, -1 for the built-in default,
+   0 one star per wave, 1 four stars per wave (FP64 permanents), 2 four stars per wave (FP32
+   largest permanent), 3 the same at four waves per SIMD, 4 two stars per wave; the output never
+   depends on it.  g4d_set_grid_table: PBA grid of a batched set without an explicit grid =
+   the smallest multiple of 8 in [32, 256] with at least voxels_per_point[k] voxels per point,
+   k the first bucket with n <= max_points[k] (ascending; the last bucket takes every larger
+   set); nbuckets = 0 restores g4d_batch_grid_size.  g4d_grid_for_set gives the grid a set of n
+   points gets under the context's table. */
+int g4d_set_star_kernel(g4d_ctx *ctx, int32_t variant);
+int g4d_set_grid_table(g4d_ctx *ctx, int32_t nbuckets, const int64_t *max_points, const float *voxels_per_point);
+int g4d_grid_for_set(const g4d_ctx *ctx, int64_t n);
+/* Bands of the y and x PBA passes (the reference's m2/m3 bands): 0 picks them by grid width,
+   1 walks whole columns, 2..16 split each column (same diagram). */
+int g4d_set_pba_bands(g4d_ctx *ctx, int32_t bands);
 /* Upload host points into a device buffer owned by ctx and return its device address. */
 int g4d_upload_points(g4d_ctx *ctx, const double *xyz, int32_t n, const double **d_xyz);
 int g4d_get_stats(const g4d_ctx *ctx, g4d_stats *st);

@@ -4,6 +4,7 @@ import numpy as np
 import pytest
 
 import gstar4d_amd as g
+from gstar4d_amd import datasets
 
 pytestmark = pytest.mark.gpu
 
@@ -131,3 +132,39 @@ def test_c5_batch_4096_sets(orc):
             print("c5: sets %d..%d (%d points, grids %d..%d) equal to single-set calls, validated" %
                   (a, b - 1, int(off[b] - off[a]), int(gs.min()), int(gs.max())), flush=True)
     assert checked == len(sets) and hashed == len(want) == 64
+
+
+def test_grid_table_and_star_kernel_variants(orc):
+    """The autotuner's hooks: a per-size grid table decides the grids of sets passed without
+    explicit grids (the same output as passing those grids), and every variant of the 32-vertex
+    star kernel gives the oracle's tetrahedra with consistent adjacency."""
+    from gstar4d_amd import autotune
+    sets = datasets.batch_sets(24, seed=5, lo=1000, hi=30000)
+    sizes = np.array([s.shape[0] for s in sets], np.int64)
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    xyz = np.concatenate(sets)
+    table = {"buckets": [{"max_points": 5000, "voxels_per_point": 8.0}, {"max_points": 200000, "voxels_per_point": 20.0}],
+             "star_kernel": 1}
+    with g.Triangulator(grid_size=64) as t:
+        autotune.apply(t, table)
+        gr = autotune.grids(t, sizes)
+        for n, gg in zip(sizes, gr):
+            vpp = 8.0 if n <= 5000 else 20.0
+            assert gg % 8 == 0 and 32 <= gg <= 256
+            assert gg == 256 or gg ** 3 >= vpp * n
+            assert gg == 32 or (gg - 8) ** 3 < vpp * n
+        a, ao, aoff, ags = t.triangulate_batch(xyz, off)
+        assert np.array_equal(ags, gr)
+        b, bo, boff, _ = t.triangulate_batch(xyz, off, gr)
+        assert np.array_equal(a, b) and np.array_equal(ao, bo) and np.array_equal(aoff, boff)
+        # the variants may list a star's tetrahedra in another order (their link triangles
+        # take other slots); the canonical set is the contract
+        x = datasets.uniform_cube(30000, seed=8)
+        ref, _, _ = orc.triangulate(x, 64)
+        for v in range(5):
+            t.set_star_kernel(v)
+            tets, _, _ = t.triangulate(x)
+            assert (g.canonical(tets) == ref).all(), v
+            rep = t.check()
+            assert rep["adjacency_failures"] == 0 and rep["euler"] == 1, v
+        t.set_star_kernel(-1)

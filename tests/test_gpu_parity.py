@@ -129,21 +129,39 @@ def test_morton_matches_oracle(tri, orc, g_, n):
 
 
 @pytest.mark.parametrize("g_,n", [(16, 40), (64, 3000), (96, 20000), (128, 8000)])
-def test_pba_matches_oracle(tri, orc, g_, n):
+@pytest.mark.parametrize("bands", [1, 2, 3, 4])
+def test_pba_matches_oracle(tri, orc, g_, n, bands):
+    """Both grid labellings, whole columns and banded y and x passes (band envelopes, their
+    merge, colouring per band; with 3 bands the last band of a 16- or 64-wide grid is empty)."""
     x = datasets.hard_mix(n, seed=3)
     perm, vox = orc.morton_order(x, g_)
     vs = vox[perm]
-    assert (tri.diag_pba(vs, g_, codes=True) == orc.pba(vs, g_, codes=True)).all()
-    ids = tri.diag_pba(vs, g_)
-    assert (ids == orc.pba(vs, g_)).all()
-    assert (tri.diag_pba(vs, g_, id_grids=True) == ids).all()  # the kernels grids above 1024 use
-    pairs = tri.diag_grid_pairs(ids)
-    assert (pairs == orc.grid_pairs(ids)).all()
+    tri.set_pba_bands(bands)
+    try:
+        assert (tri.diag_pba(vs, g_, codes=True) == orc.pba(vs, g_, codes=True)).all()
+        ids = tri.diag_pba(vs, g_)
+        assert (ids == orc.pba(vs, g_)).all()
+        assert (tri.diag_pba(vs, g_, id_grids=True) == ids).all()  # the kernels grids above 1024 use
+    finally:
+        tri.set_pba_bands(0)
+    if bands == 1:
+        pairs = tri.diag_grid_pairs(ids)
+        assert (pairs == orc.grid_pairs(ids)).all()
 
 
-def test_pba_deep_envelope_matches_oracle(tri, orc):
+@pytest.mark.parametrize("bands", [1, 3, 4])
+def test_pba_deep_envelope_matches_oracle(tri, orc, bands):
     """Sites filling whole grid lines along each axis: envelopes as deep as the grid, so the
-    per-column stack overflows LDS into the spill workspace in every pass."""
+    per-column stack overflows LDS into the spill workspace in every pass (in the banded passes,
+    the band stacks and the merged stack)."""
+    tri.set_pba_bands(bands)
+    try:
+        _deep_envelope(tri, orc)
+    finally:
+        tri.set_pba_bands(0)
+
+
+def _deep_envelope(tri, orc):
     g_ = 160
     ar = np.arange(g_)
     lines = [np.stack([np.full(g_, 5), ar, np.full(g_, 7)], 1),
@@ -189,6 +207,17 @@ def test_uniform_small(orc, n, grid):
     x = datasets.uniform_cube(n, seed=n)
     tets, opp, perm, st, rst = _parity(orc, x, grid)
     assert st["splay_rounds"] == rst["splay_rounds"]
+
+
+@pytest.mark.parametrize("bands", [1, 3, 5])
+def test_pba_bands_same_triangulation(orc, bands):
+    x = datasets.uniform_cube(20000, seed=31)
+    with g.Triangulator(grid_size=96) as t:
+        t.set_pba_bands(bands)
+        tets, _, _ = t.triangulate(x)
+        assert t.stats()["pba_bands"] == bands
+    ref, _, _ = orc.triangulate(x, 96)
+    assert (g.canonical(tets) == ref).all()
 
 
 @pytest.mark.parametrize("n,grid", [(30000, 1100), (60000, 2048)])
