@@ -24,6 +24,7 @@
 // two per-vertex maps (boundary edge starting / ending at each vertex).  The whole insertion
 // list of a star runs inside one kernel, so there is no host loop per insertion index.
 #include "g4d_predicates.h"
+#include <climits>
 #include <cstdlib>
 #include <type_traits>
 
@@ -702,6 +703,7 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
     constexpr int VMAX = StarGrpLds::VMAX, TMAX = StarGrpLds::TMAX;
     constexpr int KPL = TMAX / GL;
     __shared__ StarGrpLds Ls[NG];
+    __shared__ int order[NG * STAR_CHUNK]; // the chunk's items by insertion count
     const int lane = threadIdx.x, g = lane / GL, gl = lane % GL;
     StarGrpLds &L = Ls[g];
     unsigned long long *cnt = p.cnt;
@@ -730,9 +732,29 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
         }
         if (base >= nbatch) break;
         const int bend = one_each ? base + 1 : (base + STAR_CHUNK < nbatch ? base + STAR_CHUNK : nbatch);
+        // The chunk's stars are dealt to batches in order of their insertion counts, so that the
+        // four stars of a batch (which step through their insertions in lock-step) need about as
+        // many steps each; within a chunk of 32 consecutive stars the order keeps their locality.
+        {
+            const int nit = (bend - base) * NG;
+            const int my = base * NG + lane;
+            int key = INT_MAX;
+            if (lane < nit && my < nitems) {
+                const int s = p.star_list ? p.star_list[my] : my;
+                key = (int)(p.ins_off[s + 1] - p.ins_off[s]);
+            }
+            int rank = 0;
+#pragma unroll
+            for (int j = 0; j < NG * STAR_CHUNK; j++) {
+                const int kj = __shfl(key, j, 64);
+                rank += (kj < key || (kj == key && j < lane)) ? 1 : 0;
+            }
+            if (lane < nit) order[rank] = my;
+            wave_sync();
+        }
 #pragma unroll 1
         for (int batch = base; batch < bend; batch++) {
-            const int item = batch * NG + g;
+            const int item = order[(batch - base) * NG + g];
             // ---- this group's star: checks, then its link and insertions into LDS ----
             bool act = item < nitems;
             const int s = act ? (p.star_list ? p.star_list[item] : item) : 0;
