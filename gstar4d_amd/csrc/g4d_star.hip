@@ -36,6 +36,7 @@ static constexpr uint32_t TF_ALIVE = 1u << 27;
 static constexpr uint32_t TF_CHECKED = 1u << 28;
 static constexpr uint32_t TF_VIS = 1u << 29;
 static constexpr uint32_t TF_FRESH = 1u << 30;
+static constexpr uint32_t TF_UNSURE = 1u << 26; // global-memory kernel only (its flags word holds no vertices)
 
 __device__ __forceinline__ int tvert(uint32_t t, int i) { return (int)((t >> (9 * i)) & 511u); }
 __device__ __forceinline__ uint32_t tpack(int a, int b, int c, uint32_t fl)
@@ -1358,23 +1359,45 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
             const int gv = L.ins[q];
             const int Vn = vn + q;
             const double r[4] = {L.vc[4 * Vn], L.vc[4 * Vn + 1], L.vc[4 * Vn + 2], L.vc[4 * Vn + 3]};
-            int nv = 0;
-            bool unc = false;
+            int nv = 0, nunc = 0;
             for (int slot = tid; slot < T; slot += GSTAR_NT) {
                 const uint32_t f = L.tfl[slot];
                 if (!(f & TF_ALIVE)) continue;
-                int sg = cofactor_filter_sign(r, L.cof + 8 * slot, L.cof + 8 * slot + 4);
+                const int sg = cofactor_filter_sign(r, L.cof + 8 * slot, L.cof + 8 * slot + 4);
                 if (sg == 0) {
+                    L.tfl[slot] = f | TF_UNSURE;
+                    nunc++;
+                } else if (sg < 0) {
+                    L.tfl[slot] = f | TF_VIS;
+                    nv++;
+                }
+            }
+            // Triangles the filter cannot decide are gathered into one list (be_slot is free until
+            // the hole boundary) and decided exactly one per thread, so that the block's threads
+            // share the exact work instead of each wave stalling on its own few slots.
+            int nunc_tot;
+            int ub = block_excl(nunc, &nunc_tot, red);
+            if (nunc_tot > 0) {
+                for (int slot = tid; slot < T && nunc > 0; slot += GSTAR_NT)
+                    if (L.tfl[slot] & TF_UNSURE) {
+                        L.be_slot[ub++] = slot;
+                        nunc--;
+                    }
+                __syncthreads();
+                for (int k = tid; k < nunc_tot; k += GSTAR_NT) {
+                    const int slot = L.be_slot[k];
                     const int a = L.vid[L.tva[3 * slot]], b = L.vid[L.tva[3 * slot + 1]], c = L.vid[L.tva[3 * slot + 2]];
                     const double *pp[5] = {p.pts + 3 * (int64_t)a, p.pts + 3 * (int64_t)b, p.pts + 3 * (int64_t)c,
                                            p.pts + 3 * (int64_t)s, p.pts + 3 * (int64_t)gv};
                     const int idx[5] = {a, b, c, s, gv};
-                    sg = orient4_full(pp, idx, cnt);
+                    const int sg = orient4_full(pp, idx, cnt);
+                    const uint32_t f = L.tfl[slot] & ~TF_UNSURE;
+                    L.tfl[slot] = sg < 0 ? (f | TF_VIS) : f;
                 }
-                if (sg < 0) {
-                    L.tfl[slot] = f | TF_VIS;
-                    nv++;
-                }
+                __syncthreads();
+                // each thread counts the beneath triangles among its own slots again
+                nv = 0;
+                for (int slot = tid; slot < T; slot += GSTAR_NT) nv += (L.tfl[slot] & TF_VIS) ? 1 : 0;
             }
             int nvis;
             const int vbase = block_excl(nv, &nvis, red);
