@@ -64,7 +64,8 @@ class Stats(ctypes.Structure):
                 ("star_passes", ctypes.c_int32), ("star_t0_launches", ctypes.c_int32), ("pad1", ctypes.c_int32),
                 ("t_star_t0_ms", ctypes.c_double), ("star_t0_filter_evals", ctypes.c_int64),
                 ("star_t0_cofactors", ctypes.c_int64), ("prop_enabled", ctypes.c_int32), ("pba_bands", ctypes.c_int32),
-                ("prop_pairs", ctypes.c_int64), ("t_prop_ms", ctypes.c_double)]
+                ("prop_pairs", ctypes.c_int64), ("t_prop_ms", ctypes.c_double),
+                ("graph_launches", ctypes.c_int32), ("graph_captures", ctypes.c_int32)]
 
     def as_dict(self):
         d = {}
@@ -116,7 +117,7 @@ EXPORTS = ["g4d_default_options", "g4d_create", "g4d_destroy", "g4d_last_error",
            "g4d_struct_sizes", "g4d_check_sizes", "g4d_set_proposal", "g4d_set_sample_export", "g4d_get_samples",
            "g4d_mlp_forward", "g4d_mlp_grad", "g4d_mlp_train", "g4d_check_stars", "g4d_diag_set_link_vertex",
            "g4d_triangulate_device_noalloc", "g4d_set_star_kernel", "g4d_set_grid_table", "g4d_grid_for_set",
-           "g4d_set_pba_bands"]
+           "g4d_set_pba_bands", "g4d_set_stage_graphs"]
 
 MLP_PARAMS = 7312
 PROP_FEATURES = 12
@@ -164,6 +165,7 @@ def load_library(path=None):
     L.g4d_set_grid_table.argtypes = [_vp, ctypes.c_int32, _lp, ctypes.POINTER(ctypes.c_float)]
     L.g4d_grid_for_set.argtypes = [_vp, ctypes.c_int64]
     L.g4d_set_pba_bands.argtypes = [_vp, ctypes.c_int32]
+    L.g4d_set_stage_graphs.argtypes = [_vp, ctypes.c_int32]
     L.g4d_struct_sizes.argtypes = [_lp, ctypes.c_int32]
     L.g4d_check_sizes.argtypes = [ctypes.c_int64, ctypes.c_int64]
     _fp = ctypes.POINTER(ctypes.c_float)
@@ -365,6 +367,10 @@ class Triangulator:
     def set_pba_bands(self, bands=0):
         """Bands of the y and x PBA passes (0: by grid width; 1: whole columns)."""
         self._check(self._L.g4d_set_pba_bands(self._ctx, int(bands)))
+
+    def set_stage_graphs(self, mode=-1):
+        """Grid stage as a replayed hipGraph: 1 always, 0 never, -1 batched calls only."""
+        self._check(self._L.g4d_set_stage_graphs(self._ctx, int(mode)))
 
     def grid_for_set(self, n):
         return int(self._L.g4d_grid_for_set(self._ctx, int(n)))

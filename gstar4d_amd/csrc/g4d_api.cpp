@@ -96,6 +96,16 @@ struct g4d_ctx {
     // tuning (g4d_set_star_kernel, g4d_set_grid_table)
     int star_t0_variant = -1;
     int pba_bands = 0; // bands of the y and x PBA passes (0: by grid size)
+    int stage_graphs = -1; // grid-stage chain as a hipGraph: 1 always, 0 never, -1 batched calls only
+    // captured grid-stage chains (site scatter and the PBA passes), keyed by everything the
+    // captured launches depend on: sizes, launch shapes and buffer addresses
+    struct StageGraph {
+        std::vector<int64_t> key;
+        hipGraphExec_t exec = nullptr;
+        uint64_t last_use = 0;
+    };
+    std::vector<StageGraph> graphs;
+    uint64_t graph_clock = 0;
     std::vector<int64_t> grid_tab_n;   // ascending upper bounds of the set-size buckets
     std::vector<float> grid_tab_vpp;   // voxels per point of each bucket
 };
@@ -422,36 +432,120 @@ static int pba_bands_for(const g4d_ctx *ctx, int gmax)
     return std::max(1, std::min(b, g4d::PBA_MAX_BANDS));
 }
 
+// Device buffers of the PBA passes: the spill area and, for banded passes, the band
+// envelopes and counts (allocated before any capture: a graph cannot allocate).
+struct PbaBufs {
+    int bands = 1;
+    int *spill = nullptr, *benv = nullptr, *bcnt = nullptr, *fcnt = nullptr;
+};
+
+static int pba_buffers(g4d_ctx *ctx, const SetTotals &tot, int gmax, PbaBufs *pb)
+{
+    HCK(ctx->pbaspill.ensure(sizeof(int) * (size_t)tot.spill));
+    pb->spill = buf<int>(ctx->pbaspill);
+    pb->bands = pba_bands_for(ctx, gmax);
+    if (pb->bands > 1) {
+        HCK(ctx->pbaenv.ensure(sizeof(int) * (size_t)tot.spill));
+        HCK(ctx->pbacnt.ensure(sizeof(int) * 64 * (size_t)tot.pba_blocks * (size_t)(pb->bands + 1)));
+        pb->benv = buf<int>(ctx->pbaenv);
+        pb->bcnt = buf<int>(ctx->pbacnt);
+        pb->fcnt = pb->bcnt + 64 * tot.pba_blocks * pb->bands;
+    }
+    ctx->st.pba_bands = pb->bands;
+    return G4D_OK;
+}
+
 // The three PBA axis passes: codes in grid -> (z) gtmp -> (y) grid -> (x) gtmp, the last with
 // point ids (idgrid, or the ids the grids hold when vox is given).  The y and x passes run in
-// bands when pba_bands_for says so.
+// bands when pba_bands_for says so.  Kernel launches only (it may run under stream capture).
+static int pba_launch(g4d_ctx *ctx, const g4d::GridSets &GS, const SetTotals &tot, const PbaBufs &pb, int *grid,
+                      int *gtmp, const int *idgrid_last, const uint64_t *pvox, bool events)
+{
+    hipStream_t s = ctx->stream;
+    if (events) ev_record(ctx, EV_PBA0);
+    HCK(g4d::pba_pass(2, GS, tot.pba_blocks, grid, gtmp, nullptr, pb.spill, pvox, s));
+    if (events) ev_record(ctx, EV_PBA1);
+    if (pb.bands > 1) HCK(g4d::pba_pass_banded(1, GS, tot.pba_blocks, pb.bands, gtmp, grid, nullptr, pb.spill, pb.benv,
+                                                pb.bcnt, pb.fcnt, pvox, s));
+    else HCK(g4d::pba_pass(1, GS, tot.pba_blocks, gtmp, grid, nullptr, pb.spill, pvox, s));
+    if (events) ev_record(ctx, EV_PBA2);
+    if (pb.bands > 1) HCK(g4d::pba_pass_banded(0, GS, tot.pba_blocks, pb.bands, grid, gtmp, idgrid_last, pb.spill,
+                                                pb.benv, pb.bcnt, pb.fcnt, pvox, s));
+    else HCK(g4d::pba_pass(0, GS, tot.pba_blocks, grid, gtmp, idgrid_last, pb.spill, pvox, s));
+    if (events) ev_record(ctx, EV_PBA3);
+    return G4D_OK;
+}
+
 static int run_pba(g4d_ctx *ctx, const g4d::GridSets &GS, const SetTotals &tot, int gmax, int *grid, int *gtmp,
                    const int *idgrid_last, const uint64_t *pvox, bool events)
 {
+    PbaBufs pb;
+    CK(pba_buffers(ctx, tot, gmax, &pb));
+    return pba_launch(ctx, GS, tot, pb, grid, gtmp, idgrid_last, pvox, events);
+}
+
+static void destroy_graphs(g4d_ctx *ctx)
+{
+    for (auto &g : ctx->graphs)
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+    ctx->graphs.clear();
+}
+
+// Grid stage as one hipGraph launch: the grid fill, the site scatter and the PBA passes
+// (9-13 kernel launches) are captured once per key and replayed.  Every launch of the chain
+// is a kernel (the fill included, no memset node), and every size and address the captured
+// launches use is part of the key, so a replay runs exactly what an eager call would.
+static int grid_stage_graph(g4d_ctx *ctx, const g4d::GridSets &GS, const SetTotals &tot, int gmax, int n,
+                            const uint64_t *vox, int *grid, int *gtmp, int *idgrid, const uint64_t *pvox)
+{
     hipStream_t s = ctx->stream;
-    HCK(ctx->pbaspill.ensure(sizeof(int) * (size_t)tot.spill));
-    int *spill = buf<int>(ctx->pbaspill);
-    const int bands = pba_bands_for(ctx, gmax);
-    int *benv = nullptr, *bcnt = nullptr, *fcnt = nullptr;
-    if (bands > 1) {
-        HCK(ctx->pbaenv.ensure(sizeof(int) * (size_t)tot.spill));
-        HCK(ctx->pbacnt.ensure(sizeof(int) * 64 * (size_t)tot.pba_blocks * (size_t)(bands + 1)));
-        benv = buf<int>(ctx->pbaenv);
-        bcnt = buf<int>(ctx->pbacnt);
-        fcnt = bcnt + 64 * tot.pba_blocks * bands;
+    PbaBufs pb;
+    CK(pba_buffers(ctx, tot, gmax, &pb));
+    const std::vector<int64_t> key = {(int64_t)n, (int64_t)GS.nsets, tot.voxels, tot.pba_blocks, tot.spill,
+                                      (int64_t)pb.bands, (int64_t)(intptr_t)vox, (int64_t)(intptr_t)grid,
+                                      (int64_t)(intptr_t)gtmp, (int64_t)(intptr_t)idgrid, (int64_t)(intptr_t)pvox,
+                                      (int64_t)(intptr_t)pb.spill, (int64_t)(intptr_t)pb.benv,
+                                      (int64_t)(intptr_t)pb.bcnt, (int64_t)(intptr_t)GS.pt_off,
+                                      (int64_t)(intptr_t)GS.pt_set, (int64_t)(intptr_t)GS.g};
+    g4d_ctx::StageGraph *hit = nullptr;
+    for (auto &g : ctx->graphs)
+        if (g.key == key) hit = &g;
+    if (!hit) {
+        hipGraph_t graph = nullptr;
+        HCK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipError_t e = g4d::launch_fill_i32(grid, tot.voxels, G4D_MARKER, s);
+        if (e == hipSuccess) e = g4d::launch_scatter_sites(vox, n, GS, grid, idgrid, s);
+        int r = G4D_OK;
+        if (e == hipSuccess) r = pba_launch(ctx, GS, tot, pb, grid, gtmp, idgrid, pvox, false);
+        const hipError_t ec = hipStreamEndCapture(s, &graph);
+        if (e != hipSuccess) return hip_fail(ctx, e, "grid stage capture", __LINE__);
+        if (r != G4D_OK) {
+            if (graph) (void)hipGraphDestroy(graph);
+            return r;
+        }
+        HCK(ec);
+        hipGraphExec_t exec = nullptr;
+        const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        HCK(ei);
+        if (ctx->graphs.size() >= 16) { // drop the least recently used
+            size_t lru = 0;
+            for (size_t k = 1; k < ctx->graphs.size(); k++)
+                if (ctx->graphs[k].last_use < ctx->graphs[lru].last_use) lru = k;
+            (void)hipGraphExecDestroy(ctx->graphs[lru].exec);
+            ctx->graphs.erase(ctx->graphs.begin() + (ptrdiff_t)lru);
+        }
+        ctx->graphs.push_back(g4d_ctx::StageGraph{key, exec, 0});
+        hit = &ctx->graphs.back();
+        ctx->st.graph_captures++;
     }
-    if (events) ev_record(ctx, EV_PBA0);
-    HCK(g4d::pba_pass(2, GS, tot.pba_blocks, grid, gtmp, nullptr, spill, pvox, s));
-    if (events) ev_record(ctx, EV_PBA1);
-    if (bands > 1) HCK(g4d::pba_pass_banded(1, GS, tot.pba_blocks, bands, gtmp, grid, nullptr, spill, benv, bcnt, fcnt,
-                                             pvox, s));
-    else HCK(g4d::pba_pass(1, GS, tot.pba_blocks, gtmp, grid, nullptr, spill, pvox, s));
-    if (events) ev_record(ctx, EV_PBA2);
-    if (bands > 1) HCK(g4d::pba_pass_banded(0, GS, tot.pba_blocks, bands, grid, gtmp, idgrid_last, spill, benv, bcnt,
-                                             fcnt, pvox, s));
-    else HCK(g4d::pba_pass(0, GS, tot.pba_blocks, grid, gtmp, idgrid_last, spill, pvox, s));
-    if (events) ev_record(ctx, EV_PBA3);
-    ctx->st.pba_bands = bands;
+    hit->last_use = ++ctx->graph_clock;
+    ev_record(ctx, EV_PBA0);
+    HCK(hipGraphLaunch(hit->exec, s));
+    ev_record(ctx, EV_PBA1);
+    ev_record(ctx, EV_PBA2);
+    ev_record(ctx, EV_PBA3);
+    ctx->st.graph_launches++;
     return G4D_OK;
 }
 
@@ -536,9 +630,14 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     if (!id_mode) HCK(ctx->idgrid.ensure(sizeof(int) * (size_t)G3));
     int *grid = buf<int>(ctx->grid), *gtmp = buf<int>(ctx->gtmp), *idgrid = id_mode ? nullptr : buf<int>(ctx->idgrid);
     const uint64_t *pvox = id_mode ? vox : nullptr;
-    HCK(g4d::launch_fill_i32(grid, G3, G4D_MARKER, s));
-    HCK(g4d::launch_scatter_sites(vox, n, GS, grid, idgrid, s));
-    CK(run_pba(ctx, GS, tot, gmax, grid, gtmp, idgrid, pvox, true)); // point ids in gtmp
+    if (ctx->stage_graphs == 1 || (ctx->stage_graphs < 0 && batched)) {
+        // fill, scatter and PBA replayed from a captured graph (point ids in gtmp)
+        CK(grid_stage_graph(ctx, GS, tot, gmax, n, vox, grid, gtmp, idgrid, pvox));
+    } else {
+        HCK(g4d::launch_fill_i32(grid, G3, G4D_MARKER, s));
+        HCK(g4d::launch_scatter_sites(vox, n, GS, grid, idgrid, s));
+        CK(run_pba(ctx, GS, tot, gmax, grid, gtmp, idgrid, pvox, true)); // point ids in gtmp
+    }
     const int *idg = gtmp;
 
     // ---- D: missing points, grid pairs, worksets, pumping (GD/GDelaunay/GDelHost.cu:259-529) ----
@@ -1073,6 +1172,7 @@ void g4d_destroy(g4d_ctx *ctx)
                       &ctx->hcap, &ctx->aflags, &ctx->apos, &ctx->active, &ctx->mid, &ctx->big, &ctx->huge, &ctx->xstars, &ctx->xtasks, &ctx->xtris, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table,
                       &ctx->mlp_params, &ctx->fkeys, &ctx->ford, &ctx->frank, &ctx->pcand, &ctx->pd2, &ctx->pfeat,
                       &ctx->ppairs, &ctx->pcap, &ctx->plab, &ctx->pvcap};
+    destroy_graphs(ctx);
     for (DevBuf *b : bufs) b->release();
     ctx->store.release();
     ctx->P.release();
@@ -1159,6 +1259,18 @@ int g4d_set_grid_table(g4d_ctx *ctx, int32_t nbuckets, const int64_t *max_points
     }
     ctx->grid_tab_n.assign(max_points, max_points + nbuckets);
     ctx->grid_tab_vpp.assign(voxels_per_point, voxels_per_point + nbuckets);
+    return G4D_OK;
+}
+
+int g4d_set_stage_graphs(g4d_ctx *ctx, int32_t mode)
+{
+    if (!ctx || mode < -1 || mode > 1) return G4D_ERR_INVALID_INPUT;
+    ctx->stage_graphs = mode;
+    if (mode == 0) {
+        HCK(hipSetDevice(ctx->opt.device));
+        HCK(hipStreamSynchronize(ctx->stream));
+        destroy_graphs(ctx);
+    }
     return G4D_OK;
 }
 
@@ -1289,6 +1401,7 @@ int g4d_triangulate_device_noalloc(g4d_ctx *ctx, const double *d_xyz, int32_t n,
     tmp.stream = ctx->stream;
     tmp.star_t0_variant = ctx->star_t0_variant;
     tmp.pba_bands = ctx->pba_bands;
+    tmp.stage_graphs = 0; // instantiating a graph allocates
     tmp.ev_ok = ctx->ev_ok;
     for (int i = 0; i < EV_NUM; i++) tmp.ev[i] = ctx->ev[i];
     for (int i = 0; i < 2; i++) tmp.ev_t0[i] = ctx->ev_t0[i];

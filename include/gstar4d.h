@@ -105,6 +105,10 @@ typedef struct {
     int32_t pba_bands;         /* bands of the y and x PBA passes (1: whole columns) */
     int64_t prop_pairs;
     double t_prop_ms;
+    /* Grid stage (site scatter and PBA passes) run as a captured hipGraph: launches of a cached
+       graph in this call, and graphs captured for it. */
+    int32_t graph_launches;
+    int32_t graph_captures;
 } g4d_stats;
 
 /* Mesh validation report (GDelaunay/Common/Geometry.cu:200-393 plus adjacency and a full
@@ -181,6 +185,10 @@ int g4d_grid_for_set(const g4d_ctx *ctx, int64_t n);
 /* Bands of the y and x PBA passes (the reference's m2/m3 bands): 0 picks them by grid width,
    1 walks whole columns, 2..16 split each column (same diagram). */
 int g4d_set_pba_bands(g4d_ctx *ctx, int32_t bands);
+/* Grid stage (grid fill, site scatter, PBA passes) as a hipGraph captured once per shape and
+   buffer set and replayed (up to 16 cached per context): 1 always, 0 never (eager launches,
+   with per-pass PBA times in the stats), -1 (default) for batched calls only. */
+int g4d_set_stage_graphs(g4d_ctx *ctx, int32_t mode);
 /* Upload host points into a device buffer owned by ctx and return its device address. */
 int g4d_upload_points(g4d_ctx *ctx, const double *xyz, int32_t n, const double **d_xyz);
 int g4d_get_stats(const g4d_ctx *ctx, g4d_stats *st);

@@ -168,3 +168,34 @@ def test_grid_table_and_star_kernel_variants(orc):
             rep = t.check()
             assert rep["adjacency_failures"] == 0 and rep["euler"] == 1, v
         t.set_star_kernel(-1)
+
+
+def test_stage_graph_replays_match_eager(orc):
+    """The grid stage (fill, site scatter, PBA passes) as a captured hipGraph: captured on the
+    first call, replayed on the later ones, with eager work of another context and device
+    synchronisations between the replays; every replay's result equals the eager one's."""
+    sets = datasets.batch_sets(16, seed=12, lo=1000, hi=20000)
+    off = np.concatenate([[0], np.cumsum([s.shape[0] for s in sets])]).astype(np.int64)
+    xyz = np.concatenate(sets)
+    single = datasets.uniform_cube(30000, seed=13)
+    with g.Triangulator(grid_size=64) as eager:
+        eager.set_stage_graphs(0)
+        ref = eager.triangulate_batch(xyz, off)
+        ref1 = eager.triangulate(single)
+        assert eager.stats()["graph_launches"] == 0
+        with g.Triangulator(grid_size=64) as t:
+            t.set_stage_graphs(1)
+            for rep in range(4):
+                got = t.triangulate_batch(xyz, off)
+                st = t.stats()
+                assert st["graph_launches"] == 1 and st["graph_captures"] == (1 if rep == 0 else 0), (rep, st)
+                for a, b in zip(got, ref):
+                    assert np.array_equal(a, b), rep
+                got1 = t.triangulate(single)
+                assert t.stats()["graph_launches"] == 1
+                assert np.array_equal(got1[0], ref1[0]) and np.array_equal(got1[1], ref1[1]), rep
+                # eager work on the other context between replays
+                e1 = eager.triangulate(single)
+                assert np.array_equal(e1[0], ref1[0]), rep
+    oracle, _, _ = orc.triangulate(single, 64)
+    assert (g.canonical(ref1[0]) == oracle).all()
