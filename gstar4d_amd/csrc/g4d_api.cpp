@@ -74,7 +74,7 @@ struct g4d_ctx {
     DevBuf xyz_up, keys, pts, perm, vox, bounds, settab, ptset, grid, gtmp, idgrid, pbaspill, pbaenv, pbacnt, pcount, pairs,
         ws_off, cnt;
     Store store;
-    DevBuf settet, drown, dflag, pairbuf, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, aflags, apos, active, mid, big, huge, xstars,
+    DevBuf settet, drown, dflag, pairbuf, spcount, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, aflags, apos, active, mid, big, huge, xstars,
         xtasks, xtris, gwork, tetcnt, tetoff, tets, opp, table;
     // learned proposal: parameters, candidates, features, proposed pairs, capacity bins, labels
     DevBuf mlp_params, fkeys, ford, frank, pcand, pd2, pfeat, ppairs, pcap, plab, pvcap;
@@ -772,8 +772,10 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
             nitems = (int)(last[0] + last[1]);
             list = buf<int>(ctx->active);
         }
+        HCK(ctx->spcount.ensure(sizeof(int) * (size_t)n));
+        int *spc = buf<int>(ctx->spcount);
         HCK(g4d::launch_capacities(list, nitems, mode ? buf<int>(store.vnum) : nullptr,
-                                   mode ? buf<int>(store.hnum) : nullptr, ins_off, vcap, tcap, hcap, s));
+                                   mode ? buf<int>(store.hnum) : nullptr, ins_off, vcap, tcap, hcap, spc, s));
         int64_t need[3] = {0, 0, 0};
         int64_t *caps[3] = {vcap, tcap, hcap};
         for (int k = 0; k < 3; k++) {
@@ -831,6 +833,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         sp.dflag = buf<uint8_t>(ctx->dflag);
         sp.pairbuf = buf<uint64_t>(ctx->pairbuf);
         sp.tbase = tbase;
+        sp.pcount = spc;
         sp.cnt = cnt;
         sp.lds_limit = lds_limit;
         sp.defer_list = buf<int>(ctx->mid);
@@ -912,8 +915,17 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], mode ? "splaying" : "star creation"));
         // drowned insertions and triangle pairs, in star order
         HCK(g4d::compact_u64(P, ins_keys, buf<uint8_t>(ctx->dflag), buf<uint64_t>(ctx->drown), nins, drown_n, s));
-        HCK(g4d::compact_valid_u64(P, buf<uint64_t>(ctx->pairbuf), buf<uint64_t>(ctx->cand), npb, g4d::STAR_PAIR_NONE,
-                                   tp_n, s));
+        // each star wrote its pairs densely at the front of its region and their count: offsets
+        // in list order (the vertex capacities are done with), then one copy per star
+        HCK(g4d::launch_pair_counts(list, nitems, spc, vcap, s));
+        HCK(g4d::scan_excl_i64(P, vcap, vcap, nitems, vcap + nitems, s));
+        int64_t npairs = 0;
+        HCK(hipMemcpyAsync(&npairs, vcap + nitems, 8, hipMemcpyDeviceToHost, s));
+        HCK(hipStreamSynchronize(s));
+        if (npairs > npb) return set_err(ctx, G4D_ERR_INTERNAL, "star pairs exceed their regions");
+        HCK(g4d::launch_pair_gather(list, nitems, spc, vcap, buf<int64_t>(store.ntoff), tbase,
+                                    buf<uint64_t>(ctx->pairbuf), buf<uint64_t>(ctx->cand), s));
+        *tp_n = npairs;
         trace("star pass mode %d: %lld insertions into %d stars, over 32: %lld, over 64: %lld, exact %lld, "
               "global-memory %lld, drowned %lld, pairs %lld",
               mode, (long long)nins, nitems, (long long)nmid, (long long)nbig, (long long)nexact, (long long)nhuge,
@@ -1167,7 +1179,7 @@ void g4d_destroy(g4d_ctx *ctx)
     (void)hipSetDevice(ctx->opt.device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->xyz_up, &ctx->keys, &ctx->pts, &ctx->perm, &ctx->vox, &ctx->bounds, &ctx->settab, &ctx->ptset, &ctx->grid,
-                      &ctx->gtmp, &ctx->idgrid, &ctx->pbaspill, &ctx->pbaenv, &ctx->pbacnt, &ctx->pcount, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->settet, &ctx->drown, &ctx->dflag, &ctx->pairbuf, &ctx->dkeys,
+                      &ctx->gtmp, &ctx->idgrid, &ctx->pbaspill, &ctx->pbaenv, &ctx->pbacnt, &ctx->pcount, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->settet, &ctx->drown, &ctx->dflag, &ctx->pairbuf, &ctx->spcount, &ctx->dkeys,
                       &ctx->tasks, &ctx->cand, &ctx->keep, &ctx->ins, &ctx->ins_off, &ctx->vcap, &ctx->tcap,
                       &ctx->hcap, &ctx->aflags, &ctx->apos, &ctx->active, &ctx->mid, &ctx->big, &ctx->huge, &ctx->xstars, &ctx->xtasks, &ctx->xtris, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table,
                       &ctx->mlp_params, &ctx->fkeys, &ctx->ford, &ctx->frank, &ctx->pcand, &ctx->pd2, &ctx->pfeat,

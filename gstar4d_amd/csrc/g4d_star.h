@@ -58,8 +58,9 @@ struct StarParams {
     const int64_t *nhoff;
     // outputs without atomics (compacted by the host code after the pass)
     uint8_t *dflag;       // per insertion, indexed like ins_keys: 1 when the vertex drowned
-    uint64_t *pairbuf;    // per triangle word of this pass's regions two entries (a<<32 | b,
-    int64_t tbase;        // a < b, or STAR_PAIR_NONE), from 2 * (ntoff[s] - tbase)
+    uint64_t *pairbuf;    // per star, from 2 * (ntoff[s] - tbase): its insertion pairs (a<<32 | b,
+    int64_t tbase;        // a < b) of unchecked triangles, dense, in triangle order
+    int *pcount;          // per star: the number of those pairs
     int *defer_list;      // stars that exceed this kernel's capacity, counter `defer_counter`
     int64_t defer_cap;
     int defer_counter;
@@ -140,8 +141,15 @@ hipError_t launch_active_scatter(const uint32_t *flags, const uint32_t *pos, int
 // Region sizes of the stars of a pass (list = nullptr: stars 0..nitems-1), per item: link
 // vertices v = vn + m, triangle words 2v - 4 (twice that for stars that may need the wide
 // format), history hn + m.
+// pcount[s] = 0 for the pass's stars (their kernel writes the dense pair count).
 hipError_t launch_capacities(const int *list, int nitems, const int *vnum, const int *hnum, const int64_t *ins_off,
-                             int64_t *vcap, int64_t *tcap, int64_t *hcap, hipStream_t s);
+                             int64_t *vcap, int64_t *tcap, int64_t *hcap, int *pcount, hipStream_t s);
+// The pass's insertion pairs, star after star in list order (list = nullptr: stars 0..nitems-1):
+// cnt64[i] = pcount[list[i]] for the offsets scan, then the copy to out[off[i] ...].
+hipError_t launch_pair_counts(const int *list, int nitems, const int *pcount, int64_t *cnt64, hipStream_t s);
+hipError_t launch_pair_gather(const int *list, int nitems, const int *pcount, const int64_t *off,
+                              const int64_t *ntoff, int64_t tbase, const uint64_t *pairbuf, uint64_t *out,
+                              hipStream_t s);
 // nvoff[s] = vbase + vscan[i] (same for t, h) for s = list[i].
 hipError_t launch_assign_regions(const int *list, int nitems, const int64_t *vscan, const int64_t *tscan,
                                  const int64_t *hscan, int64_t vbase, int64_t tbase, int64_t hbase, int64_t *nvoff,

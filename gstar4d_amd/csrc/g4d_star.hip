@@ -555,22 +555,28 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
             }
             wave_sync();
             const int64_t pb = 2 * (to - p.tbase);
+            int npair = 0;
             for (int base = 0; base < T; base += 64) {
                 const int t = base + lane;
                 uint32_t tt = 0;
                 if (t < T) tt = L.tv[t];
+                TriPairs pr{STAR_PAIR_NONE, STAR_PAIR_NONE};
                 if (t < T && (tt & TF_ALIVE)) {
                     const int a = tvert(tt, 0), b = tvert(tt, 1), c = tvert(tt, 2);
                     const int d = L.remap[t];
                     p.tpool[to + d] = hpack(L.vremap[a], L.vremap[b], L.vremap[c], L.remap[L.to[t][0]],
                                             L.remap[L.to[t][1]], L.remap[L.to[t][2]], true);
-                    const TriPairs pr = star_tri_pairs(L.vid[a], L.vid[b], L.vid[c], !(tt & TF_CHECKED));
-                    p.pairbuf[pb + 2 * d] = pr.x;
-                    p.pairbuf[pb + 2 * d + 1] = pr.y;
+                    pr = star_tri_pairs(L.vid[a], L.vid[b], L.vid[c], !(tt & TF_CHECKED));
                 }
+                // the star's pairs, dense, in triangle order
+                const int np = (pr.x != STAR_PAIR_NONE ? 1 : 0) + (pr.y != STAR_PAIR_NONE ? 1 : 0);
+                int tot;
+                const int at = npair + wave_excl<2>(np, &tot);
+                if (np > 0) p.pairbuf[pb + at] = pr.x;
+                if (np > 1) p.pairbuf[pb + at + 1] = pr.y;
+                npair += tot;
             }
-            for (int64_t j = 2 * (int64_t)tcount + lane; j < 2 * star_tcap_words(need); j += 64)
-                p.pairbuf[pb + j] = STAR_PAIR_NONE;
+            if (lane == 0) p.pcount[s] = npair;
             const int hn = star_history_merge(p, s, L.ins, m, lane, 64);
             wave_sync();
             if (lane == 0) star_commit(p, s, vcount, vo, tcount, to, hn + m);
@@ -1055,24 +1061,30 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
             }
             wave_sync();
             const int64_t pb = 2 * (to - p.tbase);
+            int npair = 0;
 #pragma unroll
             for (int b0 = 0; b0 < TMAX; b0 += GL) {
                 const int t = b0 + gl;
                 const uint32_t tt = (fin && t < T) ? L.tv[t] : 0u;
+                TriPairs pr{STAR_PAIR_NONE, STAR_PAIR_NONE};
                 if (tt & TF_ALIVE) {
                     const int a = tvert(tt, 0), b = tvert(tt, 1), c = tvert(tt, 2);
                     const int d = L.remap[t];
                     p.tpool[to + d] = hpack(L.vremap[a], L.vremap[b], L.vremap[c], L.remap[L.to[t][0]],
                                             L.remap[L.to[t][1]], L.remap[L.to[t][2]], true);
-                    const TriPairs pr = star_tri_pairs(L.vid[a], L.vid[b], L.vid[c], !(tt & TF_CHECKED));
-                    p.pairbuf[pb + 2 * d] = pr.x;
-                    p.pairbuf[pb + 2 * d + 1] = pr.y;
+                    pr = star_tri_pairs(L.vid[a], L.vid[b], L.vid[c], !(tt & TF_CHECKED));
                 }
+                // the star's pairs, dense, in triangle order
+                const int np = (pr.x != STAR_PAIR_NONE ? 1 : 0) + (pr.y != STAR_PAIR_NONE ? 1 : 0);
+                int tot;
+                const int at = npair + grp_excl<GL, 2>(np, g, gl, &tot);
+                if (np > 0) p.pairbuf[pb + at] = pr.x;
+                if (np > 1) p.pairbuf[pb + at + 1] = pr.y;
+                npair += tot;
             }
             int hn = 0;
             if (fin) {
-                for (int64_t j = 2 * (int64_t)tcount + gl; j < 2 * star_tcap_words(need); j += GL)
-                    p.pairbuf[pb + j] = STAR_PAIR_NONE;
+                if (gl == 0) p.pcount[s] = npair;
                 hn = star_history_merge(p, s, L.ins, m, gl, GL);
             }
             wave_sync();
@@ -1534,25 +1546,33 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
         }
         __syncthreads();
         const int64_t pb = 2 * (to - p.tbase);
-        for (int t = tid; t < T; t += GSTAR_NT) {
-            const uint32_t f = L.tfl[t];
-            if (!(f & TF_ALIVE)) continue;
-            const int a = L.tva[3 * t], b = L.tva[3 * t + 1], c = L.tva[3 * t + 2];
-            const int ra = L.vremap[a], rb = L.vremap[b], rc = L.vremap[c];
-            const int o0 = L.remap[L.to[3 * t]], o1 = L.remap[L.to[3 * t + 1]], o2 = L.remap[L.to[3 * t + 2]];
-            const int64_t d = L.remap[t];
-            if (wide) {
-                p.tpool[to + 2 * d] = wpack3(ra, rb, rc) | (1ull << 63);
-                p.tpool[to + 2 * d + 1] = wpack3(o0, o1, o2);
-            } else {
-                p.tpool[to + d] = hpack(ra, rb, rc, o0, o1, o2, true);
+        int64_t npair = 0;
+        for (int base = 0; base < T; base += GSTAR_NT) {
+            const int t = base + tid;
+            const uint32_t f = t < T ? L.tfl[t] : 0u;
+            TriPairs pr{STAR_PAIR_NONE, STAR_PAIR_NONE};
+            if (f & TF_ALIVE) {
+                const int a = L.tva[3 * t], b = L.tva[3 * t + 1], c = L.tva[3 * t + 2];
+                const int ra = L.vremap[a], rb = L.vremap[b], rc = L.vremap[c];
+                const int o0 = L.remap[L.to[3 * t]], o1 = L.remap[L.to[3 * t + 1]], o2 = L.remap[L.to[3 * t + 2]];
+                const int64_t d = L.remap[t];
+                if (wide) {
+                    p.tpool[to + 2 * d] = wpack3(ra, rb, rc) | (1ull << 63);
+                    p.tpool[to + 2 * d + 1] = wpack3(o0, o1, o2);
+                } else {
+                    p.tpool[to + d] = hpack(ra, rb, rc, o0, o1, o2, true);
+                }
+                pr = star_tri_pairs(L.vid[a], L.vid[b], L.vid[c], !(f & TF_CHECKED));
             }
-            const TriPairs pr = star_tri_pairs(L.vid[a], L.vid[b], L.vid[c], !(f & TF_CHECKED));
-            p.pairbuf[pb + 2 * d] = pr.x;
-            p.pairbuf[pb + 2 * d + 1] = pr.y;
+            // the star's pairs, dense, in triangle order
+            const int np = (pr.x != STAR_PAIR_NONE ? 1 : 0) + (pr.y != STAR_PAIR_NONE ? 1 : 0);
+            int tot;
+            const int64_t at = npair + block_excl(np, &tot, red);
+            if (np > 0) p.pairbuf[pb + at] = pr.x;
+            if (np > 1) p.pairbuf[pb + at + 1] = pr.y;
+            npair += tot;
         }
-        for (int64_t j = 2 * (int64_t)tcount + tid; j < 2 * star_tcap_words(vn + m); j += GSTAR_NT)
-            p.pairbuf[pb + j] = STAR_PAIR_NONE;
+        if (tid == 0) p.pcount[s] = (int)npair;
         const int hn = star_history_merge(p, s, L.ins, m, tid, GSTAR_NT);
         __syncthreads();
         if (tid == 0) star_commit(p, s, vcount, vo, tcount, to, hn + m);
@@ -1798,10 +1818,11 @@ hipError_t launch_active_scatter(const uint32_t *flags, const uint32_t *pos, int
 }
 
 __global__ void k_capacities(const int *list, int nitems, const int *vnum, const int *hnum, const int64_t *ins_off,
-                             int64_t *vcap, int64_t *tcap, int64_t *hcap)
+                             int64_t *vcap, int64_t *tcap, int64_t *hcap, int *pcount)
 {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += gridDim.x * blockDim.x) {
         const int s = list ? list[i] : i;
+        pcount[s] = 0;
         const int64_t m = ins_off[s + 1] - ins_off[s];
         const int64_t v = (vnum ? vnum[s] : 0) + m;
         vcap[i] = v;
@@ -1811,11 +1832,51 @@ __global__ void k_capacities(const int *list, int nitems, const int *vnum, const
 }
 
 hipError_t launch_capacities(const int *list, int nitems, const int *vnum, const int *hnum, const int64_t *ins_off,
-                             int64_t *vcap, int64_t *tcap, int64_t *hcap, hipStream_t s)
+                             int64_t *vcap, int64_t *tcap, int64_t *hcap, int *pcount, hipStream_t s)
 {
     if (nitems <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_capacities, dim3(g4d_blocks(nitems, 256, 65536)), dim3(256), 0, s, list, nitems, vnum, hnum,
-                       ins_off, vcap, tcap, hcap);
+                       ins_off, vcap, tcap, hcap, pcount);
+    return hipGetLastError();
+}
+
+__global__ void k_pair_counts(const int *list, int nitems, const int *pcount, int64_t *cnt64)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += gridDim.x * blockDim.x)
+        cnt64[i] = pcount[list ? list[i] : i];
+}
+
+hipError_t launch_pair_counts(const int *list, int nitems, const int *pcount, int64_t *cnt64, hipStream_t s)
+{
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pair_counts, dim3(g4d_blocks(nitems, 256, 65536)), dim3(256), 0, s, list, nitems, pcount,
+                       cnt64);
+    return hipGetLastError();
+}
+
+// One quarter-wave (16 lanes) per star copies its dense pairs.
+__global__ void __launch_bounds__(256) k_pair_gather(const int *list, int nitems, const int *pcount,
+                                                     const int64_t *off, const int64_t *ntoff, int64_t tbase,
+                                                     const uint64_t *pairbuf, uint64_t *out)
+{
+    const int sub = threadIdx.x & 15;
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) >> 4; i < nitems;
+         i += ((int64_t)gridDim.x * blockDim.x) >> 4) {
+        const int s = list ? list[i] : (int)i;
+        const int c = pcount[s];
+        const uint64_t *src = pairbuf + 2 * (ntoff[s] - tbase);
+        uint64_t *dst = out + off[i];
+        for (int k = sub; k < c; k += 16) dst[k] = src[k];
+    }
+}
+
+hipError_t launch_pair_gather(const int *list, int nitems, const int *pcount, const int64_t *off,
+                              const int64_t *ntoff, int64_t tbase, const uint64_t *pairbuf, uint64_t *out,
+                              hipStream_t s)
+{
+    if (nitems <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_pair_gather, dim3(g4d_blocks((int64_t)nitems * 16, 256, 65536)), dim3(256), 0, s, list,
+                       nitems, pcount, off, ntoff, tbase, pairbuf, out);
     return hipGetLastError();
 }
 
