@@ -74,7 +74,7 @@ struct g4d_ctx {
     DevBuf xyz_up, keys, pts, perm, vox, bounds, settab, ptset, grid, gtmp, idgrid, pbaspill, pbaenv, pbacnt, pcount, pairs,
         ws_off, cnt;
     Store store;
-    DevBuf settet, drown, dflag, pairbuf, spcount, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, aflags, apos, active, mid, big, huge, xstars,
+    DevBuf settet, drown, dflag, pairbuf, spcount, hkeys, dkeys, tasks, cand, keep, ins, ins_off, vcap, tcap, hcap, aflags, apos, active, mid, big, huge, xstars,
         xtasks, xtris, gwork, tetcnt, tetoff, tets, opp, table;
     // learned proposal: parameters, candidates, features, proposed pairs, capacity bins, labels
     DevBuf mlp_params, fkeys, ford, frank, pcand, pd2, pfeat, ppairs, pcap, plab, pvcap;
@@ -757,7 +757,20 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
                          int64_t *tp_n) -> int {
         const int *list = nullptr;
         int nitems = n;
-        if (mode == 1) {
+        if (mode == 1 && nins < n / 2) {
+            // few insertions: offsets and active list from the runs of the sorted keys, O(keys)
+            // instead of O(stars)
+            HCK(ctx->active.ensure(sizeof(int) * (size_t)n));
+            HCK(ctx->hkeys.ensure(sizeof(uint64_t) * (size_t)std::max<int64_t>(nins, 1)));
+            uint64_t *hk = buf<uint64_t>(ctx->hkeys);
+            HCK(g4d::launch_sparse_runs(ins_keys, nins, const_cast<int64_t *>(ins_off), hk, s));
+            int64_t na = 0;
+            HCK(g4d::compact_valid_u64(P, hk, hk, nins, g4d::STAR_TASK_NONE, &na, s));
+            HCK(g4d::launch_u64_to_i32(hk, na, buf<int>(ctx->active), s));
+            nitems = (int)na;
+            list = buf<int>(ctx->active);
+        } else if (mode == 1) {
+            HCK(g4d::launch_csr_offsets(ins_keys, nins, n, const_cast<int64_t *>(ins_off), s));
             HCK(ctx->aflags.ensure(sizeof(uint32_t) * (size_t)n));
             HCK(ctx->apos.ensure(sizeof(uint32_t) * (size_t)n));
             HCK(ctx->active.ensure(sizeof(int) * (size_t)n));
@@ -871,7 +884,11 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
             sp.defer_list = defer;
             sp.defer_counter = defer_counter;
             sp.work_counter = queue++;
-            HCK(g4d::launch_star_kernel(sp, vmax, exact, vmax == g4d::STAR_VMAX_T1 && !exact ? 4096 : 1024, s));
+            // a tier's list is a subset of the pass's stars: late rounds with a few thousand stars
+            // dispatch a few hundred work-groups instead of thousands that would leave at once
+            const int full = vmax == g4d::STAR_VMAX_T1 && !exact ? 4096 : 1024;
+            const int nblk = std::max(64, std::min(full, (nitems + 7) / 8));
+            HCK(g4d::launch_star_kernel(sp, vmax, exact, nblk, s));
             return G4D_OK;
         };
         // filter tiers: mid list (A6) -> 64, big list (A2) -> 256, beyond -> huge (A3)
@@ -1010,8 +1027,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         ctx->st.splay_insertions += nrec;
         HCK(ctx->ins_off.ensure(sizeof(int64_t) * (size_t)(n + 1)));
         int64_t *ioff = buf<int64_t>(ctx->ins_off);
-        HCK(g4d::launch_csr_offsets(ins, nrec, n, ioff, s));
-        CK(star_pass(1, ins, ioff, nrec, &nd, &ntp));
+        CK(star_pass(1, ins, ioff, nrec, &nd, &ntp)); // builds the offsets of the stars with insertions
     }
     if (!converged) return set_err(ctx, G4D_ERR_NOT_CONVERGED, "splaying did not converge in %d rounds", rounds);
     ctx->st.splay_rounds = rounds;
@@ -1179,7 +1195,7 @@ void g4d_destroy(g4d_ctx *ctx)
     (void)hipSetDevice(ctx->opt.device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     DevBuf *bufs[] = {&ctx->xyz_up, &ctx->keys, &ctx->pts, &ctx->perm, &ctx->vox, &ctx->bounds, &ctx->settab, &ctx->ptset, &ctx->grid,
-                      &ctx->gtmp, &ctx->idgrid, &ctx->pbaspill, &ctx->pbaenv, &ctx->pbacnt, &ctx->pcount, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->settet, &ctx->drown, &ctx->dflag, &ctx->pairbuf, &ctx->spcount, &ctx->dkeys,
+                      &ctx->gtmp, &ctx->idgrid, &ctx->pbaspill, &ctx->pbaenv, &ctx->pbacnt, &ctx->pcount, &ctx->pairs, &ctx->ws_off, &ctx->cnt, &ctx->settet, &ctx->drown, &ctx->dflag, &ctx->pairbuf, &ctx->spcount, &ctx->hkeys, &ctx->dkeys,
                       &ctx->tasks, &ctx->cand, &ctx->keep, &ctx->ins, &ctx->ins_off, &ctx->vcap, &ctx->tcap,
                       &ctx->hcap, &ctx->aflags, &ctx->apos, &ctx->active, &ctx->mid, &ctx->big, &ctx->huge, &ctx->xstars, &ctx->xtasks, &ctx->xtris, &ctx->gwork, &ctx->tetcnt, &ctx->tetoff, &ctx->tets, &ctx->opp, &ctx->table,
                       &ctx->mlp_params, &ctx->fkeys, &ctx->ford, &ctx->frank, &ctx->pcand, &ctx->pd2, &ctx->pfeat,

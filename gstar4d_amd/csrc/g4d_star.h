@@ -137,6 +137,10 @@ hipError_t launch_reciprocate(uint64_t *keys, int64_t m, hipStream_t s);
 // Stars with at least one insertion (ins_off[s+1] > ins_off[s]): flags[s] in {0,1}; after
 // an exclusive scan of flags into pos, list[pos[s]] = s for the flagged stars (ascending).
 hipError_t launch_active_flags(const int64_t *ins_off, int n, uint32_t *flags, hipStream_t s);
+// Sparse alternative for few sorted keys: offsets of the stars with insertions only, and hk[i] =
+// star at the start of each run (STAR_TASK_NONE elsewhere; compact it for the active list).
+hipError_t launch_sparse_runs(const uint64_t *keys, int64_t m, int64_t *off, uint64_t *hk, hipStream_t s);
+hipError_t launch_u64_to_i32(const uint64_t *in, int64_t m, int *out, hipStream_t s);
 hipError_t launch_active_scatter(const uint32_t *flags, const uint32_t *pos, int n, int *list, hipStream_t s);
 // Region sizes of the stars of a pass (list = nullptr: stars 0..nitems-1), per item: link
 // vertices v = vn + m, triangle words 2v - 4 (twice that for stars that may need the wide

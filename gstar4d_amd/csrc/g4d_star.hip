@@ -24,6 +24,7 @@
 // two per-vertex maps (boundary edge starting / ending at each vertex).  The whole insertion
 // list of a star runs inside one kernel, so there is no host loop per insertion index.
 #include "g4d_predicates.h"
+#include <algorithm>
 #include <climits>
 #include <cstdlib>
 #include <type_traits>
@@ -1173,8 +1174,11 @@ hipError_t launch_star_kernel(const StarParams &p, int vmax, bool exact, int nbl
             hipLaunchKernelGGL(k_star32, dim3(nblocks), dim3(64), 0, s, p, nitems);
         } else {
             // one work-group per resident wave: 256 CUs x 4 SIMDs x the variant's occupancy
+            // one work-group per resident wave (256 CUs x 4 SIMDs x the variant's occupancy), or
+            // one per batch of stars when the list is shorter
             const int per_simd = v <= STAR_T0_G16F ? 3 : 4;
-            const int nb = 1024 * per_simd;
+            const int ng = v == STAR_T0_G32F ? 2 : 4;
+            const int nb = std::max(1, std::min(1024 * per_simd, (nitems + ng - 1) / ng));
             if (v == STAR_T0_G16) hipLaunchKernelGGL(k_star32g16, dim3(nb), dim3(64), 0, s, p, nitems);
             else if (v == STAR_T0_G16F) hipLaunchKernelGGL(k_star32g16f, dim3(nb), dim3(64), 0, s, p, nitems);
             else if (v == STAR_T0_G16F4) hipLaunchKernelGGL(k_star32g16f4, dim3(nb), dim3(64), 0, s, p, nitems);
@@ -1790,6 +1794,44 @@ hipError_t launch_reciprocate(uint64_t *keys, int64_t m, hipStream_t s)
 {
     if (m <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_reciprocate, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, keys, m);
+    return hipGetLastError();
+}
+
+// Sparse form of a pass's insertion offsets and active list, for sorted keys much fewer than the
+// stars: the key that starts a star's run writes the star's first offset, the key that ends it
+// the next one (equal to the following star's first when that star has insertions too), and
+// hk[i] holds the star at a run's start (STAR_TASK_NONE elsewhere), so the order-preserving
+// compaction of hk lists the stars with insertions in ascending order.  Offsets of stars
+// without insertions are not written (nothing reads them).
+__global__ void k_sparse_runs(const uint64_t *keys, int64_t m, int64_t *off, uint64_t *hk)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t st = (uint32_t)(keys[i] >> 32);
+        const bool head = i == 0 || (uint32_t)(keys[i - 1] >> 32) != st;
+        const bool tail = i == m - 1 || (uint32_t)(keys[i + 1] >> 32) != st;
+        if (head) off[st] = i;
+        if (tail) off[st + 1] = i + 1;
+        hk[i] = head ? (uint64_t)st : STAR_TASK_NONE;
+    }
+}
+
+hipError_t launch_sparse_runs(const uint64_t *keys, int64_t m, int64_t *off, uint64_t *hk, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sparse_runs, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, keys, m, off, hk);
+    return hipGetLastError();
+}
+
+__global__ void k_u64_to_i32(const uint64_t *in, int64_t m, int *out)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (int)in[i];
+}
+
+hipError_t launch_u64_to_i32(const uint64_t *in, int64_t m, int *out, hipStream_t s)
+{
+    if (m <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_u64_to_i32, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, in, m, out);
     return hipGetLastError();
 }
 
