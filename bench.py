@@ -254,8 +254,10 @@ def run_headline(args):
     d_xyz, _ = t.upload(x)
     total_s, step_ms, stages, nt = time_plain(t, d_xyz, n, args.steps, args.warmup, args.workload)
     value = n * args.steps / total_s
-    # Outside the timed region: one call through the host-buffer entry point (points copied in,
-    # tetrahedra and adjacency copied out), the PCIe-inclusive rate.
+    # Outside the timed region: the host-buffer entry point (points copied in, tetrahedra and
+    # adjacency copied out into the context's page-locked buffers), the PCIe-inclusive rate; the
+    # second call is timed (the first allocates and page-locks the result buffers).
+    t.triangulate(x)
     h0 = time.perf_counter()
     t.triangulate(x)
     host_ms = (time.perf_counter() - h0) * 1e3
@@ -309,7 +311,8 @@ def run_headline(args):
         "sos_calls": last["sos_calls"],
         "step_ms": [round(v, 3) for v in step_ms],
         "host_buffers": {"ms": round(host_ms, 3), "points_per_s": round(n / (host_ms * 1e-3), 1),
-                         "note": "g4d_triangulate: host points in, host tetrahedra + adjacency out (PCIe-inclusive)"},
+                         "note": "g4d_triangulate: host points in, host tetrahedra + adjacency out (PCIe-inclusive), "
+                                  "second call"},
     }
 
 

@@ -64,6 +64,33 @@ constexpr size_t Store::esz[3];
 enum { EV_START, EV_MORTON, EV_PBA0, EV_PBA1, EV_PBA2, EV_PBA3, EV_PROP0, EV_PROP1, EV_PAIRS, EV_STAR, EV_SPLAY, EV_OUT,
        EV_ADJ, EV_NUM };
 
+// Page-locked host buffer for results handed back through host pointers: device-to-host copies
+// into it run at the link's rate instead of through the runtime's pageable staging.
+struct PinnedBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need)
+    {
+        if (need <= bytes) return hipSuccess;
+        release();
+        const size_t cap = need + need / 8 + 256;
+        hipError_t e = hipHostMalloc(&ptr, cap, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            ptr = nullptr;
+            return e;
+        }
+        bytes = cap;
+        return hipSuccess;
+    }
+    void release()
+    {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+    int32_t *i32() const { return static_cast<int32_t *>(ptr); }
+};
+
 } // namespace
 
 struct g4d_ctx {
@@ -81,7 +108,8 @@ struct g4d_ctx {
     bool prop_on = false, prop_export = false, have_samples = false;
     float prop_threshold = 0.f;
     int samples_n = 0;
-    std::vector<int32_t> h_tets, h_opp, h_perm, h_set_grid;
+    PinnedBuf h_tets, h_opp, h_perm; // results of g4d_triangulate / g4d_triangulate_batch
+    std::vector<int32_t> h_set_grid;
     std::vector<int64_t> h_set_tet_off;
     const double *last_xyz = nullptr; // device points of the last triangulation (input order)
     int last_nsets = 1;
@@ -1202,6 +1230,9 @@ void g4d_destroy(g4d_ctx *ctx)
                       &ctx->ppairs, &ctx->pcap, &ctx->plab, &ctx->pvcap};
     destroy_graphs(ctx);
     for (DevBuf *b : bufs) b->release();
+    ctx->h_tets.release();
+    ctx->h_opp.release();
+    ctx->h_perm.release();
     ctx->store.release();
     ctx->P.release();
     if (ctx->ev_ok)
@@ -1369,13 +1400,13 @@ int g4d_triangulate_batch(g4d_ctx *ctx, const double *xyz, const int64_t *set_of
     CK(g4d_upload_points(ctx, xyz, (int32_t)n, &d_xyz));
     CK(g4d_triangulate_batch_device(ctx, d_xyz, set_offsets, nsets, grids, nullptr));
     const int64_t nt = ctx->num_tets;
-    ctx->h_tets.resize((size_t)std::max<int64_t>(4 * nt, 1));
-    ctx->h_opp.resize(ctx->opt.compute_adjacency ? (size_t)std::max<int64_t>(4 * nt, 1) : 0);
+    HCK(ctx->h_tets.ensure(sizeof(int) * (size_t)std::max<int64_t>(4 * nt, 1)));
+    if (ctx->opt.compute_adjacency) HCK(ctx->h_opp.ensure(sizeof(int) * (size_t)std::max<int64_t>(4 * nt, 1)));
     if (nt > 0)
-        HCK(hipMemcpyAsync(ctx->h_tets.data(), ctx->tets.ptr, sizeof(int) * 4 * (size_t)nt, hipMemcpyDeviceToHost,
+        HCK(hipMemcpyAsync(ctx->h_tets.ptr, ctx->tets.ptr, sizeof(int) * 4 * (size_t)nt, hipMemcpyDeviceToHost,
                            ctx->stream));
     if (nt > 0 && ctx->opt.compute_adjacency)
-        HCK(hipMemcpyAsync(ctx->h_opp.data(), ctx->opp.ptr, sizeof(int) * 4 * (size_t)nt, hipMemcpyDeviceToHost,
+        HCK(hipMemcpyAsync(ctx->h_opp.ptr, ctx->opp.ptr, sizeof(int) * 4 * (size_t)nt, hipMemcpyDeviceToHost,
                            ctx->stream));
     HCK(hipStreamSynchronize(ctx->stream));
     out->num_sets = nsets;
@@ -1383,8 +1414,8 @@ int g4d_triangulate_batch(g4d_ctx *ctx, const double *xyz, const int64_t *set_of
     out->num_points = n;
     out->num_tets = nt;
     out->tet_offsets = ctx->h_set_tet_off.data();
-    out->tets = ctx->h_tets.data();
-    out->opp = ctx->opt.compute_adjacency ? ctx->h_opp.data() : nullptr;
+    out->tets = ctx->h_tets.i32();
+    out->opp = ctx->opt.compute_adjacency ? ctx->h_opp.i32() : nullptr;
     out->grid_sizes = ctx->h_set_grid.data();
     return G4D_OK;
 }
@@ -1478,16 +1509,16 @@ int g4d_triangulate(g4d_ctx *ctx, const double *xyz, int32_t n, g4d_result *out)
     r = run_device(ctx, d_xyz, n, single_set(n, ctx->opt.grid_size));
     if (r != G4D_OK) return r;
     const int64_t nt = ctx->num_tets;
-    ctx->h_tets.resize((size_t)std::max<int64_t>(4 * nt, 1));
-    ctx->h_opp.resize(ctx->opt.compute_adjacency ? (size_t)std::max<int64_t>(4 * nt, 1) : 0);
-    ctx->h_perm.resize((size_t)n);
-    r = g4d_copy_result(ctx, ctx->h_tets.data(), ctx->opt.compute_adjacency ? ctx->h_opp.data() : nullptr,
-                        ctx->h_perm.data());
+    HCK(ctx->h_tets.ensure(sizeof(int) * (size_t)std::max<int64_t>(4 * nt, 1)));
+    if (ctx->opt.compute_adjacency) HCK(ctx->h_opp.ensure(sizeof(int) * (size_t)std::max<int64_t>(4 * nt, 1)));
+    HCK(ctx->h_perm.ensure(sizeof(int) * (size_t)n));
+    r = g4d_copy_result(ctx, ctx->h_tets.i32(), ctx->opt.compute_adjacency ? ctx->h_opp.i32() : nullptr,
+                        ctx->h_perm.i32());
     if (r != G4D_OK) return r;
     out->num_tets = nt;
-    out->tets = ctx->h_tets.data();
-    out->opp = ctx->opt.compute_adjacency ? ctx->h_opp.data() : nullptr;
-    out->perm = ctx->h_perm.data();
+    out->tets = ctx->h_tets.i32();
+    out->opp = ctx->opt.compute_adjacency ? ctx->h_opp.i32() : nullptr;
+    out->perm = ctx->h_perm.i32();
     out->num_points = n;
     return G4D_OK;
 }
