@@ -117,7 +117,7 @@ EXPORTS = ["g4d_default_options", "g4d_create", "g4d_destroy", "g4d_last_error",
            "g4d_struct_sizes", "g4d_check_sizes", "g4d_set_proposal", "g4d_set_sample_export", "g4d_get_samples",
            "g4d_mlp_forward", "g4d_mlp_grad", "g4d_mlp_train", "g4d_check_stars", "g4d_diag_set_link_vertex",
            "g4d_triangulate_device_noalloc", "g4d_set_star_kernel", "g4d_set_grid_table", "g4d_grid_for_set",
-           "g4d_set_pba_bands", "g4d_set_stage_graphs"]
+           "g4d_set_pba_bands", "g4d_set_stage_graphs", "g4d_diag_sort_pairs"]
 
 MLP_PARAMS = 7312
 PROP_FEATURES = 12
@@ -153,6 +153,7 @@ def load_library(path=None):
     L.g4d_diag_pba.argtypes = [_vp, _ip, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _ip]
     L.g4d_diag_grid_pairs.argtypes = [_vp, _ip, ctypes.c_int32, _up, ctypes.c_int64, _lp]
     L.g4d_diag_copy.argtypes = [_vp, ctypes.c_int64, _dp]
+    L.g4d_diag_sort_pairs.argtypes = [_vp, _up, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _up, _dp, _ip]
     L.g4d_check_result.argtypes = [_vp, ctypes.c_int32, ctypes.POINTER(CheckReport)]
     L.g4d_check_stars.argtypes = [_vp, _lp, _lp]
     L.g4d_diag_set_link_vertex.argtypes = [_vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]
@@ -530,6 +531,17 @@ class Triangulator:
         self._check(self._L.g4d_diag_grid_pairs(self._ctx, _ptr(grid, _ip), g, _ptr(out, _up), out.shape[0],
                                                 ctypes.byref(cnt)))
         return out[:cnt.value]
+
+    def diag_sort_pairs(self, keys, nb, method=0):
+        """Sorts pair keys a << 32 | b (a, b < 2^nb) on the device: method 0 the pipeline's
+        choice, 1 the LSD radix sort, 3 the bucket sort at any length; 2 and 4 the same as 0 and
+        3 for keys a << 33 | b with b < 2^(nb+1).  Returns (sorted keys, ms, bucket shift)."""
+        k = np.ascontiguousarray(keys, dtype=np.uint64)
+        out = np.empty(max(k.shape[0], 1), np.uint64)
+        ms, sh = ctypes.c_double(0.0), ctypes.c_int32(0)
+        self._check(self._L.g4d_diag_sort_pairs(self._ctx, _ptr(k, _up), k.shape[0], int(nb), int(method),
+                                                _ptr(out, _up), ctypes.byref(ms), ctypes.byref(sh)))
+        return out[:k.shape[0]], ms.value, sh.value
 
     def diag_copy(self, nbytes):
         """Plain device copy of nbytes; returns the kernel time in ms."""

@@ -206,10 +206,16 @@ static T *buf(DevBuf &b)
     return b.as<T>();
 }
 
-// Sort directed keys (a<<32 | b) with a, b < 2^nb.
-static hipError_t sort_pairs(g4d::Prims &P, uint64_t *k, int64_t m, int nb, hipStream_t s)
+// Sort directed keys (a<<32 | b) with a, b < n < 2^nb.
+static hipError_t sort_pairs(g4d::Prims &P, uint64_t *k, int64_t m, int nb, int64_t n, hipStream_t s)
 {
-    return g4d::radix_sort_u64_2(P, k, m, 0, nb, 32, 32 + nb, s);
+    return g4d::sort_pair_keys(P, k, m, nb, n, s);
+}
+
+// The same, then without duplicates; *m becomes the distinct count.
+static hipError_t sort_unique_pairs(g4d::Prims &P, uint64_t *k, int64_t *m, int nb, int64_t n, hipStream_t s)
+{
+    return g4d::sort_unique_pair_keys(P, k, *m, nb, n, m, s);
 }
 
 // Size limits of the device primitives: the radix sort and the compactions count with 32-bit
@@ -429,8 +435,7 @@ static int proposal_stage(g4d_ctx *ctx, int n, const g4d::GridSets &GS, const in
         CK(guard_keys(ctx, *m + 2 * np, "proposed worksets"));
         HCK(g4d::launch_reciprocate(pp, np, s));
         np *= 2;
-        HCK(sort_pairs(P, pp, np, nb, s));
-        HCK(g4d::unique_u64(P, pp, np, &np, s));
+        HCK(sort_unique_pairs(P, pp, &np, nb, n, s));
         HCK(P.sort_tmp_keys.ensure(sizeof(uint64_t) * (size_t)(*m + np)));
         uint64_t *merged = P.sort_tmp_keys.as<uint64_t>();
         HCK(g4d::merge_u64(pr, *m, pp, np, merged, s));
@@ -708,8 +713,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     const int64_t u = ngd + nmiss;
     HCK(g4d::launch_reciprocate(pr, u, s));
     int64_t m = 2 * u;
-    HCK(sort_pairs(P, pr, m, nb, s));
-    HCK(g4d::unique_u64(P, pr, m, &m, s));
+    HCK(sort_unique_pairs(P, pr, &m, nb, n, s));
     ctx->st.num_missing = nmiss;
     ctx->st.num_grid_pairs = m / 2 - nmiss;
     HCK(ctx->ws_off.ensure(sizeof(int64_t) * (size_t)(n + 1)));
@@ -735,8 +739,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         int64_t np = 2 * npump;
         CK(guard_keys(ctx, m + np, "pumped worksets"));
         uint64_t *pk = pr + m;
-        HCK(sort_pairs(P, pk, np, nb, s));
-        HCK(g4d::unique_u64(P, pk, np, &np, s));
+        HCK(sort_unique_pairs(P, pk, &np, nb, n, s));
         HCK(P.sort_tmp_keys.ensure(sizeof(uint64_t) * (size_t)(m + np)));
         uint64_t *merged = P.sort_tmp_keys.as<uint64_t>();
         HCK(g4d::merge_u64(pr, m, pk, np, merged, s));
@@ -994,7 +997,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
             HCK(ctx->tasks.ensure(sizeof(uint64_t) * (size_t)nd));
             uint64_t *dk = buf<uint64_t>(ctx->dkeys);
             HCK(g4d::launch_drown_keys(buf<uint64_t>(ctx->drown), nd, dk, s));
-            HCK(g4d::radix_sort_u64_2(P, dk, nd, 0, 1 + nb, 33, 33 + nb, s));
+            HCK(g4d::sort_keys_ab(P, dk, nd, 33, nb, 1 + nb, n, false, s));
             HCK(g4d::launch_nonmutual(dk, nd, buf<uint64_t>(ctx->tasks), s));
             HCK(g4d::compact_valid_u64(P, buf<uint64_t>(ctx->tasks), buf<uint64_t>(ctx->tasks), nd,
                                        g4d::STAR_TASK_NONE, &ntask, s));
@@ -1034,8 +1037,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
                                         buf<int64_t>(store.hoff), buf<int>(store.hnum), buf<uint8_t>(ctx->keep), s));
             HCK(g4d::compact_u64(P, buf<uint64_t>(ctx->cand), buf<uint8_t>(ctx->keep), ins, ncand, &nins, s));
             if (nins > 0) {
-                HCK(sort_pairs(P, ins, nins, nb, s));
-                HCK(g4d::unique_u64(P, ins, nins, &nins, s));
+                HCK(sort_unique_pairs(P, ins, &nins, nb, n, s));
             }
         }
         CK(read_counters(ctx, hc));
@@ -1051,7 +1053,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         uint64_t *ins = buf<uint64_t>(ctx->ins);
         HCK(g4d::launch_reciprocate(ins, nins, s));
         const int64_t nrec = 2 * nins;
-        HCK(sort_pairs(P, ins, nrec, nb, s));
+        HCK(sort_pairs(P, ins, nrec, nb, n, s));
         ctx->st.splay_insertions += nrec;
         HCK(ctx->ins_off.ensure(sizeof(int64_t) * (size_t)(n + 1)));
         int64_t *ioff = buf<int64_t>(ctx->ins_off);
@@ -1843,8 +1845,7 @@ static int check_device(g4d_ctx *ctx, const double *pts, int n, const int *tets,
     HCK(g4d::launch_check_edges(tets, nt, keys, s));
     const int nb = bits_for(n);
     int64_t ne = 6 * nt;
-    HCK(sort_pairs(P, keys, ne, nb, s));
-    HCK(g4d::unique_u64(P, keys, ne, &ne, s));
+    HCK(sort_unique_pairs(P, keys, &ne, nb, n, s));
     unsigned long long h[g4d::CHECK_NUM];
     if (full) {
         // cell grid over the bounding box
@@ -2013,6 +2014,35 @@ int g4d_diag_copy(g4d_ctx *ctx, int64_t bytes, double *ms)
     return G4D_OK;
 }
 
+int g4d_diag_sort_pairs(g4d_ctx *ctx, const uint64_t *keys, int64_t m, int32_t nb, int32_t method, uint64_t *out,
+                        double *ms, int32_t *shift)
+{
+    if (!ctx || m < 0 || (m > 0 && (!keys || !out)) || nb < 1 || nb > 31 || method < 0 || method > 4)
+        return G4D_ERR_INVALID_INPUT;
+    if (m > G4D_MAX_SORT_KEYS) return set_err(ctx, G4D_ERR_CAPACITY, "%lld keys exceed the sort limit", (long long)m);
+    HCK(hipSetDevice(ctx->opt.device));
+    hipStream_t s = ctx->stream;
+    if (shift)
+        *shift = method == 1 ? -1 : g4d::key_sort_shift(m, nb, (method == 2 || method == 4) ? nb + 1 : nb, 0, method >= 3);
+    DevBuf d;
+    HCK(d.ensure(sizeof(uint64_t) * (size_t)std::max<int64_t>(m, 1)));
+    HCK(hipMemcpyAsync(d.ptr, keys, sizeof(uint64_t) * (size_t)m, hipMemcpyHostToDevice, s));
+    if (ctx->ev_ok) HCK(hipEventRecord(ctx->ev[EV_START], s));
+    if (method == 1) {
+        if (m > 1) HCK(g4d::radix_sort_u64_2(ctx->P, buf<uint64_t>(d), m, 0, nb, 32, 32 + nb, s));
+    } else if (method == 2 || method == 4) {
+        HCK(g4d::sort_keys_ab(ctx->P, buf<uint64_t>(d), m, 33, nb, nb + 1, 0, method == 4, s));
+    } else {
+        HCK(g4d::sort_keys_ab(ctx->P, buf<uint64_t>(d), m, 32, nb, nb, 0, method == 3, s));
+    }
+    if (ctx->ev_ok) HCK(hipEventRecord(ctx->ev[EV_MORTON], s));
+    HCK(hipMemcpyAsync(out, d.ptr, sizeof(uint64_t) * (size_t)m, hipMemcpyDeviceToHost, s));
+    HCK(hipStreamSynchronize(s));
+    if (ms) *ms = ctx->ev_ok ? ev_ms(ctx, EV_START, EV_MORTON) : 0.0;
+    d.release();
+    return G4D_OK;
+}
+
 int g4d_diag_grid_pairs(g4d_ctx *ctx, const int32_t *id_grid, int32_t grid, uint64_t *out, int64_t cap, int64_t *count)
 {
     if (!ctx || !id_grid || !count || grid < 4 || grid > G4D_MAX_GRID) return G4D_ERR_INVALID_INPUT;
@@ -2040,8 +2070,7 @@ int g4d_diag_grid_pairs(g4d_ctx *ctx, const int32_t *id_grid, int32_t grid, uint
     int maxid = 0;
     for (int64_t i = 0; i < G3; i++) maxid = std::max(maxid, id_grid[i]);
     const int nb = bits_for((int64_t)maxid + 1);
-    HCK(sort_pairs(ctx->P, pr, m, nb, s));
-    HCK(g4d::unique_u64(ctx->P, pr, m, &m, s));
+    HCK(sort_unique_pairs(ctx->P, pr, &m, nb, (int64_t)maxid + 1, s));
     std::vector<uint64_t> h((size_t)std::max<int64_t>(m, 1));
     HCK(hipMemcpyAsync(h.data(), pr, sizeof(uint64_t) * (size_t)m, hipMemcpyDeviceToHost, s));
     HCK(hipStreamSynchronize(s));

@@ -241,15 +241,14 @@ __global__ void k_scatter_sites(const uint64_t *vox, int n, GridSets S, int *gri
     }
 }
 
-// Coordinates are below 2048, so squared distances stay below 2^23 and the numerators below
-// 2^24: 32-bit terms, 64-bit products.
-__device__ __forceinline__ bool env_dominated(int qu, int du, int qv, int dv, int qw, int dw)
+// Maurer's test: site v (between u and w along the sweep) leaves the lower envelope.  Each site
+// enters as its position q along the sweep and f = q^2 + (squared offset from the column), both
+// integers: coordinates are below 2048, so f < 3 * 2^22, the differences below 2^24 and the
+// products below 2^36.  In FP64 all of it is exact, and an FP64 multiply costs a quarter of a
+// 32 x 32 -> 64-bit integer multiply.
+__device__ __forceinline__ bool env_dominated(double qu, double fu, double qv, double fv, double qw, double fw)
 {
-    const int n_uv = (qv * qv + dv) - (qu * qu + du);
-    const int d_uv = qv - qu;
-    const int n_vw = (qw * qw + dw) - (qv * qv + dv);
-    const int d_vw = qw - qv;
-    return (int64_t)n_uv * d_vw >= (int64_t)n_vw * d_uv;
+    return (fv - fu) * (qw - qv) >= (fw - fv) * (qv - qu);
 }
 
 // Offset of a site from the column line (squared distance in the two non-sweep axes) and
@@ -271,16 +270,26 @@ __device__ __forceinline__ void site_pos_off(int s, const uint64_t *vox, int ca,
     if (AXIS == 2) { // sweep z; column (x=ca, y=cb)
         *q = sz;
         const int dx = ca - sx, dy = cb - sy;
-        *d = dx * dx + dy * dy;
+        *d = __mul24(dx, dx) + __mul24(dy, dy);
     } else if (AXIS == 1) { // sweep y; column (x=ca, z=cb)
         *q = sy;
         const int dx = ca - sx, dz = cb - sz;
-        *d = dx * dx + dz * dz;
+        *d = __mul24(dx, dx) + __mul24(dz, dz);
     } else { // sweep x; column (y=ca, z=cb)
         *q = sx;
         const int dy = ca - sy, dz = cb - sz;
-        *d = dy * dy + dz * dz;
+        *d = __mul24(dy, dy) + __mul24(dz, dz);
     }
+}
+
+// (q, f) of a site for env_dominated.
+template <int AXIS, bool IDS>
+__device__ __forceinline__ void site_qf(int s, const uint64_t *vox, int ca, int cb, double *q, double *f)
+{
+    int qi, di;
+    site_pos_off<AXIS, IDS>(s, vox, ca, cb, &qi, &di);
+    *q = (double)qi;
+    *f = (double)(__mul24(qi, qi) + di);
 }
 
 // One wave per tile of 64 columns along the sweep axis, one lane per column.  The column is
@@ -377,9 +386,9 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
     // is consumed, and the next chunk's loads in flight while the current one is processed.
     int pre[PBA_PER_LANE];
 
-    // ---- lower envelope; (q0, d0) = entry m-2, (q1, d1) = entry m-1 ----
+    // ---- lower envelope; (q0, f0) = entry m-2, (q1, f1) = entry m-1 ----
     int m = 0;
-    int q0 = 0, d0 = 0, q1 = 0, d1 = 0;
+    double q0 = 0, f0 = 0, q1 = 0, f1 = 0;
     pba_load_chunk<AXIS>(pre, in, g, fixed, c0, ncols, 0);
     for (int r0 = 0; r0 < g; r0 += PBA_CHUNK) {
         const int nq = min(PBA_CHUNK, g - r0);
@@ -394,20 +403,20 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
             for (int qq = 0; qq < nq; qq++) {
                 const int s = L.tile[qq][lane];
                 if (s == G4D_MARKER) continue;
-                int qs, ds;
-                site_pos_off<AXIS, IDS>(s, vox, ca, cb, &qs, &ds);
-                while (m >= 2 && env_dominated(q0, d0, q1, d1, qs, ds)) {
+                double qs, fs;
+                site_qf<AXIS, IDS>(s, vox, ca, cb, &qs, &fs);
+                while (m >= 2 && env_dominated(q0, f0, q1, f1, qs, fs)) {
                     m--;
                     q1 = q0;
-                    d1 = d0;
-                    if (m >= 2) site_pos_off<AXIS, IDS>(get(m - 2), vox, ca, cb, &q0, &d0);
+                    f1 = f0;
+                    if (m >= 2) site_qf<AXIS, IDS>(get(m - 2), vox, ca, cb, &q0, &f0);
                 }
                 if (m < PBA_STACK) L.scode[m][lane] = s;
                 else sp[(int64_t)(m - PBA_STACK) * PBA_COLS] = s;
                 q0 = q1;
-                d0 = d1;
+                f0 = f1;
                 q1 = qs;
-                d1 = ds;
+                f1 = fs;
                 m++;
             }
         }
@@ -443,8 +452,8 @@ __global__ void __launch_bounds__(64) k_pba_pass(const int *in_all, int *out_all
                     continue;
                 }
                 while (idx + 1 < m) {
-                    const int fa = (t - qa) * (t - qa) + da;
-                    const int fb = (t - qb) * (t - qb) + db;
+                    const int fa = __mul24(t - qa, t - qa) + da;
+                    const int fb = __mul24(t - qb, t - qb) + db;
                     if (fb <= fa) {
                         idx++;
                         qa = qb;
@@ -530,7 +539,7 @@ __global__ void __launch_bounds__(64) k_pba_band_env(const int *in_all, GridSets
 #define get(k) ((k) < PBA_BAND_STACK ? L.scode[(k)][lane] : pba_spill_get(sp, (k), PBA_BAND_STACK))
     int pre[PBA_PER_LANE];
     int m = 0;
-    int q0 = 0, d0 = 0, q1 = 0, d1 = 0;
+    double q0 = 0, f0 = 0, q1 = 0, f1 = 0;
     if (rb < re) pba_load_chunk<AXIS>(pre, in, g, fixed, c0, ncols, rb);
     for (int r0 = rb; r0 < re; r0 += PBA_CHUNK) {
         const int nq = min(PBA_CHUNK, re - r0);
@@ -545,20 +554,20 @@ __global__ void __launch_bounds__(64) k_pba_band_env(const int *in_all, GridSets
             for (int qq = 0; qq < nq; qq++) {
                 const int sv = L.tile[qq][lane];
                 if (sv == G4D_MARKER) continue;
-                int qs, ds;
-                site_pos_off<AXIS, IDS>(sv, vox, ca, cb, &qs, &ds);
-                while (m >= 2 && env_dominated(q0, d0, q1, d1, qs, ds)) {
+                double qs, fs;
+                site_qf<AXIS, IDS>(sv, vox, ca, cb, &qs, &fs);
+                while (m >= 2 && env_dominated(q0, f0, q1, f1, qs, fs)) {
                     m--;
                     q1 = q0;
-                    d1 = d0;
-                    if (m >= 2) site_pos_off<AXIS, IDS>(get(m - 2), vox, ca, cb, &q0, &d0);
+                    f1 = f0;
+                    if (m >= 2) site_qf<AXIS, IDS>(get(m - 2), vox, ca, cb, &q0, &f0);
                 }
                 if (m < PBA_BAND_STACK) L.scode[m][lane] = sv;
                 else sp[(int64_t)(m - PBA_BAND_STACK) * PBA_COLS] = sv;
                 q0 = q1;
-                d0 = d1;
+                f0 = f1;
                 q1 = qs;
-                d1 = ds;
+                f1 = fs;
                 m++;
             }
         }
@@ -593,26 +602,26 @@ __global__ void __launch_bounds__(64) k_pba_band_merge(GridSets S, int *spill_al
     int *ev = benv_all + base;
 #define get(k) ((k) < STACK ? scode[(k)][lane] : pba_spill_get(sp, (k), STACK))
     int m = 0;
-    int q0 = 0, d0 = 0, q1 = 0, d1 = 0;
+    double q0 = 0, f0 = 0, q1 = 0, f1 = 0;
     for (int band = 0; band < bands; band++) {
         const int nb = band * Lb < g ? bcnt[(bb * bands + band) * PBA_COLS + lane] : 0;
         const int *bev = ev + (int64_t)band * PBA_COLS * Lb;
         for (int k = 0; k < nb; k++) {
             const int sv = bev[(int64_t)k * PBA_COLS];
-            int qs, ds;
-            site_pos_off<AXIS, IDS>(sv, vox, ca, cb, &qs, &ds);
-            while (m >= 2 && env_dominated(q0, d0, q1, d1, qs, ds)) {
+            double qs, fs;
+            site_qf<AXIS, IDS>(sv, vox, ca, cb, &qs, &fs);
+            while (m >= 2 && env_dominated(q0, f0, q1, f1, qs, fs)) {
                 m--;
                 q1 = q0;
-                d1 = d0;
-                if (m >= 2) site_pos_off<AXIS, IDS>(get(m - 2), vox, ca, cb, &q0, &d0);
+                f1 = f0;
+                if (m >= 2) site_qf<AXIS, IDS>(get(m - 2), vox, ca, cb, &q0, &f0);
             }
             if (m < STACK) scode[m][lane] = sv;
             else sp[(int64_t)(m - STACK) * PBA_COLS] = sv;
             q0 = q1;
-            d0 = d1;
+            f0 = f1;
             q1 = qs;
-            d1 = ds;
+            f1 = fs;
             m++;
         }
     }
@@ -659,7 +668,7 @@ __global__ void __launch_bounds__(64) k_pba_band_color(GridSets S, const int *be
             int qa, da, qb, db;
             site_pos_off<AXIS, IDS>(ev[(int64_t)mid * PBA_COLS], vox, ca, cb, &qa, &da);
             site_pos_off<AXIS, IDS>(ev[(int64_t)(mid + 1) * PBA_COLS], vox, ca, cb, &qb, &db);
-            const int fa = (rb - qa) * (rb - qa) + da, fb = (rb - qb) * (rb - qb) + db;
+            const int fa = __mul24(rb - qa, rb - qa) + da, fb = __mul24(rb - qb, rb - qb) + db;
             if (fb > fa) hi = mid;
             else lo = mid + 1;
         }
@@ -687,8 +696,8 @@ __global__ void __launch_bounds__(64) k_pba_band_color(GridSets S, const int *be
                     continue;
                 }
                 while (idx + 1 < m) {
-                    const int fa = (t - qa) * (t - qa) + da;
-                    const int fb = (t - qb) * (t - qb) + db;
+                    const int fa = __mul24(t - qa, t - qa) + da;
+                    const int fb = __mul24(t - qb, t - qb) + db;
                     if (fb <= fa) {
                         idx++;
                         qa = qb;

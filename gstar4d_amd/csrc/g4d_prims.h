@@ -103,6 +103,23 @@ hipError_t radix_sort_u64(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int h
 hipError_t radix_sort_u64_2(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int hi_bit, int lo2, int hi2,
                             hipStream_t s);
 
+// Sort of directed pair keys a << 32 | b (a, b < 2^nb) by a, then b: a scatter into buckets of
+// consecutive a values and one LDS sort per bucket; nids (0: unknown) is the number of values a
+// takes and sizes the buckets.  Lists below 4 M keys, and ids wider than 23 bits, go through
+// radix_sort_u64_2 instead.  Result in keys.
+hipError_t sort_pair_keys(Prims &P, uint64_t *keys, int64_t m, int nb, int64_t nids, hipStream_t s);
+// sort_pair_keys, then removal of adjacent duplicates (count via host sync); the sorted list is
+// compacted from the sort's scratch buffer straight into keys.  keys must not be P.sort_tmp_keys.
+hipError_t sort_unique_pair_keys(Prims &P, uint64_t *keys, int64_t m, int nb, int64_t nids, int64_t *m_out,
+                                 hipStream_t s);
+// Bucket shift sort_pair_keys uses for m keys (buckets of 2^shift ids), -1 for the LSD sort.
+int pair_sort_shift(int64_t m, int nb, int64_t nids);
+// The same sort for keys a << ash | b with a < 2^na, b < 2^nbb and zeros between (nbb <= ash):
+// by a, then b.  force: the bucket sort also below its minimum length.
+hipError_t sort_keys_ab(Prims &P, uint64_t *keys, int64_t m, int ash, int na, int nbb, int64_t nids, bool force,
+                        hipStream_t s);
+int key_sort_shift(int64_t m, int na, int nbb, int64_t nids, bool force);
+
 // Merge of two sorted lists into out (no aliasing); equal keys keep a's before b's.
 hipError_t merge_u64(const uint64_t *a, int64_t na, const uint64_t *b, int64_t nb, uint64_t *out, hipStream_t s);
 

@@ -7,6 +7,8 @@
 // sorted list, and single-work-group versions of the sort and the compaction for short lists.
 #include "g4d_prims.h"
 
+#include <algorithm>
+
 namespace g4d {
 
 static constexpr int SCAN_THREADS = 256;
@@ -367,6 +369,376 @@ hipError_t radix_sort_u64_2(Prims &P, uint64_t *keys, int64_t n, int lo_bit, int
     }
     if (src != keys) HIP_CHECK_RET(hipMemcpyAsync(keys, src, sizeof(uint64_t) * (size_t)n, hipMemcpyDeviceToDevice, s));
     return hipSuccess;
+}
+
+// ------------------------------------------------------------------------------------
+// Sort of keys a << ash | b (a < 2^na, b < 2^nbb, the bits between zero) by buckets of
+// consecutive a: directed pairs a << 32 | b, and the drowned-insertion keys.  The LSD sort
+// above moves every key once per 8-bit digit (six times for 22-bit ids, with a histogram read
+// before each).  Here a key moves twice: a scatter into the bucket of its 2^sb consecutive a
+// values (bucket offsets from a count pass over the same buckets; a work-group's place inside
+// each bucket it touches is reserved with one atomic), then one work-group per bucket sorts it
+// in LDS by its remaining sb + nbb <= 32 bits and writes it back in order.  The
+// bucket width is chosen so that buckets average a few thousand keys; a bucket beyond the LDS
+// capacity (stars with very long lists) is sorted by its work-group through global memory
+// instead.  The order inside a bucket after the scatter depends on the atomics; the final
+// order does not (keys end sorted by their whole value).
+// ------------------------------------------------------------------------------------
+static constexpr int BK_THREADS = 1024;
+static constexpr int BK_MAX_BUCKETS = 16384;
+static constexpr int BK_TILE = 16384;  // keys per work-group of the count and scatter passes
+static constexpr int BK_TARGET = 4096;        // average bucket aimed for
+static constexpr int64_t BK_MIN_KEYS = 1 << 22; // shorter lists take the LSD sort
+
+__device__ __forceinline__ uint32_t bk_bucket(uint64_t key, int sh, int nbkt)
+{
+    return min((uint32_t)(key >> sh), (uint32_t)(nbkt - 1)); // keys outside the contract stay in bounds
+}
+
+// The count and scatter passes take tiles of `tile` keys (a multiple of BK_THREADS up to
+// BK_TILE), shorter for short lists so that the launch still fills the chip.
+__global__ void __launch_bounds__(BK_THREADS) k_bk_count(const uint64_t *keys, int64_t m, int sh, int nbkt, int tile,
+                                                         uint32_t *count)
+{
+    __shared__ uint32_t h[BK_MAX_BUCKETS];
+    for (int i = threadIdx.x; i < nbkt; i += BK_THREADS) h[i] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * tile;
+    const int R = tile / BK_THREADS;
+#pragma unroll 4
+    for (int r = 0; r < R; r++) {
+        const int64_t k = base + (int64_t)r * BK_THREADS + threadIdx.x;
+        if (k < m) atomicAdd(&h[bk_bucket(keys[k], sh, nbkt)], 1u);
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < nbkt; i += BK_THREADS)
+        if (h[i]) atomicAdd(&count[i], h[i]);
+}
+
+__global__ void __launch_bounds__(BK_THREADS) k_bk_scatter(const uint64_t *keys, int64_t m, int sh, int nbkt,
+                                                           int tile, uint32_t *cursor, uint64_t *out)
+{
+    constexpr int R = BK_TILE / BK_THREADS;
+    __shared__ uint32_t h[BK_MAX_BUCKETS];
+    for (int i = threadIdx.x; i < nbkt; i += BK_THREADS) h[i] = 0;
+    __syncthreads();
+    const int64_t base = (int64_t)blockIdx.x * tile;
+    const int64_t end = min(m, base + tile);
+    uint64_t kv[R];
+    uint32_t rk[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int64_t k = base + (int64_t)r * BK_THREADS + threadIdx.x;
+        kv[r] = k < end ? keys[k] : 0ull;
+        rk[r] = k < end ? atomicAdd(&h[bk_bucket(kv[r], sh, nbkt)], 1u) : 0u;
+    }
+    __syncthreads();
+    // this work-group's place in each bucket it touches
+    for (int i = threadIdx.x; i < nbkt; i += BK_THREADS)
+        if (h[i]) h[i] = atomicAdd(&cursor[i], h[i]);
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int64_t k = base + (int64_t)r * BK_THREADS + threadIdx.x;
+        if (k < end) out[h[bk_bucket(kv[r], sh, nbkt)] + rk[r]] = kv[r];
+    }
+}
+
+// Bits of a key below its bucket: (a mod 2^sb) << nbb | b.
+__device__ __forceinline__ uint32_t bk_local(uint64_t key, int ash, uint32_t smask, int nbb, uint32_t bmask)
+{
+    return (((uint32_t)(key >> ash) & smask) << nbb) | ((uint32_t)key & bmask);
+}
+
+// Per-bucket sort: a work-group of T threads (T / 64 waves) holds up to T * BKS_ITEMS keys in
+// registers, BKS_ITEMS per lane.  256-thread groups (8,192 keys) for buckets of a few thousand
+// keys, 512-thread groups (16,384) for larger ones: several buckets share a CU, so one bucket's
+// loads and stores overlap another's digit passes.
+static constexpr int BKS_ITEMS = 32;
+
+template <int T>
+struct BkLds {
+    static constexpr int W = T / 64;
+    uint16_t wcnt[W][256]; // per wave and digit: count, then the prefix over earlier waves
+    uint32_t dbase[256];   // keys of smaller digits
+    uint32_t dcnt[256];    // keys of the digit
+    uint32_t part[4];
+};
+
+// Stable ranking of one 8-bit digit of up to T * BKS_ITEMS keys held (wave, item, lane) in key
+// order, nvw of them valid in this wave (a prefix): rank[r] = the key's place among the keys of
+// its digit (ballot-matched peers within the wave, then per-wave counts); L.dbase and L.dcnt
+// describe the digits.  Every thread of the work-group must call it.
+template <int T>
+__device__ __forceinline__ void bk_rank_digit(const uint32_t (&kv)[BKS_ITEMS], int shift, int nvw,
+                                              uint32_t (&rank)[BKS_ITEMS], BkLds<T> &L)
+{
+    constexpr int W = T / 64;
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    for (int i = threadIdx.x; i < W * 256; i += T) (&L.wcnt[0][0])[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < BKS_ITEMS; r++) {
+        if (r * 64 >= nvw) break; // wave-uniform
+        const bool valid = r * 64 + lane < nvw;
+        const int d = (int)((kv[r] >> shift) & 255u);
+        uint64_t peers = __ballot(valid);
+#pragma unroll
+        for (int b = 0; b < 8; b++) {
+            const bool bit = (d >> b) & 1;
+            const uint64_t bal = __ballot(bit);
+            peers &= bit ? bal : ~bal;
+        }
+        const uint32_t before = valid ? L.wcnt[wid][d] : 0u;
+        rank[r] = before + (uint32_t)__popcll(peers & lt);
+        __builtin_amdgcn_wave_barrier();
+        if (valid && (peers & lt) == 0) L.wcnt[wid][d] = (uint16_t)(before + (uint32_t)__popcll(peers));
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // threads 0..255: one digit each (T >= 256)
+    uint32_t run = 0, inc = 0;
+    if (threadIdx.x < 256) {
+        const int d = threadIdx.x;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            const uint32_t t = L.wcnt[w][d];
+            L.wcnt[w][d] = (uint16_t)run;
+            run += t;
+        }
+        L.dcnt[d] = run;
+        inc = run;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(inc, o, 64);
+            if (lane >= o) inc += t;
+        }
+        if (lane == 63) L.part[wid] = inc;
+    }
+    __syncthreads();
+    if (threadIdx.x < 256) {
+        uint32_t wb = 0;
+        for (int w = 0; w < wid; w++) wb += L.part[w];
+        L.dbase[threadIdx.x] = wb + inc - run;
+    }
+#pragma unroll
+    for (int r = 0; r < BKS_ITEMS; r++) {
+        if (r * 64 >= nvw) break;
+        if (r * 64 + lane < nvw) rank[r] += L.wcnt[wid][(kv[r] >> shift) & 255u];
+    }
+    __syncthreads();
+}
+
+// A tile of n <= T * BKS_ITEMS keys is laid out (wave, item, lane) with ipw = ceil(n / T) items
+// per lane, so that every wave has work; the valid keys of a wave are a prefix of its ipw * 64
+// places.
+__device__ __forceinline__ int bk_wave_valid(uint32_t n, int ipw, int wid)
+{
+    const uint32_t lo = (uint32_t)(wid * ipw * 64);
+    return n > lo ? (int)min((uint32_t)(ipw * 64), n - lo) : 0;
+}
+
+// Bucket blockIdx.x holds src[off[b] .. off[b+1]); a bucket of at most T * BKS_ITEMS keys is
+// sorted into dst at the same places, or back into src when to_src.  Longer buckets are left to
+// k_bk_sort_big (a separate kernel, so that its registers do not cost this one occupancy).
+template <int T>
+__global__ void __launch_bounds__(T) k_bk_sort(uint64_t *src, uint64_t *dst, const uint32_t *off, int ash, int sb,
+                                               int nbb, int to_src)
+{
+    constexpr int CAP = T * BKS_ITEMS;
+    __shared__ uint32_t buf[CAP];
+    __shared__ BkLds<T> L;
+    const uint32_t o = off[blockIdx.x], sz = off[blockIdx.x + 1] - o;
+    if (sz == 0 || sz > (uint32_t)CAP) return; // work-group-uniform
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    const uint32_t smask = (1u << sb) - 1u, bmask = (1u << nbb) - 1u;
+    const int bits = sb + nbb;
+    const uint64_t hi = (uint64_t)blockIdx.x << sb; // the bucket's a values above the local bits
+    uint64_t *res = (to_src ? src : dst) + o;
+    uint32_t kv[BKS_ITEMS], rank[BKS_ITEMS];
+    const int ipw = (int)((sz + T - 1) / T);
+    const int nvw = bk_wave_valid(sz, ipw, wid);
+    const int tbase = wid * ipw * 64 + lane;
+#pragma unroll
+    for (int r = 0; r < BKS_ITEMS; r++)
+        kv[r] = r * 64 + lane < nvw ? bk_local(src[o + tbase + 64 * r], ash, smask, nbb, bmask) : 0u;
+    for (int shift = 0; shift < bits; shift += 8) {
+        bk_rank_digit<T>(kv, shift, nvw, rank, L);
+#pragma unroll
+        for (int r = 0; r < BKS_ITEMS; r++)
+            if (r * 64 + lane < nvw) buf[L.dbase[(kv[r] >> shift) & 255u] + rank[r]] = kv[r];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < BKS_ITEMS; r++)
+            if (r * 64 + lane < nvw) kv[r] = buf[tbase + 64 * r];
+        __syncthreads();
+    }
+    // every key of the bucket is in registers by now
+#pragma unroll
+    for (int r = 0; r < BKS_ITEMS; r++)
+        if (r * 64 + lane < nvw) res[tbase + 64 * r] = ((hi | (kv[r] >> nbb)) << ash) | (kv[r] & bmask);
+}
+
+// Buckets longer than cap keys: digit passes through global memory, alternating between the
+// bucket's places in src and dst, each a bucket-wide digit histogram and then tiles of T *
+// BKS_ITEMS keys in order, each ranked stably and written after the tiles before it.  The
+// result ends where k_bk_sort puts it.  Work-groups of other buckets leave at once.
+template <int T>
+__global__ void __launch_bounds__(T) k_bk_sort_big(uint64_t *src, uint64_t *dst, const uint32_t *off, int ash, int sb,
+                                                   int nbb, int to_src, uint32_t cap)
+{
+    constexpr int CAP = T * BKS_ITEMS;
+    __shared__ uint32_t goff[256]; // running digit offsets
+    __shared__ BkLds<T> L;
+    const uint32_t o = off[blockIdx.x], sz = off[blockIdx.x + 1] - o;
+    if (sz <= cap) return; // work-group-uniform
+    const int lane = __lane_id(), wid = threadIdx.x >> 6;
+    const uint32_t smask = (1u << sb) - 1u, bmask = (1u << nbb) - 1u;
+    const int bits = sb + nbb;
+    uint64_t *res = (to_src ? src : dst) + o;
+    uint64_t *a = src + o, *q = dst + o;
+    uint32_t kv[BKS_ITEMS], rank[BKS_ITEMS];
+    const int tbase = wid * BKS_ITEMS * 64 + lane;
+    for (int shift = 0; shift < bits; shift += 8) {
+        if (threadIdx.x < 256) goff[threadIdx.x] = 0;
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < sz; i += T)
+            atomicAdd(&goff[(bk_local(a[i], ash, smask, nbb, bmask) >> shift) & 255u], 1u);
+        __syncthreads();
+        uint32_t c = 0, inc = 0;
+        if (threadIdx.x < 256) {
+            c = goff[threadIdx.x];
+            inc = c;
+#pragma unroll
+            for (int t = 1; t < 64; t <<= 1) {
+                const uint32_t u = __shfl_up(inc, t, 64);
+                if (lane >= t) inc += u;
+            }
+            if (lane == 63) L.part[wid] = inc;
+        }
+        __syncthreads();
+        if (threadIdx.x < 256) {
+            uint32_t wb = 0;
+            for (int w = 0; w < wid; w++) wb += L.part[w];
+            goff[threadIdx.x] = wb + inc - c;
+        }
+        __syncthreads();
+        for (uint32_t t0 = 0; t0 < sz; t0 += CAP) {
+            const int nvw = bk_wave_valid(min((uint32_t)CAP, sz - t0), BKS_ITEMS, wid);
+#pragma unroll
+            for (int r = 0; r < BKS_ITEMS; r++)
+                kv[r] = r * 64 + lane < nvw ? bk_local(a[t0 + tbase + 64 * r], ash, smask, nbb, bmask) : 0u;
+            bk_rank_digit<T>(kv, shift, nvw, rank, L);
+            // the key is rebuilt from its local bits and the bucket (as k_bk_sort writes it)
+            const uint64_t hi = (uint64_t)blockIdx.x << sb;
+#pragma unroll
+            for (int r = 0; r < BKS_ITEMS; r++)
+                if (r * 64 + lane < nvw)
+                    q[goff[(kv[r] >> shift) & 255u] + rank[r]] = ((hi | (kv[r] >> nbb)) << ash) | (kv[r] & bmask);
+            __syncthreads();
+            if (threadIdx.x < 256) goff[threadIdx.x] += L.dcnt[threadIdx.x];
+            __syncthreads();
+        }
+        uint64_t *t = a;
+        a = q;
+        q = t;
+    }
+    if (a != res)
+        for (uint32_t i = threadIdx.x; i < sz; i += T) res[i] = a[i];
+}
+
+// Bucket shift for m keys over nids values of a (0: unknown, taken as 2^(na-1)), or -1 when the
+// bucket sort does not apply: lists below BK_MIN_KEYS (unless forced), or 2^14 buckets that
+// would not leave 32-bit local keys.  The widest buckets that average at most BK_TARGET keys,
+// else the narrowest allowed.
+int key_sort_shift(int64_t m, int na, int nbb, int64_t nids, bool force)
+{
+    if (m <= SMALL_SORT || m > (int64_t)0xFFFFFFFF || na < 1 || na > 31 || nbb < 1 || nbb > 32) return -1;
+    if (!force && m < BK_MIN_KEYS) return -1;
+    const int lo = std::max(0, na - 14), hi = std::min(na, 32 - nbb);
+    if (lo > hi) return -1;
+    const int64_t ids = nids > 0 ? nids : (int64_t)1 << (na - 1);
+    int sb = lo;
+    for (int t = hi; t > lo; t--) {
+        const int64_t nb = std::max<int64_t>(1, (ids + ((int64_t)1 << t) - 1) >> t);
+        if (m <= (int64_t)BK_TARGET * nb) {
+            sb = t;
+            break;
+        }
+    }
+    return sb;
+}
+
+int pair_sort_shift(int64_t m, int nb, int64_t nids) { return key_sort_shift(m, nb, nb, nids, false); }
+
+// Bucket sort proper; the result ends in P.sort_tmp_keys when to_tmp, else in keys.
+static hipError_t bucket_sort(Prims &P, uint64_t *keys, int64_t m, int ash, int na, int nbb, int sb, int64_t nids,
+                              bool to_tmp, hipStream_t s)
+{
+    const int nbkt = 1 << (na - sb);
+    HIP_CHECK_RET(P.sort_tmp_keys.ensure(sizeof(uint64_t) * (size_t)m));
+    HIP_CHECK_RET(P.sort_hist.ensure(sizeof(uint32_t) * (size_t)(2 * nbkt + 2)));
+    uint64_t *tmp = P.sort_tmp_keys.as<uint64_t>();
+    uint32_t *count = P.sort_hist.as<uint32_t>(), *off = count + nbkt + 1;
+    HIP_CHECK_RET(hipMemsetAsync(count, 0, sizeof(uint32_t) * (size_t)(nbkt + 1), s));
+    // tiles of up to BK_TILE keys, at least ~512 of them
+    const int64_t want = (m + 511) / 512;
+    const int tile = (int)std::min<int64_t>(
+        BK_TILE, std::max<int64_t>(BK_THREADS, (want + BK_THREADS - 1) / BK_THREADS * BK_THREADS));
+    const int64_t nblk = (m + tile - 1) / tile;
+    const int sh = ash + sb;
+    hipLaunchKernelGGL(k_bk_count, dim3((unsigned)nblk), dim3(BK_THREADS), 0, s, keys, m, sh, nbkt, tile, count);
+    HIP_CHECK_RET(hipGetLastError());
+    HIP_CHECK_RET(scan_excl_u32(P, count, off, nbkt + 1, s)); // off[nbkt] = m
+    HIP_CHECK_RET(hipMemcpyAsync(count, off, sizeof(uint32_t) * (size_t)nbkt, hipMemcpyDeviceToDevice, s));
+    hipLaunchKernelGGL(k_bk_scatter, dim3((unsigned)nblk), dim3(BK_THREADS), 0, s, keys, m, sh, nbkt, tile, count,
+                       tmp);
+    HIP_CHECK_RET(hipGetLastError());
+    // the smaller work-group when the average bucket is at most half its capacity
+    const int64_t ids = nids > 0 ? nids : (int64_t)1 << (na - 1);
+    const int64_t used = std::max<int64_t>(1, (ids + ((int64_t)1 << sb) - 1) >> sb);
+    uint32_t cap;
+    if (m <= (int64_t)128 * BKS_ITEMS * used) {
+        hipLaunchKernelGGL(k_bk_sort<256>, dim3((unsigned)nbkt), dim3(256), 0, s, tmp, keys, off, ash, sb, nbb,
+                           to_tmp ? 1 : 0);
+        cap = 256 * BKS_ITEMS;
+    } else {
+        hipLaunchKernelGGL(k_bk_sort<512>, dim3((unsigned)nbkt), dim3(512), 0, s, tmp, keys, off, ash, sb, nbb,
+                           to_tmp ? 1 : 0);
+        cap = 512 * BKS_ITEMS;
+    }
+    HIP_CHECK_RET(hipGetLastError());
+    // buckets beyond LDS, if any (the others' work-groups leave at once)
+    hipLaunchKernelGGL(k_bk_sort_big<512>, dim3((unsigned)nbkt), dim3(512), 0, s, tmp, keys, off, ash, sb, nbb,
+                       to_tmp ? 1 : 0, cap);
+    return hipGetLastError();
+}
+
+hipError_t sort_keys_ab(Prims &P, uint64_t *keys, int64_t m, int ash, int na, int nbb, int64_t nids, bool force,
+                        hipStream_t s)
+{
+    if (m <= 1) return hipSuccess;
+    const int sb = key_sort_shift(m, na, nbb, nids, force);
+    if (sb < 0 || ash < nbb || ash + na > 64) return radix_sort_u64_2(P, keys, m, 0, nbb, ash, ash + na, s);
+    return bucket_sort(P, keys, m, ash, na, nbb, sb, nids, false, s);
+}
+
+hipError_t sort_pair_keys(Prims &P, uint64_t *keys, int64_t m, int nb, int64_t nids, hipStream_t s)
+{
+    return sort_keys_ab(P, keys, m, 32, nb, nb, nids, false, s);
+}
+
+hipError_t sort_unique_pair_keys(Prims &P, uint64_t *keys, int64_t m, int nb, int64_t nids, int64_t *m_out,
+                                 hipStream_t s)
+{
+    const int sb = key_sort_shift(m, nb, nb, nids, false);
+    if (m <= 1 || sb < 0) {
+        HIP_CHECK_RET(radix_sort_u64_2(P, keys, m, 0, nb, 32, 32 + nb, s));
+        return unique_u64(P, keys, m, m_out, s);
+    }
+    HIP_CHECK_RET(bucket_sort(P, keys, m, 32, nb, nb, sb, nids, true, s));
+    return unique_copy_u64(P, P.sort_tmp_keys.as<uint64_t>(), keys, m, m_out, s);
 }
 
 // ------------------------------------------------------------------------------------

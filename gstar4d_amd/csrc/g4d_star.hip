@@ -1390,9 +1390,20 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
             }
             // Triangles the filter cannot decide are gathered into one list (be_slot is free until
             // the hole boundary) and decided exactly one per thread, so that the block's threads
-            // share the exact work instead of each wave stalling on its own few slots.
-            int nunc_tot;
-            int ub = block_excl(nunc, &nunc_tot, red);
+            // share the exact work instead of each wave stalling on its own few slots.  Below 2^15
+            // slots one scan serves both counts (each total is at most T), packed in 16-bit halves.
+            int vbase, nvis, nunc_tot, ub;
+            if (T < 32768) {
+                int both;
+                const int pre = block_excl(nv | (nunc << 16), &both, red);
+                vbase = pre & 0xffff;
+                nvis = both & 0xffff;
+                ub = pre >> 16;
+                nunc_tot = both >> 16;
+            } else {
+                ub = block_excl(nunc, &nunc_tot, red);
+                vbase = block_excl(nv, &nvis, red);
+            }
             if (nunc_tot > 0) {
                 for (int slot = tid; slot < T && nunc > 0; slot += GSTAR_NT)
                     if (L.tfl[slot] & TF_UNSURE) {
@@ -1414,9 +1425,8 @@ __global__ void __launch_bounds__(GSTAR_NT) k_star_global(StarParams p, const in
                 // each thread counts the beneath triangles among its own slots again
                 nv = 0;
                 for (int slot = tid; slot < T; slot += GSTAR_NT) nv += (L.tfl[slot] & TF_VIS) ? 1 : 0;
+                vbase = block_excl(nv, &nvis, red);
             }
-            int nvis;
-            const int vbase = block_excl(nv, &nvis, red);
             if (nvis == 0) continue; // drowned at insertion
             if (tid == 0) L.ins_local[q] = Vn;
             __syncthreads();

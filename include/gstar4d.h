@@ -325,6 +325,14 @@ int g4d_diag_copy(g4d_ctx *ctx, int64_t bytes, double *ms);
 /* Overwrites link vertex `local` of star `star` (Morton index space) of the last result with
    point id v: plants a defect for the star-consistency tests. */
 int g4d_diag_set_link_vertex(g4d_ctx *ctx, int32_t star, int32_t local, int32_t v);
+/* Sorts m directed pair keys a<<32|b (a, b < 2^nb) by a, then b: method 0 as the pipeline does
+   (the bucket sort for long lists: scatter into buckets of consecutive a, one LDS sort per
+   bucket; the LSD radix sort otherwise), 1 the LSD radix sort, 3 the bucket sort at any length;
+   methods 2 and 4 are 0 and 3 for drowned-insertion keys a<<33|b (b < 2^(nb+1)).  *ms (may be
+   NULL): device time of the sort; *shift (may be NULL): the bucket width log2 chosen, -1 where
+   the LSD sort runs. */
+int g4d_diag_sort_pairs(g4d_ctx *ctx, const uint64_t *keys, int64_t m, int32_t nb, int32_t method, uint64_t *out,
+                        double *ms, int32_t *shift);
 /* Sorted unique Voronoi-neighbour pairs (min<<32|max) from an id grid. */
 int g4d_diag_grid_pairs(g4d_ctx *ctx, const int32_t *id_grid, int32_t grid, uint64_t *out, int64_t cap,
                         int64_t *count);
