@@ -109,6 +109,11 @@ struct g4d_ctx {
     float prop_threshold = 0.f;
     int samples_n = 0;
     PinnedBuf h_tets, h_opp, h_perm; // results of g4d_triangulate / g4d_triangulate_batch
+    // Page-locked staging of the small reads back to the host (counters and counts): a pageable
+    // destination makes the runtime stage every such read itself, about 8 us more per read.
+    // Slots 0..23 the counters, 24..39 scalars, 40..41 the primitives' counts.
+    PinnedBuf hpin;
+    int64_t *hstage = nullptr; // in hpin, or the calling context's for a temporary context
     std::vector<int32_t> h_set_grid;
     std::vector<int64_t> h_set_tet_off;
     const double *last_xyz = nullptr; // device points of the last triangulation (input order)
@@ -237,9 +242,17 @@ static int guard_tets(g4d_ctx *ctx, int64_t nt)
 
 static int read_counters(g4d_ctx *ctx, unsigned long long *h)
 {
-    HCK(hipMemcpyAsync(h, ctx->cnt.ptr, sizeof(unsigned long long) * G4D_CNT_NUM, hipMemcpyDeviceToHost, ctx->stream));
+    unsigned long long *dst = ctx->hstage ? reinterpret_cast<unsigned long long *>(ctx->hstage) : h;
+    HCK(hipMemcpyAsync(dst, ctx->cnt.ptr, sizeof(unsigned long long) * G4D_CNT_NUM, hipMemcpyDeviceToHost, ctx->stream));
     HCK(hipStreamSynchronize(ctx->stream));
+    if (dst != h) memcpy(h, dst, sizeof(unsigned long long) * G4D_CNT_NUM);
     return G4D_OK;
+}
+
+// Scalar slot k (0..15) of the page-locked staging, or the caller's fallback variable.
+static int64_t *hslot(g4d_ctx *ctx, int k, int64_t *fallback)
+{
+    return ctx->hstage ? ctx->hstage + G4D_CNT_NUM + k : fallback;
 }
 
 static int check_dev_error(g4d_ctx *ctx, unsigned long long flags, const char *stage)
@@ -685,9 +698,10 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     uint2 *tcnt = reinterpret_cast<uint2 *>(ttot + ntiles + 1);
     HCK(g4d::launch_grid_pairs(idg, GS, nthr, pcnt, nullptr, false, s));
     HCK(g4d::scan_excl_i64(P, pcnt, pcnt, nthr, pcnt + nthr, s));
-    int64_t ngp = 0;
-    HCK(hipMemcpyAsync(&ngp, pcnt + nthr, 8, hipMemcpyDeviceToHost, s));
+    int64_t ngp_f = 0, *ngp_h = hslot(ctx, 0, &ngp_f);
+    HCK(hipMemcpyAsync(ngp_h, pcnt + nthr, 8, hipMemcpyDeviceToHost, s));
     HCK(hipStreamSynchronize(s));
+    const int64_t ngp = *ngp_h;
     CK(guard_keys(ctx, 2 * (ngp + (int64_t)n), "grid pairs"));
     const int64_t ucap = ngp + n;
     const int64_t prop_room = ctx->prop_on ? 2 * (int64_t)g4d::PROP_SLOTS * n : 0;
@@ -699,9 +713,10 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     HCK(g4d::launch_gather_tiles(pr + ucap, pcnt, tcnt, ntiles, ttot, pr, s));
     HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND0, 0, 8, s));
     HCK(g4d::launch_missing(vox, n, GS, idg, pr + ucap + ngp, cnt + G4D_CNT_APPEND0, n, s));
-    int64_t ngd = 0;
-    HCK(hipMemcpyAsync(&ngd, ttot + ntiles, 8, hipMemcpyDeviceToHost, s));
+    int64_t ngd_f = 0, *ngd_h = hslot(ctx, 1, &ngd_f);
+    HCK(hipMemcpyAsync(ngd_h, ttot + ntiles, 8, hipMemcpyDeviceToHost, s));
     CK(read_counters(ctx, hc));
+    const int64_t ngd = *ngd_h;
     CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], "morton"));
     const int64_t nmiss = (int64_t)hc[G4D_CNT_APPEND0];
     if (nmiss > 0)
@@ -788,17 +803,22 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
                          int64_t *tp_n) -> int {
         const int *list = nullptr;
         int nitems = n;
+        // sparse passes: the active list's length is known on the device first; the capacities
+        // and their scans run over the bound nins, and the length comes back with the needs
+        const int64_t *dev_na = nullptr;
+        int64_t na_f = 0, *na_h = hslot(ctx, 9, &na_f);
         if (mode == 1 && nins < n / 2) {
             // few insertions: offsets and active list from the runs of the sorted keys, O(keys)
             // instead of O(stars)
             HCK(ctx->active.ensure(sizeof(int) * (size_t)n));
             HCK(ctx->hkeys.ensure(sizeof(uint64_t) * (size_t)std::max<int64_t>(nins, 1)));
+            HCK(P.sort_tmp_keys.ensure(sizeof(uint64_t) * (size_t)std::max<int64_t>(nins, 1)));
             uint64_t *hk = buf<uint64_t>(ctx->hkeys);
             HCK(g4d::launch_sparse_runs(ins_keys, nins, const_cast<int64_t *>(ins_off), hk, s));
-            int64_t na = 0;
-            HCK(g4d::compact_valid_u64(P, hk, hk, nins, g4d::STAR_TASK_NONE, &na, s));
-            HCK(g4d::launch_u64_to_i32(hk, na, buf<int>(ctx->active), s));
-            nitems = (int)na;
+            uint64_t *runs = P.sort_tmp_keys.as<uint64_t>();
+            HCK(g4d::compact_valid_u64_deferred(P, hk, runs, nins, g4d::STAR_TASK_NONE, na_h, &dev_na, s));
+            HCK(g4d::launch_u64_to_i32(runs, nins, dev_na, buf<int>(ctx->active), s));
+            nitems = (int)nins; // bound until the needs come back
             list = buf<int>(ctx->active);
         } else if (mode == 1) {
             HCK(g4d::launch_csr_offsets(ins_keys, nins, n, const_cast<int64_t *>(ins_off), s));
@@ -809,7 +829,8 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
             HCK(g4d::launch_active_flags(ins_off, n, fl, s));
             HCK(g4d::scan_excl_u32(P, fl, pos, n, s));
             HCK(g4d::launch_active_scatter(fl, pos, n, buf<int>(ctx->active), s));
-            uint32_t last[2];
+            int64_t last_f = 0, *last_h = hslot(ctx, 2, &last_f);
+            uint32_t *last = reinterpret_cast<uint32_t *>(last_h);
             HCK(hipMemcpyAsync(&last[0], pos + n - 1, 4, hipMemcpyDeviceToHost, s));
             HCK(hipMemcpyAsync(&last[1], fl + n - 1, 4, hipMemcpyDeviceToHost, s));
             HCK(hipStreamSynchronize(s));
@@ -818,15 +839,18 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
         }
         HCK(ctx->spcount.ensure(sizeof(int) * (size_t)n));
         int *spc = buf<int>(ctx->spcount);
-        HCK(g4d::launch_capacities(list, nitems, mode ? buf<int>(store.vnum) : nullptr,
+        HCK(g4d::launch_capacities(list, nitems, dev_na, mode ? buf<int>(store.vnum) : nullptr,
                                    mode ? buf<int>(store.hnum) : nullptr, ins_off, vcap, tcap, hcap, spc, s));
         int64_t need[3] = {0, 0, 0};
         int64_t *caps[3] = {vcap, tcap, hcap};
+        int64_t *need_h = hslot(ctx, 3, need); // three slots
         for (int k = 0; k < 3; k++) {
             HCK(g4d::scan_excl_i64(P, caps[k], caps[k], nitems, caps[k] + nitems, s));
-            HCK(hipMemcpyAsync(&need[k], caps[k] + nitems, 8, hipMemcpyDeviceToHost, s));
+            HCK(hipMemcpyAsync(&need_h[k], caps[k] + nitems, 8, hipMemcpyDeviceToHost, s));
         }
         HCK(hipStreamSynchronize(s));
+        for (int k = 0; k < 3; k++) need[k] = need_h[k];
+        if (dev_na) nitems = (int)*na_h;
         if (mode == 0) {
             // room for the splaying rounds: their regions come to about twice the creation's;
             // with the proposal's predicted link sizes, room for that many rebuilt stars too
@@ -961,15 +985,18 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
             trace("global-memory star kernel: %lld stars, largest link plus insertions %d", (long long)nhuge, vcap_g);
         }
         CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], mode ? "splaying" : "star creation"));
-        // drowned insertions and triangle pairs, in star order
-        HCK(g4d::compact_u64(P, ins_keys, buf<uint8_t>(ctx->dflag), buf<uint64_t>(ctx->drown), nins, drown_n, s));
+        // drowned insertions and triangle pairs, in star order (both counts read back together)
+        int64_t cnt_f[2] = {0, 0}, *cnt_h = hslot(ctx, 6, cnt_f); // two slots
+        HCK(g4d::compact_u64_deferred(P, ins_keys, buf<uint8_t>(ctx->dflag), buf<uint64_t>(ctx->drown), nins,
+                                      &cnt_h[0], s));
         // each star wrote its pairs densely at the front of its region and their count: offsets
         // in list order (the vertex capacities are done with), then one copy per star
         HCK(g4d::launch_pair_counts(list, nitems, spc, vcap, s));
         HCK(g4d::scan_excl_i64(P, vcap, vcap, nitems, vcap + nitems, s));
-        int64_t npairs = 0;
-        HCK(hipMemcpyAsync(&npairs, vcap + nitems, 8, hipMemcpyDeviceToHost, s));
+        HCK(hipMemcpyAsync(&cnt_h[1], vcap + nitems, 8, hipMemcpyDeviceToHost, s));
         HCK(hipStreamSynchronize(s));
+        *drown_n = cnt_h[0];
+        const int64_t npairs = cnt_h[1];
         if (npairs > npb) return set_err(ctx, G4D_ERR_INTERNAL, "star pairs exceed their regions");
         HCK(g4d::launch_pair_gather(list, nitems, spc, vcap, buf<int64_t>(store.ntoff), tbase,
                                     buf<uint64_t>(ctx->pairbuf), buf<uint64_t>(ctx->cand), s));
@@ -1020,8 +1047,7 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
                 pp.nx = 0;
                 HCK(hipMemsetAsync(cnt + G4D_CNT_APPEND5, 0, 8, s));
                 HCK(g4d::launch_proofs(pp, false, s));
-                CK(read_counters(ctx, hc));
-                pp.nx = (int64_t)hc[G4D_CNT_APPEND5];
+                pp.nx = -1; // the exact pass reads the filter pass's count on the device
                 HCK(g4d::launch_proofs(pp, true, s));
             }
         }
@@ -1040,11 +1066,12 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
                 HCK(sort_unique_pairs(P, ins, &nins, nb, n, s));
             }
         }
-        CK(read_counters(ctx, hc));
         trace("round %d: %lld drowned, %lld proofs, %lld candidates, %lld new pairs", rounds + 1, (long long)nd,
               (long long)ntask, (long long)ncand, (long long)nins);
-        CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], "proofs"));
         if (nins == 0) {
+            // the proofs' errors; otherwise the star pass below reads them with its own counters
+            CK(read_counters(ctx, hc));
+            CK(check_dev_error(ctx, hc[G4D_CNT_ERROR], "proofs"));
             converged = true;
             break;
         }
@@ -1094,9 +1121,10 @@ static int run_device(g4d_ctx *ctx, const double *d_xyz, int n, const SetSpec &s
     HCK(g4d::launch_out_exact(op, buf<uint64_t>(ctx->xtris), nxt, s));
     int64_t *toffs = buf<int64_t>(ctx->tetoff);
     HCK(g4d::scan_excl_i64(P, buf<int64_t>(ctx->tetcnt), toffs, n, toffs + n, s));
-    int64_t nt = 0;
-    HCK(hipMemcpyAsync(&nt, toffs + n, 8, hipMemcpyDeviceToHost, s));
+    int64_t nt_f = 0, *nt_h = hslot(ctx, 8, &nt_f);
+    HCK(hipMemcpyAsync(nt_h, toffs + n, 8, hipMemcpyDeviceToHost, s));
     HCK(hipStreamSynchronize(s));
+    const int64_t nt = *nt_h;
     CK(guard_tets(ctx, nt));
     HCK(ctx->tets.ensure(sizeof(int) * 4 * (size_t)std::max<int64_t>(nt, 1)));
     op.tets = buf<int>(ctx->tets);
@@ -1215,6 +1243,11 @@ int g4d_create(g4d_ctx **out, const g4d_options *opt)
         if (hipEventCreate(&ctx->ev[i]) != hipSuccess) ctx->ev_ok = false;
     for (int i = 0; i < 2; i++)
         if (hipEventCreate(&ctx->ev_t0[i]) != hipSuccess) ctx->ev_ok = false;
+    // staging of the small host reads; without it they go to pageable variables
+    if (ctx->hpin.ensure(sizeof(int64_t) * 64) == hipSuccess) {
+        ctx->hstage = static_cast<int64_t *>(ctx->hpin.ptr);
+        ctx->P.hstage = ctx->hstage + G4D_CNT_NUM + 16;
+    }
     *out = ctx;
     return G4D_OK;
 }
@@ -1235,6 +1268,7 @@ void g4d_destroy(g4d_ctx *ctx)
     ctx->h_tets.release();
     ctx->h_opp.release();
     ctx->h_perm.release();
+    ctx->hpin.release();
     ctx->store.release();
     ctx->P.release();
     if (ctx->ev_ok)
@@ -1464,6 +1498,8 @@ int g4d_triangulate_device_noalloc(g4d_ctx *ctx, const double *d_xyz, int32_t n,
     tmp.pba_bands = ctx->pba_bands;
     tmp.stage_graphs = 0; // instantiating a graph allocates
     tmp.ev_ok = ctx->ev_ok;
+    tmp.hstage = ctx->hstage; // the caller's page-locked staging (allocated at g4d_create)
+    tmp.P.hstage = ctx->P.hstage;
     for (int i = 0; i < EV_NUM; i++) tmp.ev[i] = ctx->ev[i];
     for (int i = 0; i < 2; i++) tmp.ev_t0[i] = ctx->ev_t0[i];
     g4d::ScratchArena arena;

@@ -80,6 +80,9 @@ struct DevBuf {
 struct Prims {
     DevBuf scan_tmp[4];
     DevBuf sort_tmp_keys, sort_hist, flags, pos;
+    // Page-locked host slots (two) for the counts the compactions read back, owned by the
+    // caller; null: stack variables (a pageable destination costs the runtime a staged copy).
+    int64_t *hstage = nullptr;
     void release()
     {
         for (auto &b : scan_tmp) b.release();
@@ -130,6 +133,15 @@ hipError_t unique_u64(Prims &P, uint64_t *keys, int64_t n, int64_t *n_out, hipSt
 hipError_t unique_copy_u64(Prims &P, const uint64_t *in, uint64_t *out, int64_t n, int64_t *n_out, hipStream_t s);
 
 // Order-preserving compaction keep[i] != 0.  Count returned via host sync.
+// compact_u64 without the host synchronisation (in and out distinct): the count is copied to
+// *host_count (page-locked memory, or the copy blocks) once the stream reaches it.
+hipError_t compact_u64_deferred(Prims &P, const uint64_t *in, const uint8_t *keep, uint64_t *out, int64_t n,
+                                int64_t *host_count, hipStream_t s);
+// compact_valid_u64 from in to a distinct out without the host synchronisation (n > 0): the
+// count at *dev_count on the device (int64, valid until the next compaction) and, once the
+// stream reaches it, at *host_count (page-locked).
+hipError_t compact_valid_u64_deferred(Prims &P, const uint64_t *in, uint64_t *out, int64_t n, uint64_t none,
+                                      int64_t *host_count, const int64_t **dev_count, hipStream_t s);
 hipError_t compact_u64(Prims &P, const uint64_t *in, const uint8_t *keep, uint64_t *out, int64_t n,
                        int64_t *n_out, hipStream_t s);
 

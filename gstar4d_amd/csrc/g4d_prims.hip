@@ -856,7 +856,10 @@ __global__ void __launch_bounds__(1024) k_compact_small(const uint64_t *in, cons
         run += tot;
         __syncthreads();
     }
-    if (threadIdx.x == 0) *count = run;
+    if (threadIdx.x == 0) {
+        count[0] = run;
+        count[1] = 0; // the two words read as an int64 count too
+    }
 }
 
 template <int MODE>
@@ -868,9 +871,10 @@ static hipError_t compact_small(Prims &P, const uint64_t *in, const uint8_t *kee
     hipLaunchKernelGGL(k_compact_small<MODE>, dim3(1), dim3(1024), 0, s, in, keep, none, out, n, dcount);
     HIP_CHECK_RET(hipGetLastError());
     uint32_t c = 0;
-    HIP_CHECK_RET(hipMemcpyAsync(&c, dcount, 4, hipMemcpyDeviceToHost, s));
+    uint32_t *hc = P.hstage ? reinterpret_cast<uint32_t *>(P.hstage) : &c;
+    HIP_CHECK_RET(hipMemcpyAsync(hc, dcount, 4, hipMemcpyDeviceToHost, s));
     HIP_CHECK_RET(hipStreamSynchronize(s));
-    *n_out = c;
+    *n_out = *hc;
     return hipSuccess;
 }
 
@@ -960,11 +964,59 @@ static hipError_t compact_big(Prims &P, const uint64_t *in, const uint8_t *keep,
     hipLaunchKernelGGL(k_cmp_write<MODE>, dim3((unsigned)nt), dim3(256), 0, s, in, keep, none, n, tc, dst);
     HIP_CHECK_RET(hipGetLastError());
     int64_t m = 0;
-    HIP_CHECK_RET(hipMemcpyAsync(&m, tc + nt, 8, hipMemcpyDeviceToHost, s));
+    int64_t *hm = P.hstage ? P.hstage : &m;
+    HIP_CHECK_RET(hipMemcpyAsync(hm, tc + nt, 8, hipMemcpyDeviceToHost, s));
     HIP_CHECK_RET(hipStreamSynchronize(s));
+    m = *hm;
     if (dst != out && m > 0) HIP_CHECK_RET(hipMemcpyAsync(out, dst, sizeof(uint64_t) * (size_t)m, hipMemcpyDeviceToDevice, s));
     *n_out = m;
     return hipSuccess;
+}
+
+// The same kernels as compact_small / compact_big with in != out, the count left for the caller:
+// on the device at *dev_count (int64; valid until the next compaction) and, once the stream
+// reaches it, at *host_count (page-locked memory, or the copy blocks).
+template <int MODE>
+static hipError_t compact_deferred(Prims &P, const uint64_t *in, const uint8_t *keep, uint64_t none, uint64_t *out,
+                                   int64_t n, int64_t *host_count, const int64_t **dev_count, hipStream_t s)
+{
+    *host_count = 0;
+    if (n <= 0 || in == out) return hipErrorInvalidValue;
+    if (n <= SMALL_COMPACT) {
+        HIP_CHECK_RET(P.pos.ensure(sizeof(uint32_t) * 4));
+        uint32_t *dcount = P.pos.as<uint32_t>();
+        hipLaunchKernelGGL(k_compact_small<MODE>, dim3(1), dim3(1024), 0, s, in, keep, none, out, n, dcount);
+        HIP_CHECK_RET(hipGetLastError());
+        if (dev_count) *dev_count = reinterpret_cast<const int64_t *>(dcount);
+        return hipMemcpyAsync(host_count, dcount, 8, hipMemcpyDeviceToHost, s);
+    }
+    const int64_t nt = (n + CMP_TILE - 1) / CMP_TILE;
+    if (nt > INT32_MAX) return hipErrorInvalidValue;
+    HIP_CHECK_RET(P.flags.ensure(sizeof(int64_t) * (size_t)(nt + 1)));
+    int64_t *tc = P.flags.as<int64_t>();
+    hipLaunchKernelGGL(k_cmp_count<MODE>, dim3((unsigned)nt), dim3(256), 0, s, in, keep, none, n, tc);
+    HIP_CHECK_RET(hipGetLastError());
+    HIP_CHECK_RET(scan_excl_i64(P, tc, tc, nt, tc + nt, s));
+    hipLaunchKernelGGL(k_cmp_write<MODE>, dim3((unsigned)nt), dim3(256), 0, s, in, keep, none, n, tc, out);
+    HIP_CHECK_RET(hipGetLastError());
+    if (dev_count) *dev_count = tc + nt;
+    return hipMemcpyAsync(host_count, tc + nt, 8, hipMemcpyDeviceToHost, s);
+}
+
+hipError_t compact_u64_deferred(Prims &P, const uint64_t *in, const uint8_t *keep, uint64_t *out, int64_t n,
+                                int64_t *host_count, hipStream_t s)
+{
+    if (n <= 0) {
+        *host_count = 0;
+        return hipSuccess;
+    }
+    return compact_deferred<1>(P, in, keep, 0, out, n, host_count, nullptr, s);
+}
+
+hipError_t compact_valid_u64_deferred(Prims &P, const uint64_t *in, uint64_t *out, int64_t n, uint64_t none,
+                                      int64_t *host_count, const int64_t **dev_count, hipStream_t s)
+{
+    return compact_deferred<2>(P, in, nullptr, none, out, n, host_count, dev_count, s);
 }
 
 hipError_t unique_u64(Prims &P, uint64_t *keys, int64_t n, int64_t *n_out, hipStream_t s)

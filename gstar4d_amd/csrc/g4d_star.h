@@ -109,7 +109,7 @@ struct ProofParams {
     const uint64_t *tasks; // (star<<32 | vertex) drowned, non-mutual
     int64_t ntasks;
     int64_t *xlist;       // filter-only pass: tasks needing exact predicates, counter G4D_CNT_APPEND5;
-    int64_t nx;           // exact pass: the task indices to redo
+    int64_t nx;           // exact pass: the task indices to redo (< 0: their count from the counter)
     int64_t xcap;
     const int *vnum;
     const int64_t *voff;
@@ -140,14 +140,16 @@ hipError_t launch_active_flags(const int64_t *ins_off, int n, uint32_t *flags, h
 // Sparse alternative for few sorted keys: offsets of the stars with insertions only, and hk[i] =
 // star at the start of each run (STAR_TASK_NONE elsewhere; compact it for the active list).
 hipError_t launch_sparse_runs(const uint64_t *keys, int64_t m, int64_t *off, uint64_t *hk, hipStream_t s);
-hipError_t launch_u64_to_i32(const uint64_t *in, int64_t m, int *out, hipStream_t s);
+hipError_t launch_u64_to_i32(const uint64_t *in, int64_t m, const int64_t *dev_m, int *out, hipStream_t s);
 hipError_t launch_active_scatter(const uint32_t *flags, const uint32_t *pos, int n, int *list, hipStream_t s);
 // Region sizes of the stars of a pass (list = nullptr: stars 0..nitems-1), per item: link
 // vertices v = vn + m, triangle words 2v - 4 (twice that for stars that may need the wide
 // format), history hn + m.
-// pcount[s] = 0 for the pass's stars (their kernel writes the dense pair count).
-hipError_t launch_capacities(const int *list, int nitems, const int *vnum, const int *hnum, const int64_t *ins_off,
-                             int64_t *vcap, int64_t *tcap, int64_t *hcap, int *pcount, hipStream_t s);
+// pcount[s] = 0 for the pass's stars (their kernel writes the dense pair count).  dev_n (may be
+// null): the list's length on the device when nitems is only its bound; entries beyond get zero.
+hipError_t launch_capacities(const int *list, int nitems, const int64_t *dev_n, const int *vnum, const int *hnum,
+                             const int64_t *ins_off, int64_t *vcap, int64_t *tcap, int64_t *hcap, int *pcount,
+                             hipStream_t s);
 // The pass's insertion pairs, star after star in list order (list = nullptr: stars 0..nitems-1):
 // cnt64[i] = pcount[list[i]] for the offsets scan, then the copy to out[off[i] ...].
 hipError_t launch_pair_counts(const int *list, int nitems, const int *pcount, int64_t *cnt64, hipStream_t s);

@@ -1630,7 +1630,16 @@ template <bool EXACT>
 __global__ void __launch_bounds__(64) k_proofs(ProofParams p)
 {
     const int lane = threadIdx.x;
-    const int64_t nt = EXACT ? p.nx : p.ntasks;
+    // the exact pass reads its list length from the counter the filter pass appended with
+    // (nx < 0), so that no host round trip separates the two launches
+    int64_t nt = p.ntasks;
+    if (EXACT) {
+        nt = p.nx;
+        if (nt < 0) {
+            const int64_t c = (int64_t)p.cnt[G4D_CNT_APPEND5];
+            nt = c < p.xcap ? c : p.xcap;
+        }
+    }
     for (int64_t it = blockIdx.x; it < nt; it += gridDim.x) {
         const int64_t task = EXACT ? p.xlist[it] : it;
         const uint64_t tk = p.tasks[task];
@@ -1707,7 +1716,7 @@ __global__ void __launch_bounds__(64) k_proofs(ProofParams p)
 
 hipError_t launch_proofs(const ProofParams &p, bool exact, hipStream_t s)
 {
-    const int64_t nt = exact ? p.nx : p.ntasks;
+    const int64_t nt = exact ? (p.nx < 0 ? p.xcap : p.nx) : p.ntasks;
     if (nt <= 0) return hipSuccess;
     if (exact) {
         const int nb = (int)(nt < 1024 ? nt : 1024);
@@ -1832,16 +1841,18 @@ hipError_t launch_sparse_runs(const uint64_t *keys, int64_t m, int64_t *off, uin
     return hipGetLastError();
 }
 
-__global__ void k_u64_to_i32(const uint64_t *in, int64_t m, int *out)
+// dev_m (may be null): the count on the device, at most m.
+__global__ void k_u64_to_i32(const uint64_t *in, int64_t m, const int64_t *dev_m, int *out)
 {
+    if (dev_m) m = min(m, *dev_m);
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < m; i += (int64_t)gridDim.x * blockDim.x)
         out[i] = (int)in[i];
 }
 
-hipError_t launch_u64_to_i32(const uint64_t *in, int64_t m, int *out, hipStream_t s)
+hipError_t launch_u64_to_i32(const uint64_t *in, int64_t m, const int64_t *dev_m, int *out, hipStream_t s)
 {
     if (m <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_u64_to_i32, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, in, m, out);
+    hipLaunchKernelGGL(k_u64_to_i32, dim3(g4d_blocks(m, 256, 65536)), dim3(256), 0, s, in, m, dev_m, out);
     return hipGetLastError();
 }
 
@@ -1869,10 +1880,19 @@ hipError_t launch_active_scatter(const uint32_t *flags, const uint32_t *pos, int
     return hipGetLastError();
 }
 
-__global__ void k_capacities(const int *list, int nitems, const int *vnum, const int *hnum, const int64_t *ins_off,
-                             int64_t *vcap, int64_t *tcap, int64_t *hcap, int *pcount)
+// dev_n (may be null): the list's length on the device, at most nitems; the entries from there
+// to nitems get zero capacities (so that scans over nitems total the list's needs).
+__global__ void k_capacities(const int *list, int nitems, const int64_t *dev_n, const int *vnum, const int *hnum,
+                             const int64_t *ins_off, int64_t *vcap, int64_t *tcap, int64_t *hcap, int *pcount)
 {
+    const int nl = dev_n ? (int)min((int64_t)nitems, *dev_n) : nitems;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nitems; i += gridDim.x * blockDim.x) {
+        if (i >= nl) {
+            vcap[i] = 0;
+            tcap[i] = 0;
+            hcap[i] = 0;
+            continue;
+        }
         const int s = list ? list[i] : i;
         pcount[s] = 0;
         const int64_t m = ins_off[s + 1] - ins_off[s];
@@ -1883,12 +1903,13 @@ __global__ void k_capacities(const int *list, int nitems, const int *vnum, const
     }
 }
 
-hipError_t launch_capacities(const int *list, int nitems, const int *vnum, const int *hnum, const int64_t *ins_off,
-                             int64_t *vcap, int64_t *tcap, int64_t *hcap, int *pcount, hipStream_t s)
+hipError_t launch_capacities(const int *list, int nitems, const int64_t *dev_n, const int *vnum, const int *hnum,
+                             const int64_t *ins_off, int64_t *vcap, int64_t *tcap, int64_t *hcap, int *pcount,
+                             hipStream_t s)
 {
     if (nitems <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_capacities, dim3(g4d_blocks(nitems, 256, 65536)), dim3(256), 0, s, list, nitems, vnum, hnum,
-                       ins_off, vcap, tcap, hcap, pcount);
+    hipLaunchKernelGGL(k_capacities, dim3(g4d_blocks(nitems, 256, 65536)), dim3(256), 0, s, list, nitems, dev_n, vnum,
+                       hnum, ins_off, vcap, tcap, hcap, pcount);
     return hipGetLastError();
 }
 
