@@ -635,6 +635,7 @@ struct StarGrpLds {
     uint16_t be_a[TMAX], be_b[TMAX], be_u[TMAX], be_slot[TMAX];
     uint8_t be_j[TMAX];
     int16_t ins_local[VMAX];
+    uint8_t ord[VMAX]; // insertion order: list indices by distance to the star point
     uint8_t present[VMAX];
     uint16_t remap[TMAX];
     uint16_t vremap[VMAX];
@@ -795,17 +796,34 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                 sz = p.pts[3 * (int64_t)s + 2];
             }
             const int q0 = p.mode ? 0 : 4;
+            // Every listed vertex q takes link-vertex slot vn + q (the cone's four included).
             if (act) {
                 for (int q = gl; q < m; q += GL) {
                     const int gv = (int)(uint32_t)p.ins_keys[ib + q];
                     L.ins[q] = gv;
                     L.ins_local[q] = -1;
-                    if (q >= q0) {
-                        L.vid[vn + q] = gv;
-                        rel_coords(p.pts, gv, sx, sy, sz, L.vc[vn + q]);
-                    }
+                    L.vid[vn + q] = gv;
+                    rel_coords(p.pts, gv, sx, sy, sz, L.vc[vn + q]);
                 }
             }
+            wave_sync();
+            // Insertion order: nearest to the star point first (ties by list index).  The star
+            // that results does not depend on the order (the link is the unique convex hull of
+            // the lifted points as seen from the star point, the drowned flags mark listed
+            // vertices absent from it, the pairs come from its unchecked triangles), but near
+            // vertices are almost all in it: inserted first, they leave the far ones to be
+            // rejected by the filter alone, and fewer triangles are made and undone.
+            if (act)
+                for (int q = gl; q < m; q += GL) {
+                    const double dq = L.vc[vn + q][3];
+                    int rank = 0;
+                    for (int k = 0; k < m; k++) {
+                        const double dk = L.vc[vn + k][3];
+                        rank += (dk < dq || (dk == dq && k < q)) ? 1 : 0;
+                    }
+                    L.ord[rank] = (uint8_t)q;
+                }
+            wave_sync();
             int T = 0, nfree = 0;
             bool unsure = false, failed = false;
             if (act && p.mode == 1) {
@@ -825,31 +843,31 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                 }
                 T = tn;
             } else if (act) {
-                // initial 4-simplex (GD/GDelaunay/GDelPredKernels.cu:45-158)
-                int l[4];
-                for (int i = 0; i < 4; i++) l[i] = (int)(uint32_t)p.ins_keys[ib + i];
+                // initial 4-simplex (GD/GDelaunay/GDelPredKernels.cu:45-158) on the first four
+                // vertices of the insertion order; slot c[i] holds cone vertex i
+                int l[4], c[4];
+                for (int i = 0; i < 4; i++) {
+                    c[i] = L.ord[i];
+                    l[i] = L.ins[c[i]];
+                }
                 const double *pp[5] = {p.pts + 3 * (int64_t)l[0], p.pts + 3 * (int64_t)l[1], p.pts + 3 * (int64_t)l[2],
                                        p.pts + 3 * (int64_t)l[3], p.pts + 3 * (int64_t)s};
                 const int o = filter_insphere(pp);
                 if (o == 0) {
                     unsure = true; // pushed to the exact list below, untouched
                 } else {
-                    const bool sw = o < 0;
-                    if (sw) {
-                        const int t = l[0];
-                        l[0] = l[1];
-                        l[1] = t;
+                    if (o < 0) {
+                        const int t = c[0];
+                        c[0] = c[1];
+                        c[1] = t;
                     }
                     if (gl < 4) {
-                        L.vid[gl] = l[gl];
-                        rel_coords(p.pts, l[gl], sx, sy, sz, L.vc[gl]);
                         const int lt[4][3] = {{1, 2, 3}, {0, 3, 2}, {0, 1, 3}, {0, 2, 1}};
-                        L.tv[gl] = tpack(lt[gl][0], lt[gl][1], lt[gl][2], TF_ALIVE);
+                        L.tv[gl] = tpack(c[lt[gl][0]], c[lt[gl][1]], c[lt[gl][2]], TF_ALIVE);
                         L.to[gl][0] = (uint16_t)lt[gl][0];
                         L.to[gl][1] = (uint16_t)lt[gl][1];
                         L.to[gl][2] = (uint16_t)lt[gl][2];
-                        const int local = (gl < 2 && sw) ? 1 - gl : gl;
-                        L.ins_local[gl] = (int16_t)local;
+                        L.ins_local[c[gl]] = (int16_t)c[gl];
                     }
                     T = 4;
                 }
@@ -869,7 +887,8 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
             const int qend = wave_max_i(live ? m : 0);
             for (int q = q0; q < qend; q++) {
                 bool step = live && q < m;
-                const int Vn = vn + q;
+                const int j = step ? L.ord[q] : 0; // the list index inserted in this step
+                const int Vn = vn + j;
                 double r[4] = {0.0, 0.0, 0.0, 0.0};
                 if (step) {
                     r[0] = L.vc[Vn][0];
@@ -904,7 +923,7 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                 int nvis;
                 const int vbase = grp_excl<GL, count_bits(KPL)>(nv, g, gl, &nvis);
                 if (nvis == 0) step = false; // drowned at insertion
-                if (step && gl == 0) L.ins_local[q] = (int16_t)Vn;
+                if (step && gl == 0) L.ins_local[j] = (int16_t)Vn;
                 wave_sync();
                 // hole boundary: edges of freed triangles whose neighbour survives
                 int nb = 0;

@@ -58,6 +58,8 @@ typedef struct {
     int32_t min_workset;    /* MinWorksetSize, 8 in the reference */
     int32_t max_rounds;     /* splaying round limit (guard), e.g. 1000 */
     int32_t verbose;
+    int32_t insert_order;   /* 0: each star's list in ascending id (the reference's order);
+                               1: nearest to the star point first (the GPU grouped kernel's) */
 } orc_options;
 
 typedef struct {

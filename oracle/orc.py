@@ -15,7 +15,8 @@ _REF = None
 
 class OrcOptions(ctypes.Structure):
     _fields_ = [("grid_size", ctypes.c_int32), ("min_workset", ctypes.c_int32),
-                ("max_rounds", ctypes.c_int32), ("verbose", ctypes.c_int32)]
+                ("max_rounds", ctypes.c_int32), ("verbose", ctypes.c_int32),
+                ("insert_order", ctypes.c_int32)]
 
 
 class OrcStats(ctypes.Structure):
@@ -177,17 +178,19 @@ def set_threads(nthreads):
     return lib().orc_set_threads(int(nthreads))
 
 
-def triangulate(xyz, grid, min_workset=8, max_rounds=1000, canonical=True, threads=None, extra_pairs=None):
+def triangulate(xyz, grid, min_workset=8, max_rounds=1000, canonical=True, threads=None, extra_pairs=None,
+                insert_order=0):
     """Returns (tets, perm, stats).  tets are canonical (input index space, sorted) when
     canonical=True, else Morton-space with the reference's vertex order.  threads: OpenMP
     threads for this call (None keeps the current setting); the output does not depend on it.
     extra_pairs: uint64 (a << 32 | b) candidate pairs in Morton index space added to the
-    worksets (the proposal's plug-in point)."""
+    worksets (the proposal's plug-in point).  insert_order: 0 inserts each star's list in
+    ascending id (the reference's order), 1 nearest to the star point first."""
     if threads is not None:
         set_threads(threads)
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     n = xyz.shape[0]
-    opt = OrcOptions(grid, min_workset, max_rounds, 0)
+    opt = OrcOptions(grid, min_workset, max_rounds, 0, insert_order)
     perm = np.empty(n, np.int32)
     st = OrcStats()
     cap = 8 * n + 64

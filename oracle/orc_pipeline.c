@@ -576,6 +576,38 @@ static int star_insert(octx *c, oscratch *w, int s, int v)
     return 1;
 }
 
+/* Insertion order of a star's list (insert_order 1): nearest to the star point first, ties by
+   id.  The star that results is the same in any order: its link is the unique hull of the
+   lifted points as seen from the star point (SoS decides the degenerate cases consistently),
+   a drowned insertion is a listed vertex absent from it, and its pairs come from its triangles
+   that the pass created; the order changes only how many triangles are made and undone. */
+typedef struct {
+    double d;
+    int v;
+} odist;
+
+static int cmp_dist(const void *a, const void *b)
+{
+    const odist *x = a, *y = b;
+    if (x->d != y->d) return x->d < y->d ? -1 : 1;
+    return (x->v > y->v) - (x->v < y->v);
+}
+
+static void order_nearest(const double *pts, int s, const int *in, int m, int *out)
+{
+    odist *t = xmalloc(sizeof(odist) * (size_t)(m > 0 ? m : 1));
+    const double *ps = pts + 3 * (int64_t)s;
+    for (int i = 0; i < m; i++) {
+        const double *pv = pts + 3 * (int64_t)in[i];
+        const double x = pv[0] - ps[0], y = pv[1] - ps[1], z = pv[2] - ps[2];
+        t[i].d = x * x + y * y + z * z;
+        t[i].v = in[i];
+    }
+    qsort(t, (size_t)m, sizeof(odist), cmp_dist);
+    for (int i = 0; i < m; i++) out[i] = t[i].v;
+    free(t);
+}
+
 static int cmp_int(const void *a, const void *b)
 {
     const int x = *(const int *)a, y = *(const int *)b;
@@ -836,8 +868,16 @@ int64_t orc_triangulate_ex(const double *xyz, int32_t n, const orc_options *opt,
             const int m = (int)(off[s + 1] - off[s]);
             int *wb = scratch_w(&w, m);
             for (int k = 0; k < m; k++) wb[k] = (int)(uint32_t)ws.a[off[s] + k];
-            star_make_cone(&c, s, wb);
-            for (int k = 4; k < m; k++) star_insert(&c, &w, s, wb[k]);
+            const int *ord = wb;
+            int *ob = NULL;
+            if (opt->insert_order == 1) {
+                ob = xmalloc(sizeof(int) * (size_t)m);
+                order_nearest(pts, s, wb, m, ob);
+                ord = ob;
+            }
+            star_make_cone(&c, s, ord);
+            for (int k = 4; k < m; k++) star_insert(&c, &w, s, ord[k]);
+            free(ob);
             ostar *S = &c.st[s];
             S->hist = xmalloc(sizeof(int) * (m + 8));
             S->caphist = m + 8;
@@ -990,9 +1030,14 @@ int64_t orc_triangulate_ex(const double *xyz, int32_t n, const orc_options *opt,
                     const int s = (int)(ins.a[gs[q]] >> 32);
                     const int cnt = (int)(gs[q + 1] - gs[q]);
                     int *wb = scratch_w(&w, cnt);
-                    for (int r = 0; r < cnt; r++) {
-                        wb[r] = (int)(uint32_t)ins.a[gs[q] + r];
-                        star_insert(&c, &w, s, wb[r]);
+                    for (int r = 0; r < cnt; r++) wb[r] = (int)(uint32_t)ins.a[gs[q] + r];
+                    if (opt->insert_order == 1) {
+                        int *ob = xmalloc(sizeof(int) * (size_t)cnt);
+                        order_nearest(pts, s, wb, cnt, ob);
+                        for (int r = 0; r < cnt; r++) star_insert(&c, &w, s, ob[r]);
+                        free(ob);
+                    } else {
+                        for (int r = 0; r < cnt; r++) star_insert(&c, &w, s, wb[r]);
                     }
                     hist_add(&c.st[s], wb, cnt);
                 }

@@ -191,3 +191,32 @@ def test_min_workset_does_not_change_output(orc):
     b, _, sb = orc.triangulate(x, 32, min_workset=24)
     assert (a == b).all()
     assert sb["num_pumped"] > sa["num_pumped"]
+
+
+@pytest.mark.parametrize("kind", ["uniform", "hard_mix", "lattice", "cospherical"])
+def test_insertion_order_does_not_change_the_stars(orc, kind):
+    """The GPU's grouped star kernel inserts each star's list nearest-first instead of in
+    ascending id.  A star's link is the unique hull of the lifted points seen from the star
+    point (SoS decides degenerate cases consistently), so the order must change neither the
+    tetrahedra nor the splaying rounds and insertions (drowned flags and pairs are set
+    properties of the final stars).  Here the oracle itself runs both orders, on inputs whose
+    degenerate cases go through SoS."""
+    rng = np.random.default_rng(9)
+    if kind == "uniform":
+        x, g = datasets.uniform_cube(20000, seed=11), 32
+    elif kind == "hard_mix":
+        x, g = datasets.hard_mix(20000, seed=2), 64
+    elif kind == "lattice":
+        x = np.stack(np.meshgrid(*[np.arange(9)] * 3, indexing="ij"), -1).reshape(-1, 3) / 9.0
+        x, g = x + 0.0, 16
+    else:
+        v = rng.integers(-6, 7, size=(4000, 3)).astype(np.float64)
+        v = np.unique(v[(v ** 2).sum(1) == 29], axis=0) / 8.0  # integer points on one sphere
+        x, g = np.concatenate([v, rng.random((300, 3)) - 0.5]), 16
+    a, _, sa = orc.triangulate(x, g, insert_order=0)
+    b, _, sb = orc.triangulate(x, g, insert_order=1)
+    assert a.shape == b.shape and (a == b).all()
+    assert sa["splay_rounds"] == sb["splay_rounds"]
+    assert sa["splay_insertions"] == sb["splay_insertions"]
+    if kind in ("hard_mix", "lattice", "cospherical"):
+        assert sb["sos_calls"] > 0
