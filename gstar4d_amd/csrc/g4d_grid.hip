@@ -512,6 +512,7 @@ __host__ __device__ inline int pba_band_len(int g, int bands)
     return (L + PBA_CHUNK - 1) / PBA_CHUNK * PBA_CHUNK;
 }
 static constexpr int PBA_BAND_STACK = 48;
+static constexpr int PBA_MERGE_PF = 8; // band entries in flight per lane in the merge
 
 template <int AXIS, bool IDS>
 __global__ void __launch_bounds__(64) k_pba_band_env(const int *in_all, GridSets S, int *spill_all, int *benv_all,
@@ -603,11 +604,27 @@ __global__ void __launch_bounds__(64) k_pba_band_merge(GridSets S, int *spill_al
 #define get(k) ((k) < STACK ? scode[(k)][lane] : pba_spill_get(sp, (k), STACK))
     int m = 0;
     double q0 = 0, f0 = 0, q1 = 0, f1 = 0;
+    // A band's entries are read PBA_MERGE_PF at a time, the next group's loads in flight while
+    // the current group goes through the stack (each entry's load is otherwise one dependent
+    // round trip to memory).
     for (int band = 0; band < bands; band++) {
         const int nb = band * Lb < g ? bcnt[(bb * bands + band) * PBA_COLS + lane] : 0;
         const int *bev = ev + (int64_t)band * PBA_COLS * Lb;
-        for (int k = 0; k < nb; k++) {
-            const int sv = bev[(int64_t)k * PBA_COLS];
+        int nxt[PBA_MERGE_PF];
+#pragma unroll
+        for (int i = 0; i < PBA_MERGE_PF; i++) nxt[i] = i < nb ? bev[(int64_t)i * PBA_COLS] : 0;
+        for (int k0 = 0; k0 < nb; k0 += PBA_MERGE_PF) {
+            int cur[PBA_MERGE_PF];
+#pragma unroll
+            for (int i = 0; i < PBA_MERGE_PF; i++) {
+                cur[i] = nxt[i];
+                const int kn = k0 + PBA_MERGE_PF + i;
+                nxt[i] = kn < nb ? bev[(int64_t)kn * PBA_COLS] : 0;
+            }
+#pragma unroll
+            for (int i = 0; i < PBA_MERGE_PF; i++) {
+            if (k0 + i >= nb) break;
+            const int sv = cur[i];
             double qs, fs;
             site_qf<AXIS, IDS>(sv, vox, ca, cb, &qs, &fs);
             while (m >= 2 && env_dominated(q0, f0, q1, f1, qs, fs)) {
@@ -623,6 +640,7 @@ __global__ void __launch_bounds__(64) k_pba_band_merge(GridSets S, int *spill_al
             q1 = qs;
             f1 = fs;
             m++;
+            }
         }
     }
     // every band entry has been read: the merged envelope goes to the front of the region
