@@ -692,7 +692,9 @@ __global__ void __launch_bounds__(64) k_pba_band_color(GridSets S, const int *be
         }
         idx = lo;
     }
-    int qa = 0, qb = 0, sa = G4D_MARKER, sb = G4D_MARKER, da = 0, db = 0;
+    // The walk holds the current entry (sa), the next one decoded (sb, qb, db) and the raw
+    // entry after it (sc), loaded one advance ahead so that an advance waits on no load.
+    int qa = 0, qb = 0, sa = G4D_MARKER, sb = G4D_MARKER, sc = G4D_MARKER, da = 0, db = 0;
     if (active && m > 0) {
         sa = ev[(int64_t)idx * PBA_COLS];
         site_pos_off<AXIS, IDS>(sa, vox, ca, cb, &qa, &da);
@@ -703,6 +705,7 @@ __global__ void __launch_bounds__(64) k_pba_band_color(GridSets S, const int *be
         site_pos_off<AXIS, IDS>(sb, vox, ca, cb, &qb, &db);
         if (ids_out) sb = id_of(sb);
     }
+    if (active && idx + 2 < m) sc = ev[(int64_t)(idx + 2) * PBA_COLS];
     int pre[PBA_PER_LANE];
     for (int t0 = rb; t0 < re; t0 += PBA_CHUNK) {
         const int nt = min(PBA_CHUNK, re - t0);
@@ -722,9 +725,9 @@ __global__ void __launch_bounds__(64) k_pba_band_color(GridSets S, const int *be
                         da = db;
                         sa = sb;
                         if (idx + 1 < m) {
-                            sb = ev[(int64_t)(idx + 1) * PBA_COLS];
-                            site_pos_off<AXIS, IDS>(sb, vox, ca, cb, &qb, &db);
-                            if (ids_out) sb = id_of(sb);
+                            site_pos_off<AXIS, IDS>(sc, vox, ca, cb, &qb, &db);
+                            sb = ids_out ? id_of(sc) : sc;
+                            sc = idx + 2 < m ? ev[(int64_t)(idx + 2) * PBA_COLS] : G4D_MARKER;
                         }
                     } else {
                         break;
