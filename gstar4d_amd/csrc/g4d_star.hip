@@ -1645,10 +1645,27 @@ hipError_t launch_need_max(const StarParams &p, const int *list, int nitems, uns
 
 // EXACT = false: filters only, over all tasks; a task with an inconclusive orientation goes to
 // p.xlist.  EXACT = true: the tasks listed in p.xlist, with exact predicates and SoS.
-template <bool EXACT>
+// Minimum over the GL lanes of a group (GL a power of two dividing 64).
+template <int GL>
+__device__ __forceinline__ int grp_min(int v)
+{
+#pragma unroll
+    for (int o = GL / 2; o > 0; o >>= 1) {
+        const int w = __shfl_xor(v, o, GL);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// One task per group of GL lanes: the filter pass takes 16 lanes per task (four tasks per
+// wave; a star has about 27 link triangles, so a whole wave per task left most lanes idle),
+// the exact pass a whole wave.  The lanes of a group stay together: every loop that ends in a
+// group reduction has a trip count uniform over the group.
+template <bool EXACT, int GL>
 __global__ void __launch_bounds__(64) k_proofs(ProofParams p)
 {
-    const int lane = threadIdx.x;
+    constexpr int NG = 64 / GL;
+    const int lane = threadIdx.x, g = lane / GL, gl = lane % GL;
     // the exact pass reads its list length from the counter the filter pass appended with
     // (nx < 0), so that no host round trip separates the two launches
     int64_t nt = p.ntasks;
@@ -1659,7 +1676,7 @@ __global__ void __launch_bounds__(64) k_proofs(ProofParams p)
             nt = c < p.xcap ? c : p.xcap;
         }
     }
-    for (int64_t it = blockIdx.x; it < nt; it += gridDim.x) {
+    for (int64_t it = (int64_t)blockIdx.x * NG + g; it < nt; it += (int64_t)gridDim.x * NG) {
         const int64_t task = EXACT ? p.xlist[it] : it;
         const uint64_t tk = p.tasks[task];
         const int s = (int)(tk >> 32), v = (int)(uint32_t)tk;
@@ -1667,11 +1684,11 @@ __global__ void __launch_bounds__(64) k_proofs(ProofParams p)
         const bool wide = vn >= STAR_WIDE_MIN;
         const int64_t vo = p.voff[s], to = p.toff[s];
         int mn = 0x7fffffff;
-        for (int i = lane; i < vn; i += 64) mn = min(mn, p.vpool[vo + i]);
-        const int ex = g4d_wave_min(mn);
+        for (int i = gl; i < vn; i += GL) mn = min(mn, p.vpool[vo + i]);
+        const int ex = grp_min<GL>(mn);
         int k0 = 0x7fffffff, k1 = 0x7fffffff, k2 = 0x7fffffff;
         bool unc = false;
-        for (int t = lane; t < tn; t += 64) {
+        for (int t = gl; t < tn; t += GL) {
             int lv[3];
             star_tri_verts(p.tpool, to, t, wide, lv);
             const int a = p.vpool[vo + lv[0]], b = p.vpool[vo + lv[1]], c = p.vpool[vo + lv[2]];
@@ -1706,18 +1723,19 @@ __global__ void __launch_bounds__(64) k_proofs(ProofParams p)
                 k2 = x2;
             }
         }
-        if (!EXACT && __ballot(unc)) {
-            if (lane == 0) {
+        // the group's verdict: any inconclusive filter sends the task to the exact pass
+        if (!EXACT && grp_min<GL>(unc ? 0 : 1) == 0) {
+            if (gl == 0) {
                 const unsigned long long k = atomicAdd(&p.cnt[G4D_CNT_APPEND5], 1ull);
                 if ((int64_t)k < p.xcap) p.xlist[k] = task;
                 else atomicOr(&p.cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_BUFFER);
             }
             continue;
         }
-        const int b0 = g4d_wave_min(k0);
-        const int b1 = g4d_wave_min(k0 == b0 ? k1 : 0x7fffffff);
-        const int b2 = g4d_wave_min((k0 == b0 && k1 == b1) ? k2 : 0x7fffffff);
-        if (lane == 0) {
+        const int b0 = grp_min<GL>(k0);
+        const int b1 = grp_min<GL>(k0 == b0 ? k1 : 0x7fffffff);
+        const int b2 = grp_min<GL>((k0 == b0 && k1 == b1) ? k2 : 0x7fffffff);
+        if (gl == 0) {
             uint64_t *o = p.out + 4 * task;
             if (b0 == 0x7fffffff) {
                 atomicOr(&p.cnt[G4D_CNT_ERROR], (unsigned long long)G4D_DEV_ERR_PROOF);
@@ -1739,10 +1757,11 @@ hipError_t launch_proofs(const ProofParams &p, bool exact, hipStream_t s)
     if (nt <= 0) return hipSuccess;
     if (exact) {
         const int nb = (int)(nt < 1024 ? nt : 1024);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_proofs<true>), dim3(nb), dim3(64), 0, s, p);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_proofs<true, 64>), dim3(nb), dim3(64), 0, s, p);
     } else {
-        const int nb = (int)(nt < 65536 ? nt : 65536);
-        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_proofs<false>), dim3(nb), dim3(64), 0, s, p);
+        const int64_t nw = (nt + 3) / 4; // four tasks per wave
+        const int nb = (int)(nw < 65536 ? nw : 65536);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_proofs<false, 16>), dim3(nb), dim3(64), 0, s, p);
     }
     return hipGetLastError();
 }
