@@ -2074,25 +2074,31 @@ __device__ __forceinline__ uint64_t *out_keep_word(const OutParams &p, int64_t t
 }
 __device__ __forceinline__ uint64_t out_keep_bit(bool wide) { return wide ? (1ull << 63) : (1ull << 58); }
 
-// Decision pass, one half-wave per star and one lane per link triangle (a star has ~27 link
-// triangles, so two stars fill a wave; coalesced reads of each star's triangle words): every
-// owner triangle whose orient3d filter says "lower hull" gets its keep bit and is counted for
-// its star; inconclusive ones go to the list for the exact pass.
+// The output passes give each star a group of OUT_GL lanes, one lane per link triangle in turn
+// (a star has ~27 link triangles): four stars per wave step, so that each wave keeps four
+// stars' chains of dependent reads (star record, triangle words, link ids, points) in flight
+// instead of two.
+static constexpr int OUT_GL = 16;
+static constexpr int OUT_NG = 64 / OUT_GL;
+
+// Decision pass (coalesced reads of each star's triangle words): every owner triangle whose
+// orient3d filter says "lower hull" gets its keep bit and is counted for its star;
+// inconclusive ones go to the list for the exact pass.
 __global__ void __launch_bounds__(256) k_out_flag(OutParams p, uint64_t *list, int64_t cap)
 {
-    const int lane = threadIdx.x & 63, half = lane >> 5, hl = lane & 31;
-    const uint64_t hmask = half ? 0xffffffff00000000ull : 0x00000000ffffffffull;
+    const int lane = threadIdx.x & 63, grp = lane / OUT_GL, gl = lane % OUT_GL;
+    const uint64_t gmask = ((OUT_GL == 64 ? 0ull : (1ull << OUT_GL)) - 1ull) << (grp * OUT_GL);
     const int w0 = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), nw = (int)((gridDim.x * blockDim.x) >> 6);
-    for (int s0 = 2 * w0; s0 < p.n; s0 += 2 * nw) {
-        const int s = s0 + half;
+    for (int s0 = OUT_NG * w0; s0 < p.n; s0 += OUT_NG * nw) {
+        const int s = s0 + grp;
         const bool has = s < p.n;
         const int tn = has ? p.tnum[s] : 0;
         const bool wide = has && p.vnum[s] >= STAR_WIDE_MIN;
         const int64_t vo = has ? p.voff[s] : 0, to = has ? p.toff[s] : 0;
-        const int tmax = max(tn, __shfl_xor(tn, 32, 64));
+        const int tmax = wave_max_i(tn);
         int64_t c = 0;
-        for (int t0 = 0; t0 < tmax; t0 += 32) {
-            const int t = t0 + hl;
+        for (int t0 = 0; t0 < tmax; t0 += OUT_GL) {
+            const int t = t0 + gl;
             int o = 1;
             if (t < tn) {
                 int lv[3];
@@ -2105,11 +2111,11 @@ __global__ void __launch_bounds__(256) k_out_flag(OutParams p, uint64_t *list, i
                     if (o < 0) *out_keep_word(p, to, t, wide) |= out_keep_bit(wide);
                 }
             }
-            c += __popcll(__ballot(o < 0) & hmask);
+            c += __popcll(__ballot(o < 0) & gmask);
             const long long k = g4d_wave_append(o == 0, &p.cnt[G4D_CNT_APPEND5]);
             if (o == 0 && k < cap) list[k] = ((uint64_t)(uint32_t)s << 32) | (uint32_t)t;
         }
-        if (has && hl == 0) p.star_tet_count[s] = c;
+        if (has && gl == 0) p.star_tet_count[s] = c;
     }
 }
 
@@ -2132,24 +2138,24 @@ __global__ void k_out_exact(OutParams p, const uint64_t *list, int64_t m)
     }
 }
 
-// Write pass, one half-wave per star: the kept triangles of each star at its scanned offset,
+// Write pass, a group of OUT_GL lanes per star: the kept triangles of each star at its scanned offset,
 // in triangle order.
 __global__ void __launch_bounds__(256) k_out_write(OutParams p)
 {
-    const int lane = threadIdx.x & 63, half = lane >> 5, hl = lane & 31;
-    const uint64_t hmask = half ? 0xffffffff00000000ull : 0x00000000ffffffffull;
+    const int lane = threadIdx.x & 63, grp = lane / OUT_GL, gl = lane % OUT_GL;
+    const uint64_t gmask = ((OUT_GL == 64 ? 0ull : (1ull << OUT_GL)) - 1ull) << (grp * OUT_GL);
     const int w0 = (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6), nw = (int)((gridDim.x * blockDim.x) >> 6);
-    const uint64_t below = ((1ull << lane) - 1ull) & hmask;
-    for (int s0 = 2 * w0; s0 < p.n; s0 += 2 * nw) {
-        const int s = s0 + half;
+    const uint64_t below = ((1ull << lane) - 1ull) & gmask;
+    for (int s0 = OUT_NG * w0; s0 < p.n; s0 += OUT_NG * nw) {
+        const int s = s0 + grp;
         const bool has = s < p.n;
         const int tn = has ? p.tnum[s] : 0;
         const bool wide = has && p.vnum[s] >= STAR_WIDE_MIN;
         const int64_t vo = has ? p.voff[s] : 0, to = has ? p.toff[s] : 0;
         int64_t k = has ? p.star_tet_off[s] : 0;
-        const int tmax = max(tn, __shfl_xor(tn, 32, 64));
-        for (int t0 = 0; t0 < tmax; t0 += 32) {
-            const int t = t0 + hl;
+        const int tmax = wave_max_i(tn);
+        for (int t0 = 0; t0 < tmax; t0 += OUT_GL) {
+            const int t = t0 + gl;
             const bool keep = t < tn && (*out_keep_word(p, to, t, wide) & out_keep_bit(wide));
             const uint64_t m = __ballot(keep);
             if (keep) {
@@ -2162,7 +2168,7 @@ __global__ void __launch_bounds__(256) k_out_write(OutParams p)
                 q.w = s;
                 reinterpret_cast<int4 *>(p.tets)[k + __popcll(m & below)] = q;
             }
-            k += __popcll(m & hmask);
+            k += __popcll(m & gmask);
         }
     }
 }
