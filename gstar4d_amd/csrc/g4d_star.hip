@@ -639,7 +639,12 @@ struct StarGrpLds {
     uint8_t present[VMAX];
     uint16_t remap[TMAX];
     uint16_t vremap[VMAX];
+    // The fields above take 3072 bytes, a whole number of bank rounds: without this pad the
+    // four groups' copies of a field would sit on the same banks and every group-parallel
+    // access (16 lanes of each group at the same offsets) would conflict four ways.
+    uint8_t pad[64];
 };
+static_assert(sizeof(StarGrpLds) % 256 == 64, "group blocks start 16 banks apart");
 
 // Ballot of this lane's group of GL lanes, shifted down to bit 0.
 template <int GL>
@@ -649,10 +654,30 @@ __device__ __forceinline__ uint64_t grp_ballot(bool pred, int g)
     return GL == 64 ? m : (m >> (g * GL)) & ((1ull << GL) - 1ull);
 }
 
-// Exclusive scan of per-lane counts below 2^BITS within the group; total in *tot.
+// Inclusive prefix sums over each DPP row of 16 lanes (row_shr 1, 2, 4, 8) and every lane's row
+// total (row_ror 8, 4, 2, 1); lanes whose shifted source lies outside the row add 0.  Every lane
+// of the wave must be active.
+__device__ __forceinline__ int row16_scan(int v, int *tot)
+{
+    int x = v, a = v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    a += __builtin_amdgcn_update_dpp(0, a, 0x128, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    a += __builtin_amdgcn_update_dpp(0, a, 0x124, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    a += __builtin_amdgcn_update_dpp(0, a, 0x122, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    a += __builtin_amdgcn_update_dpp(0, a, 0x121, 0xf, 0xf, false);
+    *tot = a;
+    return x;
+}
+
+// Exclusive scan of per-lane counts below 2^BITS within the group; total in *tot.  Groups of 16
+// lanes are DPP rows and scan any counts whose group sum stays below 2^31.
 template <int GL, int BITS>
 __device__ __forceinline__ int grp_excl(int v, int g, int gl, int *tot)
 {
+    if constexpr (GL == 16) return row16_scan(v, tot) - v;
     const uint64_t below = gl == 0 ? 0ull : (~0ull >> (64 - gl));
     int pre = 0, all = 0;
 #pragma unroll
@@ -915,15 +940,8 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                         }
                     }
                 }
-                if (grp_ballot<GL>(unc, g) != 0ull) {
-                    unsure = true;
-                    live = false;
-                    step = false;
-                }
-                int nvis;
-                const int vbase = grp_excl<GL, count_bits(KPL)>(nv, g, gl, &nvis);
-                if (nvis == 0) step = false; // drowned at insertion
-                if (step && gl == 0) L.ins_local[j] = (int16_t)Vn;
+                // every group drowned its vertex at once, or idles: nothing changes in this step
+                if (__ballot(step && (nv > 0 || unc)) == 0ull) continue;
                 wave_sync();
                 // hole boundary: edges of freed triangles whose neighbour survives
                 int nb = 0;
@@ -933,8 +951,32 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                     if (step && slot < T && (L.tv[slot] & TF_VIS))
                         for (int i = 0; i < 3; i++) nb += (L.tv[L.to[slot][i]] & TF_VIS) ? 0 : 1;
                 }
-                int nbtot;
-                int e = grp_excl<GL, count_bits(3 * KPL)>(nb, g, gl, &nbtot);
+                int nvis, nbtot, vbase, e;
+                bool gunc;
+                if constexpr (GL == 16) {
+                    // one row scan of three counts: visible triangles (bits 0-7, at most TMAX),
+                    // boundary edges (bits 8-15, at most 3 TMAX) and inconclusive lanes (bits 16-20)
+                    static_assert(3 * TMAX < 256, "packed counts fit their fields");
+                    const int pk = nv | (nb << 8) | (unc ? 1 << 16 : 0);
+                    int tot;
+                    const int pre = row16_scan(pk, &tot) - pk;
+                    vbase = pre & 255;
+                    e = (pre >> 8) & 255;
+                    nvis = tot & 255;
+                    nbtot = (tot >> 8) & 255;
+                    gunc = (tot >> 16) != 0;
+                } else {
+                    gunc = grp_ballot<GL>(unc, g) != 0ull;
+                    vbase = grp_excl<GL, count_bits(KPL)>(nv, g, gl, &nvis);
+                    e = grp_excl<GL, count_bits(3 * KPL)>(nb, g, gl, &nbtot);
+                }
+                if (gunc) {
+                    unsure = true;
+                    live = false;
+                    step = false;
+                }
+                if (nvis == 0) step = false; // drowned at insertion
+                if (step && gl == 0) L.ins_local[j] = (int16_t)Vn;
                 if (step && (nbtot > TMAX || nbtot < 3)) {
                     failed = true;
                     live = false;
