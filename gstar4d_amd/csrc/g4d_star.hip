@@ -628,7 +628,10 @@ struct StarGrpLds {
     int vid[VMAX];
     int ins[VMAX];
     uint32_t tv[TMAX];
-    uint16_t to[TMAX][3];
+    union {
+        uint16_t to[TMAX][4]; // neighbours across the edges opposite vertices 0-2 (fourth unused)
+        uint64_t to64[TMAX];  // the same rows read as one word each
+    };
     int16_t vstart[VMAX];
     int16_t vend[VMAX];
     uint16_t freel[TMAX];
@@ -639,12 +642,13 @@ struct StarGrpLds {
     uint8_t present[VMAX];
     uint16_t remap[TMAX];
     uint16_t vremap[VMAX];
-    // The fields above take 3072 bytes, a whole number of bank rounds: without this pad the
-    // four groups' copies of a field would sit on the same banks and every group-parallel
-    // access (16 lanes of each group at the same offsets) would conflict four ways.
+    // The fields above take 3200 bytes.  Padded to 3264 (192 mod 256) the four groups' blocks
+    // start 48 banks apart (mod 64): group-parallel accesses (16 lanes of each group at the same
+    // offsets) do not all land on the same banks, as they would four ways at a multiple of 256.
     uint8_t pad[64];
 };
-static_assert(sizeof(StarGrpLds) % 256 == 64, "group blocks start 16 banks apart");
+static_assert(sizeof(StarGrpLds) % 256 == 192, "group blocks start 48 banks apart");
+static_assert(offsetof(StarGrpLds, to) % 8 == 0, "neighbour rows read as 64-bit words");
 
 // Ballot of this lane's group of GL lanes, shifted down to bit 0.
 template <int GL>
@@ -914,6 +918,10 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                 bool step = live && q < m;
                 const int j = step ? L.ord[q] : 0; // the list index inserted in this step
                 const int Vn = vn + j;
+                // this lane's triangle words, read together (every slot index is below TMAX)
+                uint32_t tk[KPL];
+#pragma unroll
+                for (int k = 0; k < KPL; k++) tk[k] = L.tv[gl + GL * k];
                 double r[4] = {0.0, 0.0, 0.0, 0.0};
                 if (step) {
                     r[0] = L.vc[Vn][0];
@@ -923,52 +931,38 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                 }
                 const double rs = fabs(r[0]) + fabs(r[1]) + fabs(r[2]) + r[3];
                 int nv = 0;
+                unsigned vm = 0; // bit k: slot gl + GL k sees the vertex
                 bool unc = false;
 #pragma unroll
                 for (int k = 0; k < KPL; k++) {
                     const int slot = gl + GL * k;
-                    if (step && slot < T) {
-                        const uint32_t t = L.tv[slot];
-                        if (t & TF_ALIVE) {
-                            nfilt++;
-                            const int sg = grp_filter_sign(r, rs, C[k], P[k]);
-                            if (sg == 0) unc = true;
-                            if (sg < 0) {
-                                L.tv[slot] = t | TF_VIS;
-                                nv++;
-                            }
+                    if (step && slot < T && (tk[k] & TF_ALIVE)) {
+                        nfilt++;
+                        const int sg = grp_filter_sign(r, rs, C[k], P[k]);
+                        if (sg == 0) unc = true;
+                        if (sg < 0) {
+                            L.tv[slot] = tk[k] | TF_VIS;
+                            vm |= 1u << k;
+                            nv++;
                         }
                     }
                 }
                 // every group drowned its vertex at once, or idles: nothing changes in this step
                 if (__ballot(step && (nv > 0 || unc)) == 0ull) continue;
-                wave_sync();
-                // hole boundary: edges of freed triangles whose neighbour survives
-                int nb = 0;
-#pragma unroll
-                for (int k = 0; k < KPL; k++) {
-                    const int slot = gl + GL * k;
-                    if (step && slot < T && (L.tv[slot] & TF_VIS))
-                        for (int i = 0; i < 3; i++) nb += (L.tv[L.to[slot][i]] & TF_VIS) ? 0 : 1;
-                }
-                int nvis, nbtot, vbase, e;
+                int nvis, vbase;
                 bool gunc;
                 if constexpr (GL == 16) {
-                    // one row scan of three counts: visible triangles (bits 0-7, at most TMAX),
-                    // boundary edges (bits 8-15, at most 3 TMAX) and inconclusive lanes (bits 16-20)
-                    static_assert(3 * TMAX < 256, "packed counts fit their fields");
-                    const int pk = nv | (nb << 8) | (unc ? 1 << 16 : 0);
+                    // one row scan of two counts: visible triangles (bits 0-7, at most TMAX) and
+                    // inconclusive lanes (bits 8-12)
+                    static_assert(TMAX < 256, "packed counts fit their fields");
+                    const int pk = nv | (unc ? 1 << 8 : 0);
                     int tot;
-                    const int pre = row16_scan(pk, &tot) - pk;
-                    vbase = pre & 255;
-                    e = (pre >> 8) & 255;
+                    vbase = (row16_scan(pk, &tot) - pk) & 255;
                     nvis = tot & 255;
-                    nbtot = (tot >> 8) & 255;
-                    gunc = (tot >> 16) != 0;
+                    gunc = (tot >> 8) != 0;
                 } else {
                     gunc = grp_ballot<GL>(unc, g) != 0ull;
                     vbase = grp_excl<GL, count_bits(KPL)>(nv, g, gl, &nvis);
-                    e = grp_excl<GL, count_bits(3 * KPL)>(nb, g, gl, &nbtot);
                 }
                 if (gunc) {
                     unsure = true;
@@ -977,38 +971,62 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                 }
                 if (nvis == 0) step = false; // drowned at insertion
                 if (step && gl == 0) L.ins_local[j] = (int16_t)Vn;
+                // the visible slots join the free list (positions nfree .. nfree + nvis - 1)
+                if (step) {
+                    int fpos = nfree + vbase;
+#pragma unroll
+                    for (int k = 0; k < KPL; k++)
+                        if ((vm >> k) & 1) L.freel[fpos++] = (uint16_t)(gl + GL * k);
+                }
+                wave_sync();
+                // Hole boundary: the edges of visible triangles whose neighbour is not visible, in
+                // the order of the visible list; its entry x is taken by lane x mod GL.  The edge's
+                // position in the neighbour u is that of u's vertex off the edge.
+                int nbtot = 0;
+                for (int x0 = 0; __ballot(step && x0 < nvis) != 0ull; x0 += GL) {
+                    const int x = x0 + gl;
+                    const bool own = step && x < nvis;
+                    uint32_t t = 0u;
+                    uint64_t o = 0ull;
+                    if (own) {
+                        const int slot = L.freel[nfree + x];
+                        t = L.tv[slot];
+                        o = L.to64[slot];
+                    }
+                    uint32_t tu[3] = {TF_VIS, TF_VIS, TF_VIS};
+                    if (own)
+#pragma unroll
+                        for (int i = 0; i < 3; i++) tu[i] = L.tv[(int)(o >> (16 * i)) & (TMAX - 1)];
+                    int nb = 0;
+#pragma unroll
+                    for (int i = 0; i < 3; i++) nb += (tu[i] & TF_VIS) ? 0 : 1;
+                    int tot;
+                    int e = nbtot + grp_excl<GL, 2>(nb, g, gl, &tot);
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        if (tu[i] & TF_VIS) continue;
+                        const int a = tvert(t, (i + 1) % 3), b = tvert(t, (i + 2) % 3);
+                        const int u0 = tvert(tu[i], 0), u1 = tvert(tu[i], 1);
+                        if (e < TMAX) {
+                            L.be_a[e] = (uint16_t)a;
+                            L.be_b[e] = (uint16_t)b;
+                            L.be_u[e] = (uint16_t)(o >> (16 * i));
+                            L.be_j[e] = (uint8_t)((u0 != a && u0 != b) ? 0 : ((u1 != a && u1 != b) ? 1 : 2));
+                        }
+                        e++;
+                    }
+                    nbtot += tot;
+                }
                 if (step && (nbtot > TMAX || nbtot < 3)) {
                     failed = true;
                     live = false;
                     step = false;
                 }
-                int fpos = nfree + vbase;
-#pragma unroll
-                for (int k = 0; k < KPL; k++) {
-                    const int slot = gl + GL * k;
-                    if (step && slot < T && (L.tv[slot] & TF_VIS)) {
-                        const uint32_t t = L.tv[slot];
-                        for (int i = 0; i < 3; i++) {
-                            const int u = L.to[slot][i];
-                            if (L.tv[u] & TF_VIS) continue;
-                            int j = 0;
-                            while (j < 2 && L.to[u][j] != slot) j++;
-                            L.be_a[e] = (uint16_t)tvert(t, (i + 1) % 3);
-                            L.be_b[e] = (uint16_t)tvert(t, (i + 2) % 3);
-                            L.be_u[e] = (uint16_t)u;
-                            L.be_j[e] = (uint8_t)j;
-                            e++;
-                        }
-                        L.freel[fpos++] = (uint16_t)slot;
-                    }
-                }
                 wave_sync();
                 const int nfree_tot = nfree + nvis;
 #pragma unroll
-                for (int k = 0; k < KPL; k++) {
-                    const int slot = gl + GL * k;
-                    if (step && slot < T && (L.tv[slot] & TF_VIS)) L.tv[slot] = 0u;
-                }
+                for (int k = 0; k < KPL; k++)
+                    if (step && ((vm >> k) & 1)) L.tv[gl + GL * k] = 0u;
                 if (step)
                     for (int k = gl; k < nbtot; k += GL)
                         L.be_slot[k] = (uint16_t)(k < nfree_tot ? L.freel[nfree_tot - 1 - k] : T + (k - nfree_tot));
@@ -1040,15 +1058,15 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                 }
                 wave_sync();
 #pragma unroll
+                for (int k = 0; k < KPL; k++) tk[k] = L.tv[gl + GL * k];
+#pragma unroll
                 for (int k = 0; k < KPL; k++) {
                     const int slot = gl + GL * k;
-                    if (step && slot < T) {
-                        const uint32_t t = L.tv[slot];
-                        if (t & TF_FRESH) {
-                            grp_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], C[k], P[k]);
-                            ncof++;
-                            L.tv[slot] = t & ~TF_FRESH;
-                        }
+                    if (step && slot < T && (tk[k] & TF_FRESH)) {
+                        const uint32_t t = tk[k];
+                        grp_cofactors(L.vc[tvert(t, 0)], L.vc[tvert(t, 1)], L.vc[tvert(t, 2)], C[k], P[k]);
+                        ncof++;
+                        L.tv[slot] = t & ~TF_FRESH;
                     }
                 }
                 wave_sync();
