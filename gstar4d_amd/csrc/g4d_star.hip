@@ -936,7 +936,7 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
 #pragma unroll
                 for (int k = 0; k < KPL; k++) {
                     const int slot = gl + GL * k;
-                    if (step && slot < T && (tk[k] & TF_ALIVE)) {
+                    if (step && slot < T && (tk[k] & (TF_ALIVE | TF_VIS)) == TF_ALIVE) {
                         nfilt++;
                         const int sg = grp_filter_sign(r, rs, C[k], P[k]);
                         if (sg == 0) unc = true;
@@ -1017,21 +1017,12 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                     }
                     nbtot += tot;
                 }
-                if (step && (nbtot > TMAX || nbtot < 3)) {
-                    failed = true;
-                    live = false;
-                    step = false;
-                }
-                wave_sync();
+                // The visible triangles keep their words, TF_VIS marking them dead, until a new
+                // triangle takes the slot: boundary edge k takes free slot nfree_tot - 1 - k (the
+                // visible ones first) or, past the free list, slot T + k - nfree_tot.
                 const int nfree_tot = nfree + nvis;
-#pragma unroll
-                for (int k = 0; k < KPL; k++)
-                    if (step && ((vm >> k) & 1)) L.tv[gl + GL * k] = 0u;
-                if (step)
-                    for (int k = gl; k < nbtot; k += GL)
-                        L.be_slot[k] = (uint16_t)(k < nfree_tot ? L.freel[nfree_tot - 1 - k] : T + (k - nfree_tot));
                 const int newT = T + (nbtot > nfree_tot ? nbtot - nfree_tot : 0);
-                if (step && (newT > TMAX || Vn + 1 > VMAX)) {
+                if (step && (nbtot > TMAX || nbtot < 3 || newT > TMAX || Vn + 1 > VMAX)) {
                     failed = true;
                     live = false;
                     step = false;
@@ -1039,7 +1030,9 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                 wave_sync();
                 if (step)
                     for (int k = gl; k < nbtot; k += GL) {
-                        const int slot = L.be_slot[k], a = L.be_a[k], b = L.be_b[k], u = L.be_u[k];
+                        const int slot = k < nfree_tot ? L.freel[nfree_tot - 1 - k] : T + (k - nfree_tot);
+                        const int a = L.be_a[k], b = L.be_b[k], u = L.be_u[k];
+                        L.be_slot[k] = (uint16_t)slot;
                         L.tv[slot] = tpack(a, b, Vn, TF_ALIVE | TF_FRESH);
                         L.to[slot][2] = (uint16_t)u;
                         L.to[u][L.be_j[k]] = (uint16_t)slot;
@@ -1056,7 +1049,8 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                     nfree = nfree_tot - (nbtot < nfree_tot ? nbtot : nfree_tot);
                     T = newT;
                 }
-                wave_sync();
+                // the link writes above touch only the new slots' neighbour rows: the cofactor
+                // pass below (triangle words and coordinates) needs no barrier after them
 #pragma unroll
                 for (int k = 0; k < KPL; k++) tk[k] = L.tv[gl + GL * k];
 #pragma unroll
@@ -1069,8 +1063,11 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                         L.tv[slot] = t & ~TF_FRESH;
                     }
                 }
-                wave_sync();
+                // No barrier before the next step: its filter reads and writes only this lane's
+                // own slots and the listed coordinates, and its first barrier precedes any read
+                // of another lane's words.
             }
+            wave_sync();
             // ---- outcome per group ----
             if (act && unsure && gl == 0) {
                 const unsigned long long k = atomicAdd(&cnt[p.exact_counter], 1ull);
@@ -1093,7 +1090,7 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                 const int slot = gl + GL * k;
                 if (fin && slot < T) {
                     const uint32_t t = L.tv[slot];
-                    if (t & TF_ALIVE) {
+                    if ((t & (TF_ALIVE | TF_VIS)) == TF_ALIVE) {
                         L.present[tvert(t, 0)] = 1;
                         L.present[tvert(t, 1)] = 1;
                         L.present[tvert(t, 2)] = 1;
@@ -1124,7 +1121,7 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
 #pragma unroll
             for (int b0 = 0; b0 < TMAX; b0 += GL) {
                 const int t = b0 + gl;
-                const bool al = fin && t < T && (L.tv[t] & TF_ALIVE);
+                const bool al = fin && t < T && (L.tv[t] & (TF_ALIVE | TF_VIS)) == TF_ALIVE;
                 int tot;
                 const int rk = grp_excl<GL, 1>(al ? 1 : 0, g, gl, &tot);
                 if (al) L.remap[t] = (uint16_t)(tcount + rk);
@@ -1147,7 +1144,7 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                 const int t = b0 + gl;
                 const uint32_t tt = (fin && t < T) ? L.tv[t] : 0u;
                 TriPairs pr{STAR_PAIR_NONE, STAR_PAIR_NONE};
-                if (tt & TF_ALIVE) {
+                if ((tt & (TF_ALIVE | TF_VIS)) == TF_ALIVE) {
                     const int a = tvert(tt, 0), b = tvert(tt, 1), c = tvert(tt, 2);
                     const int d = L.remap[t];
                     p.tpool[to + d] = hpack(L.vremap[a], L.vremap[b], L.vremap[c], L.remap[L.to[t][0]],
