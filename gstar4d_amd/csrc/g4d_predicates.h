@@ -662,6 +662,7 @@ __device__ __noinline__ int sos_orient4(const double *const *pin, const int *idx
 // Returns 0 when the forward-error bound cannot certify the sign.  Every monomial carries
 // at most 17 roundings (inputs 1+1+1+5 via the lift, minors 5, final sum 4), so the error
 // is below 17.1 u times the computed permanent; 32 u is used.
+__device__ __forceinline__ int filter_insphere_xyzl(const double *x, const double *y, const double *z, const double *l);
 __device__ __forceinline__ int filter_insphere(const double *const *p)
 {
     double x[4], y[4], z[4], l[4];
@@ -672,6 +673,26 @@ __device__ __forceinline__ int filter_insphere(const double *const *p)
         z[i] = p[i][2] - p[4][2];
         l[i] = x[i] * x[i] + y[i] * y[i] + z[i] * z[i];
     }
+    return filter_insphere_xyzl(x, y, z, l);
+}
+
+// filter_insphere from rows r[i] = (x, y, z, x^2 + y^2 + z^2) of p0..p3 already translated by
+// p4 with the same operations as above (rel_coords computes them so): the same sign, bit for bit.
+__device__ __forceinline__ int filter_insphere_rel(const double *const *r)
+{
+    double x[4], y[4], z[4], l[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        x[i] = r[i][0];
+        y[i] = r[i][1];
+        z[i] = r[i][2];
+        l[i] = r[i][3];
+    }
+    return filter_insphere_xyzl(x, y, z, l);
+}
+
+__device__ __forceinline__ int filter_insphere_xyzl(const double *x, const double *y, const double *z, const double *l)
+{
     double m2[4][4], q2[4][4];
 #pragma unroll
     for (int i = 0; i < 4; i++)

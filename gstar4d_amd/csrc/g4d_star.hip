@@ -846,9 +846,15 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
                 for (int q = gl; q < m; q += GL) {
                     const double dq = L.vc[vn + q][3];
                     int rank = 0;
-                    for (int k = 0; k < m; k++) {
-                        const double dk = L.vc[vn + k][3];
-                        rank += (dk < dq || (dk == dq && k < q)) ? 1 : 0;
+                    for (int k0 = 0; k0 < m; k0 += 4) { // four reads in flight (indices kept below m)
+                        double dk[4];
+#pragma unroll
+                        for (int i = 0; i < 4; i++) dk[i] = L.vc[vn + (k0 + i < m ? k0 + i : m - 1)][3];
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            const int k = k0 + i;
+                            rank += (k < m && (dk[i] < dq || (dk[i] == dq && k < q))) ? 1 : 0;
+                        }
                     }
                     L.ord[rank] = (uint8_t)q;
                 }
@@ -874,14 +880,11 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
             } else if (act) {
                 // initial 4-simplex (GD/GDelaunay/GDelPredKernels.cu:45-158) on the first four
                 // vertices of the insertion order; slot c[i] holds cone vertex i
-                int l[4], c[4];
-                for (int i = 0; i < 4; i++) {
-                    c[i] = L.ord[i];
-                    l[i] = L.ins[c[i]];
-                }
-                const double *pp[5] = {p.pts + 3 * (int64_t)l[0], p.pts + 3 * (int64_t)l[1], p.pts + 3 * (int64_t)l[2],
-                                       p.pts + 3 * (int64_t)l[3], p.pts + 3 * (int64_t)s};
-                const int o = filter_insphere(pp);
+                int c[4];
+                for (int i = 0; i < 4; i++) c[i] = L.ord[i];
+                // the cone's coordinates relative to the star point, as listed in LDS (vn = 0)
+                const double *pr[4] = {L.vc[c[0]], L.vc[c[1]], L.vc[c[2]], L.vc[c[3]]};
+                const int o = filter_insphere_rel(pr);
                 if (o == 0) {
                     unsure = true; // pushed to the exact list below, untouched
                 } else {
@@ -914,9 +917,11 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
             }
             // ---- insertions: every group steps through its list in lock-step ----
             const int qend = wave_max_i(live ? m : 0);
+            int jnext = L.ord[q0 < m ? q0 : 0];
             for (int q = q0; q < qend; q++) {
                 bool step = live && q < m;
-                const int j = step ? L.ord[q] : 0; // the list index inserted in this step
+                const int j = step ? jnext : 0; // the list index inserted in this step
+                jnext = L.ord[q + 1 < m ? q + 1 : 0]; // the next step's, read a step ahead
                 const int Vn = vn + j;
                 // this lane's triangle words, read together (every slot index is below TMAX)
                 uint32_t tk[KPL];
@@ -1084,19 +1089,22 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
             const int V = need; // drowned insertions' slots are never referenced and drop out
             if (fin)
                 for (int v = gl; v < V; v += GL) L.present[v] = 0;
-            wave_sync();
+            // this lane's live triangle words (0 for dead and unused slots), read together
+            uint32_t tf[KPL];
 #pragma unroll
             for (int k = 0; k < KPL; k++) {
                 const int slot = gl + GL * k;
-                if (fin && slot < T) {
-                    const uint32_t t = L.tv[slot];
-                    if ((t & (TF_ALIVE | TF_VIS)) == TF_ALIVE) {
-                        L.present[tvert(t, 0)] = 1;
-                        L.present[tvert(t, 1)] = 1;
-                        L.present[tvert(t, 2)] = 1;
-                    }
-                }
+                const uint32_t t = L.tv[slot];
+                tf[k] = (fin && slot < T && (t & (TF_ALIVE | TF_VIS)) == TF_ALIVE) ? t : 0u;
             }
+            wave_sync();
+#pragma unroll
+            for (int k = 0; k < KPL; k++)
+                if (tf[k]) {
+                    L.present[tvert(tf[k], 0)] = 1;
+                    L.present[tvert(tf[k], 1)] = 1;
+                    L.present[tvert(tf[k], 2)] = 1;
+                }
             wave_sync();
             if (fin)
                 for (int q = gl; q < m; q += GL) {
@@ -1121,7 +1129,7 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
 #pragma unroll
             for (int b0 = 0; b0 < TMAX; b0 += GL) {
                 const int t = b0 + gl;
-                const bool al = fin && t < T && (L.tv[t] & (TF_ALIVE | TF_VIS)) == TF_ALIVE;
+                const bool al = tf[b0 / GL] != 0u;
                 int tot;
                 const int rk = grp_excl<GL, 1>(al ? 1 : 0, g, gl, &tot);
                 if (al) L.remap[t] = (uint16_t)(tcount + rk);
@@ -1138,19 +1146,26 @@ __device__ __forceinline__ void star_body_grp(const StarParams &p, int nitems)
             }
             wave_sync();
             const int64_t pb = 2 * (to - p.tbase);
-            int npair = 0;
+            // every slot's output word and pairs first (their reads overlap), then the pair scans
+            TriPairs prk[KPL];
 #pragma unroll
-            for (int b0 = 0; b0 < TMAX; b0 += GL) {
-                const int t = b0 + gl;
-                const uint32_t tt = (fin && t < T) ? L.tv[t] : 0u;
-                TriPairs pr{STAR_PAIR_NONE, STAR_PAIR_NONE};
-                if ((tt & (TF_ALIVE | TF_VIS)) == TF_ALIVE) {
+            for (int k = 0; k < KPL; k++) {
+                const int t = gl + GL * k;
+                const uint32_t tt = fin ? tf[k] : 0u;
+                prk[k] = TriPairs{STAR_PAIR_NONE, STAR_PAIR_NONE};
+                if (tt) {
                     const int a = tvert(tt, 0), b = tvert(tt, 1), c = tvert(tt, 2);
                     const int d = L.remap[t];
-                    p.tpool[to + d] = hpack(L.vremap[a], L.vremap[b], L.vremap[c], L.remap[L.to[t][0]],
-                                            L.remap[L.to[t][1]], L.remap[L.to[t][2]], true);
-                    pr = star_tri_pairs(L.vid[a], L.vid[b], L.vid[c], !(tt & TF_CHECKED));
+                    const uint64_t o = L.to64[t];
+                    p.tpool[to + d] = hpack(L.vremap[a], L.vremap[b], L.vremap[c], L.remap[(int)(o & 0xffff)],
+                                            L.remap[(int)((o >> 16) & 0xffff)], L.remap[(int)((o >> 32) & 0xffff)], true);
+                    prk[k] = star_tri_pairs(L.vid[a], L.vid[b], L.vid[c], !(tt & TF_CHECKED));
                 }
+            }
+            int npair = 0;
+#pragma unroll
+            for (int k = 0; k < KPL; k++) {
+                const TriPairs pr = prk[k];
                 // the star's pairs, dense, in triangle order
                 const int np = (pr.x != STAR_PAIR_NONE ? 1 : 0) + (pr.y != STAR_PAIR_NONE ? 1 : 0);
                 int tot;
