@@ -82,15 +82,20 @@ __device__ __forceinline__ int g4d_wave_excl_count(bool pred, int *tot)
 }
 
 // Wave-inclusive scan of small non-negative ints via DPP-free shuffles.
+// Inclusive prefix sum over the 64 lanes: DPP row shifts (1, 2, 4, 8) within each row of 16
+// lanes, then the totals of rows 0-2 (lanes 15, 31, 47) read into scalars and added to the rows
+// above them.  No LDS traffic; every lane of the wave must be active.
 __device__ __forceinline__ int g4d_wave_incl_sum(int v)
 {
-    const int lane = __lane_id();
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int o = __shfl_up(v, off, 64);
-        if (lane >= off) v += o;
-    }
-    return v;
+    int x = v;
+    x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, false);
+    x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, false);
+    const int r0 = __builtin_amdgcn_readlane(x, 15), r1 = __builtin_amdgcn_readlane(x, 31);
+    const int r2 = __builtin_amdgcn_readlane(x, 47);
+    const int row = __lane_id() >> 4;
+    return x + (row >= 1 ? r0 : 0) + (row >= 2 ? r1 : 0) + (row >= 3 ? r2 : 0);
 }
 
 __device__ __forceinline__ int g4d_wave_sum(int v)

@@ -1020,13 +1020,8 @@ __global__ void __launch_bounds__(GP_THREADS) k_grid_pairs_dedup(const int *grid
         v[7] = grid_at(grid, g, x + 1, y + 1, z + 1);
         const uint32_t mask = cube_pairs(v, pr);
         const uint32_t c = (uint32_t)__popc(mask);
-        uint32_t inc = c;
-#pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const uint32_t t = __shfl_up(inc, o, 64);
-            if (lane >= o) inc += t;
-        }
-        const uint32_t tot = __shfl(inc, 63, 64);
+        const uint32_t inc = (uint32_t)g4d_wave_incl_sum((int)c);
+        const uint32_t tot = (uint32_t)__builtin_amdgcn_readlane((int)inc, 63);
         if (tot <= GP_STAGE) {
             uint32_t pos = inc - c;
 #pragma unroll
@@ -1055,12 +1050,7 @@ __global__ void __launch_bounds__(GP_THREADS) k_grid_pairs_dedup(const int *grid
         e[i] = tab[i * GP_THREADS + threadIdx.x];
         c += e[i] != GP_EMPTY;
     }
-    uint32_t inc = c;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t t = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += t;
-    }
+    const uint32_t inc = (uint32_t)g4d_wave_incl_sum((int)c);
     if (lane == 63) wsum[wid] = inc;
     __syncthreads();
     uint32_t base = 0, total = 0;
