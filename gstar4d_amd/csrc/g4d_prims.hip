@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t *keys,
 // digit a histogram sweep and a scatter sweep over tiles of 1024 keys in order (ballot-matched
 // ranks within a wave, wave offsets per digit, running digit offsets), ping-ponging between
 // keys and tmp; the result ends in keys.
-static constexpr int64_t SMALL_SORT = 16384;
+static constexpr int64_t SMALL_SORT = 4096;
 
 __global__ void __launch_bounds__(1024) k_sort_small(uint64_t *keys, uint64_t *tmp, int n, int lo_bit, int hi_bit,
                                                      int lo2, int hi2)
@@ -388,7 +388,7 @@ static constexpr int BK_THREADS = 1024;
 static constexpr int BK_MAX_BUCKETS = 16384;
 static constexpr int BK_TILE = 16384;  // keys per work-group of the count and scatter passes
 static constexpr int BK_TARGET = 4096;        // average bucket aimed for
-static constexpr int64_t BK_MIN_KEYS = 1 << 22; // shorter lists take the LSD sort
+static constexpr int64_t BK_MIN_KEYS = SMALL_SORT + 1; // shorter lists: one work-group
 
 __device__ __forceinline__ uint32_t bk_bucket(uint64_t key, int sh, int nbkt)
 {

@@ -1,9 +1,9 @@
 """The pair-key sort of the pipeline (directed keys a << 32 | b, a and b below 2^nb) against
 numpy's sort: uniform keys, duplicates, buckets beyond the LDS capacity (one id with a very
 long list, as for a star on a lattice or a dense cluster), few ids, runs of ids in input
-order, workset-like lists, and the short lists and wide ids that take the LSD sort.  The
-pipeline's choice (method 0), the LSD radix sort (1) and the bucket sort forced at any length
-(3) must give numpy's order."""
+order, workset-like lists, a list of a few thousand keys, and the short lists and wide ids that
+take the one-work-group and LSD sorts.  The pipeline's choice (method 0), the LSD radix sort (1)
+and the bucket sort forced at any length (3) must give numpy's order."""
 import numpy as np
 import pytest
 
@@ -41,8 +41,10 @@ def _cases():
     b = np.clip(a + rng.integers(-3000, 3000, a.shape[0]), 0, (1 << 21) - 1)
     w = np.concatenate([_keys(a, b), _keys(b, a)])
     yield "workset", w, 21
-    # the LSD fallbacks: a short list, and ids wider than 23 bits
-    yield "short", _keys(rng.integers(0, n, 9000), rng.integers(0, n, 9000)), nb
+    # a list of a few thousand keys (bucket sort), then the fallbacks: a list short enough for
+    # the one-work-group sort, and ids wider than 23 bits (LSD sort)
+    yield "mid", _keys(rng.integers(0, n, 9000), rng.integers(0, n, 9000)), nb
+    yield "short", _keys(rng.integers(0, n, 3000), rng.integers(0, n, 3000)), nb
     yield "wide_ids", _keys(rng.integers(0, 1 << 25, 200_000), rng.integers(0, 1 << 25, 200_000)), 25
 
 
