@@ -88,7 +88,10 @@ struct StarLds {
     int vid[VMAX];
     double vc[VMAX][4];
     uint32_t tv[TMAX];
-    uint16_t to[TMAX][3];
+    union {
+        uint16_t to[TMAX][4]; // neighbours across the edges opposite vertices 0-2 (fourth unused)
+        uint64_t to64[TMAX];  // the same rows read as one word each
+    };
     int16_t vstart[VMAX];
     int16_t vend[VMAX];
     uint16_t freel[TMAX];
@@ -365,13 +368,14 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                 const int Vn = vn + q;
                 const double r[4] = {L.vc[Vn][0], L.vc[Vn][1], L.vc[Vn][2], L.vc[Vn][3]};
                 int nv = 0;
+                unsigned vm = 0; // bit k: slot lane + 64 k sees the vertex
                 bool unc = false;
 #pragma unroll 1
                 for (int k = 0; k < KPL; k++) {
                     const int slot = lane + 64 * k;
                     if (slot < T) {
                         const uint32_t t = L.tv[slot];
-                        if (t & TF_ALIVE) {
+                        if ((t & (TF_ALIVE | TF_VIS)) == TF_ALIVE) {
                             nfilt++;
                             double c[8];
 #pragma unroll
@@ -391,6 +395,7 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                             }
                             if (sg < 0) {
                                 L.tv[slot] = t | TF_VIS;
+                                vm |= 1u << k;
                                 nv++;
                             }
                         }
@@ -400,62 +405,70 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                     unsure = true;
                     break;
                 }
-                int nvis;
-                const int vbase = wave_excl<count_bits(KPL)>(nv, &nvis);
+                const int vinc = g4d_wave_incl_sum(nv);
+                const int vbase = vinc - nv, nvis = __builtin_amdgcn_readlane(vinc, 63);
                 if (nvis == 0) continue; // drowned at insertion
                 if (lane == 0) L.ins_local[q] = (int16_t)Vn;
+                // the visible slots join the free list (positions nfree .. nfree + nvis - 1)
+                {
+                    int fpos = nfree + vbase;
+#pragma unroll 1
+                    for (int k = 0; k < KPL; k++)
+                        if ((vm >> k) & 1) L.freel[fpos++] = (uint16_t)(lane + 64 * k);
+                }
                 wave_sync();
-                // hole boundary: edges of freed triangles whose neighbour survives
-                int nb = 0;
-#pragma unroll 1
-                for (int k = 0; k < KPL; k++) {
-                    const int slot = lane + 64 * k;
-                    if (slot < T && (L.tv[slot] & TF_VIS))
-                        for (int i = 0; i < 3; i++) nb += (L.tv[L.to[slot][i]] & TF_VIS) ? 0 : 1;
-                }
-                int nbtot;
-                int e = wave_excl<count_bits(3 * KPL)>(nb, &nbtot);
-                if (nbtot > TMAX || nbtot < 3) {
-                    failed = true;
-                    break;
-                }
-                int fpos = nfree + vbase;
-#pragma unroll 1
-                for (int k = 0; k < KPL; k++) {
-                    const int slot = lane + 64 * k;
-                    if (slot < T && (L.tv[slot] & TF_VIS)) {
-                        const uint32_t t = L.tv[slot];
-                        for (int i = 0; i < 3; i++) {
-                            const int u = L.to[slot][i];
-                            if (L.tv[u] & TF_VIS) continue;
-                            int j = 0;
-                            while (j < 2 && L.to[u][j] != slot) j++;
-                            L.be_a[e] = (uint16_t)tvert(t, (i + 1) % 3);
-                            L.be_b[e] = (uint16_t)tvert(t, (i + 2) % 3);
-                            L.be_u[e] = (uint16_t)u;
-                            L.be_j[e] = (uint8_t)j;
-                            e++;
-                        }
-                        L.freel[fpos++] = (uint16_t)slot;
+                // Hole boundary: the edges of visible triangles whose neighbour is not visible, in
+                // the order of the visible list; entry x is taken by lane x mod 64.  The edge's
+                // position in the neighbour u is that of u's vertex off the edge.
+                int nbtot = 0;
+                for (int x0 = 0; x0 < nvis; x0 += 64) {
+                    const int x = x0 + lane;
+                    const bool own = x < nvis;
+                    uint32_t t = 0u;
+                    uint64_t o = 0ull;
+                    if (own) {
+                        const int slot = L.freel[nfree + x];
+                        t = L.tv[slot];
+                        o = L.to64[slot];
                     }
+                    uint32_t tu[3] = {TF_VIS, TF_VIS, TF_VIS};
+                    if (own)
+#pragma unroll
+                        for (int i = 0; i < 3; i++) tu[i] = L.tv[(int)(o >> (16 * i)) & (TMAX - 1)];
+                    int nb = 0;
+#pragma unroll
+                    for (int i = 0; i < 3; i++) nb += (tu[i] & TF_VIS) ? 0 : 1;
+                    const int binc = g4d_wave_incl_sum(nb);
+                    int e = nbtot + binc - nb;
+#pragma unroll
+                    for (int i = 0; i < 3; i++) {
+                        if (tu[i] & TF_VIS) continue;
+                        const int a = tvert(t, (i + 1) % 3), b = tvert(t, (i + 2) % 3);
+                        const int u0 = tvert(tu[i], 0), u1 = tvert(tu[i], 1);
+                        if (e < TMAX) {
+                            L.be_a[e] = (uint16_t)a;
+                            L.be_b[e] = (uint16_t)b;
+                            L.be_u[e] = (uint16_t)(o >> (16 * i));
+                            L.be_j[e] = (uint8_t)((u0 != a && u0 != b) ? 0 : ((u1 != a && u1 != b) ? 1 : 2));
+                        }
+                        e++;
+                    }
+                    nbtot += __builtin_amdgcn_readlane(binc, 63);
                 }
-                wave_sync();
+                // The visible triangles keep their words, TF_VIS marking them dead, until a new
+                // triangle takes the slot: boundary edge k takes free slot nfree_tot - 1 - k (the
+                // visible ones first) or, past the free list, slot T + k - nfree_tot.
                 const int nfree_tot = nfree + nvis;
-#pragma unroll 1
-                for (int k = 0; k < KPL; k++) {
-                    const int slot = lane + 64 * k;
-                    if (slot < T && (L.tv[slot] & TF_VIS)) L.tv[slot] = 0u;
-                }
-                for (int k = lane; k < nbtot; k += 64)
-                    L.be_slot[k] = (uint16_t)(k < nfree_tot ? L.freel[nfree_tot - 1 - k] : T + (k - nfree_tot));
                 const int newT = T + (nbtot > nfree_tot ? nbtot - nfree_tot : 0);
-                if (newT > TMAX || Vn + 1 > VMAX) {
+                if (nbtot > TMAX || nbtot < 3 || newT > TMAX || Vn + 1 > VMAX) {
                     failed = true;
                     break;
                 }
                 wave_sync();
                 for (int k = lane; k < nbtot; k += 64) {
-                    const int slot = L.be_slot[k], a = L.be_a[k], b = L.be_b[k], u = L.be_u[k];
+                    const int slot = k < nfree_tot ? L.freel[nfree_tot - 1 - k] : T + (k - nfree_tot);
+                    const int a = L.be_a[k], b = L.be_b[k], u = L.be_u[k];
+                    L.be_slot[k] = (uint16_t)slot;
                     L.tv[slot] = tpack(a, b, Vn, TF_ALIVE | TF_FRESH);
                     L.to[slot][2] = (uint16_t)u;
                     L.to[u][L.be_j[k]] = (uint16_t)slot;
@@ -470,7 +483,9 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                 }
                 nfree = nfree_tot - (nbtot < nfree_tot ? nbtot : nfree_tot);
                 T = newT;
-                wave_sync();
+                // the link writes above touch only the new slots' neighbour rows: the cofactor
+                // pass below (triangle words, coordinates, this lane's cofactor columns) needs no
+                // barrier after them
 #pragma unroll 1
                 for (int k = 0; k < KPL; k++) {
                     const int slot = lane + 64 * k;
@@ -484,8 +499,11 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                         }
                     }
                 }
-                wave_sync();
+                // No barrier before the next step: its filter reads and writes only this lane's
+                // own slots and cofactor columns and the listed coordinates, and its first barrier
+                // precedes any read of another lane's words.
             }
+            wave_sync();
             if (unsure) {
                 star_list_push(p.exact_list, p.exact_cap, &cnt[p.exact_counter], s, cnt);
                 wave_sync();
@@ -509,7 +527,7 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                 const int slot = lane + 64 * k;
                 if (slot < T) {
                     const uint32_t t = L.tv[slot];
-                    if (t & TF_ALIVE) {
+                    if ((t & (TF_ALIVE | TF_VIS)) == TF_ALIVE) {
                         L.present[tvert(t, 0)] = 1;
                         L.present[tvert(t, 1)] = 1;
                         L.present[tvert(t, 2)] = 1;
@@ -538,7 +556,7 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
             int tcount = 0;
             for (int base = 0; base < T; base += 64) {
                 const int t = base + lane;
-                const bool al = t < T && (L.tv[t] & TF_ALIVE);
+                const bool al = t < T && (L.tv[t] & (TF_ALIVE | TF_VIS)) == TF_ALIVE;
                 int tot;
                 const int rk = g4d_wave_excl_count(al, &tot);
                 if (al) L.remap[t] = (uint16_t)(tcount + rk);
@@ -562,7 +580,7 @@ __device__ __forceinline__ void star_body(const StarParams &p, int nitems)
                 uint32_t tt = 0;
                 if (t < T) tt = L.tv[t];
                 TriPairs pr{STAR_PAIR_NONE, STAR_PAIR_NONE};
-                if (t < T && (tt & TF_ALIVE)) {
+                if (t < T && (tt & (TF_ALIVE | TF_VIS)) == TF_ALIVE) {
                     const int a = tvert(tt, 0), b = tvert(tt, 1), c = tvert(tt, 2);
                     const int d = L.remap[t];
                     p.tpool[to + d] = hpack(L.vremap[a], L.vremap[b], L.vremap[c], L.remap[L.to[t][0]],
