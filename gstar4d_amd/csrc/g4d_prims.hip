@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(RS_THREADS) k_rs_scatter(const uint64_t *keys,
 // digit a histogram sweep and a scatter sweep over tiles of 1024 keys in order (ballot-matched
 // ranks within a wave, wave offsets per digit, running digit offsets), ping-ponging between
 // keys and tmp; the result ends in keys.
-static constexpr int64_t SMALL_SORT = 4096;
+static constexpr int64_t SMALL_SORT = 3072;
 
 __global__ void __launch_bounds__(1024) k_sort_small(uint64_t *keys, uint64_t *tmp, int n, int lo_bit, int hi_bit,
                                                      int lo2, int hi2)
@@ -556,6 +556,26 @@ __global__ void __launch_bounds__(T) k_bk_sort(uint64_t *src, uint64_t *dst, con
     const int bits = sb + nbb;
     const uint64_t hi = (uint64_t)blockIdx.x << sb; // the bucket's a values above the local bits
     uint64_t *res = (to_src ? src : dst) + o;
+    if (sz <= (uint32_t)T) {
+        // A bucket of at most one key per thread (short lists spread over many buckets): each key's
+        // place is the count of smaller keys plus that of equal keys before it, read from LDS.
+        const uint32_t i = threadIdx.x;
+        uint32_t key = 0u;
+        if (i < sz) {
+            key = bk_local(src[o + i], ash, smask, nbb, bmask);
+            buf[i] = key;
+        }
+        __syncthreads();
+        if (i < sz) {
+            uint32_t rk = 0;
+            for (uint32_t j = 0; j < sz; j++) {
+                const uint32_t kj = buf[j];
+                rk += (kj < key || (kj == key && j < i)) ? 1u : 0u;
+            }
+            res[rk] = ((hi | (key >> nbb)) << ash) | (key & bmask);
+        }
+        return;
+    }
     uint32_t kv[BKS_ITEMS], rank[BKS_ITEMS];
     const int ipw = (int)((sz + T - 1) / T);
     const int nvw = bk_wave_valid(sz, ipw, wid);
