@@ -742,14 +742,19 @@ __device__ __forceinline__ int grp_filter_sign(const double *r, double rs, const
     return val > bnd ? 1 : (-val > bnd ? -1 : 0);
 }
 
+// Largest value over the wave for non-negative v: maxima within each 16-lane row by DPP row
+// rotations (8, 4, 2, 1), then the four rows' maxima read into scalars.  Every lane must be
+// active.
 __device__ __forceinline__ int wave_max_i(int v)
 {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        const int o = __shfl_xor(v, off, 64);
-        v = o > v ? o : v;
-    }
-    return v;
+    int x = v;
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x128, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x124, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x122, 0xf, 0xf, false));
+    x = max(x, __builtin_amdgcn_update_dpp(0, x, 0x121, 0xf, 0xf, false));
+    const int r0 = __builtin_amdgcn_readlane(x, 0), r1 = __builtin_amdgcn_readlane(x, 16);
+    const int r2 = __builtin_amdgcn_readlane(x, 32), r3 = __builtin_amdgcn_readlane(x, 48);
+    return max(max(r0, r1), max(r2, r3));
 }
 
 template <int GL, bool PF>
