@@ -1744,12 +1744,18 @@ hipError_t launch_need_max(const StarParams &p, const int *list, int nitems, uns
 template <int GL>
 __device__ __forceinline__ int grp_min(int v)
 {
-#pragma unroll
-    for (int o = GL / 2; o > 0; o >>= 1) {
-        const int w = __shfl_xor(v, o, GL);
-        v = w < v ? w : v;
-    }
-    return v;
+    // minima within each 16-lane DPP row (rotations 8, 4, 2, 1); a whole-wave group then reads
+    // the four rows' minima into scalars.  Every lane of the group must be active.
+    static_assert(GL == 16 || GL == 64, "groups are DPP rows or the whole wave");
+    int x = v;
+    x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x128, 0xf, 0xf, false));
+    x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x124, 0xf, 0xf, false));
+    x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x122, 0xf, 0xf, false));
+    x = min(x, __builtin_amdgcn_update_dpp(x, x, 0x121, 0xf, 0xf, false));
+    if constexpr (GL == 64)
+        x = min(min(__builtin_amdgcn_readlane(x, 0), __builtin_amdgcn_readlane(x, 16)),
+                min(__builtin_amdgcn_readlane(x, 32), __builtin_amdgcn_readlane(x, 48)));
+    return x;
 }
 
 // One task per group of GL lanes: the filter pass takes 16 lanes per task (four tasks per
